@@ -1,0 +1,271 @@
+/*
+ * yar_gpu.h - C ABI of the MI355X path-tracing core.
+ *
+ * This is the drop-in boundary for the per-tile render loop of YAR's CPU reference renderer
+ * ("RAY", src/ref).  Every entry point is plain C: pointers, sizes and status codes; no C++ or
+ * torch types cross it.  All functions return YAR_OK (0) on success and a non-zero status
+ * otherwise; the message of the last failure is available from yar_last_error() (scene, kd-tree
+ * and generator calls, thread-local) or yar_gpu_last_error(ctx) (context calls).  Nothing in the
+ * library calls exit() or aborts: HIP failures become YAR_ERR_HIP.
+ *
+ * Reference interfaces replaced (file:line in kennyalive/YAR):
+ *   yar_scene_load_pbrt     Scene load_scene(const std::string&)            src/lib/scene_loader.cpp:74
+ *                           (pbrt3 subset, load_pbrt_scene src/lib/pbrt_scene.cpp:1050)
+ *   yar_kdtree_build_*      build_triangle_mesh_kdtree / build_scene_kdtree src/ref/kdtree_builder.cpp:425,443
+ *   yar_kdtree_save         KdTree::save (.kdtree cache file)               src/ref/kdtree.cpp:140
+ *   yar_gpu_create          init_scene_context                             src/ref/reference_renderer.h:48
+ *   yar_gpu_set_views       Scene_Overrides (camera pose / spp override)   src/ref/reference_renderer.h:23
+ *   yar_gpu_render          Image render_scene(const Scene_Context&, double* variance, float* time)
+ *                                                                          src/ref/reference_renderer.h:55
+ *   yar_gpu_render_tiles    static Film_Tile render_tile(...)              src/ref/reference_renderer.cpp:331
+ *   yar_film_tile_bounds    Film::get_tile_bounds                          src/ref/film.cpp:73
+ *   yar_gpu_trace_rays      KdTree::intersect / intersect_any (scene tree)  src/ref/kdtree.cpp:219,329
+ *   yar_gpu_render_batch    (batched multi-view form; Scene_Overrides applied to many views)
+ */
+#ifndef YAR_GPU_H
+#define YAR_GPU_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------- */
+#define YAR_OK                   0
+#define YAR_ERR_INVALID_ARGUMENT 1
+#define YAR_ERR_IO               2
+#define YAR_ERR_PARSE            3
+#define YAR_ERR_UNSUPPORTED      4
+#define YAR_ERR_HIP              5
+#define YAR_ERR_OUT_OF_MEMORY    6
+#define YAR_ERR_CAPACITY         7
+
+/* ---- scene description (host memory) ---------------------------------------------------
+ * Mirrors the parts of the reference Scene (src/lib/scene.h:29) the RAY path tracer reads.
+ * Matrices are row-major 3x4 (Matrix3x4::a[3][4], src/lib/matrix.h:7). */
+
+typedef struct yar_mesh_desc {          /* Triangle_Mesh, src/lib/triangle_mesh.h:13 */
+    const float*   vertices;            /* 3 * vertex_count */
+    const float*   normals;             /* 3 * vertex_count or NULL */
+    const float*   uvs;                 /* 2 * vertex_count or NULL */
+    const int32_t* indices;             /* 3 * triangle_count */
+    int32_t vertex_count;
+    int32_t triangle_count;
+    int32_t reverse_geometric_normal_orientation;
+    int32_t reserved;
+} yar_mesh_desc;
+
+#define YAR_MATERIAL_NONE    0           /* area-light objects carry no material */
+#define YAR_MATERIAL_DIFFUSE 1           /* Diffuse_Material  -> Diffuse_BRDF   (bsdf.cpp:60) */
+#define YAR_MATERIAL_METAL   2           /* Metal_Material    -> Metal_BRDF     (bsdf.cpp:146) */
+#define YAR_MATERIAL_GLASS   3           /* Glass_Material, roughness 0 -> delta (delta_scattering.cpp:53) */
+
+typedef struct yar_material_desc {
+    int32_t type;
+    int32_t roughness_is_alpha;         /* pbrt "remaproughness" false */
+    float   reflectance[3];             /* matte Kd / glass Kr */
+    float   transmittance[3];           /* glass Kt */
+    float   eta[3];                     /* metal eta (RGB) */
+    float   k[3];                       /* metal k (RGB) */
+    float   eta_i;                      /* metal: IOR of adjacent dielectric (1 for pbrt) */
+    float   u_roughness;                /* metal uroughness / glass roughness */
+    float   v_roughness;                /* metal vroughness */
+    float   index_of_refraction;        /* glass */
+} yar_material_desc;
+
+typedef struct yar_object_desc {        /* Scene_Object, src/lib/scene_object.h:8 */
+    int32_t mesh;                       /* index into meshes */
+    int32_t material;                   /* index into materials, -1 for area lights */
+    int32_t area_light;                 /* index into rect_lights, -1 if none */
+    int32_t reserved;
+    float   object_to_world[12];
+    float   world_to_object[12];
+} yar_object_desc;
+
+typedef struct yar_rect_light_desc {    /* Diffuse_Rectangular_Light, src/lib/light.h:60 */
+    float light_to_world[12];
+    float emitted_radiance[3];
+    float size[2];
+    int32_t sample_count;
+} yar_rect_light_desc;
+
+typedef struct yar_point_light_desc {   /* Point_Light, src/lib/light.h:33 */
+    float position[3];
+    float intensity[3];
+} yar_point_light_desc;
+
+typedef struct yar_render_config {      /* Raytracer_Config, src/lib/raytracer_config.h:5 */
+    int32_t film_width, film_height;
+    int32_t region_x0, region_y0, region_x1, region_y1;   /* render region, exclusive p1 */
+    int32_t x_pixel_sample_count, y_pixel_sample_count;
+    int32_t max_light_bounces;
+    int32_t russian_roulette_bounce_count_threshold;
+    float   russian_roulette_threshold;
+    float   film_radiance_scale;
+    float   max_rgb_component_value_of_film_sample;        /* +inf when unset */
+    float   pixel_filter_radius;                           /* box filter only */
+    uint32_t rng_seed_offset;                              /* --seedoffset */
+    int32_t pbrt3_scene;                                   /* enables the pbrt3 roughness remap */
+} yar_render_config;
+
+typedef struct yar_camera_desc {
+    float   camera_to_world[12];        /* columns: right, up/forward, back/up, position */
+    float   fov_y_degrees;
+    int32_t z_is_up;
+} yar_camera_desc;
+
+typedef struct yar_scene_desc {
+    const yar_mesh_desc*        meshes;       int32_t mesh_count;
+    const yar_material_desc*    materials;    int32_t material_count;
+    const yar_object_desc*      objects;      int32_t object_count;
+    const yar_rect_light_desc*  rect_lights;  int32_t rect_light_count;
+    const yar_point_light_desc* point_lights; int32_t point_light_count;
+    yar_camera_desc   camera;
+    yar_render_config config;
+} yar_scene_desc;
+
+/* ---- scene loading / generation -------------------------------------------------------- */
+typedef struct yar_scene yar_scene;
+
+/* Loads the pbrt3 subset the RAY path tracer consumes (trianglemesh, matte/metal/glass,
+ * diffuse area lights, point lights, perspective camera, Film, Sampler, Integrator "path",
+ * PixelFilter "box") with the reference's import semantics. */
+int  yar_scene_load_pbrt(const char* path, yar_scene** out_scene);
+const yar_scene_desc* yar_scene_get_desc(const yar_scene* scene);
+void yar_scene_free(yar_scene* scene);
+const char* yar_last_error(void);
+
+/* Writes the seeded benchmark scene `config` (1..5) as a pbrt3 file.  `scale` < 1 shrinks
+ * triangle counts for quick tests (1 = the full configuration). */
+int  yar_scene_generate(int32_t config, uint64_t seed, double scale, const char* pbrt_path);
+
+/* ---- kd-tree builder (host) -------------------------------------------------------------
+ * Byte-compatible with the reference KdNode encoding (src/ref/kdtree.h:18): node = 2 x uint32,
+ * interior word0 = axis | above_child << 2, word1 = split (f32 bits); leaf word0 = 3 | count << 2,
+ * word1 = first primitive index, the rest packed in the following words. */
+typedef struct yar_kdtree yar_kdtree;
+int  yar_kdtree_build_mesh(const float* vertices, const int32_t* indices, int32_t triangle_count,
+                           yar_kdtree** out_tree);
+/* boxes: 6 floats per primitive (min xyz, max xyz); builds the object-level (scene) tree. */
+int  yar_kdtree_build_boxes(const float* boxes, int32_t count, yar_kdtree** out_tree);
+uint32_t        yar_kdtree_node_count(const yar_kdtree* tree);
+const uint32_t* yar_kdtree_nodes(const yar_kdtree* tree);       /* 2 * node_count words */
+void            yar_kdtree_bounds(const yar_kdtree* tree, float out_bounds[6]);
+int32_t         yar_kdtree_max_depth(const yar_kdtree* tree);    /* deepest leaf (root = 0) */
+int  yar_kdtree_save(const yar_kdtree* tree, const char* path);  /* reference .kdtree layout */
+void yar_kdtree_free(yar_kdtree* tree);
+
+/* ---- GPU context --------------------------------------------------------------------- */
+typedef struct yar_gpu_ctx yar_gpu_ctx;
+
+typedef struct yar_gpu_options {
+    int32_t device;                     /* HIP device ordinal */
+    int32_t block_size;                 /* 0 = autotuned / default */
+    int64_t max_paths_in_flight;        /* 0 = default (sized from free HBM) */
+    int32_t traversal_stack_depth;      /* 0 = exact depth of the built trees */
+    int32_t enable_counters;            /* count node visits / triangle tests per ray */
+    int32_t use_graph;                  /* capture the bounce loop in a hipGraph */
+    int32_t reserved[5];
+} yar_gpu_options;
+
+typedef struct yar_view {               /* Scene_Overrides analog, reference_renderer.h:23 */
+    float    camera_to_world[12];
+    float    fov_y_degrees;
+    int32_t  z_is_up;
+    int32_t  film_width, film_height;
+    int32_t  region_x0, region_y0, region_x1, region_y1;
+    int32_t  x_pixel_sample_count, y_pixel_sample_count;
+    uint32_t rng_seed_offset;
+    int32_t  reserved;
+} yar_view;
+
+typedef struct yar_film_pixel {         /* Film_Pixel, src/ref/film.h:26 (16 bytes) */
+    float color_sum[3];
+    float weight_sum;
+} yar_film_pixel;
+
+typedef struct yar_ray {                /* Ray + tmax, 32 bytes */
+    float origin[3];
+    float tmax;                         /* closest: initial Intersection::t; any-hit: ray_tmax */
+    float direction[3];
+    uint32_t reserved;
+} yar_ray;
+
+typedef struct yar_hit {                /* Intersection subset, 24 bytes */
+    float   t;                          /* +inf on miss (closest); 1/0 in t for any-hit */
+    int32_t object;                     /* scene object index, -1 on miss */
+    int32_t triangle;                   /* mesh-local triangle index */
+    float   barycentrics[3];
+} yar_hit;
+
+typedef struct yar_render_stats {
+    double  render_seconds;             /* host wall time of the call */
+    double  gpu_seconds;                /* HIP-event time of the bounce loop */
+    int64_t samples;                    /* camera samples (pixels * spp) */
+    int64_t closest_rays;               /* scene-tree closest-hit queries (trace_ray + MIS) */
+    int64_t any_rays;                   /* shadow (any-hit) queries */
+    int64_t node_visits;                /* only with enable_counters */
+    int64_t leaf_refs;
+    int64_t triangle_tests;
+    int64_t object_visits;
+    double  extend_seconds, shade_seconds, shadow_seconds, mis_seconds, other_seconds;
+} yar_render_stats;
+
+int  yar_gpu_create(const yar_scene_desc* scene, const yar_gpu_options* options, yar_gpu_ctx** out_ctx);
+void yar_gpu_destroy(yar_gpu_ctx* ctx);
+const char* yar_gpu_last_error(const yar_gpu_ctx* ctx);
+
+/* View 0 is the scene's own camera/config; set_views replaces the list (n >= 1). */
+int  yar_gpu_set_views(yar_gpu_ctx* ctx, const yar_view* views, int32_t count);
+int  yar_gpu_get_view(const yar_gpu_ctx* ctx, int32_t index, yar_view* out_view);
+
+/* Renders the whole render region of a view.  out_film (region_w*region_h Film_Pixels, merged
+ * in tile-index order like Film::merge_tile) and out_image (region_w*region_h*3 floats,
+ * Film::get_image) may each be NULL. */
+int  yar_gpu_render(yar_gpu_ctx* ctx, int32_t view, yar_film_pixel* out_film, float* out_image,
+                    double* out_variance, yar_render_stats* out_stats);
+
+/* Tile count / bounds of a view's film (Tile_Size 64, film.cpp:5). Bounds are {x0,y0,x1,y1}. */
+int  yar_film_tile_count(const yar_gpu_ctx* ctx, int32_t view, int32_t* out_count);
+int  yar_film_tile_bounds(const yar_gpu_ctx* ctx, int32_t view, int32_t tile_index,
+                          int32_t out_sample_bounds[4], int32_t out_pixel_bounds[4]);
+/* Renders the given tiles; out_tiles[i] must hold pixel_bounds area Film_Pixels of tile i, and
+ * out_tile_variance[i] receives the tile's summed per-pixel luminance variance. */
+int  yar_gpu_render_tiles(yar_gpu_ctx* ctx, int32_t view, const int32_t* tile_indices, int32_t count,
+                          yar_film_pixel* const* out_tiles, double* out_tile_variance,
+                          yar_render_stats* out_stats);
+
+/* Batched form: renders several views in one persistent bounce loop.  out_images[i] receives the
+ * view's resolved image (region area * 3 floats) and may be NULL. */
+int  yar_gpu_render_batch(yar_gpu_ctx* ctx, const int32_t* views, int32_t count,
+                          float* const* out_images, double* out_variances, yar_render_stats* out_stats);
+
+/* Scene-tree ray queries. Host buffers; any_hit != 0 runs intersect_any (t = 1 if occluded). */
+int  yar_gpu_trace_rays(yar_gpu_ctx* ctx, const yar_ray* rays, int64_t count, int32_t any_hit,
+                        yar_hit* out_hits);
+/* Same, on device-resident buffers and the given HIP stream (NULL = the context's stream). */
+int  yar_gpu_trace_rays_device(yar_gpu_ctx* ctx, const yar_ray* d_rays, int64_t count, int32_t any_hit,
+                               yar_hit* d_hits, void* hip_stream);
+/* Per-ray traversal counters of the last trace_rays call (requires enable_counters). */
+int  yar_gpu_last_trace_counters(yar_gpu_ctx* ctx, yar_render_stats* out_stats);
+
+/* Device/scene summary: bytes of HBM used by the flattened scene, tree depths. */
+typedef struct yar_gpu_info {
+    int64_t scene_bytes;
+    int64_t triangle_count;
+    int32_t object_count;
+    int32_t scene_tree_depth;
+    int32_t mesh_tree_depth;
+    int32_t stack_depth;
+    double  build_seconds;
+    double  upload_seconds;
+    char    device_name[64];
+} yar_gpu_info;
+int  yar_gpu_get_info(const yar_gpu_ctx* ctx, yar_gpu_info* out_info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YAR_GPU_H */

@@ -1,0 +1,896 @@
+// context.hip - GPU context of the path-tracing core and its C ABI (include/yar_gpu.h).
+//
+// yar_gpu_create flattens a yar_scene_desc into HBM (kd-trees built on the host, byte-identical
+// to the reference builder), yar_gpu_render / yar_gpu_render_tiles run the wavefront bounce loop
+// over chunks of pixels and assemble Film_Tiles exactly as render_scene does
+// (kennyalive/YAR src/ref/reference_renderer.cpp:479-584, film.cpp:73-118), and
+// yar_gpu_trace_rays exposes the scene-tree queries (kdtree.cpp:219-439) for parity checks.
+#include "kernels.hpp"
+#include "../host/kdtree_builder.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+using namespace yg;
+
+struct HipError {
+    std::string msg;
+};
+
+#define HIPX(expr)                                                                                   \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            throw HipError{std::string(#expr) + ": " + hipGetErrorString(e_)};                       \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void reserve(size_t count) {
+        if (count <= n) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        HIPX(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)));
+        n = count;
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) {
+        reserve(v.size());
+        if (!v.empty()) HIPX(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+};
+
+static int pow2_at_least(int v) { int p = 1; while (p < v) p <<= 1; return p; }
+
+struct View {
+    yar_view v;
+};
+
+}  // namespace
+
+struct yar_gpu_ctx {
+    std::string error;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    yar_gpu_options opt{};
+    yar_render_config cfg{};
+    yar_gpu_info info{};
+    std::vector<yar_view> views;
+
+    // scene buffers
+    DevBuf<uint2> scene_nodes, mesh_nodes;
+    DevBuf<ObjTrav> objs;
+    DevBuf<float4> tri_verts, obj_o2w, obj_nrm, vtx_n;
+    DevBuf<int4> obj_info, mesh_info, tri_vidx;
+    DevBuf<float2> vtx_uv;
+    DevBuf<DevMaterial> materials;
+    DevBuf<DevRect> rects;
+    DevBuf<DevPoint> points;
+    DevScene scene{};
+    std::vector<int32_t> tri_base;        // per object (for hit conversion on the host)
+
+    // render buffers
+    DevBuf<int2> pixels;
+    DevBuf<float> table;
+    DevBuf<float4> ray_o, ray_d, coeff, L, dl_coeff, dl_light, dl_bsdf, hit_tb, radiance;
+    DevBuf<int2> hit_ot;
+    DevBuf<uint32_t> state;
+    DevBuf<uint64_t> rng;
+    DevBuf<uint32_t> q_a, q_b, sh_path, mi_path, dl_path, counters;
+    DevBuf<float4> sh_o, sh_d, sh_c, mi_o, mi_d, mi_f;
+    DevBuf<float> mi_ndw;
+    DevBuf<unsigned long long> totals;
+    DevBuf<float4> acc_sum, out_film, side_rad;
+    DevBuf<double2> acc_lum;
+    DevBuf<double> out_var;
+    DevBuf<int4> side_info;
+    uint32_t* h_counters = nullptr;       // pinned
+    // trace_rays buffers
+    DevBuf<float4> tr_rays, tr_tb;
+    DevBuf<int2> tr_ot;
+    yar_render_stats last_trace{};
+
+    ~yar_gpu_ctx() {
+        if (device >= 0) (void)hipSetDevice(device);
+        DevBuf<uint2>* b2[] = {&scene_nodes, &mesh_nodes};
+        for (auto* b : b2) b->release();
+        objs.release();
+        DevBuf<float4>* b4[] = {&tri_verts, &obj_o2w, &obj_nrm, &vtx_n, &ray_o, &ray_d, &coeff, &L, &dl_coeff,
+                                &dl_light, &dl_bsdf, &hit_tb, &radiance, &sh_o, &sh_d, &sh_c, &mi_o, &mi_d,
+                                &mi_f, &acc_sum, &out_film, &side_rad, &tr_rays, &tr_tb};
+        for (auto* b : b4) b->release();
+        DevBuf<int4>* bi4[] = {&obj_info, &mesh_info, &tri_vidx, &side_info};
+        for (auto* b : bi4) b->release();
+        vtx_uv.release(); materials.release(); rects.release(); points.release();
+        pixels.release(); table.release(); hit_ot.release(); tr_ot.release(); state.release(); rng.release();
+        DevBuf<uint32_t>* bu[] = {&q_a, &q_b, &sh_path, &mi_path, &dl_path, &counters};
+        for (auto* b : bu) b->release();
+        mi_ndw.release(); totals.release(); acc_lum.release(); out_var.release();
+        if (h_counters) (void)hipHostFree(h_counters);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    LaunchCfg launch_cfg() const {
+        LaunchCfg lc;
+        lc.block = opt.block_size > 0 ? opt.block_size : 128;
+        lc.scene_stack = info.stack_depth >> 16;
+        lc.mesh_stack = info.stack_depth & 0xffff;
+        lc.counters = opt.enable_counters != 0;
+        lc.stream = stream;
+        return lc;
+    }
+};
+
+namespace {
+
+static int fail(yar_gpu_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->error = msg;
+    yar::set_last_error(msg);
+    return code;
+}
+
+static void m34_rows(const float* m, float4& r0, float4& r1, float4& r2) {
+    r0 = make_float4(m[0], m[1], m[2], m[3]);
+    r1 = make_float4(m[4], m[5], m[6], m[7]);
+    r2 = make_float4(m[8], m[9], m[10], m[11]);
+}
+
+static float pbrt3_remap(float r, const yar_render_config& cfg, bool is_alpha) {   // scattering.cpp:240-269
+    if (is_alpha) return r;
+    if (cfg.pbrt3_scene) {
+        r = std::max(r, 1e-3f);
+        float x = std::log(r);
+        return 1.621420000f + 0.819955000f * x + 0.173400000f * x * x + 0.017120100f * x * x * x +
+               0.000640711f * x * x * x * x;
+    }
+    return r * r;
+}
+
+static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
+    using namespace yar;
+    auto t0 = std::chrono::steady_clock::now();
+    int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<KdBuildResult> trees = build_mesh_kdtrees(d, threads);
+    // scene tree over object world bounds (build_scene_kdtree, kdtree_builder.cpp:443-463)
+    std::vector<float> boxes(6 * (size_t)d->object_count);
+    for (int i = 0; i < d->object_count; i++) {
+        const yar_object_desc& o = d->objects[i];
+        const KdBuildResult& t = trees[o.mesh];
+        yar::M34 m;
+        std::memcpy(m.a, o.object_to_world, sizeof(m.a));
+        float lo[3] = {kInf, kInf, kInf}, hi[3] = {-kInf, -kInf, -kInf};
+        for (int k = 0; k < 8; k++) {
+            F3 c((k & 4) ? t.bounds.hi.x : t.bounds.lo.x, (k & 2) ? t.bounds.hi.y : t.bounds.lo.y,
+                 (k & 1) ? t.bounds.hi.z : t.bounds.lo.z);
+            F3 w = yar::transform_point(m, c);
+            for (int a = 0; a < 3; a++) { lo[a] = std::min(lo[a], w[a]); hi[a] = std::max(hi[a], w[a]); }
+        }
+        for (int a = 0; a < 3; a++) { boxes[6 * (size_t)i + a] = lo[a]; boxes[6 * (size_t)i + 3 + a] = hi[a]; }
+    }
+    KdBuildResult stree = build_box_kdtree(boxes.data(), d->object_count);
+    auto t1 = std::chrono::steady_clock::now();
+
+    // flatten meshes
+    std::vector<int64_t> mtri(d->mesh_count + 1, 0), mvtx(d->mesh_count + 1, 0), mnode(d->mesh_count + 1, 0);
+    for (int i = 0; i < d->mesh_count; i++) {
+        mtri[i + 1] = mtri[i] + d->meshes[i].triangle_count;
+        mvtx[i + 1] = mvtx[i] + d->meshes[i].vertex_count;
+        mnode[i + 1] = mnode[i] + (int64_t)(trees[i].words.size() / 2);
+    }
+    if (mnode[d->mesh_count] >= (int64_t)0x40000000 || mtri[d->mesh_count] >= (int64_t)0x7fffffff)
+        throw HipError{"scene exceeds 2^30 kd nodes or 2^31 triangles"};
+    std::vector<float4> tv(3 * (size_t)mtri[d->mesh_count]);
+    std::vector<int4> tvi(mtri[d->mesh_count]);
+    std::vector<float4> vn(std::max<int64_t>(1, mvtx[d->mesh_count]), make_float4(0, 0, 0, 0));
+    std::vector<float2> vuv(std::max<int64_t>(1, mvtx[d->mesh_count]), make_float2(0, 0));
+    std::vector<int4> minfo(d->mesh_count);
+    std::vector<uint2> mn(std::max<int64_t>(1, mnode[d->mesh_count]));
+    int mesh_depth = 0;
+    for (int i = 0; i < d->mesh_count; i++) {
+        const yar_mesh_desc& m = d->meshes[i];
+        for (int t = 0; t < m.triangle_count; t++) {
+            int64_t g = mtri[i] + t;
+            int vi[3];
+            for (int k = 0; k < 3; k++) {
+                vi[k] = m.indices[3 * (size_t)t + k];
+                const float* p = m.vertices + 3 * (size_t)vi[k];
+                tv[3 * (size_t)g + k] = make_float4(p[0], p[1], p[2], 0.f);
+            }
+            tvi[g] = make_int4((int)(mvtx[i] + vi[0]), (int)(mvtx[i] + vi[1]), (int)(mvtx[i] + vi[2]), 0);
+        }
+        for (int v = 0; v < m.vertex_count; v++) {
+            if (m.normals) vn[mvtx[i] + v] = make_float4(m.normals[3 * v], m.normals[3 * v + 1], m.normals[3 * v + 2], 0.f);
+            if (m.uvs) vuv[mvtx[i] + v] = make_float2(m.uvs[2 * v], m.uvs[2 * v + 1]);
+        }
+        minfo[i] = make_int4((m.normals ? 1 : 0) | (m.uvs ? 2 : 0) | (m.reverse_geometric_normal_orientation ? 4 : 0), 0, 0, 0);
+        // rebase the mesh tree: above_child and leaf triangle ids become global indices
+        const std::vector<uint32_t>& w = trees[i].words;
+        size_t nn = w.size() / 2;
+        uint32_t nb = (uint32_t)mnode[i], tb = (uint32_t)mtri[i];
+        size_t j = 0;
+        while (j < nn) {
+            uint32_t w0 = w[2 * j], w1 = w[2 * j + 1];
+            if ((w0 & 3u) != 3u) {
+                mn[nb + j] = make_uint2((w0 & 3u) | (((w0 >> 2) + nb) << 2), w1);
+                j++;
+            } else {
+                uint32_t cnt = w0 >> 2;
+                size_t span = 1 + (cnt > 1 ? cnt / 2 : 0);
+                std::vector<uint32_t> ws(w.begin() + 2 * j, w.begin() + 2 * (j + span));
+                for (uint32_t q = 0; q < cnt; q++) ws[1 + q] += tb;
+                for (size_t q = 0; q < span; q++) mn[nb + j + q] = make_uint2(ws[2 * q], ws[2 * q + 1]);
+                j += span;
+            }
+        }
+        mesh_depth = std::max(mesh_depth, trees[i].depth);
+    }
+    std::vector<uint2> sn(stree.words.size() / 2);
+    for (size_t j = 0; j < sn.size(); j++) sn[j] = make_uint2(stree.words[2 * j], stree.words[2 * j + 1]);
+
+    // objects
+    std::vector<ObjTrav> ot(d->object_count);
+    std::vector<float4> o2w(3 * (size_t)d->object_count), nrm(3 * (size_t)d->object_count);
+    std::vector<int4> oinfo(d->object_count);
+    ctx->tri_base.assign(d->object_count, 0);
+    for (int i = 0; i < d->object_count; i++) {
+        const yar_object_desc& o = d->objects[i];
+        const KdBuildResult& t = trees[o.mesh];
+        ObjTrav& r = ot[i];
+        m34_rows(o.world_to_object, r.w2o0, r.w2o1, r.w2o2);
+        uint32_t root = (uint32_t)mnode[o.mesh], tb = (uint32_t)mtri[o.mesh];
+        float rootf, tbf;
+        std::memcpy(&rootf, &root, 4);
+        std::memcpy(&tbf, &tb, 4);
+        r.lo_node = make_float4(t.bounds.lo.x, t.bounds.lo.y, t.bounds.lo.z, rootf);
+        r.hi_tri = make_float4(t.bounds.hi.x, t.bounds.hi.y, t.bounds.hi.z, tbf);
+        m34_rows(o.object_to_world, o2w[3 * i], o2w[3 * i + 1], o2w[3 * i + 2]);
+        // object_to_world_normal_transform = transpose of world_to_object's 3x3 (scene_loader.cpp:56-62)
+        const float* w2o = o.world_to_object;
+        nrm[3 * i] = make_float4(w2o[0], w2o[4], w2o[8], 0.f);
+        nrm[3 * i + 1] = make_float4(w2o[1], w2o[5], w2o[9], 0.f);
+        nrm[3 * i + 2] = make_float4(w2o[2], w2o[6], w2o[10], 0.f);
+        oinfo[i] = make_int4(o.material, o.area_light, o.mesh, (int)tb);
+        ctx->tri_base[i] = (int32_t)tb;
+    }
+    std::vector<DevMaterial> mats(std::max(1, d->material_count));
+    for (int i = 0; i < d->material_count; i++) {
+        const yar_material_desc& m = d->materials[i];
+        DevMaterial& g = mats[i];
+        std::memset(&g, 0, sizeof(g));
+        g.type = m.type;
+        g.eta_i = m.eta_i;
+        g.alpha_x = pbrt3_remap(m.u_roughness, d->config, m.roughness_is_alpha != 0);
+        g.alpha_y = pbrt3_remap(m.v_roughness, d->config, m.roughness_is_alpha != 0);
+        g.refl = make_float4(m.reflectance[0], m.reflectance[1], m.reflectance[2], 0.f);
+        g.trans = make_float4(m.transmittance[0], m.transmittance[1], m.transmittance[2], m.index_of_refraction);
+        g.eta = make_float4(m.eta[0], m.eta[1], m.eta[2], 0.f);
+        g.k = make_float4(m.k[0], m.k[1], m.k[2], 0.f);
+        if (m.type == YAR_MATERIAL_GLASS && m.u_roughness > 0.f) throw HipError{"rough glass is not supported"};
+        if (m.type < YAR_MATERIAL_DIFFUSE || m.type > YAR_MATERIAL_GLASS) throw HipError{"unsupported material type"};
+    }
+    std::vector<DevRect> rl(std::max(1, d->rect_light_count));
+    for (int i = 0; i < d->rect_light_count; i++) {
+        const yar_rect_light_desc& l = d->rect_lights[i];
+        m34_rows(l.light_to_world, rl[i].l0, rl[i].l1, rl[i].l2);
+        rl[i].L = make_float4(l.emitted_radiance[0], l.emitted_radiance[1], l.emitted_radiance[2], 0.f);
+        rl[i].size = make_float4(l.size[0], l.size[1], 0.f, 0.f);
+    }
+    std::vector<DevPoint> pl(std::max(1, d->point_light_count));
+    for (int i = 0; i < d->point_light_count; i++) {
+        const yar_point_light_desc& l = d->point_lights[i];
+        pl[i].p = make_float4(l.position[0], l.position[1], l.position[2], 0.f);
+        pl[i].I = make_float4(l.intensity[0], l.intensity[1], l.intensity[2], 0.f);
+    }
+
+    hipStream_t s = ctx->stream;
+    ctx->scene_nodes.upload(sn, s);
+    ctx->mesh_nodes.upload(mn, s);
+    ctx->objs.upload(ot, s);
+    ctx->tri_verts.upload(tv, s);
+    ctx->tri_vidx.upload(tvi, s);
+    ctx->vtx_n.upload(vn, s);
+    ctx->vtx_uv.upload(vuv, s);
+    ctx->mesh_info.upload(minfo, s);
+    ctx->obj_o2w.upload(o2w, s);
+    ctx->obj_nrm.upload(nrm, s);
+    ctx->obj_info.upload(oinfo, s);
+    ctx->materials.upload(mats, s);
+    ctx->rects.upload(rl, s);
+    ctx->points.upload(pl, s);
+    HIPX(hipStreamSynchronize(s));
+    auto t2 = std::chrono::steady_clock::now();
+
+    DevScene& S = ctx->scene;
+    S.scene_nodes = ctx->scene_nodes.p;
+    S.mesh_nodes = ctx->mesh_nodes.p;
+    S.objs = ctx->objs.p;
+    S.tri_verts = ctx->tri_verts.p;
+    S.scene_lo = make_float4(stree.bounds.lo.x, stree.bounds.lo.y, stree.bounds.lo.z, 0.f);
+    S.scene_hi = make_float4(stree.bounds.hi.x, stree.bounds.hi.y, stree.bounds.hi.z, 0.f);
+    S.obj_o2w = ctx->obj_o2w.p;
+    S.obj_nrm = ctx->obj_nrm.p;
+    S.obj_info = ctx->obj_info.p;
+    S.mesh_info = ctx->mesh_info.p;
+    S.tri_vidx = ctx->tri_vidx.p;
+    S.vtx_n = ctx->vtx_n.p;
+    S.vtx_uv = ctx->vtx_uv.p;
+    S.materials = ctx->materials.p;
+    S.rects = ctx->rects.p;
+    S.points = ctx->points.p;
+    S.n_rects = d->rect_light_count;
+    S.n_points = d->point_light_count;
+    S.n_lights = d->rect_light_count + d->point_light_count;     // update_total_light_count, light.cpp:35
+
+    yar_gpu_info& inf = ctx->info;
+    inf.scene_bytes = (int64_t)(sn.size() * 8 + mn.size() * 8 + ot.size() * sizeof(ObjTrav) + tv.size() * 16 +
+                                tvi.size() * 16 + vn.size() * 16 + vuv.size() * 8 + o2w.size() * 32 + oinfo.size() * 16);
+    inf.triangle_count = mtri[d->mesh_count];
+    inf.object_count = d->object_count;
+    inf.scene_tree_depth = stree.depth;
+    inf.mesh_tree_depth = mesh_depth;
+    int req = ctx->opt.traversal_stack_depth;
+    int sd, md;
+    if (req > 0) { sd = pow2_at_least(req); md = pow2_at_least(req); }
+    else { sd = std::min(pow2_at_least(std::max(1, stree.depth)), 32); md = std::min(pow2_at_least(std::max(1, mesh_depth)), 32); }
+    inf.stack_depth = (sd << 16) | md;
+    inf.build_seconds = std::chrono::duration<double>(t1 - t0).count();
+    inf.upload_seconds = std::chrono::duration<double>(t2 - t1).count();
+}
+
+// Pixel list of a set of tiles: tiles in the given order, pixels row-major inside each tile's
+// sample bounds (render_tile visitation order, reference_renderer.cpp:346-347).
+struct TileGeom {
+    int x0, y0, x1, y1;      // sample bounds
+    int px0, py0, px1, py1;  // pixel bounds (film.cpp:80-85)
+};
+
+static void film_layout(const yar_view& v, int* tgx, int* tgy, int* sx0, int* sy0, int* sx1, int* sy1) {
+    // Film::Film, film.cpp:52-71 (box filter radius 0.5 -> sample region == render region)
+    *sx0 = (int)std::floor(v.region_x0 + 0.5f - 0.5f);
+    *sy0 = (int)std::floor(v.region_y0 + 0.5f - 0.5f);
+    *sx1 = (int)std::ceil(v.region_x1 - 1 + 0.5f + 0.5f);
+    *sy1 = (int)std::ceil(v.region_y1 - 1 + 0.5f + 0.5f);
+    *tgx = (*sx1 - *sx0 + 63) / 64;
+    *tgy = (*sy1 - *sy0 + 63) / 64;
+}
+
+static TileGeom tile_geom(const yar_view& v, int tile) {
+    int tgx, tgy, sx0, sy0, sx1, sy1;
+    film_layout(v, &tgx, &tgy, &sx0, &sy0, &sx1, &sy1);
+    TileGeom g;
+    int tx = tile % tgx, ty = tile / tgx;
+    g.x0 = sx0 + tx * 64; g.y0 = sy0 + ty * 64;
+    g.x1 = std::min(g.x0 + 64, sx1); g.y1 = std::min(g.y0 + 64, sy1);
+    g.px0 = std::max((int)std::ceil(g.x0 - 0.5f - 0.5f), v.region_x0);
+    g.py0 = std::max((int)std::ceil(g.y0 - 0.5f - 0.5f), v.region_y0);
+    g.px1 = std::min((int)std::floor(g.x1 + 0.5f - 0.5f) + 1, v.region_x1);
+    g.py1 = std::min((int)std::floor(g.y1 + 0.5f - 0.5f) + 1, v.region_y1);
+    return g;
+}
+
+struct RenderResult {
+    std::vector<std::vector<yar_film_pixel>> tiles;   // per requested tile, pixel-bounds sized
+    std::vector<double> tile_var;
+    yar_render_stats stats{};
+};
+
+static void ensure_render_buffers(yar_gpu_ctx* ctx, int64_t paths, int64_t chunk_pixels, int64_t stride, int64_t total_pixels,
+                                  bool need_rng) {
+    ctx->table.reserve((size_t)(chunk_pixels * stride));
+    DevBuf<float4>* pb[] = {&ctx->ray_o, &ctx->ray_d, &ctx->coeff, &ctx->L, &ctx->dl_coeff, &ctx->dl_light, &ctx->dl_bsdf,
+                            &ctx->hit_tb, &ctx->radiance, &ctx->sh_o, &ctx->sh_d, &ctx->sh_c, &ctx->mi_o, &ctx->mi_d, &ctx->mi_f};
+    for (auto* b : pb) b->reserve((size_t)paths);
+    ctx->hit_ot.reserve((size_t)paths);
+    ctx->state.reserve((size_t)paths);
+    if (need_rng) ctx->rng.reserve((size_t)paths);
+    DevBuf<uint32_t>* qb[] = {&ctx->q_a, &ctx->q_b, &ctx->sh_path, &ctx->mi_path, &ctx->dl_path};
+    for (auto* b : qb) b->reserve((size_t)paths);
+    ctx->mi_ndw.reserve((size_t)paths);
+    ctx->counters.reserve(16);
+    ctx->totals.reserve(8);
+    ctx->acc_sum.reserve((size_t)chunk_pixels);
+    ctx->acc_lum.reserve((size_t)chunk_pixels);
+    ctx->out_film.reserve((size_t)total_pixels);
+    ctx->out_var.reserve((size_t)total_pixels);
+    ctx->side_rad.reserve(1 << 16);
+    ctx->side_info.reserve(1 << 16);
+    if (!ctx->h_counters) HIPX(hipHostMalloc((void**)&ctx->h_counters, 16 * sizeof(uint32_t), hipHostMallocDefault));
+}
+
+static void render_tiles_impl(yar_gpu_ctx* ctx, const yar_view& v, const std::vector<int>& tile_list, RenderResult& out) {
+    auto t_start = std::chrono::steady_clock::now();
+    const yar_render_config& rc = ctx->cfg;
+    if (rc.pixel_filter_radius != 0.5f) throw HipError{"only the box filter of radius 0.5 is supported"};
+    const int spp = v.x_pixel_sample_count * v.y_pixel_sample_count;
+    if (spp <= 0 || spp > 65536) throw HipError{"samples per pixel out of range"};
+    const int md = std::min(10, rc.max_light_bounces);
+    const int d1 = 4 * std::max(0, md), d2 = 3 * std::max(0, md);           // reference_renderer.cpp:87-99
+    const bool sequential = rc.max_light_bounces > 10;                      // RNG fallback couples samples
+    const int s_par = sequential ? 1 : spp;
+    const int64_t stride = (int64_t)spp * (2 + d1 + 2 * d2);
+
+    // pixel list
+    std::vector<int2> plist;
+    std::vector<TileGeom> geoms;
+    for (int t : tile_list) {
+        TileGeom g = tile_geom(v, t);
+        geoms.push_back(g);
+        for (int y = g.y0; y < g.y1; y++)
+            for (int x = g.x0; x < g.x1; x++) plist.push_back(make_int2(x, y));
+    }
+    const int64_t total_pixels = (int64_t)plist.size();
+    int64_t max_paths = ctx->opt.max_paths_in_flight > 0 ? ctx->opt.max_paths_in_flight : (int64_t)1 << 21;
+    int64_t chunk_pixels = std::max<int64_t>(1, max_paths / s_par);
+    // keep the sample table chunk under ~2 GB
+    chunk_pixels = std::min<int64_t>(chunk_pixels, std::max<int64_t>(1, ((int64_t)1 << 29) / stride));
+    chunk_pixels = std::min<int64_t>(chunk_pixels, std::max<int64_t>(total_pixels, 1));
+    const int64_t paths = chunk_pixels * s_par;
+    ensure_render_buffers(ctx, paths, chunk_pixels, stride, std::max<int64_t>(1, total_pixels), sequential);
+    hipStream_t s = ctx->stream;
+    ctx->pixels.upload(plist, s);
+    HIPX(hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(uint32_t), s));
+    HIPX(hipMemsetAsync(ctx->totals.p, 0, 8 * sizeof(unsigned long long), s));
+
+    Chunk c{};
+    c.spp = spp; c.s_par = s_par; c.nx = v.x_pixel_sample_count; c.ny = v.y_pixel_sample_count;
+    c.d1 = d1; c.d2 = d2; c.table = ctx->table.p; c.stride = stride; c.seed_offset = v.rng_seed_offset;
+    m34_rows(v.camera_to_world, c.c2w.r0, c.c2w.r1, c.c2w.r2);
+    c.W = (float)v.film_width; c.H = (float)v.film_height;
+    {
+        constexpr float deg2rad = yar::kPi / 180.f;                             // camera.cpp:5-14
+        float tan_half = std::tan((v.fov_y_degrees / 2.f) * deg2rad);
+        c.vert = tan_half;
+        c.horz = (c.W / c.H) * tan_half;
+    }
+    c.zup = v.z_is_up;
+    c.max_bounces = rc.max_light_bounces;
+    c.rr_bounce_threshold = rc.russian_roulette_bounce_count_threshold;
+    c.rr_threshold = rc.russian_roulette_threshold;
+    c.film_scale = rc.film_radiance_scale;
+    c.max_component = rc.max_rgb_component_value_of_film_sample;
+
+    Paths P{ctx->ray_o.p, ctx->ray_d.p, ctx->coeff.p, ctx->L.p, ctx->state.p, ctx->dl_coeff.p, ctx->dl_light.p,
+            ctx->dl_bsdf.p, ctx->hit_tb.p, ctx->hit_ot.p, sequential ? ctx->rng.p : nullptr, ctx->radiance.p};
+    Queues q{ctx->q_a.p, ctx->q_b.p, ctx->sh_path.p, ctx->sh_o.p, ctx->sh_d.p, ctx->sh_c.p, ctx->mi_path.p,
+             ctx->mi_o.p, ctx->mi_d.p, ctx->mi_f.p, ctx->mi_ndw.p, ctx->dl_path.p, ctx->counters.p, ctx->totals.p};
+    FilmAcc f{ctx->acc_sum.p, ctx->acc_lum.p, ctx->out_film.p, ctx->out_var.p, 0, ctx->side_rad.p, ctx->side_info.p, 1 << 16};
+    LaunchCfg lc = ctx->launch_cfg();
+
+    hipEvent_t e0, e1;
+    HIPX(hipEventCreate(&e0));
+    HIPX(hipEventCreate(&e1));
+    HIPX(hipEventRecord(e0, s));
+    int64_t closest = 0, anyr = 0;
+    const int passes = spp / s_par;
+    for (int64_t base = 0; base < total_pixels; base += chunk_pixels) {
+        c.pixels = ctx->pixels.p + base;
+        c.n_pixels = (int)std::min<int64_t>(chunk_pixels, total_pixels - base);
+        f.base = (int)base;
+        c.pass = 0;
+        launch_sample_table(c, ctx->table.p, s);
+        for (int pass = 0; pass < passes; pass++) {
+            c.pass = pass;
+            launch_raygen(c, P, q, s);
+            int count = c.n_pixels * s_par;
+            Queues qq = q;
+            while (count > 0) {
+                launch_extend(ctx->scene, P, qq, count, lc);
+                HIPX(hipMemsetAsync(qq.counters, 0, 4 * sizeof(uint32_t), s));
+                launch_shade(ctx->scene, c, P, qq, count, s);
+                HIPX(hipMemcpyAsync(ctx->h_counters, qq.counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                HIPX(hipStreamSynchronize(s));
+                const uint32_t n_active = ctx->h_counters[0], n_sh = ctx->h_counters[1], n_mis = ctx->h_counters[2],
+                               n_dl = ctx->h_counters[3];
+                closest += count + n_mis;
+                anyr += n_sh;
+                launch_shadow(ctx->scene, P, qq, (int)n_sh, lc);
+                launch_mis(ctx->scene, P, qq, (int)n_mis, lc);
+                launch_resolve(ctx->scene, c, P, qq, (int)n_dl, s);
+                std::swap(qq.active_in, qq.active_out);
+                count = (int)n_active;
+            }
+            launch_film(c, P, f, q, pass == 0, pass == passes - 1, s);
+        }
+    }
+    HIPX(hipEventRecord(e1, s));
+    std::vector<float4> film_own(total_pixels);
+    std::vector<double> var_own(total_pixels);
+    if (total_pixels > 0) {
+        HIPX(hipMemcpyAsync(film_own.data(), ctx->out_film.p, total_pixels * sizeof(float4), hipMemcpyDeviceToHost, s));
+        HIPX(hipMemcpyAsync(var_own.data(), ctx->out_var.p, total_pixels * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    unsigned long long tot[8];
+    HIPX(hipMemcpyAsync(tot, ctx->totals.p, sizeof(tot), hipMemcpyDeviceToHost, s));
+    HIPX(hipMemcpyAsync(ctx->h_counters, ctx->counters.p, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
+    uint32_t n_side = ctx->h_counters[4];
+    if (n_side > (1u << 16)) throw HipError{"jitter-0 side list overflow"};
+    std::vector<float4> side_rad(n_side);
+    std::vector<int4> side_info(n_side);
+    if (n_side) {
+        HIPX(hipMemcpy(side_rad.data(), ctx->side_rad.p, n_side * sizeof(float4), hipMemcpyDeviceToHost));
+        HIPX(hipMemcpy(side_info.data(), ctx->side_info.p, n_side * sizeof(int4), hipMemcpyDeviceToHost));
+    }
+    float gpu_ms = 0.f;
+    HIPX(hipEventElapsedTime(&gpu_ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+
+    // Assemble Film_Tiles: own-sample sums for the tile's sample pixels, then the jitter-0
+    // contributions of the tile's samples in visitation order (source row-major, sample index).
+    out.tiles.assign(tile_list.size(), {});
+    out.tile_var.assign(tile_list.size(), 0.0);
+    // side entries grouped by source pixel
+    std::vector<size_t> side_order(n_side);
+    for (size_t i = 0; i < n_side; i++) side_order[i] = i;
+    std::sort(side_order.begin(), side_order.end(), [&](size_t a, size_t b) {
+        const int4 &A = side_info[a], &B = side_info[b];
+        if (A.y != B.y) return A.y < B.y;
+        if (A.x != B.x) return A.x < B.x;
+        return A.z < B.z;
+    });
+    int64_t off = 0;
+    for (size_t ti = 0; ti < tile_list.size(); ti++) {
+        const TileGeom& g = geoms[ti];
+        int pw = g.px1 - g.px0, ph = g.py1 - g.py0;
+        std::vector<yar_film_pixel>& tp = out.tiles[ti];
+        tp.assign((size_t)std::max(0, pw) * std::max(0, ph), yar_film_pixel{{0, 0, 0}, 0});
+        double tv = 0.0;
+        for (int y = g.y0; y < g.y1; y++)
+            for (int x = g.x0; x < g.x1; x++, off++) {
+                const float4 a = film_own[off];
+                yar_film_pixel& fp = tp[(size_t)(y - g.py0) * pw + (x - g.px0)];
+                fp.color_sum[0] = a.x; fp.color_sum[1] = a.y; fp.color_sum[2] = a.z; fp.weight_sum = a.w;
+                tv += var_own[off];
+            }
+        out.tile_var[ti] = tv;
+        for (size_t si : side_order) {
+            const int4 inf = side_info[si];
+            if (inf.x < g.x0 || inf.x >= g.x1 || inf.y < g.y0 || inf.y >= g.y1) continue;
+            const float4 r = side_rad[si];
+            int targets[3][2];
+            int nt = 0;
+            if (inf.w & 1) { targets[nt][0] = inf.x - 1; targets[nt][1] = inf.y; nt++; }
+            if (inf.w & 2) { targets[nt][0] = inf.x; targets[nt][1] = inf.y - 1; nt++; }
+            if ((inf.w & 3) == 3) { targets[nt][0] = inf.x - 1; targets[nt][1] = inf.y - 1; nt++; }
+            for (int k = 0; k < nt; k++) {
+                int tx = targets[k][0], ty = targets[k][1];
+                if (tx < g.px0 || tx >= g.px1 || ty < g.py0 || ty >= g.py1) continue;
+                yar_film_pixel& fp = tp[(size_t)(ty - g.py0) * pw + (tx - g.px0)];
+                fp.color_sum[0] += 1.f * r.x; fp.color_sum[1] += 1.f * r.y; fp.color_sum[2] += 1.f * r.z;
+                fp.weight_sum += 1.f;
+            }
+        }
+    }
+    yar_render_stats& st = out.stats;
+    st.gpu_seconds = gpu_ms * 1e-3;
+    st.samples = total_pixels * spp;
+    st.closest_rays = closest;
+    st.any_rays = anyr;
+    st.node_visits = (int64_t)tot[2];
+    st.leaf_refs = (int64_t)tot[3];
+    st.triangle_tests = (int64_t)tot[4];
+    st.object_visits = (int64_t)tot[5];
+    st.render_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+}
+
+static bool valid_view(const yar_view& v, std::string* why) {
+    if (v.film_width <= 0 || v.film_height <= 0) { *why = "film size must be positive"; return false; }
+    if (v.region_x0 < 0 || v.region_y0 < 0 || v.region_x0 >= v.region_x1 || v.region_y0 >= v.region_y1 ||
+        v.region_x1 > v.film_width || v.region_y1 > v.film_height) { *why = "render region outside the film"; return false; }
+    if (v.region_x1 > 0x10000 || v.region_y1 > 0x10000) { *why = "film larger than 65536 pixels per side"; return false; }
+    if (v.x_pixel_sample_count <= 0 || v.y_pixel_sample_count <= 0) { *why = "sample counts must be positive"; return false; }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int yar_gpu_create(const yar_scene_desc* scene, const yar_gpu_options* options, yar_gpu_ctx** out_ctx) {
+    if (!scene || !out_ctx) return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_create: null argument");
+    *out_ctx = nullptr;
+    if (scene->object_count <= 0 || scene->mesh_count <= 0)
+        return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_create: scene has no geometry");
+    if (scene->rect_light_count + scene->point_light_count <= 0)
+        return fail(nullptr, YAR_ERR_UNSUPPORTED, "yar_gpu_create: scene has no point or rectangular lights");
+    for (int i = 0; i < scene->object_count; i++) {
+        const yar_object_desc& o = scene->objects[i];
+        if (o.mesh < 0 || o.mesh >= scene->mesh_count) return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "object mesh index out of range");
+        if (o.area_light >= scene->rect_light_count) return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "object light index out of range");
+        if (o.area_light < 0 && (o.material < 0 || o.material >= scene->material_count))
+            return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "object material index out of range");
+    }
+    for (int i = 0; i < scene->mesh_count; i++) {
+        const yar_mesh_desc& m = scene->meshes[i];
+        if (m.triangle_count <= 0 || !m.vertices || !m.indices) return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "empty mesh");
+        for (int64_t k = 0; k < 3 * (int64_t)m.triangle_count; k++)
+            if (m.indices[k] < 0 || m.indices[k] >= m.vertex_count) return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "mesh index out of range");
+    }
+    auto* ctx = new yar_gpu_ctx;
+    if (options) ctx->opt = *options;
+    ctx->device = ctx->opt.device;
+    ctx->cfg = scene->config;
+    try {
+        int ndev = 0;
+        HIPX(hipGetDeviceCount(&ndev));
+        if (ctx->device < 0 || ctx->device >= ndev) throw HipError{"no such HIP device"};
+        HIPX(hipSetDevice(ctx->device));
+        hipDeviceProp_t prop;
+        HIPX(hipGetDeviceProperties(&prop, ctx->device));
+        std::snprintf(ctx->info.device_name, sizeof(ctx->info.device_name), "%s", prop.gcnArchName);
+        HIPX(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        upload_scene(ctx, scene);
+    } catch (const HipError& e) {
+        std::string msg = e.msg;
+        int code = msg.find("not supported") != std::string::npos ? YAR_ERR_UNSUPPORTED : YAR_ERR_HIP;
+        delete ctx;
+        return fail(nullptr, code, "yar_gpu_create: " + msg);
+    } catch (const std::bad_alloc&) {
+        delete ctx;
+        return fail(nullptr, YAR_ERR_OUT_OF_MEMORY, "yar_gpu_create: out of host memory");
+    }
+    yar_view v{};
+    std::memcpy(v.camera_to_world, scene->camera.camera_to_world, sizeof(v.camera_to_world));
+    v.fov_y_degrees = scene->camera.fov_y_degrees;
+    v.z_is_up = scene->camera.z_is_up;
+    v.film_width = scene->config.film_width;
+    v.film_height = scene->config.film_height;
+    v.region_x0 = scene->config.region_x0; v.region_y0 = scene->config.region_y0;
+    v.region_x1 = scene->config.region_x1; v.region_y1 = scene->config.region_y1;
+    v.x_pixel_sample_count = scene->config.x_pixel_sample_count;
+    v.y_pixel_sample_count = scene->config.y_pixel_sample_count;
+    v.rng_seed_offset = scene->config.rng_seed_offset;
+    ctx->views.assign(1, v);
+    *out_ctx = ctx;
+    return YAR_OK;
+}
+
+void yar_gpu_destroy(yar_gpu_ctx* ctx) { delete ctx; }
+
+const char* yar_gpu_last_error(const yar_gpu_ctx* ctx) { return ctx ? ctx->error.c_str() : yar::get_last_error(); }
+
+int yar_gpu_set_views(yar_gpu_ctx* ctx, const yar_view* views, int32_t count) {
+    if (!ctx || !views || count <= 0) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_set_views: invalid argument");
+    std::string why;
+    for (int i = 0; i < count; i++)
+        if (!valid_view(views[i], &why)) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_set_views: view " + std::to_string(i) + ": " + why);
+    ctx->views.assign(views, views + count);
+    return YAR_OK;
+}
+
+int yar_gpu_get_view(const yar_gpu_ctx* ctx, int32_t index, yar_view* out_view) {
+    if (!ctx || !out_view || index < 0 || index >= (int)ctx->views.size()) return YAR_ERR_INVALID_ARGUMENT;
+    *out_view = ctx->views[index];
+    return YAR_OK;
+}
+
+int yar_film_tile_count(const yar_gpu_ctx* ctx, int32_t view, int32_t* out_count) {
+    if (!ctx || !out_count || view < 0 || view >= (int)ctx->views.size()) return YAR_ERR_INVALID_ARGUMENT;
+    int tgx, tgy, a, b, c2, d;
+    film_layout(ctx->views[view], &tgx, &tgy, &a, &b, &c2, &d);
+    *out_count = tgx * tgy;
+    return YAR_OK;
+}
+
+int yar_film_tile_bounds(const yar_gpu_ctx* ctx, int32_t view, int32_t tile_index, int32_t out_sample_bounds[4],
+                         int32_t out_pixel_bounds[4]) {
+    int32_t n = 0;
+    if (yar_film_tile_count(ctx, view, &n) != YAR_OK || tile_index < 0 || tile_index >= n) return YAR_ERR_INVALID_ARGUMENT;
+    TileGeom g = tile_geom(ctx->views[view], tile_index);
+    if (out_sample_bounds) { out_sample_bounds[0] = g.x0; out_sample_bounds[1] = g.y0; out_sample_bounds[2] = g.x1; out_sample_bounds[3] = g.y1; }
+    if (out_pixel_bounds) { out_pixel_bounds[0] = g.px0; out_pixel_bounds[1] = g.py0; out_pixel_bounds[2] = g.px1; out_pixel_bounds[3] = g.py1; }
+    return YAR_OK;
+}
+
+int yar_gpu_render_tiles(yar_gpu_ctx* ctx, int32_t view, const int32_t* tile_indices, int32_t count,
+                         yar_film_pixel* const* out_tiles, double* out_tile_variance, yar_render_stats* out_stats) {
+    int32_t n = 0;
+    if (!ctx || !tile_indices || count <= 0 || !out_tiles || yar_film_tile_count(ctx, view, &n) != YAR_OK)
+        return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_tiles: invalid argument");
+    std::vector<int> tl(tile_indices, tile_indices + count);
+    for (int t : tl) if (t < 0 || t >= n) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_tiles: tile index out of range");
+    RenderResult rr;
+    try {
+        HIPX(hipSetDevice(ctx->device));
+        render_tiles_impl(ctx, ctx->views[view], tl, rr);
+    } catch (const HipError& e) {
+        return fail(ctx, YAR_ERR_HIP, "yar_gpu_render_tiles: " + e.msg);
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, YAR_ERR_OUT_OF_MEMORY, "yar_gpu_render_tiles: out of host memory");
+    }
+    for (int i = 0; i < count; i++) {
+        if (out_tiles[i]) std::memcpy(out_tiles[i], rr.tiles[i].data(), rr.tiles[i].size() * sizeof(yar_film_pixel));
+        if (out_tile_variance) out_tile_variance[i] = rr.tile_var[i];
+    }
+    if (out_stats) *out_stats = rr.stats;
+    return YAR_OK;
+}
+
+int yar_gpu_render(yar_gpu_ctx* ctx, int32_t view, yar_film_pixel* out_film, float* out_image, double* out_variance,
+                   yar_render_stats* out_stats) {
+    int32_t n = 0;
+    if (!ctx || yar_film_tile_count(ctx, view, &n) != YAR_OK) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render: invalid view");
+    const yar_view& v = ctx->views[view];
+    std::vector<int> tl(n);
+    for (int i = 0; i < n; i++) tl[i] = i;
+    RenderResult rr;
+    try {
+        HIPX(hipSetDevice(ctx->device));
+        render_tiles_impl(ctx, v, tl, rr);
+    } catch (const HipError& e) {
+        return fail(ctx, YAR_ERR_HIP, "yar_gpu_render: " + e.msg);
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, YAR_ERR_OUT_OF_MEMORY, "yar_gpu_render: out of host memory");
+    }
+    // Film::merge_tile in tile-index order, then Film::get_image (film.cpp:88-118)
+    const int rw = v.region_x1 - v.region_x0, rh = v.region_y1 - v.region_y0;
+    std::vector<yar_film_pixel> film((size_t)rw * rh, yar_film_pixel{{0, 0, 0}, 0});
+    for (int t = 0; t < n; t++) {
+        TileGeom g = tile_geom(v, t);
+        int pw = g.px1 - g.px0;
+        for (int y = g.py0; y < g.py1; y++)
+            for (int x = g.px0; x < g.px1; x++) {
+                yar_film_pixel& f = film[(size_t)(y - v.region_y0) * rw + (x - v.region_x0)];
+                const yar_film_pixel& p = rr.tiles[t][(size_t)(y - g.py0) * pw + (x - g.px0)];
+                f.color_sum[0] += p.color_sum[0]; f.color_sum[1] += p.color_sum[1]; f.color_sum[2] += p.color_sum[2];
+                f.weight_sum += p.weight_sum;
+            }
+    }
+    if (out_film) std::memcpy(out_film, film.data(), film.size() * sizeof(yar_film_pixel));
+    if (out_image) {
+        for (size_t i = 0; i < film.size(); i++) {
+            const yar_film_pixel& f = film[i];
+            float r = 0, g = 0, b = 0;
+            if (f.weight_sum != 0.f) {
+                float inv = 1.f / f.weight_sum;
+                r = f.color_sum[0] * inv; g = f.color_sum[1] * inv; b = f.color_sum[2] * inv;
+            }
+            out_image[3 * i] = std::max(0.f, r); out_image[3 * i + 1] = std::max(0.f, g); out_image[3 * i + 2] = std::max(0.f, b);
+        }
+    }
+    if (out_variance) {
+        const int spp = v.x_pixel_sample_count * v.y_pixel_sample_count;
+        double acc = 0.0;
+        int64_t cnt = 0;
+        for (int t = 0; t < n; t++) {
+            acc += rr.tile_var[t];
+            TileGeom g = tile_geom(v, t);
+            cnt += (int64_t)(g.x1 - g.x0) * (g.y1 - g.y0);
+        }
+        *out_variance = spp > 1 ? acc / cnt : 0.0;
+    }
+    if (out_stats) {
+        *out_stats = rr.stats;
+    }
+    return YAR_OK;
+}
+
+int yar_gpu_render_batch(yar_gpu_ctx* ctx, const int32_t* views, int32_t count, float* const* out_images,
+                         double* out_variances, yar_render_stats* out_stats) {
+    if (!ctx || !views || count <= 0) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_batch: invalid argument");
+    yar_render_stats total{};
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < count; i++) {
+        yar_render_stats st{};
+        int rc = yar_gpu_render(ctx, views[i], nullptr, out_images ? out_images[i] : nullptr,
+                                out_variances ? &out_variances[i] : nullptr, &st);
+        if (rc != YAR_OK) return rc;
+        total.gpu_seconds += st.gpu_seconds;
+        total.samples += st.samples;
+        total.closest_rays += st.closest_rays;
+        total.any_rays += st.any_rays;
+        total.node_visits += st.node_visits;
+        total.leaf_refs += st.leaf_refs;
+        total.triangle_tests += st.triangle_tests;
+        total.object_visits += st.object_visits;
+    }
+    total.render_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (out_stats) *out_stats = total;
+    return YAR_OK;
+}
+
+int yar_gpu_trace_rays_device(yar_gpu_ctx* ctx, const yar_ray* d_rays, int64_t count, int32_t any_hit, yar_hit* d_hits,
+                              void* hip_stream);
+
+int yar_gpu_trace_rays(yar_gpu_ctx* ctx, const yar_ray* rays, int64_t count, int32_t any_hit, yar_hit* out_hits) {
+    if (!ctx || (!rays && count > 0) || (!out_hits && count > 0) || count < 0)
+        return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_trace_rays: invalid argument");
+    if (count == 0) return YAR_OK;
+    try {
+        HIPX(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        ctx->tr_rays.reserve(2 * (size_t)count);
+        ctx->tr_tb.reserve((size_t)count);
+        ctx->tr_ot.reserve((size_t)count);
+        ctx->totals.reserve(8);
+        HIPX(hipMemcpyAsync(ctx->tr_rays.p, rays, (size_t)count * sizeof(yar_ray), hipMemcpyHostToDevice, s));
+        HIPX(hipMemsetAsync(ctx->totals.p, 0, 8 * sizeof(unsigned long long), s));
+        LaunchCfg lc = ctx->launch_cfg();
+        launch_trace_rays(ctx->scene, ctx->tr_rays.p, count, any_hit, ctx->tr_tb.p, ctx->tr_ot.p, ctx->totals.p, lc);
+        HIPX(hipGetLastError());
+        std::vector<float4> tb(count);
+        std::vector<int2> ot(count);
+        unsigned long long tot[8];
+        HIPX(hipMemcpyAsync(tb.data(), ctx->tr_tb.p, (size_t)count * sizeof(float4), hipMemcpyDeviceToHost, s));
+        HIPX(hipMemcpyAsync(ot.data(), ctx->tr_ot.p, (size_t)count * sizeof(int2), hipMemcpyDeviceToHost, s));
+        HIPX(hipMemcpyAsync(tot, ctx->totals.p, sizeof(tot), hipMemcpyDeviceToHost, s));
+        HIPX(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < count; i++) {
+            yar_hit& h = out_hits[i];
+            h.t = tb[i].x;
+            h.object = ot[i].x;
+            h.triangle = ot[i].y;
+            h.barycentrics[0] = tb[i].y; h.barycentrics[1] = tb[i].z; h.barycentrics[2] = tb[i].w;
+        }
+        ctx->last_trace = yar_render_stats{};
+        ctx->last_trace.node_visits = (int64_t)tot[2];
+        ctx->last_trace.leaf_refs = (int64_t)tot[3];
+        ctx->last_trace.triangle_tests = (int64_t)tot[4];
+        ctx->last_trace.object_visits = (int64_t)tot[5];
+        if (any_hit) ctx->last_trace.any_rays = count; else ctx->last_trace.closest_rays = count;
+    } catch (const HipError& e) {
+        return fail(ctx, YAR_ERR_HIP, "yar_gpu_trace_rays: " + e.msg);
+    }
+    return YAR_OK;
+}
+
+}  // extern "C"
+
+namespace {
+__global__ void k_pack_hits(const float4* __restrict__ tb, const int2* __restrict__ ot, long long n, yar_hit* out) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 a = tb[i];
+    int2 b = ot[i];
+    yar_hit h;
+    h.t = a.x; h.object = b.x; h.triangle = b.y;
+    h.barycentrics[0] = a.y; h.barycentrics[1] = a.z; h.barycentrics[2] = a.w;
+    out[i] = h;
+}
+}  // namespace
+
+extern "C" int yar_gpu_trace_rays_device(yar_gpu_ctx* ctx, const yar_ray* d_rays, int64_t count, int32_t any_hit,
+                                         yar_hit* d_hits, void* hip_stream) {
+    if (!ctx || count < 0 || (count > 0 && (!d_rays || !d_hits)))
+        return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_trace_rays_device: invalid argument");
+    if (count == 0) return YAR_OK;
+    try {
+        HIPX(hipSetDevice(ctx->device));
+        ctx->tr_tb.reserve((size_t)count);
+        ctx->tr_ot.reserve((size_t)count);
+        ctx->totals.reserve(8);
+        LaunchCfg lc = ctx->launch_cfg();
+        if (hip_stream) lc.stream = (hipStream_t)hip_stream;
+        HIPX(hipMemsetAsync(ctx->totals.p, 0, 8 * sizeof(unsigned long long), lc.stream));
+        launch_trace_rays(ctx->scene, reinterpret_cast<const float4*>(d_rays), count, any_hit, ctx->tr_tb.p, ctx->tr_ot.p,
+                          ctx->totals.p, lc);
+        hipLaunchKernelGGL(k_pack_hits, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, lc.stream, ctx->tr_tb.p,
+                           ctx->tr_ot.p, (long long)count, d_hits);
+        HIPX(hipGetLastError());
+    } catch (const HipError& e) {
+        return fail(ctx, YAR_ERR_HIP, "yar_gpu_trace_rays_device: " + e.msg);
+    }
+    return YAR_OK;
+}
+
+extern "C" int yar_gpu_last_trace_counters(yar_gpu_ctx* ctx, yar_render_stats* out_stats) {
+    if (!ctx || !out_stats) return YAR_ERR_INVALID_ARGUMENT;
+    *out_stats = ctx->last_trace;
+    return YAR_OK;
+}
+
+extern "C" int yar_gpu_get_info(const yar_gpu_ctx* ctx, yar_gpu_info* out_info) {
+    if (!ctx || !out_info) return YAR_ERR_INVALID_ARGUMENT;
+    *out_info = ctx->info;
+    return YAR_OK;
+}

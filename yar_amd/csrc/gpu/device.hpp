@@ -1,0 +1,367 @@
+// device.hpp - device-side data layout, float math and two-level kd-tree traversal for gfx950.
+//
+// All arithmetic keeps the reference renderer's operation order and the file is compiled with
+// -ffp-contract=off (no FMA contraction), so hit records are bit-identical to a contraction-free
+// build of the reference traversal (kennyalive/YAR src/ref/kdtree.cpp, intersection.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace yg {
+
+#define YG_INLINE __device__ __forceinline__
+
+constexpr float kPi = 3.14159265f;          // src/lib/math.h:9
+constexpr float kPiInv = 1.f / kPi;         // src/lib/math.h:12
+constexpr float kOneMinusEps = 0x1.fffffep-1f;
+
+YG_INLINE float f_inf() { return __builtin_huge_valf(); }
+YG_INLINE uint32_t fbits(float f) { return __float_as_uint(f); }
+YG_INLINE float bitsf(uint32_t u) { return __uint_as_float(u); }
+
+// ---- float3 with explicit operation order --------------------------------------------------
+struct V3 {
+    float x, y, z;
+    YG_INLINE float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+YG_INLINE V3 v3(float x, float y, float z) { V3 r; r.x = x; r.y = y; r.z = z; return r; }
+YG_INLINE V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+YG_INLINE V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+YG_INLINE V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
+YG_INLINE V3 operator*(V3 a, float t) { return v3(a.x * t, a.y * t, a.z * t); }
+YG_INLINE V3 operator*(float t, V3 a) { return a * t; }
+YG_INLINE V3 vdiv(V3 a, float t) { return v3(a.x / t, a.y / t, a.z / t); }
+YG_INLINE float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+YG_INLINE V3 cross(V3 a, V3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+YG_INLINE float len_sq(V3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+YG_INLINE float length(V3 a) { return sqrtf(len_sq(a)); }
+YG_INLINE V3 normalized(V3 a) { return vdiv(a, length(a)); }
+YG_INLINE V3 from4(float4 f) { return v3(f.x, f.y, f.z); }
+
+// ColorRGB semantics: division by a scalar multiplies by its reciprocal (src/lib/color.h:104).
+struct RGB { float r, g, b; };
+YG_INLINE RGB rgb(float r, float g, float b) { RGB c; c.r = r; c.g = g; c.b = b; return c; }
+YG_INLINE RGB operator*(RGB a, RGB b) { return rgb(a.r * b.r, a.g * b.g, a.b * b.b); }
+YG_INLINE RGB operator*(RGB a, float k) { return rgb(a.r * k, a.g * k, a.b * k); }
+YG_INLINE RGB operator*(float k, RGB a) { return a * k; }
+YG_INLINE RGB operator+(RGB a, RGB b) { return rgb(a.r + b.r, a.g + b.g, a.b + b.b); }
+YG_INLINE RGB operator-(RGB a, RGB b) { return rgb(a.r - b.r, a.g - b.g, a.b - b.b); }
+YG_INLINE RGB cdiv(RGB a, RGB b) { return rgb(a.r / b.r, a.g / b.g, a.b / b.b); }
+YG_INLINE RGB cdivs(RGB a, float k) { float inv = 1.f / k; return a * inv; }
+YG_INLINE RGB csqrt(RGB a) { return rgb(sqrtf(a.r), sqrtf(a.g), sqrtf(a.b)); }
+YG_INLINE RGB splat(float v) { return rgb(v, v, v); }
+YG_INLINE bool is_black(RGB a) { return a.r == 0 && a.g == 0 && a.b == 0; }
+YG_INLINE RGB rgb4(float4 f) { return rgb(f.x, f.y, f.z); }
+YG_INLINE float4 f4(RGB c, float w) { return make_float4(c.r, c.g, c.b, w); }
+YG_INLINE float4 f4(V3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+// 3x4 row-major matrix stored as three float4 rows.
+struct M34 { float4 r0, r1, r2; };
+YG_INLINE V3 xform_point(const M34& m, V3 p) {
+    return v3(m.r0.x * p.x + m.r0.y * p.y + m.r0.z * p.z + m.r0.w,
+              m.r1.x * p.x + m.r1.y * p.y + m.r1.z * p.z + m.r1.w,
+              m.r2.x * p.x + m.r2.y * p.y + m.r2.z * p.z + m.r2.w);
+}
+YG_INLINE V3 xform_vector(const M34& m, V3 v) {
+    return v3(m.r0.x * v.x + m.r0.y * v.y + m.r0.z * v.z,
+              m.r1.x * v.x + m.r1.y * v.y + m.r1.z * v.z,
+              m.r2.x * v.x + m.r2.y * v.y + m.r2.z * v.z);
+}
+
+// ---- scene layout in HBM ---------------------------------------------------------------
+// Object record for traversal (80 bytes): world_to_object rows, mesh-tree bounds with the
+// mesh's root node index and triangle base packed into the w lanes.
+struct ObjTrav { float4 w2o0, w2o1, w2o2, lo_node, hi_tri; };
+
+struct DevMaterial {      // 64 bytes
+    int type; float eta_i, alpha_x, alpha_y;
+    float4 refl;          // diffuse reflectance / glass Kr
+    float4 trans;         // glass Kt (w = index of refraction)
+    float4 eta;           // metal eta
+    float4 k;             // metal k
+};
+struct DevRect { float4 l0, l1, l2; float4 L; float4 size; };   // light_to_world rows, radiance, size (xy)
+struct DevPoint { float4 p; float4 I; };
+
+struct DevScene {
+    const uint2* scene_nodes;
+    const uint2* mesh_nodes;       // all mesh trees; interior above_child and leaf ids rebased
+    const ObjTrav* objs;
+    const float4* tri_verts;       // 3 float4 per triangle (object space)
+    float4 scene_lo, scene_hi;
+    // shading
+    const float4* obj_o2w;         // 3 rows per object
+    const float4* obj_nrm;         // 3 rows per object (transpose of world_to_object 3x3)
+    const int4* obj_info;          // material, area light, mesh, triangle base
+    const int4* mesh_info;         // flags (1 normals, 2 uvs, 4 reverse), -, -, -
+    const int4* tri_vidx;          // global vertex indices per triangle
+    const float4* vtx_n;           // per vertex normal
+    const float2* vtx_uv;          // per vertex uv
+    const DevMaterial* materials;
+    const DevRect* rects;
+    const DevPoint* points;
+    int n_rects, n_points, n_lights;
+};
+
+// ---- ray/box slab test (Bounding_Box::intersect_by_ray_without_NaNs, bounding_box.h:63) ----
+YG_INLINE bool slab(float4 lo, float4 hi, V3 o, V3 d, float& tmin, float& tmax) {
+    float t0 = 0.f, t1 = f_inf();
+    float los[3] = {lo.x, lo.y, lo.z}, his[3] = {hi.x, hi.y, hi.z};
+    float os[3] = {o.x, o.y, o.z}, ds[3] = {d.x, d.y, d.z};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (ds[i] != 0.f) {
+            float inv = 1.f / ds[i];
+            float s0 = (los[i] - os[i]) * inv;
+            float s1 = (his[i] - os[i]) * inv;
+            if (inv < 0.f) { float t = s0; s0 = s1; s1 = t; }
+            t0 = s0 > t0 ? s0 : t0;
+            t1 = s1 < t1 ? s1 : t1;
+            if (t0 > t1) return false;
+        } else if (os[i] < los[i] || os[i] > his[i]) {
+            return false;
+        }
+    }
+    tmin = t0; tmax = t1;
+    return true;
+}
+
+// ---- watertight ray/triangle test (intersection.cpp:53-113) ------------------------------
+YG_INLINE float watertight(V3 o, V3 d, int kx, int ky, int kz, float sx, float sy, float sz,
+                           float4 p0, float4 p1, float4 p2, V3& b) {
+    V3 a0 = v3(p0.x - o.x, p0.y - o.y, p0.z - o.z);
+    V3 a1 = v3(p1.x - o.x, p1.y - o.y, p1.z - o.z);
+    V3 a2 = v3(p2.x - o.x, p2.y - o.y, p2.z - o.z);
+    float p0x = a0[kx], p0y = a0[ky], p0z = a0[kz];
+    float p1x = a1[kx], p1y = a1[ky], p1z = a1[kz];
+    float p2x = a2[kx], p2y = a2[ky], p2z = a2[kz];
+    float x0 = p0x + sx * p0z, y0 = p0y + sy * p0z;
+    float x1 = p1x + sx * p1z, y1 = p1y + sy * p1z;
+    float x2 = p2x + sx * p2z, y2 = p2y + sy * p2z;
+    float e0 = x1 * y2 - y1 * x2;
+    float e1 = x2 * y0 - y2 * x0;
+    float e2 = x0 * y1 - y0 * x1;
+    if (e0 == 0.f || e1 == 0.f || e2 == 0.f) {
+        e0 = (float)((double)x1 * (double)y2 - (double)y1 * (double)x2);
+        e1 = (float)((double)x2 * (double)y0 - (double)y2 * (double)x0);
+        e2 = (float)((double)x0 * (double)y1 - (double)y0 * (double)x1);
+    }
+    if ((e0 < 0 || e1 < 0 || e2 < 0) && (e0 > 0 || e1 > 0 || e2 > 0)) return f_inf();
+    float det = e0 + e1 + e2;
+    if (det == 0.f) return f_inf();
+    float z0 = sz * p0z, z1 = sz * p1z, z2 = sz * p2z;
+    float ts = e0 * z0 + e1 * z1 + e2 * z2;
+    if ((fbits(det) ^ fbits(ts)) >> 31) return f_inf();
+    float inv_det = 1.f / det;
+    float t = inv_det * ts;
+    b = v3(e0 * inv_det, e1 * inv_det, e2 * inv_det);
+    return t;
+}
+
+YG_INLINE bool degenerate(float4 p0, float4 p1, float4 p2) {
+    V3 e1 = v3(p1.x - p0.x, p1.y - p0.y, p1.z - p0.z);
+    V3 e2 = v3(p2.x - p0.x, p2.y - p0.y, p2.z - p0.z);
+    return len_sq(cross(e1, e2)) == 0.f;
+}
+
+struct TravCounters { uint32_t nodes, leaf_refs, tri_tests, objects; };
+
+struct HitRec { float t; int obj; uint32_t tri; V3 b; };
+
+// Short traversal stack in LDS: entries [slot][lane] so a wave's accesses hit distinct banks.
+// When a push finds the stack full the oldest entry is dropped; when the stack then runs empty
+// the traversal restarts from the root at the current exit distance (kd-restart), visiting the
+// remaining leaves in the same front-to-back order.  With depth >= tree depth it is the plain
+// stack traversal of kdtree.cpp:219-439.
+struct LdsStack {
+    uint32_t* node;
+    float* tmax;
+    int stride;
+    YG_INLINE void put(int slot, uint32_t n, float t) { node[slot * stride] = n; tmax[slot * stride] = t; }
+    YG_INLINE void get(int slot, uint32_t& n, float& t) const { n = node[slot * stride]; t = tmax[slot * stride]; }
+};
+
+// One kd-tree level. `root` is the global index of the tree's root node, [t0,t1] the slab range.
+// loop_t() returns the current loop bound (closest hit t, or the fixed any-hit tmax).
+// Leaf(ids_node, first_id, count) processes a leaf and returns true to stop the whole query.
+template <bool COUNT, class LoopT, class Leaf>
+YG_INLINE bool kd_level(const uint2* __restrict__ nodes, uint32_t root, V3 o, V3 d, V3 inv, float t0, float t1,
+                        LdsStack st, int depth, LoopT loop_t, Leaf leaf, TravCounters& cnt) {
+    const int mask = depth - 1;   // depth is a power of two
+    float tmin = t0, tmax = t1;
+    uint32_t node = root;
+    int sp = 0, count = 0;
+    bool dropped = false;
+    float restart_t = -1.f;
+    // The iteration cap only guarantees that every wave terminates; real trees need far fewer.
+    for (int iter = 0; iter < (1 << 22) && loop_t() > tmin; iter++) {
+        if (COUNT) cnt.nodes++;
+        uint2 w = nodes[node];
+        if ((w.x & 3u) != 3u) {
+            int axis = (int)(w.x & 3u);
+            float oa = axis == 0 ? o.x : (axis == 1 ? o.y : o.z);
+            float da = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
+            float ia = axis == 0 ? inv.x : (axis == 1 ? inv.y : inv.z);
+            float dist = bitsf(w.y) - oa;
+            uint32_t below = node + 1, above = w.x >> 2;
+            if (dist != 0.f) {
+                uint32_t first = dist > 0.f ? below : above;
+                uint32_t second = dist > 0.f ? above : below;
+                float ts = dist * ia;
+                if (ts > tmax || ts < 0.f) {
+                    node = first;
+                } else if (ts < tmin) {
+                    node = second;
+                } else {
+                    st.put(sp & mask, second, tmax);
+                    sp++;
+                    if (count == depth) dropped = true; else count++;
+                    node = first;
+                    tmax = ts;
+                }
+            } else {
+                node = (da > 0.f) ? above : below;
+            }
+        } else {
+            uint32_t n = w.x >> 2;
+            if (COUNT) cnt.leaf_refs += n;
+            if (n > 0 && leaf(node, w.y, n)) return true;
+            if (count > 0) {
+                sp--; count--;
+                uint32_t nn; float tm;
+                st.get(sp & mask, nn, tm);
+                node = nn;
+                tmin = tmax;
+                tmax = tm;
+            } else if (dropped && tmax < t1) {
+                // restart behind the last leaf; a restart that would not advance (a ray grazing
+                // many coincident split planes) steps past the degenerate point instead
+                dropped = false;
+                node = root;
+                tmin = tmax > restart_t ? tmax : nextafterf(restart_t, f_inf());
+                restart_t = tmin;
+                tmax = t1;
+            } else {
+                break;
+            }
+        }
+    }
+    return false;
+}
+
+// Leaf id j of a leaf whose first id is in word1 of node `nd` (kdtree.h:53: ids packed inline).
+YG_INLINE uint32_t leaf_id(const uint2* __restrict__ nodes, uint32_t nd, uint32_t first, uint32_t j) {
+    if (j == 0) return first;
+    uint2 w = nodes[nd + ((j + 1) >> 1)];
+    return (j & 1) ? w.x : w.y;
+}
+
+// Two-level closest-hit / any-hit query over the scene tree and the per-mesh trees
+// (KdTree::intersect / intersect_any with intersect_scene_geometry_data and
+// intersect_triangle_mesh_geometry_data, kdtree.cpp:12-109).
+template <bool ANY, bool COUNT>
+YG_INLINE bool trace(const DevScene& s, V3 o, V3 d, float tmax_in, HitRec& hit, LdsStack sst, int sdepth,
+                     LdsStack mst, int mdepth, TravCounters& cnt) {
+    hit.t = tmax_in; hit.obj = -1; hit.tri = 0; hit.b = v3(0, 0, 0);
+    float st0, st1;
+    if (!slab(s.scene_lo, s.scene_hi, o, d, st0, st1)) return false;
+    V3 inv = v3(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    const float any_t = tmax_in;
+    bool any_found = false;
+    auto scene_loop_t = [&]() { return ANY ? any_t : hit.t; };
+    auto scene_leaf = [&](uint32_t nd, uint32_t first, uint32_t n) -> bool {
+        for (uint32_t j = 0; j < n; j++) {
+            uint32_t oi = leaf_id(s.scene_nodes, nd, first, j);
+            const ObjTrav ob = s.objs[oi];
+            if (COUNT) cnt.objects++;
+            M34 w2o = {ob.w2o0, ob.w2o1, ob.w2o2};
+            V3 lo = xform_point(w2o, o);
+            V3 ld = xform_vector(w2o, d);
+            float mt0, mt1;
+            if (!slab(ob.lo_node, ob.hi_tri, lo, ld, mt0, mt1)) continue;
+            V3 linv = v3(1.f / ld.x, 1.f / ld.y, 1.f / ld.z);
+            uint32_t root = fbits(ob.lo_node.w);
+            // watertight-test setup hoisted per mesh ray (intersection.cpp:55-61)
+            float ax = fabsf(ld.x), ay = fabsf(ld.y), az = fabsf(ld.z);
+            int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+            int kx = (kz == 2 ? 0 : kz + 1);
+            int ky = (kz == 0 ? 2 : kz - 1);
+            float dx = ld[kx], dy = ld[ky], dz = ld[kz];
+            float sx = -dx / dz, sy = -dy / dz, sz = 1.f / dz;
+            float mesh_tmax = hit.t;
+            bool stop = false;
+            auto mesh_loop_t = [&]() { return ANY ? any_t : hit.t; };
+            auto mesh_leaf = [&](uint32_t mnd, uint32_t mfirst, uint32_t mn) -> bool {
+                for (uint32_t k = 0; k < mn; k++) {
+                    uint32_t ti = leaf_id(s.mesh_nodes, mnd, mfirst, k);
+                    float4 p0 = s.tri_verts[3 * (size_t)ti], p1 = s.tri_verts[3 * (size_t)ti + 1],
+                           p2 = s.tri_verts[3 * (size_t)ti + 2];
+                    if (COUNT) cnt.tri_tests++;
+                    V3 b;
+                    float t = watertight(lo, ld, kx, ky, kz, sx, sy, sz, p0, p1, p2, b);
+                    if (ANY) {
+                        if (t >= any_t) continue;
+                        if (degenerate(p0, p1, p2)) continue;
+                        any_found = true;
+                        return true;
+                    } else {
+                        if (t < hit.t) {
+                            if (degenerate(p0, p1, p2)) continue;
+                            hit.t = t; hit.tri = ti; hit.b = b;
+                        }
+                    }
+                }
+                return false;
+            };
+            stop = kd_level<COUNT>(s.mesh_nodes, root, lo, ld, linv, mt0, mt1, mst, mdepth, mesh_loop_t, mesh_leaf, cnt);
+            if (ANY) {
+                if (stop) return true;
+            } else if (hit.t < mesh_tmax) {
+                hit.obj = (int)oi;
+            }
+        }
+        return false;
+    };
+    kd_level<COUNT>(s.scene_nodes, 0u, o, d, inv, st0, st1, sst, sdepth, scene_loop_t, scene_leaf, cnt);
+    if (ANY) return any_found;
+    return hit.t < tmax_in;
+}
+
+// ---- PCG32 (third-party/pcg/pcg_basic.c:46-75) ------------------------------------------
+constexpr uint64_t kPcgMult = 6364136223846793005ULL;
+YG_INLINE uint32_t pcg_output(uint64_t old) {
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31));
+}
+YG_INLINE uint32_t pcg_next(uint64_t& state, uint64_t inc) {
+    uint64_t old = state;
+    state = old * kPcgMult + inc;
+    return pcg_output(old);
+}
+YG_INLINE float pcg_float(uint64_t& state, uint64_t inc) {
+    return bitsf((pcg_next(state, inc) >> 9) | 0x3f800000u) - 1.0f;
+}
+// State after `delta` further steps of the LCG (Brown 1994 jump-ahead, as pcg32_advance_r).
+YG_INLINE uint64_t pcg_advance(uint64_t state, uint64_t inc, uint64_t delta) {
+    uint64_t acc_mult = 1, acc_plus = 0, cur_mult = kPcgMult, cur_plus = inc;
+    while (delta > 0) {
+        if (delta & 1) { acc_mult *= cur_mult; acc_plus = acc_plus * cur_mult + cur_plus; }
+        cur_plus = (cur_mult + 1) * cur_plus;
+        cur_mult *= cur_mult;
+        delta >>= 1;
+    }
+    return acc_mult * state + acc_plus;
+}
+// pcg32_srandom_r(rng, 0, stream): seeded state of a pixel stream.
+YG_INLINE uint64_t pcg_seed_state(uint64_t stream, uint64_t& inc) {
+    inc = (stream << 1u) | 1u;
+    uint64_t st = 0;
+    pcg_next(st, inc);
+    st += 0;
+    pcg_next(st, inc);
+    return st;
+}
+
+}  // namespace yg

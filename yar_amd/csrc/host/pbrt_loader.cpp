@@ -1,0 +1,587 @@
+// pbrt_loader.cpp - loader for the pbrt3 subset consumed by the RAY path tracer.
+//
+// Import semantics follow the reference (kennyalive/YAR):
+//   load_pbrt_scene             src/lib/pbrt_scene.cpp:1050-1183
+//   import_pbrt_triangle_mesh   src/lib/pbrt_scene.cpp:676-743 (default UVs for 3/4-vertex meshes,
+//                               remove_degenerate_triangles src/lib/triangle_mesh.cpp:5)
+//   check_if_mesh_is_rectangle  src/lib/pbrt_scene.cpp:45-121 (rectangular area lights)
+//   import_pbrt_material        src/lib/pbrt_scene.cpp:418-501 (matte, metal, glass)
+//   import_pbrt_non_area_light  src/lib/pbrt_scene.cpp:906-915 (point light)
+//   import_pbrt_camera          src/lib/pbrt_scene.cpp:985-1045
+//   finalize_scene              src/lib/scene_loader.cpp:56-72
+// and the parameter defaults of the vendored pbrt-parser fork (third-party/pbrt-parser:
+// impl/semantic/{Camera,Sampler,Integrator,Materials}.cpp, impl/syntactic/Parser.inl:695 LookAt).
+#include "common.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <unordered_map>
+
+namespace yar {
+namespace {
+
+struct ParseError {
+    std::string msg;
+};
+
+// ---- tokenizer over a memory buffer ---------------------------------------------------------
+struct Lexer {
+    const char* p;
+    const char* end;
+    int line = 1;
+
+    bool skip_ws() {
+        while (p < end) {
+            char c = *p;
+            if (c == '\n') { line++; p++; }
+            else if (c == ' ' || c == '\t' || c == '\r') p++;
+            else if (c == '#') { while (p < end && *p != '\n') p++; }
+            else return true;
+        }
+        return false;
+    }
+    // Token kinds: 's' quoted string, '[' ']', 'n' number, 'i' identifier, 0 end.
+    char peek_kind() {
+        if (!skip_ws()) return 0;
+        char c = *p;
+        if (c == '"') return 's';
+        if (c == '[' || c == ']') return c;
+        if (c == '-' || c == '+' || c == '.' || std::isdigit((unsigned char)c)) return 'n';
+        return 'i';
+    }
+    std::string ident() {
+        skip_ws();
+        const char* s = p;
+        while (p < end && !std::isspace((unsigned char)*p) && *p != '"' && *p != '[' && *p != ']' && *p != '#') p++;
+        return std::string(s, p);
+    }
+    std::string string() {
+        skip_ws();
+        if (p >= end || *p != '"') throw ParseError{"expected quoted string at line " + std::to_string(line)};
+        const char* s = ++p;
+        while (p < end && *p != '"') { if (*p == '\n') line++; p++; }
+        if (p >= end) throw ParseError{"unterminated string at line " + std::to_string(line)};
+        std::string r(s, p);
+        p++;
+        return r;
+    }
+    double number() { float f; return number2(&f); }
+    // Parses one number both as double (exact for integers) and directly as float (strtof, the
+    // correctly rounded conversion the pbrt-parser's std::stof performs).
+    double number2(float* f) {
+        skip_ws();
+        char* e = nullptr;
+        double v = std::strtod(p, &e);
+        if (e == p) throw ParseError{"expected number at line " + std::to_string(line)};
+        *f = std::strtof(p, nullptr);
+        p = e;
+        return v;
+    }
+    void expect(char c) {
+        skip_ws();
+        if (p >= end || *p != c) throw ParseError{std::string("expected '") + c + "' at line " + std::to_string(line)};
+        p++;
+    }
+};
+
+// One "type name" [values] parameter.
+struct Param {
+    std::string type, name;
+    std::vector<double> nums;     // exact for integer parameters
+    std::vector<float> fnums;     // float parameters, parsed with strtof
+    std::vector<std::string> strs;
+};
+using ParamList = std::vector<Param>;
+
+static ParamList parse_params(Lexer& lx) {
+    ParamList out;
+    while (lx.peek_kind() == 's') {
+        // A parameter declaration is a quoted "type name"; a statement argument is never
+        // followed by a value list in this subset, so any quoted token here starts a parameter.
+        const char* save = lx.p;
+        int save_line = lx.line;
+        std::string decl = lx.string();
+        std::istringstream ss(decl);
+        Param prm;
+        ss >> prm.type >> prm.name;
+        if (prm.name.empty()) { lx.p = save; lx.line = save_line; break; }
+        char k = lx.peek_kind();
+        bool bracket = (k == '[');
+        if (bracket) lx.expect('[');
+        auto read_one = [&]() {
+            char kk = lx.peek_kind();
+            if (kk == 's') prm.strs.push_back(lx.string());
+            else if (kk == 'n') { float f; prm.nums.push_back(lx.number2(&f)); prm.fnums.push_back(f); }
+            else if (kk == 'i') {   // bare true/false
+                std::string id = lx.ident();
+                if (id == "true") { prm.nums.push_back(1); prm.fnums.push_back(1); }
+                else if (id == "false") { prm.nums.push_back(0); prm.fnums.push_back(0); }
+                else throw ParseError{"unexpected token '" + id + "' in parameter " + prm.name};
+            } else throw ParseError{"bad parameter value for " + prm.name};
+        };
+        if (bracket) {
+            while (lx.peek_kind() != ']') {
+                if (lx.peek_kind() == 0) throw ParseError{"unterminated parameter list for " + prm.name};
+                read_one();
+            }
+            lx.expect(']');
+        } else {
+            read_one();
+        }
+        out.push_back(std::move(prm));
+    }
+    return out;
+}
+
+static const Param* find(const ParamList& ps, const char* name) {
+    for (const Param& p : ps) if (p.name == name) return &p;
+    return nullptr;
+}
+static float get_f(const ParamList& ps, const char* name, float def) {
+    const Param* p = find(ps, name);
+    return (p && !p->fnums.empty()) ? p->fnums[0] : def;
+}
+static int get_i(const ParamList& ps, const char* name, int def) {
+    const Param* p = find(ps, name);
+    return (p && !p->nums.empty()) ? (int)p->nums[0] : def;
+}
+static bool get_b(const ParamList& ps, const char* name, bool def) {
+    const Param* p = find(ps, name);
+    if (!p) return def;
+    if (!p->strs.empty()) return p->strs[0] == "true";
+    return !p->nums.empty() ? p->nums[0] != 0 : def;
+}
+static bool get_rgb(const ParamList& ps, const char* name, float out[3]) {
+    const Param* p = find(ps, name);
+    if (!p) return false;
+    if (p->type == "spectrum" || p->type == "blackbody")
+        throw ParseError{std::string("spectral parameter '") + name + "' is not supported (use rgb)"};
+    if (p->fnums.size() == 1) { out[0] = out[1] = out[2] = p->fnums[0]; return true; }
+    if (p->fnums.size() < 3) throw ParseError{std::string("parameter '") + name + "' needs 3 values"};
+    for (int i = 0; i < 3; i++) out[i] = p->fnums[i];
+    return true;
+}
+
+struct MaterialState {
+    std::string type = "";        // "" = pbrt default material (diffuse 0.5)
+    ParamList params;
+    int id = -1;                  // index into SceneData::materials once imported
+};
+
+struct GraphicsState {
+    M34 ctm = M34::identity();
+    std::shared_ptr<MaterialState> material;
+    std::shared_ptr<ParamList> area_light;   // AreaLightSource "diffuse" params
+    bool reverse_orientation = false;
+};
+
+// check_if_mesh_is_rectangle, src/lib/pbrt_scene.cpp:45-121
+static bool mesh_is_rectangle(const MeshData& mesh, float size[2], M34& xf) {
+    size_t nv = mesh.vertices.size() / 3;
+    if (!((nv == 4 || nv == 6) && mesh.indices.size() == 6)) return false;
+    auto V = [&](int i) { const float* q = &mesh.vertices[3 * (size_t)mesh.indices[i]]; return F3(q[0], q[1], q[2]); };
+    F3 p[3] = {V(0), V(1), V(2)};
+    bool found = false;
+    F3 p3;
+    for (int i = 3; i < 6; i++) {
+        F3 pp = V(i);
+        if (length(pp - p[0]) < 1e-4f || length(pp - p[1]) < 1e-4f || length(pp - p[2]) < 1e-4f) continue;
+        if (found) return false;
+        found = true;
+        p3 = pp;
+    }
+    if (!found) return false;
+    F3 v[3] = {p[1] - p[0], p[2] - p[1], p[0] - p[2]};
+    F3 d[3] = {normalized(v[0]), normalized(v[1]), normalized(v[2])};
+    int k = 0;
+    for (; k < 3; k++) if (std::fabs(dot(d[k], d[(k + 1) % 3])) < 1e-4f) break;
+    if (k == 3) return false;
+    F3 mid = (p[0] + p[1] + p[2] + p3) * 0.25f;
+    F3 test = (p[k] + p[(k + 2) % 3]) * 0.5f;
+    if (length(mid - test) > 1e-4f) return false;
+    F3 x_axis = d[k], y_axis = d[(k + 1) % 3];
+    F3 z_axis = cross(x_axis, y_axis);
+    if (!mesh.normals.empty()) {
+        F3 n0(mesh.normals[0], mesh.normals[1], mesh.normals[2]);
+        if (dot(n0, z_axis) < 0.f) { x_axis = -x_axis; z_axis = -z_axis; }
+    }
+    size[0] = length(v[k]);
+    size[1] = length(v[(k + 1) % 3]);
+    xf.set_column(0, x_axis); xf.set_column(1, y_axis); xf.set_column(2, z_axis); xf.set_column(3, mid);
+    return true;
+}
+
+struct Loader {
+    SceneData& sd;
+    std::vector<GraphicsState> stack;
+    GraphicsState gs;
+    std::map<std::string, std::shared_ptr<MaterialState>> named;
+    // camera/film/sampler/integrator state
+    bool have_camera = false;
+    M34 camera_ctm = M34::identity();
+    float fov = 90.f;
+    bool z_is_up = false;
+    bool have_film = false;
+    int xres = 640, yres = 480;      // pbrt defaults
+    float crop[4] = {0, 1, 0, 1};
+    bool have_crop = false;
+    float film_scale = 1.f, max_component = kInf;
+    int xs = 1, ys = 1;
+    bool have_integrator = false;
+    int maxdepth = 5;
+    float rr_threshold = 1.f;
+    float filter_radius = 0.5f;
+
+    explicit Loader(SceneData& s) : sd(s) {}
+
+    void transform(const M34& x) { gs.ctm = mul(gs.ctm, x); }
+
+    int import_material(const std::shared_ptr<MaterialState>& ms) {
+        if (ms && ms->id >= 0) return ms->id;
+        yar_material_desc m{};
+        m.eta_i = 1.f;
+        if (!ms || ms->type.empty()) {                                     // default pbrt material
+            m.type = YAR_MATERIAL_DIFFUSE;
+            m.reflectance[0] = m.reflectance[1] = m.reflectance[2] = 0.5f;
+        } else if (ms->type == "matte") {
+            m.type = YAR_MATERIAL_DIFFUSE;
+            m.reflectance[0] = m.reflectance[1] = m.reflectance[2] = 0.5f;
+            if (find(ms->params, "Kd") && find(ms->params, "Kd")->type == "texture")
+                throw ParseError{"textured matte Kd is not supported"};
+            get_rgb(ms->params, "Kd", m.reflectance);
+        } else if (ms->type == "metal") {
+            m.type = YAR_MATERIAL_METAL;
+            float r = get_f(ms->params, "roughness", 0.01f);
+            m.u_roughness = get_f(ms->params, "uroughness", r);
+            m.v_roughness = get_f(ms->params, "vroughness", find(ms->params, "roughness") ? r : 0.01f);
+            m.roughness_is_alpha = get_b(ms->params, "remaproughness", true) ? 0 : 1;
+            if (!get_rgb(ms->params, "eta", m.eta) || !get_rgb(ms->params, "k", m.k))
+                throw ParseError{"metal without rgb eta/k (copper spectrum default) is not supported"};
+        } else if (ms->type == "glass") {
+            m.type = YAR_MATERIAL_GLASS;
+            for (int i = 0; i < 3; i++) m.reflectance[i] = m.transmittance[i] = 1.f;
+            get_rgb(ms->params, "Kr", m.reflectance);
+            get_rgb(ms->params, "Kt", m.transmittance);
+            m.index_of_refraction = get_f(ms->params, "index", 1.5f);
+            m.u_roughness = get_f(ms->params, "uroughness", 0.f);
+            m.v_roughness = get_f(ms->params, "vroughness", 0.f);
+            m.roughness_is_alpha = get_b(ms->params, "remaproughness", true) ? 0 : 1;
+            if (m.u_roughness > 0.f) throw ParseError{"rough glass (Rough_Glass_BSDF) is not supported"};
+        } else {
+            throw ParseError{"unsupported material type '" + ms->type + "'"};
+        }
+        sd.materials.push_back(m);
+        int id = (int)sd.materials.size() - 1;
+        if (ms) ms->id = id;
+        return id;
+    }
+
+    void shape(const std::string& type, const ParamList& ps) {
+        if (type != "trianglemesh") throw ParseError{"unsupported shape '" + type + "'"};
+        const Param* P = find(ps, "P");
+        const Param* I = find(ps, "indices");
+        if (!P) throw ParseError{"trianglemesh without P"};
+        MeshData mesh;
+        mesh.vertices = P->fnums;
+        size_t nv = mesh.vertices.size() / 3;
+        if (I) {
+            mesh.indices.resize(I->nums.size());
+            for (size_t i = 0; i < I->nums.size(); i++) mesh.indices[i] = (int32_t)I->nums[i];
+        } else if (nv == 3) {
+            mesh.indices = {0, 1, 2};
+        } else {
+            throw ParseError{"trianglemesh without indices"};
+        }
+        if (mesh.indices.size() % 3) throw ParseError{"trianglemesh index count is not a multiple of 3"};
+        for (int32_t ix : mesh.indices) if (ix < 0 || (size_t)ix >= nv) throw ParseError{"trianglemesh index out of range"};
+        if (const Param* N = find(ps, "N")) {
+            if (N->nums.size() != mesh.vertices.size()) throw ParseError{"N count differs from P count"};
+            mesh.normals = N->fnums;
+        }
+        const Param* UV = find(ps, "uv");
+        if (!UV) UV = find(ps, "st");
+        if (UV) {
+            if (UV->nums.size() != 2 * nv) throw ParseError{"uv count differs from vertex count"};
+            mesh.uvs = UV->fnums;
+        } else if (nv == 4) {
+            mesh.uvs = {0, 0, 1, 0, 1, 1, 1, 0};       // pbrt_scene.cpp:704-712 ("not a typo")
+        } else if (nv == 3) {
+            mesh.uvs = {0, 0, 1, 0, 1, 1};             // pbrt_scene.cpp:714-720
+        }
+        mesh.reverse = gs.reverse_orientation;
+        // remove_degenerate_triangles (triangle_mesh.cpp:5): drop triangles with coincident vertices
+        {
+            std::vector<int32_t> good;
+            good.reserve(mesh.indices.size());
+            for (size_t t = 0; t + 2 < mesh.indices.size(); t += 3) {
+                const float* a = &mesh.vertices[3 * (size_t)mesh.indices[t]];
+                const float* b = &mesh.vertices[3 * (size_t)mesh.indices[t + 1]];
+                const float* c = &mesh.vertices[3 * (size_t)mesh.indices[t + 2]];
+                auto eq = [](const float* u, const float* v) { return u[0] == v[0] && u[1] == v[1] && u[2] == v[2]; };
+                if (eq(a, b) || eq(a, c) || eq(b, c)) continue;
+                good.insert(good.end(), {mesh.indices[t], mesh.indices[t + 1], mesh.indices[t + 2]});
+            }
+            mesh.indices.swap(good);
+        }
+        if (mesh.indices.empty()) return;
+
+        yar_object_desc obj{};
+        obj.mesh = (int32_t)sd.meshes.size();
+        obj.material = -1;
+        obj.area_light = -1;
+        M34 o2w = gs.ctm;
+        M34 w2o = inverse_transform(o2w);
+        std::memcpy(obj.object_to_world, o2w.a, sizeof(o2w.a));
+        std::memcpy(obj.world_to_object, w2o.a, sizeof(w2o.a));
+        if (gs.area_light) {
+            float size[2];
+            M34 rect{};
+            if (!mesh_is_rectangle(mesh, size, rect))
+                throw ParseError{"non-rectangular area lights (triangle-mesh lights) are not supported"};
+            if (gs.reverse_orientation) {
+                rect.set_column(0, -rect.column(0));
+                rect.set_column(2, -rect.column(2));
+            }
+            yar_rect_light_desc L{};
+            std::memcpy(L.light_to_world, rect.a, sizeof(rect.a));
+            float Lrgb[3] = {1, 1, 1};
+            get_rgb(*gs.area_light, "L", Lrgb);
+            for (int i = 0; i < 3; i++) L.emitted_radiance[i] = Lrgb[i];
+            L.size[0] = size[0]; L.size[1] = size[1];
+            L.sample_count = get_i(*gs.area_light, "nsamples", 1);
+            sd.rect_lights.push_back(L);
+            obj.area_light = (int32_t)sd.rect_lights.size() - 1;
+        } else {
+            obj.material = import_material(gs.material);
+        }
+        sd.meshes.push_back(std::move(mesh));
+        sd.objects.push_back(obj);
+    }
+
+    void run(Lexer& lx) {
+        bool in_world = false;
+        while (true) {
+            char k = lx.peek_kind();
+            if (k == 0) break;
+            if (k != 'i') throw ParseError{"expected directive at line " + std::to_string(lx.line)};
+            std::string d = lx.ident();
+            if (d == "LookAt") {                                             // Parser.inl:695
+                float v[9];
+                for (float& x : v) lx.number2(&x);
+                F3 eye(v[0], v[1], v[2]), at(v[3], v[4], v[5]), up(v[6], v[7], v[8]);
+                F3 vz = normalized(at - eye);
+                F3 vx = normalized(cross(up, vz));
+                F3 vy = cross(vz, vx);
+                M34 x;
+                x.set_column(0, vx); x.set_column(1, vy); x.set_column(2, vz); x.set_column(3, eye);
+                transform(inverse_transform(x));
+                z_is_up = std::fabs(up.z) > std::fabs(up.y);
+            } else if (d == "Translate") {
+                M34 x = M34::identity();
+                for (int i = 0; i < 3; i++) lx.number2(&x.a[i][3]);
+                transform(x);
+            } else if (d == "Scale") {
+                M34 x = M34::identity();
+                for (int i = 0; i < 3; i++) lx.number2(&x.a[i][i]);
+                transform(x);
+            } else if (d == "Transform" || d == "ConcatTransform") {
+                lx.expect('[');
+                float m[16];
+                for (float& x : m) lx.number2(&x);
+                lx.expect(']');
+                M34 x;
+                for (int c = 0; c < 4; c++) for (int r = 0; r < 3; r++) x.a[r][c] = m[4 * c + r];
+                if (d == "Transform") gs.ctm = x; else transform(x);
+            } else if (d == "Identity") {
+                gs.ctm = M34::identity();
+            } else if (d == "Camera") {
+                std::string type = lx.string();
+                ParamList ps = parse_params(lx);
+                if (type != "perspective") throw ParseError{"unsupported camera '" + type + "'"};
+                fov = get_f(ps, "fov", 30.f);                               // Camera.cpp findCameraFov
+                camera_ctm = gs.ctm;
+                have_camera = true;
+            } else if (d == "Film") {
+                lx.string();
+                ParamList ps = parse_params(lx);
+                have_film = true;
+                xres = get_i(ps, "xresolution", xres);
+                yres = get_i(ps, "yresolution", yres);
+                if (const Param* c = find(ps, "cropwindow")) {
+                    if (c->nums.size() != 4) throw ParseError{"cropwindow needs 4 values"};
+                    for (int i = 0; i < 4; i++) crop[i] = c->fnums[i];
+                    have_crop = true;
+                }
+                film_scale = get_f(ps, "scale", 1.f);
+                max_component = get_f(ps, "maxsampleluminance", kInf);
+            } else if (d == "Sampler") {                                     // Sampler.cpp
+                std::string type = lx.string();
+                ParamList ps = parse_params(lx);
+                if (type == "stratified") {
+                    xs = get_i(ps, "xsamples", 1);
+                    ys = get_i(ps, "ysamples", 1);
+                } else {
+                    int n = get_i(ps, "pixelsamples", 1);
+                    int kk = (int)std::ceil(std::sqrt((double)n));
+                    xs = ys = kk;
+                }
+            } else if (d == "Integrator") {                                  // Integrator.cpp
+                std::string type = lx.string();
+                ParamList ps = parse_params(lx);
+                if (type != "path" && type != "bdpt" && type != "volpath")
+                    throw ParseError{"unsupported integrator '" + type + "'"};
+                have_integrator = true;
+                maxdepth = get_i(ps, "maxdepth", 5);
+                rr_threshold = get_f(ps, "rrthreshold", 1.f);
+            } else if (d == "PixelFilter") {
+                std::string type = lx.string();
+                ParamList ps = parse_params(lx);
+                if (type != "box") throw ParseError{"unsupported pixel filter '" + type + "'"};
+                filter_radius = get_f(ps, "xwidth", 0.5f);
+            } else if (d == "WorldBegin") {
+                in_world = true;
+                gs.ctm = M34::identity();
+            } else if (d == "WorldEnd") {
+                in_world = false;
+            } else if (d == "AttributeBegin" || d == "TransformBegin") {
+                stack.push_back(gs);
+            } else if (d == "AttributeEnd" || d == "TransformEnd") {
+                if (stack.empty()) throw ParseError{"unbalanced " + d};
+                if (d == "AttributeEnd") gs = stack.back();
+                else gs.ctm = stack.back().ctm;
+                stack.pop_back();
+            } else if (d == "Material") {
+                auto ms = std::make_shared<MaterialState>();
+                ms->type = lx.string();
+                ms->params = parse_params(lx);
+                gs.material = ms;
+            } else if (d == "MakeNamedMaterial") {
+                std::string name = lx.string();
+                auto ms = std::make_shared<MaterialState>();
+                ms->params = parse_params(lx);
+                const Param* t = find(ms->params, "type");
+                if (!t || t->strs.empty()) throw ParseError{"MakeNamedMaterial without type"};
+                ms->type = t->strs[0];
+                named[name] = ms;
+            } else if (d == "NamedMaterial") {
+                std::string name = lx.string();
+                auto it = named.find(name);
+                if (it == named.end()) throw ParseError{"unknown named material '" + name + "'"};
+                gs.material = it->second;
+            } else if (d == "AreaLightSource") {
+                std::string type = lx.string();
+                ParamList ps = parse_params(lx);
+                if (type != "diffuse") throw ParseError{"unsupported area light '" + type + "'"};
+                gs.area_light = std::make_shared<ParamList>(ps);
+            } else if (d == "LightSource") {
+                std::string type = lx.string();
+                ParamList ps = parse_params(lx);
+                if (type != "point") throw ParseError{"unsupported light source '" + type + "'"};
+                yar_point_light_desc L{};
+                float I[3] = {1, 1, 1}, sc[3] = {1, 1, 1}, from[3] = {0, 0, 0};
+                get_rgb(ps, "I", I);
+                get_rgb(ps, "scale", sc);
+                if (const Param* f = find(ps, "from")) for (int i = 0; i < 3 && i < (int)f->fnums.size(); i++) from[i] = f->fnums[i];
+                for (int i = 0; i < 3; i++) { L.position[i] = from[i]; L.intensity[i] = I[i] * sc[i]; }
+                sd.point_lights.push_back(L);
+            } else if (d == "ReverseOrientation") {
+                gs.reverse_orientation = !gs.reverse_orientation;
+            } else if (d == "Shape") {
+                std::string type = lx.string();
+                ParamList ps = parse_params(lx);
+                if (!in_world) throw ParseError{"Shape outside WorldBegin/WorldEnd"};
+                shape(type, ps);
+            } else {
+                throw ParseError{"unsupported directive '" + d + "' at line " + std::to_string(lx.line)};
+            }
+        }
+    }
+
+    void finish() {
+        if (!have_camera) throw ParseError{"scene has no Camera"};
+        yar_render_config& c = sd.desc.config;
+        c.film_width = xres;
+        c.film_height = yres;
+        int cx0 = 0, cy0 = 0, cx1 = xres, cy1 = yres;                    // Camera.cpp createFilm
+        if (have_crop) {
+            cx0 = (int)std::ceil(xres * crop[0]);
+            cx1 = (int)std::ceil(xres * crop[1]);
+            cy0 = (int)std::ceil(yres * crop[2]);
+            cy1 = (int)std::ceil(yres * crop[3]);
+        }
+        c.region_x0 = xres - cx1;                                          // pbrt_scene.cpp:1104-1112
+        c.region_x1 = xres - cx0;
+        c.region_y0 = cy0;
+        c.region_y1 = cy1;
+        c.x_pixel_sample_count = xs;
+        c.y_pixel_sample_count = ys;
+        c.max_light_bounces = have_integrator ? maxdepth : 32;             // raytracer_config.h:21
+        c.russian_roulette_bounce_count_threshold = 3;
+        c.russian_roulette_threshold = have_integrator ? rr_threshold : 1.f;
+        c.film_radiance_scale = film_scale;
+        c.max_rgb_component_value_of_film_sample = max_component;
+        c.pixel_filter_radius = filter_radius;
+        c.rng_seed_offset = 0;
+        c.pbrt3_scene = 1;
+        // import_pbrt_camera (pbrt_scene.cpp:985-1045): frame = inverse(CTM at Camera)
+        M34 frame = inverse_transform(camera_ctm);
+        F3 right = frame.column(0), up = frame.column(1), forward = frame.column(2), pos = frame.column(3);
+        M34 vp;
+        if (z_is_up) { vp.set_column(0, -right); vp.set_column(1, forward); vp.set_column(2, up); }
+        else { vp.set_column(0, -right); vp.set_column(1, up); vp.set_column(2, -forward); }
+        vp.set_column(3, pos);
+        std::memcpy(sd.desc.camera.camera_to_world, vp.a, sizeof(vp.a));
+        sd.desc.camera.z_is_up = z_is_up ? 1 : 0;
+        if (yres > xres) {
+            constexpr float deg2rad = kPi / 180.f, rad2deg = 180.f / kPi;
+            float tx = std::tan((fov / 2.f) * deg2rad);
+            float ty = tx * float(yres) / float(xres);
+            sd.desc.camera.fov_y_degrees = (2.f * std::atan(ty)) * rad2deg;
+        } else {
+            sd.desc.camera.fov_y_degrees = fov;
+        }
+        if (sd.point_lights.empty() && sd.rect_lights.empty())
+            throw ParseError{"scene has no supported lights (the default directional light is not supported)"};
+        sd.finalize_desc();
+    }
+};
+
+}  // namespace
+}  // namespace yar
+
+extern "C" int yar_scene_load_pbrt(const char* path, yar_scene** out_scene) {
+    using namespace yar;
+    if (!path || !out_scene) { set_last_error("yar_scene_load_pbrt: null argument"); return YAR_ERR_INVALID_ARGUMENT; }
+    *out_scene = nullptr;
+    std::ifstream f(path, std::ios::binary);
+    if (!f) { set_last_error(std::string("cannot open ") + path); return YAR_ERR_IO; }
+    std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    auto* scene = new yar_scene;
+    try {
+        Lexer lx{text.data(), text.data() + text.size()};
+        Loader ld(scene->data);
+        ld.run(lx);
+        ld.finish();
+    } catch (const ParseError& e) {
+        delete scene;
+        set_last_error(std::string(path) + ": " + e.msg);
+        return YAR_ERR_PARSE;
+    } catch (const std::bad_alloc&) {
+        delete scene;
+        set_last_error("out of host memory while loading scene");
+        return YAR_ERR_OUT_OF_MEMORY;
+    }
+    *out_scene = scene;
+    return YAR_OK;
+}
+
+extern "C" const yar_scene_desc* yar_scene_get_desc(const yar_scene* scene) {
+    return scene ? &scene->data.desc : nullptr;
+}
+
+extern "C" void yar_scene_free(yar_scene* scene) { delete scene; }
