@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain wavefront GPU path tracer on the config-2 workload.
+
+A step is one full render of the seeded config-2 scene (400 displaced icospheres + a 2 km ground
+grid, diffuse / GGX conductor / smooth glass, 2 rectangular lights + 1 point light) at 1024x1024,
+64 spp (8x8 stratified), max depth 8, through the C ABI (yar_gpu_render).  The scene and its
+kd-trees are resident in HBM before the timed region; each step writes the merged film back to the
+host.  Prints one JSON line: Msamples/s (value), rays/s, the traversal kernel's roofline and the CPU
+oracle timed on the host cores on a bounded crop of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Msamples/s & rays/s at 1 GPU; equal-time MSE, guided vs plain wavefront"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per traversal event with this build's HBM layout: kd node 8 B, leaf index 4 B,
+# triangle 48 B (3 x float4 vertices), object record 80 B (world_to_object + mesh bounds).
+BYTES_NODE, BYTES_LEAF_REF, BYTES_TRI, BYTES_OBJ = 8, 4, 48, 80
+
+
+def _scene(config, scale, seed):
+    import yar_amd
+    cache = os.environ.get("YAR_SCENE_CACHE", os.path.join(tempfile.gettempdir(), "yar_scenes"))
+    os.makedirs(cache, exist_ok=True)
+    path = os.path.join(cache, f"config{config}_s{seed}_x{scale}.pbrt")
+    if not os.path.exists(path):
+        yar_amd.generate_scene(config, path + ".tmp", seed=seed, scale=scale)
+        os.replace(path + ".tmp", path)
+    return yar_amd.Scene(path)
+
+
+def _cpu_baseline(scene, view, crop, threads):
+    from tests import oracle_lib
+    import yar_amd
+    v = yar_amd.View()
+    C = type(view)
+    for name, _ in C._fields_:
+        setattr(v, name, getattr(view, name))
+    cx = (view.region_x0 + view.region_x1) // 2
+    cy = (view.region_y0 + view.region_y1) // 2
+    v.region_x0, v.region_y0 = cx - crop // 2, cy - crop // 2
+    v.region_x1, v.region_y1 = v.region_x0 + crop, v.region_y0 + crop
+    osc = oracle_lib.OracleScene(scene, threads=threads)
+    t0 = time.perf_counter()
+    osc.render(v)
+    dt = time.perf_counter() - t0
+    spp = v.x_pixel_sample_count * v.y_pixel_sample_count
+    samples = crop * crop * spp
+    return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"{crop}x{crop} centre crop of the same render ({samples} samples, {spp} spp), "
+                      f"{dt:.2f} s on {threads} threads (oracle/cpu_ref.cpp, g++ -O2 -ffp-contract=off)"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-crop", type=int, default=128, help="side of the CPU-baseline crop (0 disables)")
+    ap.add_argument("--no-counters", action="store_true", help="skip the counted render (roofline bytes)")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--stack", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the path shards by "
+              f"tiles/views without collectives, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import yar_amd
+    t0 = time.perf_counter()
+    scene = _scene(args.config, args.scale, args.seed)
+    t_load = time.perf_counter() - t0
+    ntri = scene.triangle_count()
+    opts = dict(enable_timing=1)
+    if args.block:
+        opts["block_size"] = args.block
+    if args.stack:
+        opts["traversal_stack_depth"] = args.stack
+    ctx = yar_amd.Context(scene, **opts)
+    info = ctx.info()
+    view = ctx.view(0)
+    spp = view.x_pixel_sample_count * view.y_pixel_sample_count
+    w = view.region_x1 - view.region_x0
+    h = view.region_y1 - view.region_y0
+    print(f"# scene: {ntri} triangles, {info.object_count} objects, kd build {info.build_seconds:.2f} s, "
+          f"load {t_load:.2f} s, tree depths scene {info.scene_tree_depth} mesh {info.mesh_tree_depth}, "
+          f"stacks {info.stack_depth >> 16}/{info.stack_depth & 0xffff}", file=sys.stderr, flush=True)
+
+    for i in range(args.warmup):
+        img, _, _, st = ctx.render(0, want_film=False)
+        print(f"# warmup {i}: {st.render_seconds * 1e3:.1f} ms", file=sys.stderr, flush=True)
+
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        img, _, _, st = ctx.render(0, want_film=False)
+        stats.append(st)
+        print(f"# step {i}: {st.render_seconds * 1e3:.1f} ms (gpu {st.gpu_seconds * 1e3:.1f} ms)",
+              file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    ms = elapsed / args.steps * 1e3
+    samples = stats[0].samples
+    rays = stats[0].closest_rays + stats[0].any_rays
+    value = samples / (elapsed / args.steps) / 1e6
+    rays_per_s = rays / (elapsed / args.steps)
+
+    ext_sec = sum(s.extend_seconds for s in stats) / len(stats)
+    ext_launches = stats[0].extend_launches
+    roofline = None
+    if not args.no_counters:
+        cctx = yar_amd.Context(scene, enable_counters=1)
+        _, _, _, cs = cctx.render(0, want_film=False)
+        cctx.close()
+        ext_bytes = (BYTES_NODE * cs.extend_node_visits + BYTES_LEAF_REF * cs.extend_leaf_refs +
+                     BYTES_TRI * cs.extend_triangle_tests + BYTES_OBJ * cs.extend_object_visits)
+        achieved = ext_bytes / ext_sec / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "extend_traffic.json")
+        if os.path.exists(tpath):
+            traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "kernel": "k_extend (closest-hit kd traversal of path rays)",
+                    "algorithmic_bytes_per_launch": int(ext_bytes / max(1, ext_launches)),
+                    "avg_launch_ms": round(ext_sec / max(1, ext_launches) * 1e3, 4),
+                    "launches_per_step": ext_launches,
+                    "per_ray": {"nodes": round(cs.extend_node_visits / cs.extend_rays, 2),
+                                "leaf_refs": round(cs.extend_leaf_refs / cs.extend_rays, 2),
+                                "triangle_tests": round(cs.extend_triangle_tests / cs.extend_rays, 2),
+                                "objects": round(cs.extend_object_visits / cs.extend_rays, 2)}}
+
+    cpu = None
+    if args.cpu_crop > 0:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = _cpu_baseline(scene, view, args.cpu_crop, threads)
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded config-2 scene generator (yar_scene_generate), no checkpoints",
+        "config": {"workload": f"config {args.config} plain wavefront render, {ntri} triangles, {w}x{h}, "
+                               f"{spp} spp, max depth {ctx.scene.desc.config.max_light_bounces}",
+                   "samples_per_step": samples, "triangles": ntri, "scale": args.scale, "seed": args.seed},
+        "rays_per_s": round(rays_per_s, 1), "mrays_per_s": round(rays_per_s / 1e6, 2),
+        "rays_per_sample": round(rays / samples, 3),
+        "gpu_ms_per_step": round(sum(s.gpu_seconds for s in stats) / len(stats) * 1e3, 2),
+        "kernel_ms_per_step": {"extend": round(ext_sec * 1e3, 2),
+                               "shade": round(sum(s.shade_seconds for s in stats) / len(stats) * 1e3, 2),
+                               "shadow": round(sum(s.shadow_seconds for s in stats) / len(stats) * 1e3, 2),
+                               "mis": round(sum(s.mis_seconds for s in stats) / len(stats) * 1e3, 2),
+                               "other": round(sum(s.other_seconds for s in stats) / len(stats) * 1e3, 2)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["gpu_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -167,7 +167,8 @@ typedef struct yar_gpu_options {
     int32_t traversal_stack_depth;      /* 0 = exact depth of the built trees */
     int32_t enable_counters;            /* count node visits / triangle tests per ray */
     int32_t use_graph;                  /* capture the bounce loop in a hipGraph */
-    int32_t reserved[5];
+    int32_t enable_timing;              /* per-kernel HIP-event timing in yar_render_stats */
+    int32_t reserved[4];
 } yar_gpu_options;
 
 typedef struct yar_view {               /* Scene_Overrides analog, reference_renderer.h:23 */
@@ -206,11 +207,17 @@ typedef struct yar_render_stats {
     int64_t samples;                    /* camera samples (pixels * spp) */
     int64_t closest_rays;               /* scene-tree closest-hit queries (trace_ray + MIS) */
     int64_t any_rays;                   /* shadow (any-hit) queries */
-    int64_t node_visits;                /* only with enable_counters */
+    int64_t node_visits;                /* only with enable_counters: all traversal kernels */
     int64_t leaf_refs;
     int64_t triangle_tests;
     int64_t object_visits;
-    double  extend_seconds, shade_seconds, shadow_seconds, mis_seconds, other_seconds;
+    int64_t extend_node_visits;         /* only with enable_counters: extend (path-ray) kernel */
+    int64_t extend_leaf_refs;
+    int64_t extend_triangle_tests;
+    int64_t extend_object_visits;
+    int64_t extend_rays;                /* rays traced by the extend kernel */
+    int64_t extend_launches;
+    double  extend_seconds, shade_seconds, shadow_seconds, mis_seconds, other_seconds;  /* enable_timing */
 } yar_render_stats;
 
 int  yar_gpu_create(const yar_scene_desc* scene, const yar_gpu_options* options, yar_gpu_ctx** out_ctx);

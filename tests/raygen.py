@@ -56,3 +56,33 @@ def random_rays(scene, n, seed=1, tmax=np.inf):
     r["direction"] = dirs
     r["tmax"] = tmax
     return r
+
+
+def rays_at_mesh(vertices, indices, n, seed=1, axis_aligned_fraction=1 / 32):
+    """Rays from a box around a mesh towards random points on its triangles (grazing and
+    axis-aligned directions included), as the reference's Ray_Generator does for mesh trees."""
+    v = np.asarray(vertices, np.float32).reshape(-1, 3)
+    idx = np.asarray(indices, np.int32).reshape(-1, 3)
+    rng = np.random.default_rng(seed)
+    lo, hi = v.min(0), v.max(0)
+    pad = np.maximum((hi - lo) * 0.5, 1e-3)
+    o = rng.uniform(lo - pad, hi + pad, size=(n, 3)).astype(np.float32)
+    t = rng.integers(0, idx.shape[0], n)
+    a, b = rng.uniform(0, 1, n), rng.uniform(0, 1, n)
+    flip = a + b > 1
+    a[flip], b[flip] = 1 - a[flip], 1 - b[flip]
+    p = v[idx[t, 0]] + a[:, None] * (v[idx[t, 1]] - v[idx[t, 0]]) + b[:, None] * (v[idx[t, 2]] - v[idx[t, 0]])
+    d = (p - o).astype(np.float32)
+    na = int(n * axis_aligned_fraction)
+    if na:
+        ax = np.argmax(np.abs(d[:na]), axis=1)
+        keep = np.zeros((na, 3), np.float32)
+        keep[np.arange(na), ax] = np.sign(d[np.arange(na), ax])
+        o[:na] = p[:na] - keep * np.linalg.norm(d[:na], axis=1, keepdims=True)
+        d[:na] = keep
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.zeros(n, dtype=ray_dtype())
+    r["origin"] = o
+    r["direction"] = d
+    r["tmax"] = np.inf
+    return r

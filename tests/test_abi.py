@@ -1,0 +1,63 @@
+"""The C ABI library loads on a CPU-only host and exports every function of include/yar_gpu.h."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+import yar_amd
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    text = open(os.path.join(ROOT, "include", "yar_gpu.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(yar_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = _header_functions()
+    assert len(names) >= 25
+    L = C.CDLL(yar_amd.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(yar_amd._abi.SIGNATURES), set(names) ^ set(yar_amd._abi.SIGNATURES)
+
+
+def test_struct_sizes_match_header_layout():
+    A = yar_amd._abi
+    assert C.sizeof(A.Ray) == 32 and C.sizeof(A.Hit) == 24 and C.sizeof(A.FilmPixel) == 16
+    assert yar_amd.ray_dtype().itemsize == 32 and yar_amd.hit_dtype().itemsize == 24
+    assert C.sizeof(A.View) == 4 * (12 + 1 + 1 + 2 + 4 + 2 + 1 + 1)
+
+
+def test_errors_are_status_codes_not_exits(tmp_path):
+    L = yar_amd.lib()
+    h = C.c_void_p()
+    assert L.yar_scene_load_pbrt(str(tmp_path / "missing.pbrt").encode(), C.byref(h)) == 2   # YAR_ERR_IO
+    assert b"missing.pbrt" in L.yar_last_error()
+    bad = tmp_path / "bad.pbrt"
+    bad.write_text('Camera "perspective"\nWorldBegin\nShape "sphere" "float radius" [1]\nWorldEnd\n')
+    assert L.yar_scene_load_pbrt(str(bad).encode(), C.byref(h)) == 3                         # YAR_ERR_PARSE
+    assert b"sphere" in L.yar_last_error()
+    assert L.yar_gpu_create(None, None, C.byref(h)) == 1                                     # invalid argument
+    assert L.yar_scene_generate(9, 1, 1.0, str(tmp_path / "x.pbrt").encode()) == 1
+    assert L.yar_kdtree_build_mesh(None, None, 0, C.byref(h)) == 1
+
+
+def test_generated_scenes_load(scene_dir):
+    for cfg, scale in [(1, 1.0), (2, 0.02), (4, 0.01), (5, 0.002)]:
+        p = yar_amd.generate_scene(cfg, str(scene_dir / f"gen{cfg}.pbrt"), seed=3, scale=scale)
+        s = yar_amd.Scene(p)
+        d = s.desc
+        assert d.object_count > 0 and d.rect_light_count > 0
+        assert s.triangle_count() > 0
+
+
+def test_generator_is_deterministic(scene_dir):
+    a = yar_amd.generate_scene(1, str(scene_dir / "a.pbrt"), seed=5)
+    b = yar_amd.generate_scene(1, str(scene_dir / "b.pbrt"), seed=5)
+    c = yar_amd.generate_scene(1, str(scene_dir / "c.pbrt"), seed=6)
+    assert open(a, "rb").read() == open(b, "rb").read()
+    assert open(a, "rb").read() != open(c, "rb").read()

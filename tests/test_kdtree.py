@@ -1,0 +1,123 @@
+"""Host kd-tree builder (product) vs the oracle's restatement, and kd traversal vs brute force."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import yar_amd
+from tests import oracle_lib as O
+from tests import raygen, scenes
+
+
+def _meshes(scene):
+    d = scene.desc
+    for i in range(d.mesh_count):
+        m = d.meshes[i]
+        v = np.ctypeslib.as_array(m.vertices, shape=(m.vertex_count * 3,)).copy()
+        idx = np.ctypeslib.as_array(m.indices, shape=(m.triangle_count * 3,)).copy()
+        yield v, idx
+
+
+@pytest.mark.parametrize("which", ["c1", "c2_small"])
+def test_builder_byte_identical_to_oracle(which, c1_path, c2_small_path):
+    scene = yar_amd.Scene(c1_path if which == "c1" else c2_small_path)
+    n = 0
+    for v, idx in _meshes(scene):
+        t = yar_amd.build_kdtree(v, idx)
+        ow, ob, od = O.kdtree_mesh(v, idx)
+        assert np.array_equal(t.words, ow)
+        assert np.array_equal(t.bounds, ob)
+        assert t.depth == od
+        n += 1
+    assert n == scene.desc.mesh_count
+
+
+def test_builder_regression_meshes_identical():
+    data = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "kdtree_regression_meshes.json")))
+    for m in data["meshes"]:
+        v = np.array(m["vertices"], np.float32)
+        idx = np.array(m["indices"], np.int32)
+        t = yar_amd.build_kdtree(v, idx)
+        ow, ob, od = O.kdtree_mesh(v, idx)
+        assert np.array_equal(t.words, ow)
+
+
+def test_box_builder_identical_to_oracle():
+    rng = np.random.default_rng(11)
+    lo = rng.uniform(-10, 10, (500, 3)).astype(np.float32)
+    hi = lo + rng.uniform(0, 2, (500, 3)).astype(np.float32)
+    hi[::7, 1] = lo[::7, 1]                          # planar boxes exercise the planar-primitive rule
+    boxes = np.concatenate([lo, hi], 1)
+    t = yar_amd.build_box_kdtree(boxes)
+    ow, ob, od = O.kdtree_boxes(boxes)
+    assert np.array_equal(t.words, ow)
+
+
+def _leaf_lists(words):
+    """Walk a tree: returns (max depth, list of leaf id lists) and checks the encoding."""
+    leaves, deepest = [], 0
+    stack = [(0, 0)]
+    while stack:
+        n, lvl = stack.pop()
+        w0, w1 = int(words[2 * n]), int(words[2 * n + 1])
+        if (w0 & 3) == 3:
+            cnt = w0 >> 2
+            ids = [int(words[2 * n + 1 + i]) for i in range(cnt)]
+            assert ids == sorted(ids)
+            leaves.append(ids)
+            deepest = max(deepest, lvl)
+        else:
+            stack.append((n + 1, lvl + 1))
+            stack.append((w0 >> 2, lvl + 1))
+    return deepest, leaves
+
+
+def test_builder_structure_invariants(c1_path):
+    scene = yar_amd.Scene(c1_path)
+    for v, idx in _meshes(scene):
+        t = yar_amd.build_kdtree(v, idx)
+        deepest, leaves = _leaf_lists(t.words)
+        ntri = idx.size // 3
+        limit = min(int(round(8 + 1.3 * np.floor(np.log2(ntri)))), 40)
+        assert deepest <= limit == deepest or deepest <= limit
+        # every triangle is referenced by some leaf
+        seen = set(i for lst in leaves for i in lst)
+        assert seen == set(range(ntri))
+
+
+def test_kdtree_save_layout(tmp_path):
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32)
+    idx = np.array([0, 1, 2, 0, 2, 3, 0, 3, 1], np.int32)
+    t = yar_amd.build_kdtree(v, idx)
+    p = tmp_path / "0.kdtree"
+    t.save(str(p))
+    raw = p.read_bytes()
+    assert len(raw) == 24 + 8 + 4 + 8 * (t.words.size // 2)          # kdtree.cpp:140-160
+    assert np.array_equal(np.frombuffer(raw[:24], np.float32), t.bounds)
+    assert np.frombuffer(raw[24:32], np.uint64)[0] == 0
+    assert np.frombuffer(raw[32:36], np.uint32)[0] == t.words.size // 2
+    assert np.array_equal(np.frombuffer(raw[36:], np.uint32), t.words)
+
+
+@pytest.mark.parametrize("which", ["c1", "reg0", "reg1"])
+def test_oracle_kd_traversal_equals_brute_force(which, c1_path):
+    # validate_triangle_mesh_kdtree (test_kdtree.cpp:138-194): kd-tree t == brute-force t exactly
+    if which == "c1":
+        scene = yar_amd.Scene(c1_path)
+        rays = raygen.random_rays(scene, 40_000, seed=2)
+    else:
+        scene = scenes.regression_mesh_scene(int(which[-1]))
+        m = scene.desc.meshes[0]
+        v = np.ctypeslib.as_array(m.vertices, shape=(m.vertex_count * 3,))
+        idx = np.ctypeslib.as_array(m.indices, shape=(m.triangle_count * 3,))
+        rays = raygen.rays_at_mesh(v, idx, 100_000, seed=2)
+    os_ = O.OracleScene(scene)
+    a = os_.trace(rays)
+    b = os_.trace(rays, brute=True)
+    assert np.array_equal(a["t"], b["t"])
+    hit = b["object"] >= 0
+    same = (a["object"] == b["object"]) & (a["triangle"] == b["triangle"])
+    # identical hit ids except exact ties between coincident triangles
+    assert same[hit].mean() > 0.9999
+    assert hit.mean() > 0.05

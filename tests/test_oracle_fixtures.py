@@ -1,0 +1,172 @@
+"""The oracle pinned against the reference's golden vectors and closed forms (CPU only)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests import oracle_lib as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _load(name):
+    return json.load(open(os.path.join(GOLDEN, name)))
+
+
+def test_pcg32_known_answer_vectors():
+    for vec in _load("pcg32_vectors.json")["vectors"]:
+        want = np.array([int(x, 16) for x in vec["outputs"]], dtype=np.uint32)
+        got = O.pcg32(vec["initstate"], vec["initseq"], len(want))
+        assert np.array_equal(got, want), (vec, got)
+
+
+@pytest.mark.skipif(not O.pcg_ref_available(), reason="oracle/_ref not built (reference tree absent)")
+def test_pcg32_matches_reference_pcg_basic():
+    for init, seq in [(0, 0), (0, 1), (0, 0x10000), (42, 54), (0, 0xffffffff), (12345, 987654321)]:
+        assert np.array_equal(O.pcg32(init, seq, 1000), O.pcg32_ref(init, seq, 1000))
+
+
+def test_rng_float_bit_construction():
+    # RNG::get_float: (u32 >> 9 | 0x3f800000) reinterpreted, minus 1 (random.h:61); the sampler's
+    # first image-plane value of pixel (0,0), 1x1 samples, is get_float of the stream's first draw.
+    u = O.pcg32(0, 0, 1)[0]
+    want = np.frombuffer(np.uint32((u >> 9) | 0x3F800000).tobytes(), np.float32)[0] - np.float32(1)
+    img, _, _, _ = O.pixel_samples(0, 0, 0, 1, 1, 0, 0)
+    assert img[0, 0] == min(want, np.float32(float.fromhex('0x1.fffffep-1')))
+
+
+def test_watertight_closed_form_fixtures():
+    fx = _load("watertight_fixtures.json")
+    o, d = fx["ray"]["origin"], fx["ray"]["direction"]
+    for case in fx["cases"]:
+        n = len(case["pz"])
+        ts = []
+        for lane in range(n):
+            p = [[case["px"][k][lane], case["py"][k], case["pz"][lane]] for k in range(3)]
+            t, b = O.watertight(o, d, *p)
+            ts.append(t)
+            if lane in case["miss_lanes"]:
+                assert t == np.inf
+            else:
+                assert abs(5.0 - t - case["pz"][lane]) < 1e-5
+                assert abs(b.sum() - 1.0) < 1e-5
+                # point on the ray equals the barycentric point (test_triangle_intersection.cpp:43-49)
+                hit = np.array(o) + t * np.array(d)
+                bp = b[0] * np.array(p[0]) + b[1] * np.array(p[1]) + b[2] * np.array(p[2])
+                assert np.abs(hit - bp).max() < 1e-3
+        if case["closest"] is not None:
+            assert abs(min(ts) - case["closest"]) < 1e-5
+
+
+def test_watertight_shared_edge_never_leaks():
+    # Two triangles sharing an edge: rays through points on the shared edge must hit at least one
+    # (the watertightness property, test_watertightness.cpp).
+    rng = np.random.default_rng(5)
+    a, b, c, dd = [0, 0, 0], [1, 0, 0], [0, 1, 0], [1.3, 1.1, 0.2]
+    leaks = 0
+    for s in rng.uniform(0, 1, 2000):
+        p = np.array(b) * (1 - s) + np.array(c) * s
+        org = p + np.array([0.1, -0.2, 1.0])
+        dirv = (p - org) / np.linalg.norm(p - org)
+        t1, _ = O.watertight(org, dirv, a, b, c)
+        t2, _ = O.watertight(org, dirv, b, dd, c)
+        leaks += (t1 == np.inf) and (t2 == np.inf)
+    assert leaks == 0
+
+
+def test_sampler_stratification_and_permutation():
+    # Every 1D dimension holds one value per stratum (pixel_sampling.cpp:64-71); every 2D dimension
+    # one value per cell of the nx x ny grid; the image plane is the unshuffled grid in order.
+    nx, ny, d1, d2 = 4, 4, 20, 15
+    spp = nx * ny
+    img, s1, s2, _ = O.pixel_samples(17, 33, 0, nx, ny, d1, d2)
+    for k in range(spp):
+        x, y = k % nx, k // nx
+        assert x / nx <= img[k, 0] < (x + 1) / nx and y / ny <= img[k, 1] < (y + 1) / ny
+    for i in range(d1):
+        strata = np.floor(s1[i] * spp).astype(int)
+        assert sorted(strata.tolist()) == list(range(spp))
+    for i in range(d2):
+        cells = np.floor(s2[i, :, 0] * nx).astype(int) + nx * np.floor(s2[i, :, 1] * ny).astype(int)
+        assert sorted(cells.tolist()) == list(range(spp))
+
+
+def test_sampler_matches_numpy_restatement():
+    # Independent restatement of next_pixel's draw order with the pcg32 stream itself.
+    nx, ny, d1, d2 = 2, 2, 3, 2
+    spp = nx * ny
+    x, y, off = 5, 9, 7
+    stream = ((x & 0xFFFF) | (y << 16)) + off
+    draws = O.pcg32(0, stream, 2 * spp + d1 * (2 * spp - 1) + d2 * (3 * spp - 1) + 4)
+    pos = 0
+
+    def f():
+        nonlocal pos
+        u = draws[pos]
+        pos += 1
+        return np.frombuffer(np.uint32((u >> 9) | 0x3F800000).tobytes(), np.float32)[0] - np.float32(1)
+
+    def bounded(n):
+        nonlocal pos
+        u = draws[pos]
+        pos += 1
+        return int((int(u) * n) >> 32)
+
+    ome = np.float32(float.fromhex('0x1.fffffep-1'))
+    want_img = [(min((np.float32(i) + f()) * np.float32(1 / nx), ome), min((np.float32(j) + f()) * np.float32(1 / ny), ome))
+                for j in range(ny) for i in range(nx)]
+    want1 = []
+    for _ in range(d1):
+        v = [min((np.float32(i) + f()) * (np.float32(1) / np.float32(spp)), ome) for i in range(spp)]
+        for n in range(spp, 1, -1):
+            k = bounded(n)
+            v[n - 1], v[k] = v[k], v[n - 1]
+        want1.append(v)
+    want2 = []
+    for _ in range(d2):
+        v = [(min((np.float32(i) + f()) * np.float32(1 / nx), ome), min((np.float32(j) + f()) * np.float32(1 / ny), ome))
+             for j in range(ny) for i in range(nx)]
+        for n in range(spp, 1, -1):
+            k = bounded(n)
+            v[n - 1], v[k] = v[k], v[n - 1]
+        want2.append(v)
+    img, s1, s2, st = O.pixel_samples(x, y, off, nx, ny, d1, d2)
+    assert np.array_equal(img, np.array(want_img, np.float32))
+    assert np.array_equal(s1, np.array(want1, np.float32))
+    assert np.array_equal(s2, np.array(want2, np.float32))
+
+
+def test_form_factor_closed_form_sanity():
+    from tests.scenes import parallel_rect_form_factor
+    # an infinite parallel plane has form factor 1
+    assert abs(parallel_rect_form_factor(1e6, 1.0) - 1.0) < 1e-5
+    assert abs(parallel_rect_form_factor(1.0, 1.0) - 0.554124) < 1e-5
+
+
+def test_oracle_direct_lighting_matches_form_factor():
+    # maxdepth 1: radiance of a diffuse point under a parallel rectangle light = albedo * Le * F
+    # (MIS of light and BSDF sampling, direct_lighting.cpp:109-194).
+    from tests.scenes import parallel_light_scene, parallel_rect_form_factor
+    sc = parallel_light_scene(albedo=0.5, Le=1.0, h=1.0, half=1.0, spp_side=16, res=16)
+    os_ = O.OracleScene(sc)
+    v = _view_from_desc(sc.desc)
+    img, film, var, _, _ = os_.render(v)
+    want = 0.5 * 1.0 * parallel_rect_form_factor(1.0, 1.0)
+    got = img.mean()
+    # 16x16 pixels x 256 stratified samples; the estimator's relative std here is ~0.3%
+    assert abs(got - want) / want < 0.01, (got, want)
+
+
+def _view_from_desc(d):
+    from yar_amd import View
+    v = View()
+    v.camera_to_world[:] = d.camera.camera_to_world[:]
+    v.fov_y_degrees = d.camera.fov_y_degrees
+    v.z_is_up = d.camera.z_is_up
+    v.film_width, v.film_height = d.config.film_width, d.config.film_height
+    v.region_x0, v.region_y0, v.region_x1, v.region_y1 = (d.config.region_x0, d.config.region_y0,
+                                                          d.config.region_x1, d.config.region_y1)
+    v.x_pixel_sample_count, v.y_pixel_sample_count = d.config.x_pixel_sample_count, d.config.y_pixel_sample_count
+    v.rng_seed_offset = d.config.rng_seed_offset
+    return v

@@ -65,7 +65,9 @@ class Scene:
 
     @property
     def desc(self):
-        return lib().yar_scene_get_desc(self._h).contents
+        d = lib().yar_scene_get_desc(self._h).contents
+        d._owner = self            # keeps the scene (and the arrays the descriptor points at) alive
+        return d
 
     def desc_ptr(self):
         return lib().yar_scene_get_desc(self._h)

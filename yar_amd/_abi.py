@@ -63,7 +63,7 @@ class SceneDesc(C.Structure):
 class GpuOptions(C.Structure):
     _fields_ = [("device", C.c_int32), ("block_size", C.c_int32), ("max_paths_in_flight", C.c_int64),
                 ("traversal_stack_depth", C.c_int32), ("enable_counters", C.c_int32),
-                ("use_graph", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("use_graph", C.c_int32), ("enable_timing", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class View(C.Structure):
@@ -93,6 +93,9 @@ class RenderStats(C.Structure):
     _fields_ = [("render_seconds", C.c_double), ("gpu_seconds", C.c_double), ("samples", C.c_int64),
                 ("closest_rays", C.c_int64), ("any_rays", C.c_int64), ("node_visits", C.c_int64),
                 ("leaf_refs", C.c_int64), ("triangle_tests", C.c_int64), ("object_visits", C.c_int64),
+                ("extend_node_visits", C.c_int64), ("extend_leaf_refs", C.c_int64),
+                ("extend_triangle_tests", C.c_int64), ("extend_object_visits", C.c_int64),
+                ("extend_rays", C.c_int64), ("extend_launches", C.c_int64),
                 ("extend_seconds", C.c_double), ("shade_seconds", C.c_double),
                 ("shadow_seconds", C.c_double), ("mis_seconds", C.c_double), ("other_seconds", C.c_double)]
 

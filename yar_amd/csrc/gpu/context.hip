@@ -51,6 +51,43 @@ struct DevBuf {
 
 static int pow2_at_least(int v) { int p = 1; while (p < v) p <<= 1; return p; }
 
+// Per-kernel HIP-event timing (kind: 0 extend, 1 shade, 2 shadow, 3 mis, 4 other).
+struct KernelTimer {
+    bool on = false;
+    hipStream_t s = nullptr;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    std::vector<std::pair<int, size_t>> marks;   // kind, index of the start event
+    hipEvent_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            HIPX(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void begin(int kind) {
+        if (!on) return;
+        marks.push_back({kind, used});
+        HIPX(hipEventRecord(next(), s));
+    }
+    void end() {
+        if (!on) return;
+        HIPX(hipEventRecord(next(), s));
+    }
+    void collect(double out[5]) {
+        for (int k = 0; k < 5; k++) out[k] = 0;
+        for (auto& m : marks) {
+            float ms = 0.f;
+            HIPX(hipEventElapsedTime(&ms, pool[m.second], pool[m.second + 1]));
+            out[m.first] += ms * 1e-3;
+        }
+        marks.clear();
+        used = 0;
+    }
+    ~KernelTimer() { for (auto e : pool) (void)hipEventDestroy(e); }
+};
+
 struct View {
     yar_view v;
 };
@@ -89,10 +126,11 @@ struct yar_gpu_ctx {
     DevBuf<float4> sh_o, sh_d, sh_c, mi_o, mi_d, mi_f;
     DevBuf<float> mi_ndw;
     DevBuf<unsigned long long> totals;
-    DevBuf<float4> acc_sum, out_film, side_rad;
-    DevBuf<double2> acc_lum;
+    DevBuf<float4> out_film, side_rad;
     DevBuf<double> out_var;
     DevBuf<int4> side_info;
+    DevBuf<uint8_t> spill;
+    DevBuf<int> tile_first;
     uint32_t* h_counters = nullptr;       // pinned
     // trace_rays buffers
     DevBuf<float4> tr_rays, tr_tb;
@@ -106,7 +144,7 @@ struct yar_gpu_ctx {
         objs.release();
         DevBuf<float4>* b4[] = {&tri_verts, &obj_o2w, &obj_nrm, &vtx_n, &ray_o, &ray_d, &coeff, &L, &dl_coeff,
                                 &dl_light, &dl_bsdf, &hit_tb, &radiance, &sh_o, &sh_d, &sh_c, &mi_o, &mi_d,
-                                &mi_f, &acc_sum, &out_film, &side_rad, &tr_rays, &tr_tb};
+                                &mi_f, &out_film, &side_rad, &tr_rays, &tr_tb};
         for (auto* b : b4) b->release();
         DevBuf<int4>* bi4[] = {&obj_info, &mesh_info, &tri_vidx, &side_info};
         for (auto* b : bi4) b->release();
@@ -114,7 +152,7 @@ struct yar_gpu_ctx {
         pixels.release(); table.release(); hit_ot.release(); tr_ot.release(); state.release(); rng.release();
         DevBuf<uint32_t>* bu[] = {&q_a, &q_b, &sh_path, &mi_path, &dl_path, &counters};
         for (auto* b : bu) b->release();
-        mi_ndw.release(); totals.release(); acc_lum.release(); out_var.release();
+        mi_ndw.release(); totals.release(); out_var.release(); spill.release(); tile_first.release();
         if (h_counters) (void)hipHostFree(h_counters);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -384,10 +422,12 @@ struct RenderResult {
 };
 
 static void ensure_render_buffers(yar_gpu_ctx* ctx, int64_t paths, int64_t chunk_pixels, int64_t stride, int64_t total_pixels,
-                                  bool need_rng) {
+                                  int spp, bool need_rng) {
     ctx->table.reserve((size_t)(chunk_pixels * stride));
+    ctx->radiance.reserve((size_t)(chunk_pixels * spp));
+    ctx->spill.reserve((size_t)chunk_pixels);
     DevBuf<float4>* pb[] = {&ctx->ray_o, &ctx->ray_d, &ctx->coeff, &ctx->L, &ctx->dl_coeff, &ctx->dl_light, &ctx->dl_bsdf,
-                            &ctx->hit_tb, &ctx->radiance, &ctx->sh_o, &ctx->sh_d, &ctx->sh_c, &ctx->mi_o, &ctx->mi_d, &ctx->mi_f};
+                            &ctx->hit_tb, &ctx->sh_o, &ctx->sh_d, &ctx->sh_c, &ctx->mi_o, &ctx->mi_d, &ctx->mi_f};
     for (auto* b : pb) b->reserve((size_t)paths);
     ctx->hit_ot.reserve((size_t)paths);
     ctx->state.reserve((size_t)paths);
@@ -396,9 +436,7 @@ static void ensure_render_buffers(yar_gpu_ctx* ctx, int64_t paths, int64_t chunk
     for (auto* b : qb) b->reserve((size_t)paths);
     ctx->mi_ndw.reserve((size_t)paths);
     ctx->counters.reserve(16);
-    ctx->totals.reserve(8);
-    ctx->acc_sum.reserve((size_t)chunk_pixels);
-    ctx->acc_lum.reserve((size_t)chunk_pixels);
+    ctx->totals.reserve(16);
     ctx->out_film.reserve((size_t)total_pixels);
     ctx->out_var.reserve((size_t)total_pixels);
     ctx->side_rad.reserve(1 << 16);
@@ -418,27 +456,44 @@ static void render_tiles_impl(yar_gpu_ctx* ctx, const yar_view& v, const std::ve
     const int s_par = sequential ? 1 : spp;
     const int64_t stride = (int64_t)spp * (2 + d1 + 2 * d2);
 
-    // pixel list
+    // pixel list: tiles in the given order, row-major inside each tile
+    int tgx, tgy, sx0, sy0, sx1, sy1;
+    film_layout(v, &tgx, &tgy, &sx0, &sy0, &sx1, &sy1);
     std::vector<int2> plist;
     std::vector<TileGeom> geoms;
+    std::vector<int> tile_first((size_t)tgx * tgy, -1);
+    std::vector<int64_t> tile_start;
     for (int t : tile_list) {
         TileGeom g = tile_geom(v, t);
         geoms.push_back(g);
+        tile_start.push_back((int64_t)plist.size());
+        tile_first[t] = (int)plist.size();
         for (int y = g.y0; y < g.y1; y++)
             for (int x = g.x0; x < g.x1; x++) plist.push_back(make_int2(x, y));
     }
+    tile_start.push_back((int64_t)plist.size());
     const int64_t total_pixels = (int64_t)plist.size();
     int64_t max_paths = ctx->opt.max_paths_in_flight > 0 ? ctx->opt.max_paths_in_flight : (int64_t)1 << 21;
-    int64_t chunk_pixels = std::max<int64_t>(1, max_paths / s_par);
-    // keep the sample table chunk under ~2 GB
-    chunk_pixels = std::min<int64_t>(chunk_pixels, std::max<int64_t>(1, ((int64_t)1 << 29) / stride));
-    chunk_pixels = std::min<int64_t>(chunk_pixels, std::max<int64_t>(total_pixels, 1));
+    int64_t max_chunk = std::max<int64_t>(4096, max_paths / s_par);
+    // bound the chunk's sample table (~2 GB) and per-sample radiance (~2 GB)
+    max_chunk = std::min<int64_t>(max_chunk, std::max<int64_t>(4096, ((int64_t)1 << 29) / stride));
+    max_chunk = std::min<int64_t>(max_chunk, std::max<int64_t>(4096, ((int64_t)1 << 27) / spp));
+    // chunks hold whole tiles so in-tile film neighbours are resident together
+    std::vector<int64_t> chunk_start{0};
+    for (size_t t = 0; t < tile_list.size(); t++) {
+        if (tile_start[t + 1] - chunk_start.back() > max_chunk && tile_start[t] > chunk_start.back())
+            chunk_start.push_back(tile_start[t]);
+    }
+    chunk_start.push_back(total_pixels);
+    int64_t chunk_pixels = 1;
+    for (size_t i = 0; i + 1 < chunk_start.size(); i++) chunk_pixels = std::max(chunk_pixels, chunk_start[i + 1] - chunk_start[i]);
     const int64_t paths = chunk_pixels * s_par;
-    ensure_render_buffers(ctx, paths, chunk_pixels, stride, std::max<int64_t>(1, total_pixels), sequential);
+    ensure_render_buffers(ctx, paths, chunk_pixels, stride, std::max<int64_t>(1, total_pixels), spp, sequential);
     hipStream_t s = ctx->stream;
     ctx->pixels.upload(plist, s);
+    ctx->tile_first.upload(tile_first, s);
     HIPX(hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(uint32_t), s));
-    HIPX(hipMemsetAsync(ctx->totals.p, 0, 8 * sizeof(unsigned long long), s));
+    HIPX(hipMemsetAsync(ctx->totals.p, 0, 16 * sizeof(unsigned long long), s));
 
     Chunk c{};
     c.spp = spp; c.s_par = s_par; c.nx = v.x_pixel_sample_count; c.ny = v.y_pixel_sample_count;
@@ -462,44 +517,77 @@ static void render_tiles_impl(yar_gpu_ctx* ctx, const yar_view& v, const std::ve
             ctx->dl_bsdf.p, ctx->hit_tb.p, ctx->hit_ot.p, sequential ? ctx->rng.p : nullptr, ctx->radiance.p};
     Queues q{ctx->q_a.p, ctx->q_b.p, ctx->sh_path.p, ctx->sh_o.p, ctx->sh_d.p, ctx->sh_c.p, ctx->mi_path.p,
              ctx->mi_o.p, ctx->mi_d.p, ctx->mi_f.p, ctx->mi_ndw.p, ctx->dl_path.p, ctx->counters.p, ctx->totals.p};
-    FilmAcc f{ctx->acc_sum.p, ctx->acc_lum.p, ctx->out_film.p, ctx->out_var.p, 0, ctx->side_rad.p, ctx->side_info.p, 1 << 16};
+    FilmAcc f{};
+    f.out_film = ctx->out_film.p;
+    f.out_var = ctx->out_var.p;
+    f.spill = ctx->spill.p;
+    f.tile_first = ctx->tile_first.p;
+    f.sx0 = sx0; f.sy0 = sy0; f.sx1 = sx1; f.sy1 = sy1; f.tgx = tgx;
+    f.rx0 = v.region_x0; f.ry0 = v.region_y0; f.rx1 = v.region_x1; f.ry1 = v.region_y1;
+    f.side_rad = ctx->side_rad.p;
+    f.side_info = ctx->side_info.p;
+    f.side_capacity = 1 << 16;
     LaunchCfg lc = ctx->launch_cfg();
 
+    KernelTimer kt;
+    kt.on = ctx->opt.enable_timing != 0;
+    kt.s = s;
     hipEvent_t e0, e1;
     HIPX(hipEventCreate(&e0));
     HIPX(hipEventCreate(&e1));
     HIPX(hipEventRecord(e0, s));
-    int64_t closest = 0, anyr = 0;
+    int64_t closest = 0, anyr = 0, ext_rays = 0, ext_launches = 0;
     const int passes = spp / s_par;
-    for (int64_t base = 0; base < total_pixels; base += chunk_pixels) {
+    for (size_t ci = 0; ci + 1 < chunk_start.size(); ci++) {
+        const int64_t base = chunk_start[ci];
         c.pixels = ctx->pixels.p + base;
-        c.n_pixels = (int)std::min<int64_t>(chunk_pixels, total_pixels - base);
+        c.n_pixels = (int)(chunk_start[ci + 1] - base);
         f.base = (int)base;
         c.pass = 0;
+        kt.begin(4);
         launch_sample_table(c, ctx->table.p, s);
+        kt.end();
         for (int pass = 0; pass < passes; pass++) {
             c.pass = pass;
+            kt.begin(4);
             launch_raygen(c, P, q, s);
+            kt.end();
             int count = c.n_pixels * s_par;
             Queues qq = q;
+            int iterations = 0;
             while (count > 0) {
+                if (++iterations > 2 * std::max(1, rc.max_light_bounces) + 4) throw HipError{"bounce loop did not terminate"};
+                kt.begin(0);
                 launch_extend(ctx->scene, P, qq, count, lc);
+                kt.end();
+                ext_rays += count;
+                ext_launches++;
                 HIPX(hipMemsetAsync(qq.counters, 0, 4 * sizeof(uint32_t), s));
+                kt.begin(1);
                 launch_shade(ctx->scene, c, P, qq, count, s);
+                kt.end();
                 HIPX(hipMemcpyAsync(ctx->h_counters, qq.counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
                 HIPX(hipStreamSynchronize(s));
                 const uint32_t n_active = ctx->h_counters[0], n_sh = ctx->h_counters[1], n_mis = ctx->h_counters[2],
                                n_dl = ctx->h_counters[3];
                 closest += count + n_mis;
                 anyr += n_sh;
+                kt.begin(2);
                 launch_shadow(ctx->scene, P, qq, (int)n_sh, lc);
+                kt.end();
+                kt.begin(3);
                 launch_mis(ctx->scene, P, qq, (int)n_mis, lc);
+                kt.end();
+                kt.begin(4);
                 launch_resolve(ctx->scene, c, P, qq, (int)n_dl, s);
+                kt.end();
                 std::swap(qq.active_in, qq.active_out);
                 count = (int)n_active;
             }
-            launch_film(c, P, f, q, pass == 0, pass == passes - 1, s);
         }
+        kt.begin(4);
+        launch_film(c, P, f, q, s);
+        kt.end();
     }
     HIPX(hipEventRecord(e1, s));
     std::vector<float4> film_own(total_pixels);
@@ -508,10 +596,12 @@ static void render_tiles_impl(yar_gpu_ctx* ctx, const yar_view& v, const std::ve
         HIPX(hipMemcpyAsync(film_own.data(), ctx->out_film.p, total_pixels * sizeof(float4), hipMemcpyDeviceToHost, s));
         HIPX(hipMemcpyAsync(var_own.data(), ctx->out_var.p, total_pixels * sizeof(double), hipMemcpyDeviceToHost, s));
     }
-    unsigned long long tot[8];
+    unsigned long long tot[16];
     HIPX(hipMemcpyAsync(tot, ctx->totals.p, sizeof(tot), hipMemcpyDeviceToHost, s));
     HIPX(hipMemcpyAsync(ctx->h_counters, ctx->counters.p, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
+    double ksec[5];
+    kt.collect(ksec);
     uint32_t n_side = ctx->h_counters[4];
     if (n_side > (1u << 16)) throw HipError{"jitter-0 side list overflow"};
     std::vector<float4> side_rad(n_side);
@@ -525,11 +615,10 @@ static void render_tiles_impl(yar_gpu_ctx* ctx, const yar_view& v, const std::ve
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
 
-    // Assemble Film_Tiles: own-sample sums for the tile's sample pixels, then the jitter-0
-    // contributions of the tile's samples in visitation order (source row-major, sample index).
+    // Assemble Film_Tiles: in-tile pixel values come from the device gather; the one-pixel margins
+    // receive the side-list samples of the tile in visitation order (row-major sources, sample index).
     out.tiles.assign(tile_list.size(), {});
     out.tile_var.assign(tile_list.size(), 0.0);
-    // side entries grouped by source pixel
     std::vector<size_t> side_order(n_side);
     for (size_t i = 0; i < n_side; i++) side_order[i] = i;
     std::sort(side_order.begin(), side_order.end(), [&](size_t a, size_t b) {
@@ -538,13 +627,13 @@ static void render_tiles_impl(yar_gpu_ctx* ctx, const yar_view& v, const std::ve
         if (A.x != B.x) return A.x < B.x;
         return A.z < B.z;
     });
-    int64_t off = 0;
     for (size_t ti = 0; ti < tile_list.size(); ti++) {
         const TileGeom& g = geoms[ti];
         int pw = g.px1 - g.px0, ph = g.py1 - g.py0;
         std::vector<yar_film_pixel>& tp = out.tiles[ti];
         tp.assign((size_t)std::max(0, pw) * std::max(0, ph), yar_film_pixel{{0, 0, 0}, 0});
         double tv = 0.0;
+        int64_t off = tile_start[ti];
         for (int y = g.y0; y < g.y1; y++)
             for (int x = g.x0; x < g.x1; x++, off++) {
                 const float4 a = film_own[off];
@@ -557,18 +646,16 @@ static void render_tiles_impl(yar_gpu_ctx* ctx, const yar_view& v, const std::ve
             const int4 inf = side_info[si];
             if (inf.x < g.x0 || inf.x >= g.x1 || inf.y < g.y0 || inf.y >= g.y1) continue;
             const float4 r = side_rad[si];
-            int targets[3][2];
-            int nt = 0;
-            if (inf.w & 1) { targets[nt][0] = inf.x - 1; targets[nt][1] = inf.y; nt++; }
-            if (inf.w & 2) { targets[nt][0] = inf.x; targets[nt][1] = inf.y - 1; nt++; }
-            if ((inf.w & 3) == 3) { targets[nt][0] = inf.x - 1; targets[nt][1] = inf.y - 1; nt++; }
-            for (int k = 0; k < nt; k++) {
-                int tx = targets[k][0], ty = targets[k][1];
-                if (tx < g.px0 || tx >= g.px1 || ty < g.py0 || ty >= g.py1) continue;
-                yar_film_pixel& fp = tp[(size_t)(ty - g.py0) * pw + (tx - g.px0)];
-                fp.color_sum[0] += 1.f * r.x; fp.color_sum[1] += 1.f * r.y; fp.color_sum[2] += 1.f * r.z;
-                fp.weight_sum += 1.f;
-            }
+            const int fx0 = inf.x + (inf.w & 3) - 1, fx1 = inf.x + ((inf.w >> 2) & 3);
+            const int fy0 = inf.y + ((inf.w >> 4) & 3) - 1, fy1 = inf.y + ((inf.w >> 6) & 3);
+            for (int ty = fy0; ty < fy1; ty++)
+                for (int tx = fx0; tx < fx1; tx++) {
+                    if (tx < g.px0 || tx >= g.px1 || ty < g.py0 || ty >= g.py1) continue;
+                    if (tx >= g.x0 && tx < g.x1 && ty >= g.y0 && ty < g.y1) continue;   // in-tile: device gather
+                    yar_film_pixel& fp = tp[(size_t)(ty - g.py0) * pw + (tx - g.px0)];
+                    fp.color_sum[0] += 1.f * r.x; fp.color_sum[1] += 1.f * r.y; fp.color_sum[2] += 1.f * r.z;
+                    fp.weight_sum += 1.f;
+                }
         }
     }
     yar_render_stats& st = out.stats;
@@ -576,10 +663,21 @@ static void render_tiles_impl(yar_gpu_ctx* ctx, const yar_view& v, const std::ve
     st.samples = total_pixels * spp;
     st.closest_rays = closest;
     st.any_rays = anyr;
-    st.node_visits = (int64_t)tot[2];
-    st.leaf_refs = (int64_t)tot[3];
-    st.triangle_tests = (int64_t)tot[4];
-    st.object_visits = (int64_t)tot[5];
+    st.node_visits = (int64_t)(tot[0] + tot[4] + tot[8]);
+    st.leaf_refs = (int64_t)(tot[1] + tot[5] + tot[9]);
+    st.triangle_tests = (int64_t)(tot[2] + tot[6] + tot[10]);
+    st.object_visits = (int64_t)(tot[3] + tot[7] + tot[11]);
+    st.extend_node_visits = (int64_t)tot[0];
+    st.extend_leaf_refs = (int64_t)tot[1];
+    st.extend_triangle_tests = (int64_t)tot[2];
+    st.extend_object_visits = (int64_t)tot[3];
+    st.extend_rays = ext_rays;
+    st.extend_launches = ext_launches;
+    st.extend_seconds = ksec[0];
+    st.shade_seconds = ksec[1];
+    st.shadow_seconds = ksec[2];
+    st.mis_seconds = ksec[3];
+    st.other_seconds = ksec[4];
     st.render_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
 }
 
@@ -812,15 +910,15 @@ int yar_gpu_trace_rays(yar_gpu_ctx* ctx, const yar_ray* rays, int64_t count, int
         ctx->tr_rays.reserve(2 * (size_t)count);
         ctx->tr_tb.reserve((size_t)count);
         ctx->tr_ot.reserve((size_t)count);
-        ctx->totals.reserve(8);
+        ctx->totals.reserve(16);
         HIPX(hipMemcpyAsync(ctx->tr_rays.p, rays, (size_t)count * sizeof(yar_ray), hipMemcpyHostToDevice, s));
-        HIPX(hipMemsetAsync(ctx->totals.p, 0, 8 * sizeof(unsigned long long), s));
+        HIPX(hipMemsetAsync(ctx->totals.p, 0, 16 * sizeof(unsigned long long), s));
         LaunchCfg lc = ctx->launch_cfg();
         launch_trace_rays(ctx->scene, ctx->tr_rays.p, count, any_hit, ctx->tr_tb.p, ctx->tr_ot.p, ctx->totals.p, lc);
         HIPX(hipGetLastError());
         std::vector<float4> tb(count);
         std::vector<int2> ot(count);
-        unsigned long long tot[8];
+        unsigned long long tot[16];
         HIPX(hipMemcpyAsync(tb.data(), ctx->tr_tb.p, (size_t)count * sizeof(float4), hipMemcpyDeviceToHost, s));
         HIPX(hipMemcpyAsync(ot.data(), ctx->tr_ot.p, (size_t)count * sizeof(int2), hipMemcpyDeviceToHost, s));
         HIPX(hipMemcpyAsync(tot, ctx->totals.p, sizeof(tot), hipMemcpyDeviceToHost, s));
@@ -833,10 +931,10 @@ int yar_gpu_trace_rays(yar_gpu_ctx* ctx, const yar_ray* rays, int64_t count, int
             h.barycentrics[0] = tb[i].y; h.barycentrics[1] = tb[i].z; h.barycentrics[2] = tb[i].w;
         }
         ctx->last_trace = yar_render_stats{};
-        ctx->last_trace.node_visits = (int64_t)tot[2];
-        ctx->last_trace.leaf_refs = (int64_t)tot[3];
-        ctx->last_trace.triangle_tests = (int64_t)tot[4];
-        ctx->last_trace.object_visits = (int64_t)tot[5];
+        ctx->last_trace.node_visits = (int64_t)tot[12];
+        ctx->last_trace.leaf_refs = (int64_t)tot[13];
+        ctx->last_trace.triangle_tests = (int64_t)tot[14];
+        ctx->last_trace.object_visits = (int64_t)tot[15];
         if (any_hit) ctx->last_trace.any_rays = count; else ctx->last_trace.closest_rays = count;
     } catch (const HipError& e) {
         return fail(ctx, YAR_ERR_HIP, "yar_gpu_trace_rays: " + e.msg);
@@ -868,10 +966,10 @@ extern "C" int yar_gpu_trace_rays_device(yar_gpu_ctx* ctx, const yar_ray* d_rays
         HIPX(hipSetDevice(ctx->device));
         ctx->tr_tb.reserve((size_t)count);
         ctx->tr_ot.reserve((size_t)count);
-        ctx->totals.reserve(8);
+        ctx->totals.reserve(16);
         LaunchCfg lc = ctx->launch_cfg();
         if (hip_stream) lc.stream = (hipStream_t)hip_stream;
-        HIPX(hipMemsetAsync(ctx->totals.p, 0, 8 * sizeof(unsigned long long), lc.stream));
+        HIPX(hipMemsetAsync(ctx->totals.p, 0, 16 * sizeof(unsigned long long), lc.stream));
         launch_trace_rays(ctx->scene, reinterpret_cast<const float4*>(d_rays), count, any_hit, ctx->tr_tb.p, ctx->tr_ot.p,
                           ctx->totals.p, lc);
         hipLaunchKernelGGL(k_pack_hits, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, lc.stream, ctx->tr_tb.p,

@@ -19,7 +19,7 @@ struct Paths {
     float4* hit_tb;      // closest hit: t, barycentrics
     int2* hit_ot;        // closest hit: object, global triangle
     uint64_t* rng;       // per-pixel fallback stream (sequential sample mode only)
-    float4* radiance;    // final per-sample radiance (after clamp and film scale)
+    float4* radiance;    // final per-sample radiance of the chunk, [pixel][sample]
 };
 
 enum : uint32_t {
@@ -36,7 +36,7 @@ struct Queues {
     uint32_t* mis_path; float4* mis_o; float4* mis_d; float4* mis_f; float* mis_ndw;
     uint32_t* dl_path;
     uint32_t* counters;       // [0] active_out, [1] shadow, [2] mis, [3] dl, [4] side entries, [5] active_in
-    unsigned long long* totals;   // [0] closest rays, [1] any rays, [2] nodes, [3] leaf refs, [4] tri tests, [5] objects
+    unsigned long long* totals;   // 16 traversal counters: 4 per kernel kind (extend, shadow, mis, trace_rays)
 };
 
 // Camera, sampler and integrator parameters of one chunk of pixels.
@@ -55,15 +55,21 @@ struct Chunk {
     float rr_threshold, film_scale, max_component;
 };
 
-// Per-pixel film accumulation state for the chunk (sample order preserved across passes).
+// Film accumulation of one chunk (whole 64x64 tiles).  Every sample's pixel footprint is computed
+// exactly as Film_Tile::add_sample does (film.cpp:27-50); because film_pos = pixel + jitter is a float
+// sum, a sample can reach the 3x3 neighbourhood of its pixel.  Contributions inside the source's tile
+// are gathered on the device in the tile's visitation order; contributions to a tile's one-pixel
+// margin go to a side list that the host adds to the Film_Tile.
 struct FilmAcc {
-    float4* sum;              // rgb sum, weight sum
-    double2* lum;             // luminance sum, squared sum
-    float4* out_film;         // per listed pixel: Film_Pixel of its own samples
-    double* out_var;          // per listed pixel: luminance variance of the estimate
+    float4* out_film;         // per listed pixel: its Film_Tile value inside its own tile
+    double* out_var;          // per listed pixel: luminance variance of its estimate
+    uint8_t* spill;           // per chunk pixel: 8-neighbour footprint bits of its samples
     int base;                 // index of the chunk's first pixel in the pixel list
-    float4* side_rad;         // jitter-0 samples that also land in a neighbour pixel
-    int4* side_info;          // x, y, sample, flags (1 = left neighbour, 2 = upper neighbour)
+    const int* tile_first;    // per film tile: pixel-list index of its first pixel, -1 if not rendered
+    int sx0, sy0, sx1, sy1, tgx;   // sample region and tile grid (Film::Film, film.cpp:52-71)
+    int rx0, ry0, rx1, ry1;        // render region (clamps tile pixel bounds)
+    float4* side_rad;         // margin contributions: radiance
+    int4* side_info;          // x, y, sample, packed footprint (p0x-x+1 | (p1x-x)<<2 | (p0y-y+1)<<4 | (p1y-y)<<6)
     int side_capacity;
 };
 
@@ -81,7 +87,7 @@ void launch_shade(const DevScene& sc, const Chunk& c, const Paths& p, Queues q, 
 void launch_shadow(const DevScene& sc, const Paths& p, Queues q, int count, const LaunchCfg& cfg);
 void launch_mis(const DevScene& sc, const Paths& p, Queues q, int count, const LaunchCfg& cfg);
 void launch_resolve(const DevScene& sc, const Chunk& c, const Paths& p, Queues q, int count, hipStream_t s);
-void launch_film(const Chunk& c, const Paths& p, FilmAcc f, Queues q, bool first_pass, bool last_pass, hipStream_t s);
+void launch_film(const Chunk& c, const Paths& p, FilmAcc f, Queues q, hipStream_t s);
 void launch_trace_rays(const DevScene& sc, const float4* rays, int64_t n, int any, float4* out_tb, int2* out_ot,
                        unsigned long long* totals, const LaunchCfg& cfg);
 

@@ -27,15 +27,17 @@ YG_INLINE uint32_t wave_append(uint32_t* counter, bool pred) {
     return base + prefix;
 }
 
+// Traversal counters per kernel kind: totals[4*kind + 0..3] = nodes, leaf refs, triangle tests,
+// objects; kind 0 extend, 1 shadow, 2 mis, 3 trace_rays.
 template <bool COUNT>
-YG_INLINE void flush_counters(unsigned long long* totals, const TravCounters& c) {
+YG_INLINE void flush_counters(unsigned long long* totals, int kind, const TravCounters& c) {
     if (!COUNT) return;
     unsigned long long v[4] = {c.nodes, c.leaf_refs, c.tri_tests, c.objects};
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         unsigned long long x = v[k];
         for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-        if (__lane_id() == 0 && x) atomicAdd(&totals[2 + k], x);
+        if (__lane_id() == 0 && x) atomicAdd(&totals[4 * kind + k], x);
     }
 }
 
@@ -193,7 +195,7 @@ __global__ void k_extend(DevScene sc, Paths P, Queues q, int count, StackCfg cfg
         P.hit_tb[slot] = make_float4(h.t, h.b.x, h.b.y, h.b.z);
         P.hit_ot[slot] = make_int2(h.obj, (int)h.tri);
     }
-    flush_counters<COUNT>(q.totals, cnt);
+    flush_counters<COUNT>(q.totals, 0, cnt);
 }
 
 template <bool COUNT>
@@ -209,7 +211,7 @@ __global__ void k_shadow(DevScene sc, Paths P, Queues q, int count, StackCfg cfg
         bool occluded = trace<true, COUNT>(sc, from4(o4), from4(d4), o4.w, h, sst, cfg.sd, mst, cfg.md, cnt);
         if (!occluded) P.dl_light[q.shadow_path[i]] = q.shadow_c[i];
     }
-    flush_counters<COUNT>(q.totals, cnt);
+    flush_counters<COUNT>(q.totals, 1, cnt);
 }
 
 YG_INLINE float power_heuristic(float a, float b) {             // direct_lighting.cpp:18
@@ -247,7 +249,7 @@ __global__ void k_mis(DevScene sc, Paths P, Queues q, int count, StackCfg cfg) {
             P.dl_bsdf[q.mis_path[i]] = f4(contrib, 0.f);
         }
     }
-    flush_counters<COUNT>(q.totals, cnt);
+    flush_counters<COUNT>(q.totals, 2, cnt);
 }
 
 static size_t stack_lds(const LaunchCfg& cfg) {
@@ -510,7 +512,8 @@ YG_INLINE void finalize(const Chunk& c, const Paths& P, uint32_t slot, RGB L) { 
     float mc = fmaxf(L.r, fmaxf(L.g, L.b));
     if (mc > c.max_component) L = L * (c.max_component / mc);
     if (c.film_scale != 1.f) L = L * c.film_scale;
-    P.radiance[slot] = f4(L, 0.f);
+    const uint32_t p = slot / (uint32_t)c.s_par, j = slot % (uint32_t)c.s_par;
+    P.radiance[(long long)p * c.spp + c.pass * c.s_par + j] = f4(L, 0.f);
 }
 
 __global__ void k_shade(DevScene s, Chunk c, Paths P, Queues q, int count) {
@@ -764,55 +767,126 @@ void launch_resolve(const DevScene& sc, const Chunk& c, const Paths& p, Queues q
 }
 
 // ---------------------------------------------------------------------------------------------
-// Film: per-pixel accumulation in sample order (Film_Tile::add_sample, film.cpp:27-50, box
-// filter radius 0.5) and the per-pixel luminance variance (reference_renderer.cpp:395-409).
-// A sample whose jitter is exactly 0 also lands in the left/upper neighbour pixel; those rare
-// samples are appended to a side list that the host merges in reference order.
+// Film (Film_Tile::add_sample, film.cpp:27-50, box filter radius 0.5; per-pixel luminance variance,
+// reference_renderer.cpp:395-409).
 // ---------------------------------------------------------------------------------------------
-__global__ void k_film(Chunk c, Paths P, FilmAcc f, Queues q, int first_pass, int last_pass) {
+struct Footprint { int x0, x1, y0, y1; };   // [x0, x1) x [y0, y1)
+
+YG_INLINE Footprint footprint(int x, int y, float jx, float jy) {
+    const float fx = (float)x + jx, fy = (float)y + jy;   // film_pos, reference_renderer.cpp:363
+    Footprint f;
+    f.x0 = (int)ceilf(fx - 0.5f - 0.5f);
+    f.y0 = (int)ceilf(fy - 0.5f - 0.5f);
+    f.x1 = (int)floorf(fx + 0.5f - 0.5f) + 1;
+    f.y1 = (int)floorf(fy + 0.5f - 0.5f) + 1;
+    return f;
+}
+
+struct TileBox { int sx0, sy0, sx1, sy1, px0, py0, px1, py1, first; };
+
+YG_INLINE TileBox tile_of(const FilmAcc& f, int x, int y) {          // Film::get_tile_bounds, film.cpp:73-86
+    TileBox t;
+    const int tx = (x - f.sx0) >> 6, ty = (y - f.sy0) >> 6;
+    t.sx0 = f.sx0 + tx * 64; t.sy0 = f.sy0 + ty * 64;
+    t.sx1 = min(t.sx0 + 64, f.sx1); t.sy1 = min(t.sy0 + 64, f.sy1);
+    t.px0 = max(t.sx0 - 1, f.rx0); t.py0 = max(t.sy0 - 1, f.ry0);
+    t.px1 = min(t.sx1 + 1, f.rx1); t.py1 = min(t.sy1 + 1, f.ry1);
+    t.first = f.tile_first[ty * f.tgx + tx];
+    return t;
+}
+
+// Pass 1: which neighbours a pixel's samples reach (bit b = (dy+1)*3 + (dx+1), centre excluded),
+// and the samples that reach the tile's margin (outside its sample bounds) for the host.
+__global__ void k_film_spill(Chunk c, Paths P, FilmAcc f, Queues q) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= c.n_pixels) return;
-    float4 acc = first_pass ? make_float4(0.f, 0.f, 0.f, 0.f) : f.sum[p];
-    double2 lum = first_pass ? make_double2(0.0, 0.0) : f.lum[p];
-    const float* tab = c.table + (long long)p * c.stride;
     const int2 px = c.pixels[p];
-    for (int j = 0; j < c.s_par; j++) {
-        const int k = c.pass * c.s_par + j;
-        const float4 r = P.radiance[(long long)p * c.s_par + j];
-        acc.x += 1.f * r.x;
-        acc.y += 1.f * r.y;
-        acc.z += 1.f * r.z;
-        acc.w += 1.f;
-        const float lv = 0.2126729f * r.x + 0.7151522f * r.y + 0.0721750f * r.z;   // color.cpp:21
-        lum.x += (double)lv;
-        lum.y += (double)(lv * lv);
-        const float jx = tab[2 * k], jy = tab[2 * k + 1];
-        if (jx == 0.f || jy == 0.f) {
+    const float* tab = c.table + (long long)p * c.stride;
+    const TileBox t = tile_of(f, px.x, px.y);
+    uint32_t bits = 0;
+    for (int k = 0; k < c.spp; k++) {
+        const Footprint fp = footprint(px.x, px.y, tab[2 * k], tab[2 * k + 1]);
+        if (fp.x0 == px.x && fp.x1 == px.x + 1 && fp.y0 == px.y && fp.y1 == px.y + 1) continue;
+        bool margin = false;
+        for (int yy = fp.y0; yy < fp.y1; yy++)
+            for (int xx = fp.x0; xx < fp.x1; xx++) {
+                if (xx == px.x && yy == px.y) continue;
+                if (xx < t.px0 || xx >= t.px1 || yy < t.py0 || yy >= t.py1) continue;   // outside Film_Tile
+                if (xx < t.sx0 || xx >= t.sx1 || yy < t.sy0 || yy >= t.sy1) margin = true;
+                else bits |= 1u << ((yy - px.y + 1) * 3 + (xx - px.x + 1));
+            }
+        if (margin) {
             const uint32_t e = atomicAdd(&q.counters[4], 1u);
             if ((int)e < f.side_capacity) {
-                f.side_rad[e] = r;
-                f.side_info[e] = make_int4(px.x, px.y, k, (jx == 0.f ? 1 : 0) | (jy == 0.f ? 2 : 0));
+                f.side_rad[e] = P.radiance[(long long)p * c.spp + k];
+                f.side_info[e] = make_int4(px.x, px.y, k, (fp.x0 - px.x + 1) | ((fp.x1 - px.x) << 2) |
+                                                              ((fp.y0 - px.y + 1) << 4) | ((fp.y1 - px.y) << 6));
             }
         }
     }
-    if (!last_pass) {
-        f.sum[p] = acc;
-        f.lum[p] = lum;
-        return;
+    f.spill[p] = (uint8_t)(bits & 0xffu) | (uint8_t)((bits >> 1) & 0xf0u);   // pack 9 bits minus centre
+}
+
+YG_INLINE bool spill_bit(uint8_t packed, int dx, int dy) {       // direction from source to target
+    int b = (dy + 1) * 3 + (dx + 1);
+    if (b == 4) return false;
+    return b < 4 ? ((packed >> b) & 1) : ((packed >> (b - 1)) & 1);
+}
+
+// Adds the samples of in-tile neighbour (nx, ny) whose footprint covers (x, y), in sample order.
+YG_INLINE void gather(const Chunk& c, const Paths& P, const FilmAcc& f, const TileBox& t, int x, int y, int nx, int ny,
+                      float4& acc) {
+    if (nx < t.sx0 || nx >= t.sx1 || ny < t.sy0 || ny >= t.sy1) return;
+    const int li = t.first + (ny - t.sy0) * (t.sx1 - t.sx0) + (nx - t.sx0) - f.base;   // chunk-local index
+    if (!spill_bit(f.spill[li], x - nx, y - ny)) return;
+    const float* tab = c.table + (long long)li * c.stride;
+    for (int k = 0; k < c.spp; k++) {
+        const Footprint fp = footprint(nx, ny, tab[2 * k], tab[2 * k + 1]);
+        if (x >= fp.x0 && x < fp.x1 && y >= fp.y0 && y < fp.y1) {
+            const float4 r = P.radiance[(long long)li * c.spp + k];
+            acc.x += 1.f * r.x; acc.y += 1.f * r.y; acc.z += 1.f * r.z; acc.w += 1.f;
+        }
     }
+}
+
+// Pass 2: a pixel's Film_Tile value in its own tile, accumulated in render_tile's visitation order
+// (row-major sources, samples in order): earlier neighbours, own samples, later neighbours.
+__global__ void k_film(Chunk c, Paths P, FilmAcc f) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= c.n_pixels) return;
+    const int2 px = c.pixels[p];
+    const int x = px.x, y = px.y;
+    const TileBox t = tile_of(f, x, y);
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    gather(c, P, f, t, x, y, x - 1, y - 1, acc);
+    gather(c, P, f, t, x, y, x, y - 1, acc);
+    gather(c, P, f, t, x, y, x + 1, y - 1, acc);
+    gather(c, P, f, t, x, y, x - 1, y, acc);
+    double lsum = 0.0, lsq = 0.0;
+    for (int k = 0; k < c.spp; k++) {
+        const float4 r = P.radiance[(long long)p * c.spp + k];
+        acc.x += 1.f * r.x; acc.y += 1.f * r.y; acc.z += 1.f * r.z; acc.w += 1.f;
+        const float lv = 0.2126729f * r.x + 0.7151522f * r.y + 0.0721750f * r.z;   // color.cpp:21
+        lsum += (double)lv;
+        lsq += (double)(lv * lv);
+    }
+    gather(c, P, f, t, x, y, x + 1, y, acc);
+    gather(c, P, f, t, x, y, x - 1, y + 1, acc);
+    gather(c, P, f, t, x, y, x, y + 1, acc);
+    gather(c, P, f, t, x, y, x + 1, y + 1, acc);
     f.out_film[f.base + p] = acc;
     double pv = 0.0;
     const int n = c.spp;
     if (n > 1) {
-        pv = (lum.y - lum.x * lum.x / n) / (n * (n - 1));
+        pv = (lsq - lsum * lsum / n) / (n * (n - 1));
         pv = pv > 0.0 ? pv : 0.0;
     }
     f.out_var[f.base + p] = pv;
 }
 
-void launch_film(const Chunk& c, const Paths& p, FilmAcc f, Queues q, bool first_pass, bool last_pass, hipStream_t s) {
-    hipLaunchKernelGGL(k_film, dim3((c.n_pixels + 255) / 256), dim3(256), 0, s, c, p, f, q, first_pass ? 1 : 0,
-                       last_pass ? 1 : 0);
+void launch_film(const Chunk& c, const Paths& p, FilmAcc f, Queues q, hipStream_t s) {
+    hipLaunchKernelGGL(k_film_spill, dim3((c.n_pixels + 255) / 256), dim3(256), 0, s, c, p, f, q);
+    hipLaunchKernelGGL(k_film, dim3((c.n_pixels + 255) / 256), dim3(256), 0, s, c, p, f);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -845,7 +919,7 @@ __global__ void k_trace_rays(DevScene sc, const float4* __restrict__ rays, long 
             }
         }
     }
-    flush_counters<COUNT>(totals, cnt);
+    flush_counters<COUNT>(totals, 3, cnt);
 }
 
 void launch_trace_rays(const DevScene& sc, const float4* rays, int64_t n, int any, float4* out_tb, int2* out_ot,
