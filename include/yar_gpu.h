@@ -168,7 +168,8 @@ typedef struct yar_gpu_options {
     int32_t enable_counters;            /* count node visits / triangle tests per ray */
     int32_t use_graph;                  /* capture the bounce loop in a hipGraph */
     int32_t enable_timing;              /* per-kernel HIP-event timing in yar_render_stats */
-    int32_t reserved[4];
+    int32_t stack_mode;                 /* traversal stack storage: 0 = default, 1 = LDS, 2 = private */
+    int32_t reserved[3];
 } yar_gpu_options;
 
 typedef struct yar_view {               /* Scene_Overrides analog, reference_renderer.h:23 */
@@ -271,6 +272,47 @@ typedef struct yar_gpu_info {
     char    device_name[64];
 } yar_gpu_info;
 int  yar_gpu_get_info(const yar_gpu_ctx* ctx, yar_gpu_info* out_info);
+
+/* ---- learned directional proposal (guided form) ------------------------------------------
+ * No counterpart in the reference: a small MLP (9 inputs -> 16-frequency encoding, 297 features ->
+ * 4 x 64 ReLU -> 40 outputs = 8 von Mises-Fisher lobes) with bf16 weights and fp32 accumulation,
+ * trained online with Adam on the tracer's own radiance samples.  Inputs are position in scene
+ * bounds, outgoing direction and shading normal, each mapped to [0,1]^3.  The parameters travel in
+ * a canonical fp32 order (W1[304][64], b1, W2[64][64], b2, W3, b3, W4, b4, W5[64][48], b5; padding
+ * rows/columns of W1 and W5 are zero). */
+typedef struct yar_guide yar_guide;
+
+typedef struct yar_guide_config {
+    uint32_t seed;                      /* parameter initialisation and batch selection */
+    int32_t  batch_size;                /* training samples per Adam step (0 = 16384) */
+    float    learning_rate;             /* 0 = 5e-3 */
+    float    beta1, beta2, epsilon;     /* 0 = 0.9, 0.99, 1e-8 */
+    int32_t  steps_per_pass;            /* Adam steps after each guided render pass (0 = 32) */
+    float    guide_fraction;            /* one-sample MIS probability of the learned mixture (0 = 0.5) */
+    int32_t  first_pass_spp;            /* samples per pixel of the first (untrained) pass (0 = 1) */
+    int32_t  record_capacity;           /* training records kept per pass (0 = 2^22) */
+    int32_t  reserved[6];
+} yar_guide_config;
+
+int64_t yar_guide_param_count(void);
+int  yar_guide_create(int32_t device, const yar_guide_config* config, yar_guide** out_guide);
+void yar_guide_destroy(yar_guide* guide);
+const char* yar_guide_last_error(const yar_guide* guide);
+int  yar_guide_get_params(yar_guide* guide, float* out_params);
+/* Replaces the parameters (padding forced to zero) and resets the Adam state. */
+int  yar_guide_set_params(yar_guide* guide, const float* params);
+/* Network outputs (40 floats per query) for n queries of 9 inputs. */
+int  yar_guide_forward(yar_guide* guide, const float* inputs, int64_t n, float* out_outputs);
+/* Mean loss -wt log q(dir | inputs) over the n samples and its gradient in canonical order. */
+int  yar_guide_gradients(yar_guide* guide, const float* inputs, const float* dirs, const float* weights, int64_t n,
+                         float* out_gradient, float* out_loss);
+/* `steps` Adam steps on batches drawn from the given samples; out_loss = loss of the last batch. */
+int  yar_guide_train(yar_guide* guide, const float* inputs, const float* dirs, const float* weights, int64_t n,
+                     int32_t steps, float* out_loss);
+/* Mixture density at given directions / directions sampled with 3 uniforms per query. */
+int  yar_guide_pdf(yar_guide* guide, const float* inputs, const float* dirs, int64_t n, float* out_pdf);
+int  yar_guide_sample(yar_guide* guide, const float* inputs, const float* uniforms, int64_t n, float* out_dirs,
+                      float* out_pdf);
 
 #ifdef __cplusplus
 }

@@ -63,7 +63,8 @@ class SceneDesc(C.Structure):
 class GpuOptions(C.Structure):
     _fields_ = [("device", C.c_int32), ("block_size", C.c_int32), ("max_paths_in_flight", C.c_int64),
                 ("traversal_stack_depth", C.c_int32), ("enable_counters", C.c_int32),
-                ("use_graph", C.c_int32), ("enable_timing", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("use_graph", C.c_int32), ("enable_timing", C.c_int32), ("stack_mode", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class View(C.Structure):
@@ -98,6 +99,13 @@ class RenderStats(C.Structure):
                 ("extend_rays", C.c_int64), ("extend_launches", C.c_int64),
                 ("extend_seconds", C.c_double), ("shade_seconds", C.c_double),
                 ("shadow_seconds", C.c_double), ("mis_seconds", C.c_double), ("other_seconds", C.c_double)]
+
+
+class GuideConfig(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("batch_size", C.c_int32), ("learning_rate", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
+                ("steps_per_pass", C.c_int32), ("guide_fraction", C.c_float), ("first_pass_spp", C.c_int32),
+                ("record_capacity", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class GpuInfo(C.Structure):
@@ -155,4 +163,15 @@ SIGNATURES = {
     "yar_gpu_trace_rays_device": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P]),
     "yar_gpu_last_trace_counters": (C.c_int, [_P, C.POINTER(RenderStats)]),
     "yar_gpu_get_info": (C.c_int, [_P, C.POINTER(GpuInfo)]),
+    "yar_guide_param_count": (C.c_int64, []),
+    "yar_guide_create": (C.c_int, [C.c_int32, C.POINTER(GuideConfig), C.POINTER(_P)]),
+    "yar_guide_destroy": (None, [_P]),
+    "yar_guide_last_error": (C.c_char_p, [_P]),
+    "yar_guide_get_params": (C.c_int, [_P, _P]),
+    "yar_guide_set_params": (C.c_int, [_P, _P]),
+    "yar_guide_forward": (C.c_int, [_P, _P, C.c_int64, _P]),
+    "yar_guide_gradients": (C.c_int, [_P, _P, _P, _P, C.c_int64, _P, _P]),
+    "yar_guide_train": (C.c_int, [_P, _P, _P, _P, C.c_int64, C.c_int32, _P]),
+    "yar_guide_pdf": (C.c_int, [_P, _P, _P, C.c_int64, _P]),
+    "yar_guide_sample": (C.c_int, [_P, _P, _P, C.c_int64, _P, _P]),
 }

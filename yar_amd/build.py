@@ -15,7 +15,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libyar_gpu.so")
 
 HOST_SOURCES = ["host/common.cpp", "host/pbrt_loader.cpp", "host/kdtree_builder.cpp", "host/scene_gen.cpp"]
-HIP_SOURCES = ["gpu/render.hip", "gpu/context.hip"]
+HIP_SOURCES = ["gpu/render.hip", "gpu/context.hip", "gpu/guide.hip"]
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")

@@ -88,15 +88,12 @@ struct KernelTimer {
     ~KernelTimer() { for (auto e : pool) (void)hipEventDestroy(e); }
 };
 
-struct View {
-    yar_view v;
-};
-
 }  // namespace
 
 struct yar_gpu_ctx {
     std::string error;
     int device = 0;
+    int cu_count = 1;
     hipStream_t stream = nullptr;
     yar_gpu_options opt{};
     yar_render_config cfg{};
@@ -117,12 +114,17 @@ struct yar_gpu_ctx {
 
     // render buffers
     DevBuf<int2> pixels;
+    DevBuf<int> pix_view, tile_first, side_view;
+    DevBuf<long long> pix_first;
+    DevBuf<DevView> dviews;
+    DevBuf<Chunk> chunks, chunk_slot;
     DevBuf<float> table;
     DevBuf<float4> ray_o, ray_d, coeff, L, dl_coeff, dl_light, dl_bsdf, hit_tb, radiance;
     DevBuf<int2> hit_ot;
+    DevBuf<uint2> pk;
     DevBuf<uint32_t> state;
     DevBuf<uint64_t> rng;
-    DevBuf<uint32_t> q_a, q_b, sh_path, mi_path, dl_path, counters;
+    DevBuf<uint32_t> q_a, q_b, sh_path, mi_path, dl_path, counts;
     DevBuf<float4> sh_o, sh_d, sh_c, mi_o, mi_d, mi_f;
     DevBuf<float> mi_ndw;
     DevBuf<unsigned long long> totals;
@@ -130,16 +132,30 @@ struct yar_gpu_ctx {
     DevBuf<double> out_var;
     DevBuf<int4> side_info;
     DevBuf<uint8_t> spill;
-    DevBuf<int> tile_first;
-    uint32_t* h_counters = nullptr;       // pinned
+    uint32_t* h_poll = nullptr;           // pinned: active counts read back while the loop runs
+    size_t h_poll_n = 0;
+    // bounce-loop graph (captured per job shape)
+    hipGraphExec_t graph = nullptr;
+    long long graph_paths = -1;
+    int graph_iters = -1;
+    Paths graph_P{};
+    Queues graph_q{};
     // trace_rays buffers
     DevBuf<float4> tr_rays, tr_tb;
     DevBuf<int2> tr_ot;
     yar_render_stats last_trace{};
 
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        graph_paths = -1;
+        graph_iters = -1;
+    }
+
     ~yar_gpu_ctx() {
         if (device >= 0) (void)hipSetDevice(device);
-        DevBuf<uint2>* b2[] = {&scene_nodes, &mesh_nodes};
+        drop_graph();
+        DevBuf<uint2>* b2[] = {&scene_nodes, &mesh_nodes, &pk};
         for (auto* b : b2) b->release();
         objs.release();
         DevBuf<float4>* b4[] = {&tri_verts, &obj_o2w, &obj_nrm, &vtx_n, &ray_o, &ray_d, &coeff, &L, &dl_coeff,
@@ -148,12 +164,15 @@ struct yar_gpu_ctx {
         for (auto* b : b4) b->release();
         DevBuf<int4>* bi4[] = {&obj_info, &mesh_info, &tri_vidx, &side_info};
         for (auto* b : bi4) b->release();
+        DevBuf<int>* bi[] = {&pix_view, &tile_first, &side_view};
+        for (auto* b : bi) b->release();
         vtx_uv.release(); materials.release(); rects.release(); points.release();
-        pixels.release(); table.release(); hit_ot.release(); tr_ot.release(); state.release(); rng.release();
-        DevBuf<uint32_t>* bu[] = {&q_a, &q_b, &sh_path, &mi_path, &dl_path, &counters};
+        pixels.release(); pix_first.release(); dviews.release(); chunks.release(); chunk_slot.release();
+        table.release(); hit_ot.release(); tr_ot.release(); state.release(); rng.release();
+        DevBuf<uint32_t>* bu[] = {&q_a, &q_b, &sh_path, &mi_path, &dl_path, &counts};
         for (auto* b : bu) b->release();
-        mi_ndw.release(); totals.release(); out_var.release(); spill.release(); tile_first.release();
-        if (h_counters) (void)hipHostFree(h_counters);
+        mi_ndw.release(); totals.release(); out_var.release(); spill.release();
+        if (h_poll) (void)hipHostFree(h_poll);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -162,7 +181,10 @@ struct yar_gpu_ctx {
         lc.block = opt.block_size > 0 ? opt.block_size : 128;
         lc.scene_stack = info.stack_depth >> 16;
         lc.mesh_stack = info.stack_depth & 0xffff;
+        lc.priv_stack = opt.stack_mode == 2;
         lc.counters = opt.enable_counters != 0;
+        lc.trav_grid = 1;
+        lc.path_grid = 1;
         lc.stream = stream;
         return lc;
     }
@@ -384,15 +406,14 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
     inf.upload_seconds = std::chrono::duration<double>(t2 - t1).count();
 }
 
-// Pixel list of a set of tiles: tiles in the given order, pixels row-major inside each tile's
-// sample bounds (render_tile visitation order, reference_renderer.cpp:346-347).
+// Film layout of a view (Film::Film, film.cpp:52-71; box filter radius 0.5 -> sample region ==
+// render region) and the bounds of one tile (Film::get_tile_bounds, film.cpp:73-86).
 struct TileGeom {
     int x0, y0, x1, y1;      // sample bounds
     int px0, py0, px1, py1;  // pixel bounds (film.cpp:80-85)
 };
 
 static void film_layout(const yar_view& v, int* tgx, int* tgy, int* sx0, int* sy0, int* sx1, int* sy1) {
-    // Film::Film, film.cpp:52-71 (box filter radius 0.5 -> sample region == render region)
     *sx0 = (int)std::floor(v.region_x0 + 0.5f - 0.5f);
     *sy0 = (int)std::floor(v.region_y0 + 0.5f - 0.5f);
     *sx1 = (int)std::ceil(v.region_x1 - 1 + 0.5f + 0.5f);
@@ -415,178 +436,295 @@ static TileGeom tile_geom(const yar_view& v, int tile) {
     return g;
 }
 
-struct RenderResult {
-    std::vector<std::vector<yar_film_pixel>> tiles;   // per requested tile, pixel-bounds sized
-    std::vector<double> tile_var;
-    yar_render_stats stats{};
+// A render job: tiles of one or more views.  The pixel list holds the parts in order, tiles in
+// the given order inside a part, pixels row-major inside a tile's sample bounds (render_tile
+// visitation order, reference_renderer.cpp:346-347).
+struct JobPart {
+    int view;
+    std::vector<int> tiles;
 };
 
-static void ensure_render_buffers(yar_gpu_ctx* ctx, int64_t paths, int64_t chunk_pixels, int64_t stride, int64_t total_pixels,
-                                  int spp, bool need_rng) {
-    ctx->table.reserve((size_t)(chunk_pixels * stride));
-    ctx->radiance.reserve((size_t)(chunk_pixels * spp));
-    ctx->spill.reserve((size_t)chunk_pixels);
-    DevBuf<float4>* pb[] = {&ctx->ray_o, &ctx->ray_d, &ctx->coeff, &ctx->L, &ctx->dl_coeff, &ctx->dl_light, &ctx->dl_bsdf,
-                            &ctx->hit_tb, &ctx->sh_o, &ctx->sh_d, &ctx->sh_c, &ctx->mi_o, &ctx->mi_d, &ctx->mi_f};
-    for (auto* b : pb) b->reserve((size_t)paths);
-    ctx->hit_ot.reserve((size_t)paths);
-    ctx->state.reserve((size_t)paths);
-    if (need_rng) ctx->rng.reserve((size_t)paths);
-    DevBuf<uint32_t>* qb[] = {&ctx->q_a, &ctx->q_b, &ctx->sh_path, &ctx->mi_path, &ctx->dl_path};
-    for (auto* b : qb) b->reserve((size_t)paths);
-    ctx->mi_ndw.reserve((size_t)paths);
-    ctx->counters.reserve(16);
-    ctx->totals.reserve(16);
-    ctx->out_film.reserve((size_t)total_pixels);
-    ctx->out_var.reserve((size_t)total_pixels);
-    ctx->side_rad.reserve(1 << 16);
-    ctx->side_info.reserve(1 << 16);
-    if (!ctx->h_counters) HIPX(hipHostMalloc((void**)&ctx->h_counters, 16 * sizeof(uint32_t), hipHostMallocDefault));
+struct PartResult {
+    std::vector<std::vector<yar_film_pixel>> tiles;   // per requested tile, pixel-bounds sized
+    std::vector<double> tile_var;
+};
+
+static DevView make_dev_view(const yar_view& v, int tile_base) {
+    DevView d{};
+    m34_rows(v.camera_to_world, d.c0, d.c1, d.c2);
+    d.W = (float)v.film_width;
+    d.H = (float)v.film_height;
+    constexpr float deg2rad = yar::kPi / 180.f;                                 // camera.cpp:5-14
+    const float tan_half = std::tan((v.fov_y_degrees / 2.f) * deg2rad);
+    d.vert = tan_half;
+    d.horz = (d.W / d.H) * tan_half;
+    d.zup = v.z_is_up;
+    d.nx = v.x_pixel_sample_count;
+    d.ny = v.y_pixel_sample_count;
+    d.spp = d.nx * d.ny;
+    d.seed_offset = v.rng_seed_offset;
+    d.tile_base = tile_base;
+    int tgy;
+    film_layout(v, &d.tgx, &tgy, &d.sx0, &d.sy0, &d.sx1, &d.sy1);
+    d.rx0 = v.region_x0; d.ry0 = v.region_y0; d.rx1 = v.region_x1; d.ry1 = v.region_y1;
+    return d;
 }
 
-static void render_tiles_impl(yar_gpu_ctx* ctx, const yar_view& v, const std::vector<int>& tile_list, RenderResult& out) {
+// Bounce-loop iterations that always suffice: every shade pass raises a path's bounce count by at
+// least one except the first pass after a delta (glass) event, and paths end at max_light_bounces.
+static int loop_iterations(int max_bounces) { return std::max(1, max_bounces) + 2; }
+
+static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std::vector<PartResult>& out,
+                       yar_render_stats& st) {
     auto t_start = std::chrono::steady_clock::now();
     const yar_render_config& rc = ctx->cfg;
     if (rc.pixel_filter_radius != 0.5f) throw HipError{"only the box filter of radius 0.5 is supported"};
-    const int spp = v.x_pixel_sample_count * v.y_pixel_sample_count;
-    if (spp <= 0 || spp > 65536) throw HipError{"samples per pixel out of range"};
-    const int md = std::min(10, rc.max_light_bounces);
-    const int d1 = 4 * std::max(0, md), d2 = 3 * std::max(0, md);           // reference_renderer.cpp:87-99
-    const bool sequential = rc.max_light_bounces > 10;                      // RNG fallback couples samples
-    const int s_par = sequential ? 1 : spp;
-    const int64_t stride = (int64_t)spp * (2 + d1 + 2 * d2);
+    const int mdim = std::min(10, rc.max_light_bounces);
+    const int d1 = 4 * std::max(0, mdim), d2 = 3 * std::max(0, mdim);          // reference_renderer.cpp:87-99
+    const int dps = 2 + d1 + 2 * d2;
+    const bool sequential = rc.max_light_bounces > 10;                         // RNG fallback couples samples
 
-    // pixel list: tiles in the given order, row-major inside each tile
-    int tgx, tgy, sx0, sy0, sx1, sy1;
-    film_layout(v, &tgx, &tgy, &sx0, &sy0, &sx1, &sy1);
+    // views, tile offsets and the pixel list
+    std::vector<DevView> dv;
+    std::vector<int> tile_first;
     std::vector<int2> plist;
-    std::vector<TileGeom> geoms;
-    std::vector<int> tile_first((size_t)tgx * tgy, -1);
-    std::vector<int64_t> tile_start;
-    for (int t : tile_list) {
-        TileGeom g = tile_geom(v, t);
-        geoms.push_back(g);
-        tile_start.push_back((int64_t)plist.size());
-        tile_first[t] = (int)plist.size();
-        for (int y = g.y0; y < g.y1; y++)
-            for (int x = g.x0; x < g.x1; x++) plist.push_back(make_int2(x, y));
+    std::vector<int> pview;
+    struct Unit { int part, tile; int64_t start, count; };
+    std::vector<Unit> units;
+    for (size_t pi = 0; pi < parts.size(); pi++) {
+        const yar_view& v = ctx->views[parts[pi].view];
+        const int spp = v.x_pixel_sample_count * v.y_pixel_sample_count;
+        if (spp <= 0 || spp > 65536) throw HipError{"samples per pixel out of range"};
+        int tgx, tgy, sx0, sy0, sx1, sy1;
+        film_layout(v, &tgx, &tgy, &sx0, &sy0, &sx1, &sy1);
+        const int base = (int)tile_first.size();
+        dv.push_back(make_dev_view(v, base));
+        tile_first.resize(tile_first.size() + (size_t)tgx * tgy, -1);
+        for (int t : parts[pi].tiles) {
+            const TileGeom g = tile_geom(v, t);
+            if (plist.size() + (size_t)(g.x1 - g.x0) * (g.y1 - g.y0) > (size_t)0x7fffffff)
+                throw HipError{"more than 2^31 pixels in one job"};
+            tile_first[base + t] = (int)plist.size();
+            units.push_back({(int)pi, t, (int64_t)plist.size(), (int64_t)(g.x1 - g.x0) * (g.y1 - g.y0)});
+            for (int y = g.y0; y < g.y1; y++)
+                for (int x = g.x0; x < g.x1; x++) {
+                    plist.push_back(make_int2(x, y));
+                    pview.push_back((int)pi);
+                }
+        }
     }
-    tile_start.push_back((int64_t)plist.size());
     const int64_t total_pixels = (int64_t)plist.size();
-    int64_t max_paths = ctx->opt.max_paths_in_flight > 0 ? ctx->opt.max_paths_in_flight : (int64_t)1 << 21;
-    int64_t max_chunk = std::max<int64_t>(4096, max_paths / s_par);
-    // bound the chunk's sample table (~2 GB) and per-sample radiance (~2 GB)
-    max_chunk = std::min<int64_t>(max_chunk, std::max<int64_t>(4096, ((int64_t)1 << 29) / stride));
-    max_chunk = std::min<int64_t>(max_chunk, std::max<int64_t>(4096, ((int64_t)1 << 27) / spp));
-    // chunks hold whole tiles so in-tile film neighbours are resident together
-    std::vector<int64_t> chunk_start{0};
-    for (size_t t = 0; t < tile_list.size(); t++) {
-        if (tile_start[t + 1] - chunk_start.back() > max_chunk && tile_start[t] > chunk_start.back())
-            chunk_start.push_back(tile_start[t]);
-    }
-    chunk_start.push_back(total_pixels);
-    int64_t chunk_pixels = 1;
-    for (size_t i = 0; i + 1 < chunk_start.size(); i++) chunk_pixels = std::max(chunk_pixels, chunk_start[i + 1] - chunk_start[i]);
-    const int64_t paths = chunk_pixels * s_par;
-    ensure_render_buffers(ctx, paths, chunk_pixels, stride, std::max<int64_t>(1, total_pixels), spp, sequential);
-    hipStream_t s = ctx->stream;
-    ctx->pixels.upload(plist, s);
-    ctx->tile_first.upload(tile_first, s);
-    HIPX(hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(uint32_t), s));
-    HIPX(hipMemsetAsync(ctx->totals.p, 0, 16 * sizeof(unsigned long long), s));
 
-    Chunk c{};
-    c.spp = spp; c.s_par = s_par; c.nx = v.x_pixel_sample_count; c.ny = v.y_pixel_sample_count;
-    c.d1 = d1; c.d2 = d2; c.table = ctx->table.p; c.stride = stride; c.seed_offset = v.rng_seed_offset;
-    m34_rows(v.camera_to_world, c.c2w.r0, c.c2w.r1, c.c2w.r2);
-    c.W = (float)v.film_width; c.H = (float)v.film_height;
+    // chunks of whole tiles, bounded by paths in flight, sample-table and radiance sizes
+    const int64_t max_paths = ctx->opt.max_paths_in_flight > 0 ? ctx->opt.max_paths_in_flight : (int64_t)1 << 22;
+    const int64_t max_samples = std::min<int64_t>(((int64_t)1 << 29) / dps, (int64_t)1 << 27);
+    struct ChunkInfo { int64_t start, pixels, samples, paths; int max_spp; };
+    std::vector<ChunkInfo> cks;
     {
-        constexpr float deg2rad = yar::kPi / 180.f;                             // camera.cpp:5-14
-        float tan_half = std::tan((v.fov_y_degrees / 2.f) * deg2rad);
-        c.vert = tan_half;
-        c.horz = (c.W / c.H) * tan_half;
+        ChunkInfo cur{0, 0, 0, 0, 1};
+        for (const Unit& u : units) {
+            const int spp = dv[u.part].spp;
+            const int64_t us = u.count * spp, up = sequential ? u.count : us;
+            if (cur.pixels > 0 && (cur.paths + up > max_paths || cur.samples + us > max_samples)) {
+                cks.push_back(cur);
+                cur = ChunkInfo{u.start, 0, 0, 0, 1};
+            }
+            cur.pixels += u.count;
+            cur.samples += us;
+            cur.paths += up;
+            cur.max_spp = std::max(cur.max_spp, spp);
+        }
+        if (cur.pixels > 0) cks.push_back(cur);
     }
-    c.zup = v.z_is_up;
-    c.max_bounces = rc.max_light_bounces;
-    c.rr_bounce_threshold = rc.russian_roulette_bounce_count_threshold;
-    c.rr_threshold = rc.russian_roulette_threshold;
-    c.film_scale = rc.film_radiance_scale;
-    c.max_component = rc.max_rgb_component_value_of_film_sample;
+    // chunk-local spp prefix sums, one (n_pixels + 1)-array per chunk
+    std::vector<long long> pf;
+    std::vector<size_t> pf_off;
+    int64_t max_chunk_pixels = 1, max_chunk_samples = 1, max_chunk_paths = 1;
+    for (const ChunkInfo& ck : cks) {
+        pf_off.push_back(pf.size());
+        long long acc = 0;
+        for (int64_t i = 0; i < ck.pixels; i++) {
+            pf.push_back(acc);
+            acc += dv[pview[ck.start + i]].spp;
+        }
+        pf.push_back(acc);
+        max_chunk_pixels = std::max(max_chunk_pixels, ck.pixels);
+        max_chunk_samples = std::max(max_chunk_samples, ck.samples);
+        max_chunk_paths = std::max(max_chunk_paths, ck.paths);
+    }
+    const int64_t side_cap = std::max<int64_t>(1 << 16, std::min<int64_t>((int64_t)1 << 24, max_chunk_samples / 64 * (int64_t)cks.size()));
+
+    // buffers
+    hipStream_t s = ctx->stream;
+    const size_t np = (size_t)max_chunk_paths;
+    ctx->table.reserve((size_t)(max_chunk_samples * dps));
+    ctx->radiance.reserve((size_t)max_chunk_samples);
+    ctx->spill.reserve((size_t)max_chunk_pixels);
+    DevBuf<float4>* pb[] = {&ctx->ray_o, &ctx->ray_d, &ctx->coeff, &ctx->L, &ctx->dl_coeff, &ctx->dl_light, &ctx->dl_bsdf,
+                            &ctx->hit_tb, &ctx->sh_o, &ctx->sh_d, &ctx->sh_c, &ctx->mi_o, &ctx->mi_d, &ctx->mi_f};
+    for (auto* b : pb) b->reserve(np);
+    ctx->hit_ot.reserve(np);
+    ctx->pk.reserve(np);
+    ctx->state.reserve(np);
+    if (sequential) ctx->rng.reserve(np);
+    DevBuf<uint32_t>* qb[] = {&ctx->q_a, &ctx->q_b, &ctx->sh_path, &ctx->mi_path, &ctx->dl_path};
+    for (auto* b : qb) b->reserve(np);
+    ctx->mi_ndw.reserve(np);
+    ctx->counts.reserve(16);
+    ctx->totals.reserve(24);
+    ctx->out_film.reserve((size_t)std::max<int64_t>(1, total_pixels));
+    ctx->out_var.reserve((size_t)std::max<int64_t>(1, total_pixels));
+    ctx->side_rad.reserve((size_t)side_cap);
+    ctx->side_info.reserve((size_t)side_cap);
+    ctx->side_view.reserve((size_t)side_cap);
+    ctx->chunk_slot.reserve(1);
+    ctx->pixels.upload(plist, s);
+    ctx->pix_view.upload(pview, s);
+    ctx->pix_first.upload(pf, s);
+    ctx->tile_first.upload(tile_first, s);
+    ctx->dviews.upload(dv, s);
+
+    // per (chunk, pass) parameter blocks
+    std::vector<Chunk> hc;
+    std::vector<int> passes_of;
+    for (size_t ci = 0; ci < cks.size(); ci++) {
+        const ChunkInfo& ck = cks[ci];
+        Chunk c{};
+        c.pixels = ctx->pixels.p + ck.start;
+        c.pix_view = ctx->pix_view.p + ck.start;
+        c.pix_first = ctx->pix_first.p + pf_off[ci];
+        c.views = ctx->dviews.p;
+        c.table = ctx->table.p;
+        c.n_pixels = (int)ck.pixels;
+        c.max_spp = ck.max_spp;
+        c.d1 = d1; c.d2 = d2; c.dps = dps;
+        c.sequential = sequential ? 1 : 0;
+        c.n_paths = ck.paths;
+        c.max_bounces = rc.max_light_bounces;
+        c.rr_bounce_threshold = rc.russian_roulette_bounce_count_threshold;
+        c.rr_threshold = rc.russian_roulette_threshold;
+        c.film_scale = rc.film_radiance_scale;
+        c.max_component = rc.max_rgb_component_value_of_film_sample;
+        const int passes = sequential ? ck.max_spp : 1;
+        passes_of.push_back(passes);
+        for (int pass = 0; pass < passes; pass++) {
+            c.pass = pass;
+            hc.push_back(c);
+        }
+    }
+    ctx->chunks.upload(hc, s);
+    HIPX(hipMemsetAsync(ctx->counts.p, 0, 16 * sizeof(uint32_t), s));
+    HIPX(hipMemsetAsync(ctx->totals.p, 0, 24 * sizeof(unsigned long long), s));
 
     Paths P{ctx->ray_o.p, ctx->ray_d.p, ctx->coeff.p, ctx->L.p, ctx->state.p, ctx->dl_coeff.p, ctx->dl_light.p,
-            ctx->dl_bsdf.p, ctx->hit_tb.p, ctx->hit_ot.p, sequential ? ctx->rng.p : nullptr, ctx->radiance.p};
+            ctx->dl_bsdf.p, ctx->hit_tb.p, ctx->hit_ot.p, ctx->pk.p, sequential ? ctx->rng.p : nullptr, ctx->radiance.p};
     Queues q{ctx->q_a.p, ctx->q_b.p, ctx->sh_path.p, ctx->sh_o.p, ctx->sh_d.p, ctx->sh_c.p, ctx->mi_path.p,
-             ctx->mi_o.p, ctx->mi_d.p, ctx->mi_f.p, ctx->mi_ndw.p, ctx->dl_path.p, ctx->counters.p, ctx->totals.p};
-    FilmAcc f{};
+             ctx->mi_o.p, ctx->mi_d.p, ctx->mi_f.p, ctx->mi_ndw.p, ctx->dl_path.p, ctx->counts.p, ctx->totals.p};
+    FilmParams f{};
     f.out_film = ctx->out_film.p;
     f.out_var = ctx->out_var.p;
     f.spill = ctx->spill.p;
     f.tile_first = ctx->tile_first.p;
-    f.sx0 = sx0; f.sy0 = sy0; f.sx1 = sx1; f.sy1 = sy1; f.tgx = tgx;
-    f.rx0 = v.region_x0; f.ry0 = v.region_y0; f.rx1 = v.region_x1; f.ry1 = v.region_y1;
     f.side_rad = ctx->side_rad.p;
     f.side_info = ctx->side_info.p;
-    f.side_capacity = 1 << 16;
+    f.side_view = ctx->side_view.p;
+    f.side_capacity = (int)side_cap;
     LaunchCfg lc = ctx->launch_cfg();
+    lc.trav_grid = ctx->cu_count * traversal_blocks_per_cu(lc);
+    lc.path_grid = (int)std::max<int64_t>(1, std::min<int64_t>((max_chunk_paths + 255) / 256, (int64_t)ctx->cu_count * 8));
+    const int iters = loop_iterations(rc.max_light_bounces);
+    const bool use_graph = ctx->opt.use_graph != 0;
+    Chunk* slot = ctx->chunk_slot.p;
+
+    // polling slots: one per iteration (eager) or per (chunk, pass) completion check (graph)
+    const size_t poll_n = std::max<size_t>(hc.size() + 1, (size_t)iters + 1);
+    if (ctx->h_poll_n < poll_n) {
+        if (ctx->h_poll) (void)hipHostFree(ctx->h_poll);
+        ctx->h_poll = nullptr;
+        ctx->h_poll_n = 0;
+        HIPX(hipHostMalloc((void**)&ctx->h_poll, poll_n * sizeof(uint32_t), hipHostMallocDefault));
+        ctx->h_poll_n = poll_n;
+    }
+    uint32_t* hp = ctx->h_poll;
+
+    // the captured loop bakes in buffer addresses and sizes: capture again when any changed
+    const bool graph_stale = ctx->graph_paths != max_chunk_paths || ctx->graph_iters != iters ||
+                             std::memcmp(&ctx->graph_P, &P, sizeof(P)) != 0 || std::memcmp(&ctx->graph_q, &q, sizeof(q)) != 0;
+    if (use_graph && (graph_stale || !ctx->graph)) {
+        ctx->drop_graph();
+        hipGraph_t g;
+        HIPX(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        launch_raygen(slot, max_chunk_paths, P, q, s);
+        for (int it = 0; it < iters; it++) launch_iteration(ctx->scene, slot, P, q, it & 1, lc);
+        HIPX(hipStreamEndCapture(s, &g));
+        hipError_t e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) { ctx->graph = nullptr; throw HipError{std::string("hipGraphInstantiate: ") + hipGetErrorString(e)}; }
+        ctx->graph_paths = max_chunk_paths;
+        ctx->graph_iters = iters;
+        ctx->graph_P = P;
+        ctx->graph_q = q;
+    }
 
     KernelTimer kt;
-    kt.on = ctx->opt.enable_timing != 0;
+    kt.on = ctx->opt.enable_timing != 0 && !use_graph;
     kt.s = s;
+    std::vector<hipEvent_t> ev((size_t)iters + 1);
+    for (auto& e : ev) HIPX(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     hipEvent_t e0, e1;
     HIPX(hipEventCreate(&e0));
     HIPX(hipEventCreate(&e1));
     HIPX(hipEventRecord(e0, s));
-    int64_t closest = 0, anyr = 0, ext_rays = 0, ext_launches = 0;
-    const int passes = spp / s_par;
-    for (size_t ci = 0; ci + 1 < chunk_start.size(); ci++) {
-        const int64_t base = chunk_start[ci];
-        c.pixels = ctx->pixels.p + base;
-        c.n_pixels = (int)(chunk_start[ci + 1] - base);
-        f.base = (int)base;
-        c.pass = 0;
-        kt.begin(4);
-        launch_sample_table(c, ctx->table.p, s);
-        kt.end();
-        for (int pass = 0; pass < passes; pass++) {
-            c.pass = pass;
-            kt.begin(4);
-            launch_raygen(c, P, q, s);
-            kt.end();
-            int count = c.n_pixels * s_par;
-            Queues qq = q;
-            int iterations = 0;
-            while (count > 0) {
-                if (++iterations > 2 * std::max(1, rc.max_light_bounces) + 4) throw HipError{"bounce loop did not terminate"};
-                kt.begin(0);
-                launch_extend(ctx->scene, P, qq, count, lc);
-                kt.end();
-                ext_rays += count;
-                ext_launches++;
-                HIPX(hipMemsetAsync(qq.counters, 0, 4 * sizeof(uint32_t), s));
-                kt.begin(1);
-                launch_shade(ctx->scene, c, P, qq, count, s);
-                kt.end();
-                HIPX(hipMemcpyAsync(ctx->h_counters, qq.counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-                HIPX(hipStreamSynchronize(s));
-                const uint32_t n_active = ctx->h_counters[0], n_sh = ctx->h_counters[1], n_mis = ctx->h_counters[2],
-                               n_dl = ctx->h_counters[3];
-                closest += count + n_mis;
-                anyr += n_sh;
-                kt.begin(2);
-                launch_shadow(ctx->scene, P, qq, (int)n_sh, lc);
-                kt.end();
-                kt.begin(3);
-                launch_mis(ctx->scene, P, qq, (int)n_mis, lc);
-                kt.end();
+    size_t hci = 0;
+    bool loop_ok = true;
+    for (size_t ci = 0; ci < cks.size(); ci++) {
+        for (int pass = 0; pass < passes_of[ci]; pass++, hci++) {
+            HIPX(hipMemcpyAsync(slot, ctx->chunks.p + hci, sizeof(Chunk), hipMemcpyDeviceToDevice, s));
+            if (pass == 0) {
                 kt.begin(4);
-                launch_resolve(ctx->scene, c, P, qq, (int)n_dl, s);
+                launch_sample_table(hc[hci], slot, ctx->table.p, s);
                 kt.end();
-                std::swap(qq.active_in, qq.active_out);
-                count = (int)n_active;
+            }
+            if (use_graph) {
+                HIPX(hipGraphLaunch(ctx->graph, s));
+                // the active list the last iteration wrote must be empty
+                HIPX(hipMemcpyAsync(hp + hci, ctx->counts.p + (iters & 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                continue;
+            }
+            kt.begin(4);
+            launch_raygen(slot, hc[hci].n_paths, P, q, s);
+            kt.end();
+            // Queue kernels take their counts from device memory; the host reads each iteration's
+            // active count asynchronously and stops enqueueing two iterations after it saw zero.
+            int it = 0;
+            for (; it < iters; it++) {
+                if (it >= 2) {
+                    HIPX(hipEventSynchronize(ev[it - 2]));
+                    if (hp[it - 2] == 0) break;
+                }
+                const int parity = it & 1;
+                if (kt.on) {
+                    static const int kind_of_stage[kStageCount] = {4, 0, 1, 2, 3, 4};
+                    for (int stage = 0; stage < kStageCount; stage++) {
+                        kt.begin(kind_of_stage[stage]);
+                        launch_stage(stage, ctx->scene, slot, P, q, parity, lc);
+                        kt.end();
+                    }
+                } else {
+                    launch_iteration(ctx->scene, slot, P, q, parity, lc);
+                }
+                HIPX(hipMemcpyAsync(hp + it, ctx->counts.p + (parity ^ 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                HIPX(hipEventRecord(ev[it], s));
+            }
+            if (it == iters) {
+                HIPX(hipEventSynchronize(ev[it - 1]));
+                if (hp[it - 1] != 0) loop_ok = false;
             }
         }
+        HIPX(hipGetLastError());
+        f.base = cks[ci].start;
         kt.begin(4);
-        launch_film(c, P, f, q, s);
+        launch_film(slot, (int)cks[ci].pixels, P, f, q, s);
         kt.end();
     }
     HIPX(hipEventRecord(e1, s));
@@ -596,73 +734,97 @@ static void render_tiles_impl(yar_gpu_ctx* ctx, const yar_view& v, const std::ve
         HIPX(hipMemcpyAsync(film_own.data(), ctx->out_film.p, total_pixels * sizeof(float4), hipMemcpyDeviceToHost, s));
         HIPX(hipMemcpyAsync(var_own.data(), ctx->out_var.p, total_pixels * sizeof(double), hipMemcpyDeviceToHost, s));
     }
-    unsigned long long tot[16];
+    unsigned long long tot[24];
+    uint32_t cnt[16];
     HIPX(hipMemcpyAsync(tot, ctx->totals.p, sizeof(tot), hipMemcpyDeviceToHost, s));
-    HIPX(hipMemcpyAsync(ctx->h_counters, ctx->counters.p, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPX(hipMemcpyAsync(cnt, ctx->counts.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
+    for (auto e : ev) (void)hipEventDestroy(e);
+    if (use_graph)
+        for (size_t i = 0; i < hc.size(); i++)
+            if (hp[i] != 0) loop_ok = false;
+    if (!loop_ok) throw HipError{"bounce loop did not terminate within max_light_bounces + 2 iterations"};
     double ksec[5];
     kt.collect(ksec);
-    uint32_t n_side = ctx->h_counters[4];
-    if (n_side > (1u << 16)) throw HipError{"jitter-0 side list overflow"};
+    const uint32_t n_side = cnt[5];
+    if ((int64_t)n_side > side_cap) throw HipError{"film side list overflow"};
     std::vector<float4> side_rad(n_side);
     std::vector<int4> side_info(n_side);
+    std::vector<int> side_view(n_side);
     if (n_side) {
         HIPX(hipMemcpy(side_rad.data(), ctx->side_rad.p, n_side * sizeof(float4), hipMemcpyDeviceToHost));
         HIPX(hipMemcpy(side_info.data(), ctx->side_info.p, n_side * sizeof(int4), hipMemcpyDeviceToHost));
+        HIPX(hipMemcpy(side_view.data(), ctx->side_view.p, n_side * sizeof(int), hipMemcpyDeviceToHost));
     }
     float gpu_ms = 0.f;
     HIPX(hipEventElapsedTime(&gpu_ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
 
-    // Assemble Film_Tiles: in-tile pixel values come from the device gather; the one-pixel margins
-    // receive the side-list samples of the tile in visitation order (row-major sources, sample index).
-    out.tiles.assign(tile_list.size(), {});
-    out.tile_var.assign(tile_list.size(), 0.0);
-    std::vector<size_t> side_order(n_side);
-    for (size_t i = 0; i < n_side; i++) side_order[i] = i;
-    std::sort(side_order.begin(), side_order.end(), [&](size_t a, size_t b) {
-        const int4 &A = side_info[a], &B = side_info[b];
-        if (A.y != B.y) return A.y < B.y;
-        if (A.x != B.x) return A.x < B.x;
-        return A.z < B.z;
-    });
-    for (size_t ti = 0; ti < tile_list.size(); ti++) {
-        const TileGeom& g = geoms[ti];
-        int pw = g.px1 - g.px0, ph = g.py1 - g.py0;
-        std::vector<yar_film_pixel>& tp = out.tiles[ti];
-        tp.assign((size_t)std::max(0, pw) * std::max(0, ph), yar_film_pixel{{0, 0, 0}, 0});
-        double tv = 0.0;
-        int64_t off = tile_start[ti];
-        for (int y = g.y0; y < g.y1; y++)
-            for (int x = g.x0; x < g.x1; x++, off++) {
-                const float4 a = film_own[off];
-                yar_film_pixel& fp = tp[(size_t)(y - g.py0) * pw + (x - g.px0)];
-                fp.color_sum[0] = a.x; fp.color_sum[1] = a.y; fp.color_sum[2] = a.z; fp.weight_sum = a.w;
-                tv += var_own[off];
-            }
-        out.tile_var[ti] = tv;
-        for (size_t si : side_order) {
-            const int4 inf = side_info[si];
-            if (inf.x < g.x0 || inf.x >= g.x1 || inf.y < g.y0 || inf.y >= g.y1) continue;
-            const float4 r = side_rad[si];
-            const int fx0 = inf.x + (inf.w & 3) - 1, fx1 = inf.x + ((inf.w >> 2) & 3);
-            const int fy0 = inf.y + ((inf.w >> 4) & 3) - 1, fy1 = inf.y + ((inf.w >> 6) & 3);
-            for (int ty = fy0; ty < fy1; ty++)
-                for (int tx = fx0; tx < fx1; tx++) {
-                    if (tx < g.px0 || tx >= g.px1 || ty < g.py0 || ty >= g.py1) continue;
-                    if (tx >= g.x0 && tx < g.x1 && ty >= g.y0 && ty < g.y1) continue;   // in-tile: device gather
-                    yar_film_pixel& fp = tp[(size_t)(ty - g.py0) * pw + (tx - g.px0)];
-                    fp.color_sum[0] += 1.f * r.x; fp.color_sum[1] += 1.f * r.y; fp.color_sum[2] += 1.f * r.z;
-                    fp.weight_sum += 1.f;
+    // Side entries by the tile of their source pixel, in visitation order (row-major sources, then
+    // sample index), so each tile's one-pixel margin receives them as render_tile adds them.
+    std::vector<std::vector<size_t>> side_of_tile(tile_first.size());
+    for (size_t i = 0; i < n_side; i++) {
+        const DevView& v = dv[side_view[i]];
+        const int tx = (side_info[i].x - v.sx0) >> 6, ty = (side_info[i].y - v.sy0) >> 6;
+        side_of_tile[(size_t)v.tile_base + (size_t)ty * v.tgx + tx].push_back(i);
+    }
+    for (auto& l : side_of_tile)
+        std::sort(l.begin(), l.end(), [&](size_t a, size_t b) {
+            const int4 &A = side_info[a], &B = side_info[b];
+            if (A.y != B.y) return A.y < B.y;
+            if (A.x != B.x) return A.x < B.x;
+            return A.z < B.z;
+        });
+
+    // Assemble Film_Tiles: in-tile values from the device gather, margins from the side list.
+    out.assign(parts.size(), {});
+    for (size_t pi = 0; pi < parts.size(); pi++) {
+        out[pi].tiles.assign(parts[pi].tiles.size(), {});
+        out[pi].tile_var.assign(parts[pi].tiles.size(), 0.0);
+    }
+    {
+        std::vector<size_t> next(parts.size(), 0);
+        for (const Unit& u : units) {
+            const yar_view& v = ctx->views[parts[u.part].view];
+            const TileGeom g = tile_geom(v, u.tile);
+            const size_t ti = next[u.part]++;
+            const int pw = g.px1 - g.px0, ph = g.py1 - g.py0;
+            std::vector<yar_film_pixel>& tp = out[u.part].tiles[ti];
+            tp.assign((size_t)std::max(0, pw) * std::max(0, ph), yar_film_pixel{{0, 0, 0}, 0});
+            double tv = 0.0;
+            int64_t off = u.start;
+            for (int y = g.y0; y < g.y1; y++)
+                for (int x = g.x0; x < g.x1; x++, off++) {
+                    const float4 a = film_own[off];
+                    yar_film_pixel& fp = tp[(size_t)(y - g.py0) * pw + (x - g.px0)];
+                    fp.color_sum[0] = a.x; fp.color_sum[1] = a.y; fp.color_sum[2] = a.z; fp.weight_sum = a.w;
+                    tv += var_own[off];
                 }
+            out[u.part].tile_var[ti] = tv;
+            for (size_t si : side_of_tile[(size_t)dv[u.part].tile_base + u.tile]) {
+                const int4 inf = side_info[si];
+                const float4 r = side_rad[si];
+                const int fx0 = inf.x + (inf.w & 3) - 1, fx1 = inf.x + ((inf.w >> 2) & 3);
+                const int fy0 = inf.y + ((inf.w >> 4) & 3) - 1, fy1 = inf.y + ((inf.w >> 6) & 3);
+                for (int ty = fy0; ty < fy1; ty++)
+                    for (int tx = fx0; tx < fx1; tx++) {
+                        if (tx < g.px0 || tx >= g.px1 || ty < g.py0 || ty >= g.py1) continue;
+                        if (tx >= g.x0 && tx < g.x1 && ty >= g.y0 && ty < g.y1) continue;   // in-tile: device gather
+                        yar_film_pixel& fp = tp[(size_t)(ty - g.py0) * pw + (tx - g.px0)];
+                        fp.color_sum[0] += 1.f * r.x; fp.color_sum[1] += 1.f * r.y; fp.color_sum[2] += 1.f * r.z;
+                        fp.weight_sum += 1.f;
+                    }
+            }
         }
     }
-    yar_render_stats& st = out.stats;
+    int64_t samples = 0;
+    for (const Unit& u : units) samples += u.count * dv[u.part].spp;
+    st = yar_render_stats{};
     st.gpu_seconds = gpu_ms * 1e-3;
-    st.samples = total_pixels * spp;
-    st.closest_rays = closest;
-    st.any_rays = anyr;
+    st.samples = samples;
+    st.closest_rays = (int64_t)(tot[16] + tot[18]);
+    st.any_rays = (int64_t)tot[17];
     st.node_visits = (int64_t)(tot[0] + tot[4] + tot[8]);
     st.leaf_refs = (int64_t)(tot[1] + tot[5] + tot[9]);
     st.triangle_tests = (int64_t)(tot[2] + tot[6] + tot[10]);
@@ -671,14 +833,57 @@ static void render_tiles_impl(yar_gpu_ctx* ctx, const yar_view& v, const std::ve
     st.extend_leaf_refs = (int64_t)tot[1];
     st.extend_triangle_tests = (int64_t)tot[2];
     st.extend_object_visits = (int64_t)tot[3];
-    st.extend_rays = ext_rays;
-    st.extend_launches = ext_launches;
+    st.extend_rays = (int64_t)tot[16];
+    st.extend_launches = (int64_t)tot[19];
     st.extend_seconds = ksec[0];
     st.shade_seconds = ksec[1];
     st.shadow_seconds = ksec[2];
     st.mis_seconds = ksec[3];
     st.other_seconds = ksec[4];
     st.render_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+}
+
+// Film::merge_tile of every tile in index order, then Film::get_image (film.cpp:88-118), and the
+// variance estimate of render_scene (reference_renderer.cpp:566-576).
+static void merge_view(const yar_view& v, const PartResult& r, yar_film_pixel* out_film, float* out_image,
+                       double* out_variance) {
+    const int n = (int)r.tiles.size();
+    const int rw = v.region_x1 - v.region_x0, rh = v.region_y1 - v.region_y0;
+    std::vector<yar_film_pixel> film((size_t)rw * rh, yar_film_pixel{{0, 0, 0}, 0});
+    for (int t = 0; t < n; t++) {
+        const TileGeom g = tile_geom(v, t);
+        const int pw = g.px1 - g.px0;
+        for (int y = g.py0; y < g.py1; y++)
+            for (int x = g.px0; x < g.px1; x++) {
+                yar_film_pixel& fo = film[(size_t)(y - v.region_y0) * rw + (x - v.region_x0)];
+                const yar_film_pixel& p = r.tiles[t][(size_t)(y - g.py0) * pw + (x - g.px0)];
+                fo.color_sum[0] += p.color_sum[0]; fo.color_sum[1] += p.color_sum[1]; fo.color_sum[2] += p.color_sum[2];
+                fo.weight_sum += p.weight_sum;
+            }
+    }
+    if (out_film) std::memcpy(out_film, film.data(), film.size() * sizeof(yar_film_pixel));
+    if (out_image) {
+        for (size_t i = 0; i < film.size(); i++) {
+            const yar_film_pixel& fp = film[i];
+            float cr = 0, cg = 0, cb = 0;
+            if (fp.weight_sum != 0.f) {
+                const float inv = 1.f / fp.weight_sum;
+                cr = fp.color_sum[0] * inv; cg = fp.color_sum[1] * inv; cb = fp.color_sum[2] * inv;
+            }
+            out_image[3 * i] = std::max(0.f, cr); out_image[3 * i + 1] = std::max(0.f, cg); out_image[3 * i + 2] = std::max(0.f, cb);
+        }
+    }
+    if (out_variance) {
+        const int spp = v.x_pixel_sample_count * v.y_pixel_sample_count;
+        double acc = 0.0;
+        int64_t cnt = 0;
+        for (int t = 0; t < n; t++) {
+            acc += r.tile_var[t];
+            const TileGeom g = tile_geom(v, t);
+            cnt += (int64_t)(g.x1 - g.x0) * (g.y1 - g.y0);
+        }
+        *out_variance = spp > 1 ? acc / cnt : 0.0;
+    }
 }
 
 static bool valid_view(const yar_view& v, std::string* why) {
@@ -726,6 +931,7 @@ int yar_gpu_create(const yar_scene_desc* scene, const yar_gpu_options* options, 
         hipDeviceProp_t prop;
         HIPX(hipGetDeviceProperties(&prop, ctx->device));
         std::snprintf(ctx->info.device_name, sizeof(ctx->info.device_name), "%s", prop.gcnArchName);
+        ctx->cu_count = std::max(1, prop.multiProcessorCount);
         HIPX(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         upload_scene(ctx, scene);
     } catch (const HipError& e) {
@@ -795,22 +1001,30 @@ int yar_gpu_render_tiles(yar_gpu_ctx* ctx, int32_t view, const int32_t* tile_ind
     int32_t n = 0;
     if (!ctx || !tile_indices || count <= 0 || !out_tiles || yar_film_tile_count(ctx, view, &n) != YAR_OK)
         return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_tiles: invalid argument");
-    std::vector<int> tl(tile_indices, tile_indices + count);
-    for (int t : tl) if (t < 0 || t >= n) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_tiles: tile index out of range");
-    RenderResult rr;
+    std::vector<JobPart> parts(1);
+    parts[0].view = view;
+    parts[0].tiles.assign(tile_indices, tile_indices + count);
+    std::vector<char> seen(n, 0);
+    for (int t : parts[0].tiles) {
+        if (t < 0 || t >= n) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_tiles: tile index out of range");
+        if (seen[t]) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_tiles: tile listed twice");
+        seen[t] = 1;
+    }
+    std::vector<PartResult> rr;
+    yar_render_stats st{};
     try {
         HIPX(hipSetDevice(ctx->device));
-        render_tiles_impl(ctx, ctx->views[view], tl, rr);
+        render_job(ctx, parts, rr, st);
     } catch (const HipError& e) {
         return fail(ctx, YAR_ERR_HIP, "yar_gpu_render_tiles: " + e.msg);
     } catch (const std::bad_alloc&) {
         return fail(ctx, YAR_ERR_OUT_OF_MEMORY, "yar_gpu_render_tiles: out of host memory");
     }
     for (int i = 0; i < count; i++) {
-        if (out_tiles[i]) std::memcpy(out_tiles[i], rr.tiles[i].data(), rr.tiles[i].size() * sizeof(yar_film_pixel));
-        if (out_tile_variance) out_tile_variance[i] = rr.tile_var[i];
+        if (out_tiles[i]) std::memcpy(out_tiles[i], rr[0].tiles[i].data(), rr[0].tiles[i].size() * sizeof(yar_film_pixel));
+        if (out_tile_variance) out_tile_variance[i] = rr[0].tile_var[i];
     }
-    if (out_stats) *out_stats = rr.stats;
+    if (out_stats) *out_stats = st;
     return YAR_OK;
 }
 
@@ -818,82 +1032,49 @@ int yar_gpu_render(yar_gpu_ctx* ctx, int32_t view, yar_film_pixel* out_film, flo
                    yar_render_stats* out_stats) {
     int32_t n = 0;
     if (!ctx || yar_film_tile_count(ctx, view, &n) != YAR_OK) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render: invalid view");
-    const yar_view& v = ctx->views[view];
-    std::vector<int> tl(n);
-    for (int i = 0; i < n; i++) tl[i] = i;
-    RenderResult rr;
+    std::vector<JobPart> parts(1);
+    parts[0].view = view;
+    for (int i = 0; i < n; i++) parts[0].tiles.push_back(i);
+    std::vector<PartResult> rr;
+    yar_render_stats st{};
     try {
         HIPX(hipSetDevice(ctx->device));
-        render_tiles_impl(ctx, v, tl, rr);
+        render_job(ctx, parts, rr, st);
+        merge_view(ctx->views[view], rr[0], out_film, out_image, out_variance);
     } catch (const HipError& e) {
         return fail(ctx, YAR_ERR_HIP, "yar_gpu_render: " + e.msg);
     } catch (const std::bad_alloc&) {
         return fail(ctx, YAR_ERR_OUT_OF_MEMORY, "yar_gpu_render: out of host memory");
     }
-    // Film::merge_tile in tile-index order, then Film::get_image (film.cpp:88-118)
-    const int rw = v.region_x1 - v.region_x0, rh = v.region_y1 - v.region_y0;
-    std::vector<yar_film_pixel> film((size_t)rw * rh, yar_film_pixel{{0, 0, 0}, 0});
-    for (int t = 0; t < n; t++) {
-        TileGeom g = tile_geom(v, t);
-        int pw = g.px1 - g.px0;
-        for (int y = g.py0; y < g.py1; y++)
-            for (int x = g.px0; x < g.px1; x++) {
-                yar_film_pixel& f = film[(size_t)(y - v.region_y0) * rw + (x - v.region_x0)];
-                const yar_film_pixel& p = rr.tiles[t][(size_t)(y - g.py0) * pw + (x - g.px0)];
-                f.color_sum[0] += p.color_sum[0]; f.color_sum[1] += p.color_sum[1]; f.color_sum[2] += p.color_sum[2];
-                f.weight_sum += p.weight_sum;
-            }
-    }
-    if (out_film) std::memcpy(out_film, film.data(), film.size() * sizeof(yar_film_pixel));
-    if (out_image) {
-        for (size_t i = 0; i < film.size(); i++) {
-            const yar_film_pixel& f = film[i];
-            float r = 0, g = 0, b = 0;
-            if (f.weight_sum != 0.f) {
-                float inv = 1.f / f.weight_sum;
-                r = f.color_sum[0] * inv; g = f.color_sum[1] * inv; b = f.color_sum[2] * inv;
-            }
-            out_image[3 * i] = std::max(0.f, r); out_image[3 * i + 1] = std::max(0.f, g); out_image[3 * i + 2] = std::max(0.f, b);
-        }
-    }
-    if (out_variance) {
-        const int spp = v.x_pixel_sample_count * v.y_pixel_sample_count;
-        double acc = 0.0;
-        int64_t cnt = 0;
-        for (int t = 0; t < n; t++) {
-            acc += rr.tile_var[t];
-            TileGeom g = tile_geom(v, t);
-            cnt += (int64_t)(g.x1 - g.x0) * (g.y1 - g.y0);
-        }
-        *out_variance = spp > 1 ? acc / cnt : 0.0;
-    }
-    if (out_stats) {
-        *out_stats = rr.stats;
-    }
+    if (out_stats) *out_stats = st;
     return YAR_OK;
 }
 
 int yar_gpu_render_batch(yar_gpu_ctx* ctx, const int32_t* views, int32_t count, float* const* out_images,
                          double* out_variances, yar_render_stats* out_stats) {
     if (!ctx || !views || count <= 0) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_batch: invalid argument");
-    yar_render_stats total{};
-    auto t0 = std::chrono::steady_clock::now();
+    std::vector<JobPart> parts(count);
     for (int i = 0; i < count; i++) {
-        yar_render_stats st{};
-        int rc = yar_gpu_render(ctx, views[i], nullptr, out_images ? out_images[i] : nullptr,
-                                out_variances ? &out_variances[i] : nullptr, &st);
-        if (rc != YAR_OK) return rc;
-        total.gpu_seconds += st.gpu_seconds;
-        total.samples += st.samples;
-        total.closest_rays += st.closest_rays;
-        total.any_rays += st.any_rays;
-        total.node_visits += st.node_visits;
-        total.leaf_refs += st.leaf_refs;
-        total.triangle_tests += st.triangle_tests;
-        total.object_visits += st.object_visits;
+        int32_t n = 0;
+        if (yar_film_tile_count(ctx, views[i], &n) != YAR_OK)
+            return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_batch: invalid view index");
+        parts[i].view = views[i];
+        for (int t = 0; t < n; t++) parts[i].tiles.push_back(t);
     }
-    total.render_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (out_stats) *out_stats = total;
+    std::vector<PartResult> rr;
+    yar_render_stats st{};
+    try {
+        HIPX(hipSetDevice(ctx->device));
+        render_job(ctx, parts, rr, st);
+        for (int i = 0; i < count; i++)
+            merge_view(ctx->views[views[i]], rr[i], nullptr, out_images ? out_images[i] : nullptr,
+                       out_variances ? &out_variances[i] : nullptr);
+    } catch (const HipError& e) {
+        return fail(ctx, YAR_ERR_HIP, "yar_gpu_render_batch: " + e.msg);
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, YAR_ERR_OUT_OF_MEMORY, "yar_gpu_render_batch: out of host memory");
+    }
+    if (out_stats) *out_stats = st;
     return YAR_OK;
 }
 

@@ -182,12 +182,22 @@ struct LdsStack {
     YG_INLINE void get(int slot, uint32_t& n, float& t) const { n = node[slot * stride]; t = tmax[slot * stride]; }
 };
 
+// The same stack in per-lane private memory (scratch): no LDS footprint, so occupancy is bounded by
+// registers only; accesses are lane-interleaved by the hardware and stay in the vector L1.
+template <int N>
+struct PrivStack {
+    uint32_t node[N];
+    float tmax[N];
+    YG_INLINE void put(int slot, uint32_t n, float t) { node[slot] = n; tmax[slot] = t; }
+    YG_INLINE void get(int slot, uint32_t& n, float& t) const { n = node[slot]; t = tmax[slot]; }
+};
+
 // One kd-tree level. `root` is the global index of the tree's root node, [t0,t1] the slab range.
 // loop_t() returns the current loop bound (closest hit t, or the fixed any-hit tmax).
 // Leaf(ids_node, first_id, count) processes a leaf and returns true to stop the whole query.
-template <bool COUNT, class LoopT, class Leaf>
+template <bool COUNT, class Stack, class LoopT, class Leaf>
 YG_INLINE bool kd_level(const uint2* __restrict__ nodes, uint32_t root, V3 o, V3 d, V3 inv, float t0, float t1,
-                        LdsStack st, int depth, LoopT loop_t, Leaf leaf, TravCounters& cnt) {
+                        Stack& st, int depth, LoopT loop_t, Leaf leaf, TravCounters& cnt) {
     const int mask = depth - 1;   // depth is a power of two
     float tmin = t0, tmax = t1;
     uint32_t node = root;
@@ -260,9 +270,9 @@ YG_INLINE uint32_t leaf_id(const uint2* __restrict__ nodes, uint32_t nd, uint32_
 // Two-level closest-hit / any-hit query over the scene tree and the per-mesh trees
 // (KdTree::intersect / intersect_any with intersect_scene_geometry_data and
 // intersect_triangle_mesh_geometry_data, kdtree.cpp:12-109).
-template <bool ANY, bool COUNT>
-YG_INLINE bool trace(const DevScene& s, V3 o, V3 d, float tmax_in, HitRec& hit, LdsStack sst, int sdepth,
-                     LdsStack mst, int mdepth, TravCounters& cnt) {
+template <bool ANY, bool COUNT, class SStack, class MStack>
+YG_INLINE bool trace(const DevScene& s, V3 o, V3 d, float tmax_in, HitRec& hit, SStack& sst, int sdepth,
+                     MStack& mst, int mdepth, TravCounters& cnt) {
     hit.t = tmax_in; hit.obj = -1; hit.tri = 0; hit.b = v3(0, 0, 0);
     float st0, st1;
     if (!slab(s.scene_lo, s.scene_hi, o, d, st0, st1)) return false;

@@ -6,6 +6,19 @@
 
 namespace yg {
 
+// One camera view of a render (the Scene_Overrides analog, reference_renderer.h:23): camera,
+// sampler grid, seed offset and film layout (Film::Film / get_tile_bounds, film.cpp:52-86).
+struct DevView {
+    float4 c0, c1, c2;              // camera_to_world rows
+    float W, H, horz, vert;         // film extent and tan(fov/2) terms (camera.cpp:5-14)
+    int zup, spp, nx, ny;
+    uint32_t seed_offset;
+    int tile_base;                  // offset of this view's tiles in the tile_first array
+    int sx0, sy0;                   // sample region (== render region for the radius-0.5 box filter)
+    int sx1, sy1, tgx, pad;
+    int rx0, ry0, rx1, ry1;         // render region
+};
+
 // Per-path state, structure of arrays indexed by path slot.
 struct Paths {
     float4* ray_o;       // origin xyz
@@ -18,8 +31,9 @@ struct Paths {
     float4* dl_bsdf;     // BSDF-sampling half
     float4* hit_tb;      // closest hit: t, barycentrics
     int2* hit_ot;        // closest hit: object, global triangle
+    uint2* pk;           // chunk pixel and sample index of the path
     uint64_t* rng;       // per-pixel fallback stream (sequential sample mode only)
-    float4* radiance;    // final per-sample radiance of the chunk, [pixel][sample]
+    float4* radiance;    // final per-sample radiance of the chunk, at pix_first[pixel] + sample
 };
 
 enum : uint32_t {
@@ -28,67 +42,69 @@ enum : uint32_t {
     kDone = 1u << 26,         // path finished; finalize after the resolve pass
 };
 
-// Work queues of one bounce.
+// Work queues.  The active path list ping-pongs between two buffers; `parity` selects the input.
+// counts: [0], [1] active lists, [2] shadow, [3] mis, [4] dl, [5] film side entries,
+//         [6] [7] [8] work-fetch cursors of extend, shadow and mis.
 struct Queues {
-    uint32_t* active_in;
-    uint32_t* active_out;
+    uint32_t* active0;
+    uint32_t* active1;
     uint32_t* shadow_path; float4* shadow_o; float4* shadow_d; float4* shadow_c;
     uint32_t* mis_path; float4* mis_o; float4* mis_d; float4* mis_f; float* mis_ndw;
     uint32_t* dl_path;
-    uint32_t* counters;       // [0] active_out, [1] shadow, [2] mis, [3] dl, [4] side entries, [5] active_in
-    unsigned long long* totals;   // 16 traversal counters: 4 per kernel kind (extend, shadow, mis, trace_rays)
+    uint32_t* counts;
+    // [0..15] traversal counters (4 per kernel kind), [16] extend rays, [17] shadow rays, [18] mis rays,
+    // [19] non-empty extend launches
+    unsigned long long* totals;
 };
 
-// Camera, sampler and integrator parameters of one chunk of pixels.
+// One chunk of pixels (whole film tiles, possibly of several views).  Lives in device memory so
+// the bounce loop can be captured once in a hipGraph and replayed for every chunk and pass.
 struct Chunk {
-    const int2* pixels;       // film coordinates of the chunk's sample pixels
-    int n_pixels;
-    int spp, s_par, pass;     // samples per pixel; samples in flight per pixel; pass index
-    int nx, ny, d1, d2;       // stratification grid and sample-vector dimensions
-    const float* table;       // per-pixel sample tables, `stride` floats each
-    long long stride;
-    uint32_t seed_offset;
-    M34 c2w;
-    float W, H, horz, vert;
-    int zup;
+    const int2* pixels;             // film coordinates of the chunk's pixels
+    const int* pix_view;            // view of each pixel
+    const long long* pix_first;     // prefix sums of spp over the chunk's pixels (n_pixels + 1)
+    const DevView* views;
+    const float* table;             // sample tables, pixel p at pix_first[p] * dps floats
+    long long n_paths;              // paths of the current pass
+    int n_pixels, max_spp;
+    int d1, d2, dps;                // sample-vector dims; floats per sample = 2 + d1 + 2*d2
+    int pass, sequential;           // sequential: one sample per pixel per pass (RNG fallback)
     int max_bounces, rr_bounce_threshold;
     float rr_threshold, film_scale, max_component;
 };
 
-// Film accumulation of one chunk (whole 64x64 tiles).  Every sample's pixel footprint is computed
-// exactly as Film_Tile::add_sample does (film.cpp:27-50); because film_pos = pixel + jitter is a float
-// sum, a sample can reach the 3x3 neighbourhood of its pixel.  Contributions inside the source's tile
-// are gathered on the device in the tile's visitation order; contributions to a tile's one-pixel
-// margin go to a side list that the host adds to the Film_Tile.
-struct FilmAcc {
+// Film accumulation of one chunk (every sample's footprint as Film_Tile::add_sample computes it).
+struct FilmParams {
     float4* out_film;         // per listed pixel: its Film_Tile value inside its own tile
     double* out_var;          // per listed pixel: luminance variance of its estimate
     uint8_t* spill;           // per chunk pixel: 8-neighbour footprint bits of its samples
-    int base;                 // index of the chunk's first pixel in the pixel list
-    const int* tile_first;    // per film tile: pixel-list index of its first pixel, -1 if not rendered
-    int sx0, sy0, sx1, sy1, tgx;   // sample region and tile grid (Film::Film, film.cpp:52-71)
-    int rx0, ry0, rx1, ry1;        // render region (clamps tile pixel bounds)
-    float4* side_rad;         // margin contributions: radiance
+    const int* tile_first;    // per (view, tile): pixel-list index of its first pixel, -1 if absent
+    long long base;           // pixel-list index of the chunk's first pixel
+    float4* side_rad;         // samples reaching a tile's one-pixel margin: radiance
     int4* side_info;          // x, y, sample, packed footprint (p0x-x+1 | (p1x-x)<<2 | (p0y-y+1)<<4 | (p1y-y)<<6)
+    int* side_view;           // view of the side entry
     int side_capacity;
 };
 
 struct LaunchCfg {
     int block;                // threads per block for traversal kernels
     int scene_stack, mesh_stack;
+    bool priv_stack;          // stacks in private (scratch) memory instead of LDS
     bool counters;
+    int trav_grid;            // persistent traversal grid (blocks)
+    int path_grid;            // grid-stride grid of per-path kernels (blocks of 256)
     hipStream_t stream;
 };
 
-void launch_sample_table(const Chunk& c, float* table, hipStream_t s);
-void launch_raygen(const Chunk& c, const Paths& p, Queues q, hipStream_t s);
-void launch_extend(const DevScene& sc, const Paths& p, Queues q, int count, const LaunchCfg& cfg);
-void launch_shade(const DevScene& sc, const Chunk& c, const Paths& p, Queues q, int count, hipStream_t s);
-void launch_shadow(const DevScene& sc, const Paths& p, Queues q, int count, const LaunchCfg& cfg);
-void launch_mis(const DevScene& sc, const Paths& p, Queues q, int count, const LaunchCfg& cfg);
-void launch_resolve(const DevScene& sc, const Chunk& c, const Paths& p, Queues q, int count, hipStream_t s);
-void launch_film(const Chunk& c, const Paths& p, FilmAcc f, Queues q, hipStream_t s);
+void launch_sample_table(const Chunk& host_chunk, const Chunk* dev_chunk, float* table, hipStream_t s);
+void launch_raygen(const Chunk* c, long long n_paths_max, const Paths& p, Queues q, hipStream_t s);
+// Stages of one bounce-loop iteration, in launch order.
+enum IterStage { kStageReset, kStageExtend, kStageShade, kStageShadow, kStageMis, kStageResolve, kStageCount };
+void launch_stage(int stage, const DevScene& sc, const Chunk* c, const Paths& p, Queues q, int parity, const LaunchCfg& cfg);
+void launch_iteration(const DevScene& sc, const Chunk* c, const Paths& p, Queues q, int parity, const LaunchCfg& cfg);
+void launch_film(const Chunk* c, int n_pixels, const Paths& p, FilmParams f, Queues q, hipStream_t s);
 void launch_trace_rays(const DevScene& sc, const float4* rays, int64_t n, int any, float4* out_tb, int2* out_ot,
                        unsigned long long* totals, const LaunchCfg& cfg);
+int traversal_blocks_per_cu(const LaunchCfg& cfg);
 
 }  // namespace yg

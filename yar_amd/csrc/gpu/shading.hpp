@@ -1,0 +1,245 @@
+// shading.hpp - shading frame, BSDFs and MIS helpers of the wavefront tracer (device inline code).
+//
+// Restates src/ref/shading_context.cpp:17-240 (local geometry, shading-normal adaptation, ray-origin
+// offsets), src/ref/bsdf.cpp:15-200 (Diffuse_BRDF, Metal_BRDF), src/ref/scattering.cpp (Fresnel,
+// anisotropic GGX) and src/ref/sampling.cpp:43-53,311-359 (cosine and VNDF sampling) with the
+// reference's operation order.
+#pragma once
+
+#include "device.hpp"
+
+namespace yg {
+
+YG_INLINE float power_heuristic(float a, float b) {             // direct_lighting.cpp:18
+    if (a == f_inf()) return 1.f;
+    return a * a / (a * a + b * b);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Shading context (shading_context.cpp:17-240, 405-418; math.cpp:21-120)
+// ---------------------------------------------------------------------------------------------
+struct Shading {
+    V3 wo, pos, ng, n, T, B, org_pos, org_neg;
+    bool flipped;
+    int material, light;
+};
+
+YG_INLINE void coordinate_system(V3 v, V3& v1) {                                   // math.cpp:21
+    v1 = (fabsf(v.x) > fabsf(v.y)) ? v3(-v.z, 0.f, v.x) : v3(0.f, -v.z, v.y);
+    v1 = vdiv(v1, length(v1));
+}
+YG_INLINE V3 reflect(V3 w, V3 n) { return (2.f * dot(w, n)) * n - w; }             // math.h:111
+YG_INLINE V3 origin_dir(const Shading& sc, V3 d) { return dot(d, sc.ng) > 0.f ? sc.org_pos : sc.org_neg; }
+
+YG_INLINE void offset_origins(V3 p, V3 ng, V3& pos, V3& neg) {                      // math.cpp:81-120
+    const float int_scale = 256.f, origin = 1.0f / 32.0f, float_scale = 1.0f / 65536.0f;
+    float pa[3] = {p.x, p.y, p.z}, na[3] = {ng.x, ng.y, ng.z}, po[3], ne[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        int di = (int)(int_scale * na[a]);
+        int ib = (int)fbits(pa[a]);
+        int i1 = ib - di, i2 = ib + di;
+        po[a] = bitsf((uint32_t)(pa[a] < 0 ? i1 : i2));
+        ne[a] = bitsf((uint32_t)(pa[a] < 0 ? i2 : i1));
+        if (fabsf(pa[a]) < origin) {
+            float dlt = float_scale * na[a];
+            po[a] = pa[a] + dlt;
+            ne[a] = pa[a] - dlt;
+        }
+    }
+    pos = v3(po[0], po[1], po[2]);
+    neg = v3(ne[0], ne[1], ne[2]);
+}
+
+YG_INLINE void adjust_normal(V3 wo, V3 ng, V3& n) {                                 // shading_context.cpp:17
+    V3 R = reflect(wo, n);
+    float a = dot(R, ng);
+    if (a >= 0.f) return;
+    if (dot(wo, ng) < 0.0017453f) { n = ng; return; }
+    float b = dot(n, ng);
+    V3 tangent;
+    if (b > 1e-4f) {
+        float dist = fabsf(a) / b;
+        tangent = R + dist * n;
+        tangent = vdiv(tangent, length(tangent));
+    } else {
+        tangent = n;
+    }
+    tangent = tangent + 1e-4f * ng;
+    n = normalized(wo + tangent);
+}
+
+YG_INLINE void init_shading(const DevScene& s, V3 ray_d, int obj, uint32_t tri, V3 b, Shading& sc) {
+    const int4 oi = s.obj_info[obj];
+    const int mflags = s.mesh_info[oi.z].x;
+    sc.wo = -normalized(ray_d);
+    const V3 p0 = from4(s.tri_verts[3 * (size_t)tri]), p1 = from4(s.tri_verts[3 * (size_t)tri + 1]),
+             p2 = from4(s.tri_verts[3 * (size_t)tri + 2]);
+    V3 position = b.x * p0 + b.y * p1 + b.z * p2;
+    V3 ng = normalized(cross(p1 - p0, p2 - p0));
+    if (mflags & 4) ng = -ng;
+    V3 n = ng;
+    const int4 vi = s.tri_vidx[tri];
+    if (mflags & 1) {
+        V3 n0 = from4(s.vtx_n[vi.x]), n1 = from4(s.vtx_n[vi.y]), n2 = from4(s.vtx_n[vi.z]);
+        V3 in = b.x * n0 + b.y * n1 + b.z * n2;
+        if (len_sq(in) != 0.f) n = normalized(in);
+    }
+    const bool has_uv = (mflags & 2) != 0;
+    V3 dpdu = v3(0.f, 0.f, 0.f);
+    if (has_uv) {
+        const float2 t0 = s.vtx_uv[vi.x], t1 = s.vtx_uv[vi.y], t2 = s.vtx_uv[vi.z];
+        float a00 = t1.x - t0.x, a01 = t1.y - t0.y, a10 = t2.x - t0.x, a11 = t2.y - t0.y;
+        V3 bp0 = p1 - p0, bp1 = p2 - p0;
+        float det = a00 * a11 - a01 * a10;                                            // math.cpp:8
+        if (det != 0.f) {
+            float inv = 1.f / det;
+            dpdu = (a11 * bp0 - a01 * bp1) * inv;
+        }
+        if (fabsf(det) < 1e-9f) coordinate_system(ng, dpdu);
+    }
+    const M34 o2w = {s.obj_o2w[3 * obj], s.obj_o2w[3 * obj + 1], s.obj_o2w[3 * obj + 2]};
+    const M34 nrm = {s.obj_nrm[3 * obj], s.obj_nrm[3 * obj + 1], s.obj_nrm[3 * obj + 2]};
+    sc.pos = xform_point(o2w, position);
+    n = xform_vector(nrm, n); n = vdiv(n, length(n));
+    ng = xform_vector(nrm, ng); ng = vdiv(ng, length(ng));
+    dpdu = xform_vector(o2w, dpdu);
+    if (dot(n, ng) == 0.f) ng = n;
+    if (dot(ng, sc.wo) < 0.f) ng = -ng;
+    sc.flipped = false;
+    if (dot(n, ng) < 0.f) { n = -n; sc.flipped = true; }
+    offset_origins(sc.pos, ng, sc.org_pos, sc.org_neg);
+    adjust_normal(sc.wo, ng, n);
+    V3 dpdu_s;
+    if (!has_uv) coordinate_system(n, dpdu_s);
+    else if (fabsf(dot(n, ng) - 1.f) < 1e-6f) dpdu_s = dpdu;
+    else {
+        V3 t = cross(dpdu, n);                                                         // math.cpp:34
+        dpdu_s = normalized(cross(n, t));
+        dpdu_s = dpdu_s * length(dpdu);
+    }
+    sc.ng = ng;
+    sc.n = n;
+    sc.B = normalized(cross(n, dpdu_s));                                               // bsdf.cpp:15-21
+    sc.T = cross(sc.B, n);
+    sc.material = oi.x;
+    sc.light = oi.y;
+}
+
+// ---------------------------------------------------------------------------------------------
+// BSDFs: Diffuse_BRDF (bsdf.cpp:60-84), Metal_BRDF (bsdf.cpp:146-200) with scattering.cpp and
+// sampling.cpp:43-53, 311-359
+// ---------------------------------------------------------------------------------------------
+YG_INLINE V3 to_world(const Shading& sc, V3 l) {
+    return v3(sc.T.x * l.x + sc.B.x * l.y + sc.n.x * l.z, sc.T.y * l.x + sc.B.y * l.y + sc.n.y * l.z,
+              sc.T.z * l.x + sc.B.z * l.y + sc.n.z * l.z);
+}
+YG_INLINE V3 to_local(const Shading& sc, V3 w) { return v3(dot(w, sc.T), dot(w, sc.B), dot(w, sc.n)); }
+
+YG_INLINE RGB conductor_fresnel(float cos_i, float eta_i, RGB eta_t, RGB k_t) {
+    cos_i = fabsf(fminf(fmaxf(cos_i, -1.f), 1.f));
+    float c2 = cos_i * cos_i, s2 = 1.f - c2;
+    RGB eta = cdivs(eta_t, eta_i), k = cdivs(k_t, eta_i);
+    RGB eta2 = eta * eta, k2 = k * k;
+    RGB t0 = eta2 - k2 - splat(s2);
+    RGB a2b2 = csqrt(t0 * t0 + 4.f * eta2 * k2);
+    RGB t1 = a2b2 + splat(c2);
+    RGB a = csqrt(0.5f * (a2b2 + t0));
+    RGB t2 = (2.f * cos_i) * a;
+    RGB Rs = cdiv(t1 - t2, t1 + t2);
+    RGB t3 = c2 * a2b2 + splat(s2 * s2);
+    RGB t4 = t2 * s2;
+    RGB Rp = cdiv(Rs * (t3 - t4), t3 + t4);
+    return 0.5f * (Rp + Rs);
+}
+YG_INLINE float dielectric_fresnel(float cos_i, float eta) {
+    cos_i = fminf(fabsf(cos_i), 1.f);
+    float sin_i = sqrtf(1.f - cos_i * cos_i);
+    float sin_t = (1.f / eta) * sin_i;
+    if (sin_t >= 1.f) return 1.f;
+    float cos_t = sqrtf(1.f - sin_t * sin_t);
+    float Rp = (eta * cos_i - cos_t) / (eta * cos_i + cos_t);
+    float Rs = (cos_i - eta * cos_t) / (cos_i + eta * cos_t);
+    return 0.5f * (Rp * Rp + Rs * Rs);
+}
+YG_INLINE float D_aniso(V3 wh, float ax, float ay) {
+    float c = fminf(fmaxf(wh.z, -1.f), 1.f);
+    float c2 = c * c, s2 = 1.f - c2;
+    float s2inv = 1.f / s2;
+    float cp = (s2 == 0.f) ? 1.f : (wh.x * wh.x) * s2inv;
+    float sp = (s2 == 0.f) ? 0.f : (wh.y * wh.y) * s2inv;
+    float k = c2 + s2 * (cp / (ax * ax) + sp / (ay * ay));
+    return 1.f / (kPi * ax * ay * k * k);
+}
+YG_INLINE float lambda_aniso(V3 v, float ax, float ay) {
+    float c = fminf(fmaxf(v.z, -1.f), 1.f);
+    float c2 = c * c, s2 = 1.f - c2;
+    float tan2 = s2 / c2;
+    float s2inv = 1.f / s2;
+    float cp = (s2 == 0.f) ? 1.f : (v.x * v.x) * s2inv;
+    float sp = (s2 == 0.f) ? 0.f : (v.y * v.y) * s2inv;
+    float alpha2 = cp * ax * ax + sp * ay * ay;
+    return 0.5f * (-1.f + sqrtf(1.f + alpha2 * tan2));
+}
+YG_INLINE float G_aniso(V3 wi, V3 wo, float ax, float ay) {
+    return 1.f / (1.f + lambda_aniso(wi, ax, ay) + lambda_aniso(wo, ax, ay));
+}
+YG_INLINE float G1_aniso(V3 v, float ax, float ay) { return 1.f / (1.f + lambda_aniso(v, ax, ay)); }
+
+YG_INLINE V3 vndf_sample(float u0, float u1, V3 wo, float ax, float ay) {
+    V3 N = normalized(v3(ax * wo.x, ay * wo.y, wo.z));
+    float lsq = N.x * N.x + N.y * N.y;
+    V3 T1 = lsq > 0.f ? vdiv(v3(-N.y, N.x, 0.f), sqrtf(lsq)) : v3(1.f, 0.f, 0.f);
+    V3 T2 = cross(N, T1);
+    float r = sqrtf(u0);
+    float phi = 2.f * kPi * u1;
+    float t1 = r * cosf(phi);
+    float t2 = r * sinf(phi);
+    float sgl = 0.5f * (1.f + N.z);
+    t2 = (1.f - sgl) * sqrtf(1.f - t1 * t1) + sgl * t2;
+    V3 nh = t1 * T1 + t2 * T2 + sqrtf(fmaxf(0.f, 1.f - t1 * t1 - t2 * t2)) * N;
+    return normalized(v3(ax * nh.x, ay * nh.y, fmaxf(1e-6f, nh.z)));
+}
+YG_INLINE float vndf_wi_pdf(V3 wo, V3 wh, float ax, float ay) {
+    float G1 = G1_aniso(wo, ax, ay);
+    float D = D_aniso(wh, ax, ay);
+    float wh_pdf = G1 * D * fmaxf(0.f, dot(wo, wh)) / wo.z;
+    return wh_pdf / (4 * dot(wh, wo));
+}
+
+YG_INLINE RGB bsdf_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
+    if (m.type == 1) return kPiInv * rgb4(m.refl);
+    V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
+    V3 wh = normalized(wol + wil);
+    float ci = dot(wil, wh);
+    RGB F = conductor_fresnel(ci, m.eta_i, rgb4(m.eta), rgb4(m.k));
+    float G = G_aniso(wil, wol, m.alpha_x, m.alpha_y);
+    float D = D_aniso(wh, m.alpha_x, m.alpha_y);
+    return F * ((G * D) / (4.f * wol.z * wil.z));
+}
+YG_INLINE float bsdf_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
+    if (m.type == 1) return dot(sc.n, wi) / kPi;
+    V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
+    V3 wh = normalized(wol + wil);
+    return vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
+}
+YG_INLINE RGB bsdf_sample(const DevMaterial& m, const Shading& sc, float u0, float u1, V3 wo, V3& wi, float& pdf) {
+    if (m.type == 1) {
+        float z = sqrtf(1.f - u0);                                                     // sampling.cpp:43
+        float r = sqrtf(u0);
+        float phi = 2.f * kPi * u1;
+        V3 l = v3(r * cosf(phi), r * sinf(phi), z);
+        wi = to_world(sc, l);
+        pdf = bsdf_pdf(m, sc, wo, wi);
+        return bsdf_eval(m, sc, wo, wi);
+    }
+    V3 wol = to_local(sc, wo);
+    V3 wh = vndf_sample(u0, u1, wol, m.alpha_x, m.alpha_y);
+    V3 wil = reflect(wol, wh);
+    if (wil.z <= 0.f) return rgb(0.f, 0.f, 0.f);
+    wi = to_world(sc, wil);
+    pdf = vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
+    return bsdf_eval(m, sc, wo, wi);
+}
+
+}  // namespace yg
