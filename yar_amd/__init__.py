@@ -10,10 +10,10 @@ import os
 import numpy as np
 
 from . import _abi
-from ._abi import (FilmPixel, GpuInfo, GpuOptions, Hit, Ray, RenderStats, SceneDesc, View,
+from ._abi import (FilmPixel, GpuInfo, GpuOptions, GuideConfig, Hit, Ray, RenderStats, SceneDesc, View,
                    film_dtype, hit_dtype, ray_dtype)
 
-__all__ = ["lib", "YarError", "Scene", "Context", "generate_scene", "build_kdtree", "build_box_kdtree",
+__all__ = ["lib", "YarError", "Scene", "Context", "Guide", "GuideConfig", "generate_scene", "build_kdtree", "build_box_kdtree",
            "GpuOptions", "View", "RenderStats", "ray_dtype", "hit_dtype", "film_dtype"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libyar_gpu.so")
@@ -241,3 +241,87 @@ class Context:
         st = RenderStats()
         _check(lib().yar_gpu_last_trace_counters(self._h, C.byref(st)), self._h)
         return st
+
+
+class Guide:
+    """Learned directional proposal (yar_guide_*): the MLP + vMF mixture with its own training."""
+
+    def __init__(self, device=0, config=None, **kw):
+        cfg = config or _abi.GuideConfig()
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        h = C.c_void_p()
+        rc = lib().yar_guide_create(int(device), C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise YarError(rc, (lib().yar_last_error() or b"").decode(errors="replace"))
+        self._h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise YarError(rc, (lib().yar_guide_last_error(self._h) or b"").decode(errors="replace"))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().yar_guide_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def param_count():
+        return int(lib().yar_guide_param_count())
+
+    def params(self):
+        out = np.empty(self.param_count(), dtype=np.float32)
+        self._check(lib().yar_guide_get_params(self._h, out.ctypes.data))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.param_count()
+        self._check(lib().yar_guide_set_params(self._h, p.ctypes.data))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 9)
+        out = np.empty((x.shape[0], 40), dtype=np.float32)
+        self._check(lib().yar_guide_forward(self._h, x.ctypes.data, x.shape[0], out.ctypes.data))
+        return out
+
+    def gradients(self, x, dirs, weights):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 9)
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        w = np.ascontiguousarray(weights, dtype=np.float32).reshape(-1)
+        g = np.empty(self.param_count(), dtype=np.float32)
+        loss = C.c_float()
+        self._check(lib().yar_guide_gradients(self._h, x.ctypes.data, d.ctypes.data, w.ctypes.data, x.shape[0],
+                                              g.ctypes.data, C.byref(loss)))
+        return loss.value, g
+
+    def train(self, x, dirs, weights, steps):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 9)
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        w = np.ascontiguousarray(weights, dtype=np.float32).reshape(-1)
+        loss = C.c_float()
+        self._check(lib().yar_guide_train(self._h, x.ctypes.data, d.ctypes.data, w.ctypes.data, x.shape[0],
+                                          int(steps), C.byref(loss)))
+        return loss.value
+
+    def pdf(self, x, dirs):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 9)
+        d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+        out = np.empty(x.shape[0], dtype=np.float32)
+        self._check(lib().yar_guide_pdf(self._h, x.ctypes.data, d.ctypes.data, x.shape[0], out.ctypes.data))
+        return out
+
+    def sample(self, x, u):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 9)
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1, 3)
+        dirs = np.empty((x.shape[0], 3), dtype=np.float32)
+        pdf = np.empty(x.shape[0], dtype=np.float32)
+        self._check(lib().yar_guide_sample(self._h, x.ctypes.data, u.ctypes.data, x.shape[0], dirs.ctypes.data,
+                                           pdf.ctypes.data))
+        return dirs, pdf
