@@ -69,7 +69,7 @@ def test_gradients_match_oracle(n):
     names = ["W1", "b1", "W2", "b2", "W3", "b3", "W4", "b4", "W5", "b5"]
     errs = []
     for name, idx in zip(names, parts):
-        idx = idx.reshape(-1)
+        idx = idx.reshape(-1).astype(np.int64)
         errs.append((name, _rel_l2(grad[idx], ref[idx])))
     print(f"n={n}: loss gpu {loss:.6f} oracle {ref_loss:.6f}; rel L2 per tensor " +
           ", ".join(f"{a} {b:.2e}" for a, b in errs))
