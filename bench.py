@@ -58,6 +58,92 @@ def _cpu_baseline(scene, view, crop, threads):
                       f"{dt:.2f} s on {threads} threads (oracle/cpu_ref.cpp, g++ -O2 -ffp-contract=off)"}
 
 
+def _lum(img):
+    import numpy as np
+    return 0.2126729 * img[..., 0] + 0.7151522 * img[..., 1] + 0.0721750 * img[..., 2]
+
+
+def _config1_leg(threads, cpu):
+    """Config 1 (256x256, 16 spp, depth 5): GPU samples/s of the whole render beside the CPU oracle
+    on the same render."""
+    import yar_amd
+    scene = _scene(1, 1.0, 1)
+    ctx = yar_amd.Context(scene)
+    ctx.render(0, want_film=False)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        _, _, _, st = ctx.render(0, want_film=False)
+        ts.append(time.perf_counter() - t0)
+    gpu = st.samples / min(ts) / 1e6
+    out = {"workload": "config 1 plain render, 256x256, 16 spp, max depth 5", "gpu_msamples_s": round(gpu, 2),
+           "gpu_ms": round(min(ts) * 1e3, 2)}
+    if cpu:
+        from tests import oracle_lib
+        osc = oracle_lib.OracleScene(scene, threads=threads)
+        t0 = time.perf_counter()
+        osc.render(ctx.view(0))
+        dt = time.perf_counter() - t0
+        c = st.samples / dt / 1e6
+        out.update({"cpu_msamples_s": round(c, 3), "cpu_threads": threads, "cpu_s": round(dt, 2),
+                    "gpu_vs_cpu": round(gpu / c, 1)})
+    ctx.close()
+    return out
+
+
+def _batch_leg():
+    """Config 3: the config-1 scene from 32 seeded poses (sides 128..1024, 4/16/64 spp) in one
+    batched render with the bounce loop replayed from a hipGraph."""
+    import yar_amd
+    from yar_amd import views as yv
+    scene = _scene(1, 1.0, 1)
+    ctx = yar_amd.Context(scene, use_graph=1)
+    views = yv.batch_views(ctx.view(0), 32)
+    ctx.set_views(views)
+    idx = list(range(len(views)))
+    ctx.render_batch(idx)
+    t0 = time.perf_counter()
+    _, _, st = ctx.render_batch(idx)
+    dt = time.perf_counter() - t0
+    ctx.close()
+    return {"workload": "config 3: config-1 scene, 32 views, sides {128..1024}, spp {4,16,64}, hipGraph bounce loop",
+            "samples": st.samples, "ms": round(dt * 1e3, 1), "msamples_s": round(st.samples / dt / 1e6, 2),
+            "mrays_s": round((st.closest_rays + st.any_rays) / dt / 1e6, 1)}
+
+
+def _guided_leg(n_views=4, side=256, grid=8, ref_grid=32):
+    """Guided vs plain on config-3 views at equal sample count: MSE against a plain reference of
+    ref_grid^2 spp, wall times, and the time ratio guided/plain to reach the same MSE (MSE ~ 1/time)."""
+    import numpy as np
+    import yar_amd
+    from yar_amd import views as yv
+    scene = _scene(1, 1.0, 1)
+    ctx = yar_amd.Context(scene)
+    base = yv.batch_views(ctx.view(0), n_views)
+    vs = [yv.with_film(v, side, side, grid, seed_offset=v.rng_seed_offset) for v in base]
+    refs = [yv.with_film(v, side, side, ref_grid, seed_offset=1000 + i) for i, v in enumerate(base)]
+    ctx.set_views(vs + refs)
+    ref, _, _ = ctx.render_batch(list(range(n_views, 2 * n_views)))
+    idx = list(range(n_views))
+    ctx.render_batch(idx)
+    t0 = time.perf_counter()
+    plain, _, _ = ctx.render_batch(idx)
+    tp = time.perf_counter() - t0
+    guide = yar_amd.Guide(seed=1)
+    ctx.render_guided(guide, idx)
+    guide = yar_amd.Guide(seed=1)
+    t0 = time.perf_counter()
+    guided, _, gst = ctx.render_guided(guide, idx)
+    tg = time.perf_counter() - t0
+    ctx.close()
+    mp = float(np.mean([((a - b) ** 2).mean() for a, b in zip(plain, ref)]))
+    mg = float(np.mean([((a - b) ** 2).mean() for a, b in zip(guided, ref)]))
+    return {"workload": f"config-3 scene, {n_views} views at {side}x{side}, {grid * grid} spp, reference {ref_grid ** 2} spp plain",
+            "plain_ms": round(tp * 1e3, 1), "guided_ms": round(tg * 1e3, 1), "mse_plain": mp, "mse_guided": mg,
+            "equal_spp_mse_ratio": round(mg / mp, 4), "time_ratio_to_equal_mse": round((mg * tg) / (mp * tp), 3),
+            "train_nats": round(gst.train_nats, 3), "passes": gst.passes}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +156,7 @@ def main():
     ap.add_argument("--no-counters", action="store_true", help="skip the counted render (roofline bytes)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the config-1, batch and guided legs")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -141,9 +228,17 @@ def main():
                                 "objects": round(cs.extend_object_visits / cs.extend_rays, 2)}}
 
     cpu = None
+    threads = min(16, os.cpu_count() or 1)
     if args.cpu_crop > 0:
-        threads = min(16, os.cpu_count() or 1)
         cpu = _cpu_baseline(scene, view, args.cpu_crop, threads)
+    extra = {}
+    if not args.no_extra:
+        extra["config1"] = _config1_leg(threads, args.cpu_crop > 0)
+        print(f"# config 1: {extra['config1']}", file=sys.stderr, flush=True)
+        extra["batch"] = _batch_leg()
+        print(f"# batch: {extra['batch']}", file=sys.stderr, flush=True)
+        extra["guided"] = _guided_leg()
+        print(f"# guided: {extra['guided']}", file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1,
@@ -166,6 +261,7 @@ def main():
     }
     if cpu:
         line["gpu_vs_cpu"] = round(value / cpu["value"], 1)
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 

@@ -219,6 +219,11 @@ typedef struct yar_render_stats {
     int64_t extend_rays;                /* rays traced by the extend kernel */
     int64_t extend_launches;
     double  extend_seconds, shade_seconds, shadow_seconds, mis_seconds, other_seconds;  /* enable_timing */
+    double  guide_seconds;              /* enable_timing: guide network + guided sampling stages */
+    double  train_seconds;              /* guided: host wall time of the training between passes */
+    int64_t train_records;              /* guided: training records collected over all passes */
+    int64_t passes;                     /* guided: sample passes (1 for plain renders) */
+    double  train_nats;                 /* guided: last batch's log-likelihood gain over uniform */
 } yar_render_stats;
 
 int  yar_gpu_create(const yar_scene_desc* scene, const yar_gpu_options* options, yar_gpu_ctx** out_ctx);
@@ -313,6 +318,13 @@ int  yar_guide_train(yar_guide* guide, const float* inputs, const float* dirs, c
 int  yar_guide_pdf(yar_guide* guide, const float* inputs, const float* dirs, int64_t n, float* out_pdf);
 int  yar_guide_sample(yar_guide* guide, const float* inputs, const float* uniforms, int64_t n, float* out_dirs,
                       float* out_pdf);
+
+/* Guided form of yar_gpu_render_batch: renders the views in sample passes (sizes first_pass_spp,
+ * first_pass_spp, 2x, 4x, ...), traces continuations with one-sample MIS between the guide's
+ * mixture and the BSDF (the first pass uses the BSDF alone), and trains the guide on the pass's
+ * radiance samples between passes.  The image converges to the plain render's mean. */
+int  yar_gpu_render_guided(yar_gpu_ctx* ctx, yar_guide* guide, const int32_t* views, int32_t count,
+                           float* const* out_images, double* out_variances, yar_render_stats* out_stats);
 
 #ifdef __cplusplus
 }

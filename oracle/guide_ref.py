@@ -9,8 +9,8 @@ kernels implement (there is no reference file:line to follow):
 
   inputs   x in [0,1]^9 (position in scene bounds, outgoing direction and shading normal mapped
            from [-1,1]), clamped to [0,1]
-  encoding e = [x (9), for input i, frequency k<16: sin(2 pi frac(x_i 2^k)), cos(...)] -> 297
-           features, zero padded to 304
+  encoding e = [x (9), for input i, frequency k<16: sin(2 pi frac(x_i f_k)), cos(...)] with
+           f_k = 2^(k/2) in fp32 (half-octave steps, 1 .. 181) -> 297 features, zero padded to 304
   network  h0 = bf16(e); z_l = h_{l-1} W_l + b_l (fp32 accumulate); h_l = bf16(relu(z_l)), l=1..4;
            o = h4 W5 + b5 (fp32), 48 outputs of which 40 are used; W_l in bf16, b_l in fp32
   mixture  8 lobes, lobe j reads o[5j .. 5j+4] = (logit, r_xyz, s): pi = softmax(logits),
@@ -69,6 +69,12 @@ def bf16(x):
     return np.where(nan, np.nan, r)
 
 
+def freq(k):
+    """f_k = 2^(k/2) as the fp32 value the kernels use."""
+    base = np.float32(np.sqrt(2.0)) if k & 1 else np.float32(1.0)
+    return np.float32(base * np.float32(2.0 ** (k >> 1)))
+
+
 def encode(x):
     x = np.clip(np.asarray(x, dtype=np.float32), 0.0, 1.0)
     n = x.shape[0]
@@ -76,7 +82,7 @@ def encode(x):
     e[:, :N_IN] = x
     for i in range(N_IN):
         for k in range(N_FREQ):
-            t = x[:, i].astype(np.float32) * np.float32(2.0 ** k)        # exact in fp32
+            t = x[:, i].astype(np.float32) * freq(k)                     # one fp32 rounding
             f = (t - np.floor(t)).astype(np.float64)
             e[:, N_IN + 32 * i + 2 * k] = np.sin(2 * np.pi * f)
             e[:, N_IN + 32 * i + 2 * k + 1] = np.cos(2 * np.pi * f)

@@ -143,3 +143,42 @@ def regression_mesh_scene(index):
                      materials=[{"reflectance": [0.5, 0.5, 0.5]}],
                      objects=[{"mesh": 0, "material": 0}, {"mesh": 1, "light": 0}],
                      rects=[{"l2w": [1, 0, 0, 0, 0, 0, -1, 40, 0, 1, 0, 0], "L": [1, 1, 1], "size": [0.2, 0.2]}])
+
+
+def _quad(a, b, c, d):
+    return {"vertices": [a, b, c, d], "indices": [0, 1, 2, 0, 2, 3]}
+
+
+def indirect_room_scene(res=64, spp_side=4, maxdepth=5, albedo=0.7, Le=50.0):
+    """Closed 4 x 3 x 4 diffuse room lit only indirectly: a 0.4 x 0.4 emitter at height 2.6 faces
+    the ceiling, so every point below it receives light by way of the bright ceiling patch (next-event
+    estimation sees the emitter only from above its plane).  The camera looks at the far wall."""
+    W, Hh, D = 4.0, 3.0, 4.0
+    walls = [
+        _quad([0, 0, 0], [W, 0, 0], [W, 0, D], [0, 0, D]),          # floor
+        _quad([0, Hh, 0], [0, Hh, D], [W, Hh, D], [W, Hh, 0]),      # ceiling
+        _quad([0, 0, 0], [0, Hh, 0], [W, Hh, 0], [W, 0, 0]),        # z = 0
+        _quad([0, 0, D], [W, 0, D], [W, Hh, D], [0, Hh, D]),        # z = D
+        _quad([0, 0, 0], [0, 0, D], [0, Hh, D], [0, Hh, 0]),        # x = 0
+        _quad([W, 0, 0], [W, Hh, 0], [W, Hh, D], [W, 0, D]),        # x = W
+    ]
+    h, s = 2.6, 0.2
+    light = _quad([2 - s, h, 2 - s], [2 - s, h, 2 + s], [2 + s, h, 2 + s], [2 + s, h, 2 - s])
+    # light frame: local x -> world x, local y -> world -z, normal (local z) -> world +y
+    l2w = [1, 0, 0, 2,
+           0, 0, 1, h,
+           0, -1, 0, 2]
+    # camera at (2, 1.4, 3.9) looking down -z: columns right (1,0,0), up (0,1,0), back (0,0,1)
+    c2w = [1, 0, 0, 2,
+           0, 1, 0, 1.4,
+           0, 0, 1, 3.9]
+    meshes = walls + [light]
+    objects = [{"mesh": i, "material": 0} for i in range(len(walls))] + [{"mesh": len(walls), "light": 0}]
+    return DescScene(
+        meshes=meshes,
+        materials=[{"type": _abi.YAR_MATERIAL_DIFFUSE, "reflectance": [albedo] * 3}],
+        objects=objects,
+        rects=[{"l2w": l2w, "L": [Le] * 3, "size": [2 * s, 2 * s]}],
+        camera={"c2w": c2w, "fov": 70.0},
+        config={"film_width": res, "film_height": res, "region": (0, 0, res, res), "nx": spp_side, "ny": spp_side,
+                "maxdepth": maxdepth})

@@ -38,9 +38,12 @@ def test_encoding_closed_form():
     e = G.encode(x)
     assert e.shape == (1, 304) and np.all(e[:, 297:] == 0)
     assert np.array_equal(e[0, :9], x[0].astype(np.float64))
-    # input 1 (0.25), frequency 0: sin(pi/2) = 1, cos = 0; frequency 1: sin(pi) = 0, cos = -1
+    # input 1 (0.25), frequency 1: sin(pi/2) = 1, cos = 0; frequency 2 (k = 2): sin(pi) = 0, cos = -1
     assert np.isclose(e[0, 9 + 32 + 0], 1.0) and np.isclose(e[0, 9 + 32 + 1], 0.0, atol=1e-12)
-    assert np.isclose(e[0, 9 + 32 + 2], 0.0, atol=1e-12) and np.isclose(e[0, 9 + 32 + 3], -1.0)
+    assert np.isclose(e[0, 9 + 32 + 4], 0.0, atol=1e-12) and np.isclose(e[0, 9 + 32 + 5], -1.0)
+    # k = 1 is 2^(1/2) cycles: 0.25 * sqrt(2) of a revolution
+    assert np.isclose(e[0, 9 + 32 + 2], np.sin(2 * np.pi * 0.25 * np.sqrt(2)), atol=1e-6)
+    assert [float(G.freq(k)) for k in (0, 2, 4, 15)] == [1.0, 2.0, 4.0, float(np.float32(np.sqrt(2)) * 128)]
     # sin^2 + cos^2 = 1 for every pair
     s, c = e[0, 9:297:2], e[0, 10:297:2]
     assert np.allclose(s * s + c * c, 1.0)

@@ -225,6 +225,19 @@ class Context:
         _check(lib().yar_gpu_render_batch(self._h, idx, len(views), ptrs, var, C.byref(st)), self._h)
         return imgs, list(var), st
 
+    def render_guided(self, guide, views):
+        """Guided form of render_batch (the guide trains between sample passes)."""
+        imgs = []
+        for vi in views:
+            v = self.view(vi)
+            imgs.append(np.empty((v.region_y1 - v.region_y0, v.region_x1 - v.region_x0, 3), dtype=np.float32))
+        ptrs = (C.c_void_p * len(views))(*[i.ctypes.data for i in imgs])
+        idx = (C.c_int32 * len(views))(*views)
+        var = (C.c_double * len(views))()
+        st = RenderStats()
+        _check(lib().yar_gpu_render_guided(self._h, guide._h, idx, len(views), ptrs, var, C.byref(st)), self._h)
+        return imgs, list(var), st
+
     def trace_rays(self, rays, any_hit=False):
         """rays: structured array of ray_dtype(); returns structured array of hit_dtype()."""
         r = np.ascontiguousarray(rays, dtype=ray_dtype())

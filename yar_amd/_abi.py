@@ -98,7 +98,9 @@ class RenderStats(C.Structure):
                 ("extend_triangle_tests", C.c_int64), ("extend_object_visits", C.c_int64),
                 ("extend_rays", C.c_int64), ("extend_launches", C.c_int64),
                 ("extend_seconds", C.c_double), ("shade_seconds", C.c_double),
-                ("shadow_seconds", C.c_double), ("mis_seconds", C.c_double), ("other_seconds", C.c_double)]
+                ("shadow_seconds", C.c_double), ("mis_seconds", C.c_double), ("other_seconds", C.c_double),
+                ("guide_seconds", C.c_double), ("train_seconds", C.c_double), ("train_records", C.c_int64),
+                ("passes", C.c_int64), ("train_nats", C.c_double)]
 
 
 class GuideConfig(C.Structure):
@@ -163,6 +165,7 @@ SIGNATURES = {
     "yar_gpu_trace_rays_device": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P]),
     "yar_gpu_last_trace_counters": (C.c_int, [_P, C.POINTER(RenderStats)]),
     "yar_gpu_get_info": (C.c_int, [_P, C.POINTER(GpuInfo)]),
+    "yar_gpu_render_guided": (C.c_int, [_P, _P, C.POINTER(C.c_int32), C.c_int32, _P, _P, _P]),
     "yar_guide_param_count": (C.c_int64, []),
     "yar_guide_create": (C.c_int, [C.c_int32, C.POINTER(GuideConfig), C.POINTER(_P)]),
     "yar_guide_destroy": (None, [_P]),

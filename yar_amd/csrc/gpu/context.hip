@@ -51,7 +51,9 @@ struct DevBuf {
 
 static int pow2_at_least(int v) { int p = 1; while (p < v) p <<= 1; return p; }
 
-// Per-kernel HIP-event timing (kind: 0 extend, 1 shade, 2 shadow, 3 mis, 4 other).
+// Per-kernel HIP-event timing (kind: 0 extend, 1 shade, 2 shadow, 3 mis, 4 other, 5 guide network
+// and sampling, 6 guide training).
+constexpr int kTimerKinds = 7;
 struct KernelTimer {
     bool on = false;
     hipStream_t s = nullptr;
@@ -75,8 +77,8 @@ struct KernelTimer {
         if (!on) return;
         HIPX(hipEventRecord(next(), s));
     }
-    void collect(double out[5]) {
-        for (int k = 0; k < 5; k++) out[k] = 0;
+    void collect(double out[kTimerKinds]) {
+        for (int k = 0; k < kTimerKinds; k++) out[k] = 0;
         for (auto& m : marks) {
             float ms = 0.f;
             HIPX(hipEventElapsedTime(&ms, pool[m.second], pool[m.second + 1]));
@@ -132,6 +134,12 @@ struct yar_gpu_ctx {
     DevBuf<double> out_var;
     DevBuf<int4> side_info;
     DevBuf<uint8_t> spill;
+    // guided form
+    DevBuf<uint64_t> grng;
+    DevBuf<float> rec, guide_in, guide_out;
+    DevBuf<uint32_t> guide_path;
+    DevBuf<GuideRecord> train;
+    DevBuf<double> m1, m2;
     uint32_t* h_poll = nullptr;           // pinned: active counts read back while the loop runs
     size_t h_poll_n = 0;
     // bounce-loop graph (captured per job shape)
@@ -172,12 +180,14 @@ struct yar_gpu_ctx {
         DevBuf<uint32_t>* bu[] = {&q_a, &q_b, &sh_path, &mi_path, &dl_path, &counts};
         for (auto* b : bu) b->release();
         mi_ndw.release(); totals.release(); out_var.release(); spill.release();
+        grng.release(); rec.release(); guide_in.release(); guide_out.release(); guide_path.release(); train.release();
+        m1.release(); m2.release();
         if (h_poll) (void)hipHostFree(h_poll);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
     LaunchCfg launch_cfg() const {
-        LaunchCfg lc;
+        LaunchCfg lc{};
         lc.block = opt.block_size > 0 ? opt.block_size : 128;
         lc.scene_stack = info.stack_depth >> 16;
         lc.mesh_stack = info.stack_depth & 0xffff;
@@ -186,6 +196,9 @@ struct yar_gpu_ctx {
         lc.trav_grid = 1;
         lc.path_grid = 1;
         lc.stream = stream;
+        lc.guided = false;
+        lc.net = GuideNet{nullptr, nullptr, nullptr};
+        lc.guide_grid = 0;
         return lc;
     }
 };
@@ -474,15 +487,36 @@ static DevView make_dev_view(const yar_view& v, int tile_base) {
 // least one except the first pass after a delta (glass) event, and paths end at max_light_bounces.
 static int loop_iterations(int max_bounces) { return std::max(1, max_bounces) + 2; }
 
+// Guided render: the learned proposal, its pass schedule and training-record storage.
+struct GuidedRun {
+    yar_guide* guide = nullptr;
+    std::vector<std::pair<int, int>> passes;      // sample ranges [k0, k1)
+};
+
+static std::vector<std::pair<int, int>> guided_passes(int max_spp, int first) {
+    std::vector<std::pair<int, int>> p;
+    int k0 = 0, size = std::max(1, first);
+    while (k0 < max_spp) {
+        const int k1 = std::min(max_spp, k0 + size);
+        p.push_back({k0, k1});
+        if (k0 > 0) size *= 2;                    // sizes first, first, 2 first, 4 first, ...
+        k0 = k1;
+    }
+    return p;
+}
+
 static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std::vector<PartResult>& out,
-                       yar_render_stats& st) {
+                       yar_render_stats& st, GuidedRun* gr = nullptr) {
     auto t_start = std::chrono::steady_clock::now();
     const yar_render_config& rc = ctx->cfg;
     if (rc.pixel_filter_radius != 0.5f) throw HipError{"only the box filter of radius 0.5 is supported"};
+    const bool guided = gr != nullptr;
     const int mdim = std::min(10, rc.max_light_bounces);
     const int d1 = 4 * std::max(0, mdim), d2 = 3 * std::max(0, mdim);          // reference_renderer.cpp:87-99
     const int dps = 2 + d1 + 2 * d2;
-    const bool sequential = rc.max_light_bounces > 10;                         // RNG fallback couples samples
+    // RNG fallback past 10 bounces couples a pixel's samples; guided passes draw those dimensions
+    // from each path's own stream instead
+    const bool sequential = rc.max_light_bounces > 10 && !guided;
 
     // views, tile offsets and the pixel list
     std::vector<DevView> dv;
@@ -491,10 +525,12 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     std::vector<int> pview;
     struct Unit { int part, tile; int64_t start, count; };
     std::vector<Unit> units;
+    int job_max_spp = 1;
     for (size_t pi = 0; pi < parts.size(); pi++) {
         const yar_view& v = ctx->views[parts[pi].view];
         const int spp = v.x_pixel_sample_count * v.y_pixel_sample_count;
         if (spp <= 0 || spp > 65536) throw HipError{"samples per pixel out of range"};
+        job_max_spp = std::max(job_max_spp, spp);
         int tgx, tgy, sx0, sy0, sx1, sy1;
         film_layout(v, &tgx, &tgy, &sx0, &sy0, &sx1, &sy1);
         const int base = (int)tile_first.size();
@@ -514,6 +550,10 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         }
     }
     const int64_t total_pixels = (int64_t)plist.size();
+    if (guided) gr->passes = guided_passes(job_max_spp, guide_config(gr->guide).first_pass_spp);
+    int pass_max = 1;                                                          // widest guided pass
+    if (guided)
+        for (auto& pr : gr->passes) pass_max = std::max(pass_max, pr.second - pr.first);
 
     // chunks of whole tiles, bounded by paths in flight, sample-table and radiance sizes
     const int64_t max_paths = ctx->opt.max_paths_in_flight > 0 ? ctx->opt.max_paths_in_flight : (int64_t)1 << 22;
@@ -524,7 +564,8 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         ChunkInfo cur{0, 0, 0, 0, 1};
         for (const Unit& u : units) {
             const int spp = dv[u.part].spp;
-            const int64_t us = u.count * spp, up = sequential ? u.count : us;
+            const int64_t us = u.count * spp;
+            const int64_t up = guided ? u.count * pass_max : sequential ? u.count : us;
             if (cur.pixels > 0 && (cur.paths + up > max_paths || cur.samples + us > max_samples)) {
                 cks.push_back(cur);
                 cur = ChunkInfo{u.start, 0, 0, 0, 1};
@@ -550,9 +591,12 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         pf.push_back(acc);
         max_chunk_pixels = std::max(max_chunk_pixels, ck.pixels);
         max_chunk_samples = std::max(max_chunk_samples, ck.samples);
-        max_chunk_paths = std::max(max_chunk_paths, ck.paths);
+        const int64_t paths = guided ? ck.pixels * pass_max : ck.paths;
+        max_chunk_paths = std::max(max_chunk_paths, paths);
     }
-    const int64_t side_cap = std::max<int64_t>(1 << 16, std::min<int64_t>((int64_t)1 << 24, max_chunk_samples / 64 * (int64_t)cks.size()));
+    const int n_passes = guided ? (int)gr->passes.size() : 1;
+    const int64_t side_cap = std::max<int64_t>(
+        1 << 16, std::min<int64_t>((int64_t)1 << 24, max_chunk_samples / 64 * (int64_t)cks.size() * n_passes));
 
     // buffers
     hipStream_t s = ctx->stream;
@@ -578,16 +622,31 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     ctx->side_info.reserve((size_t)side_cap);
     ctx->side_view.reserve((size_t)side_cap);
     ctx->chunk_slot.reserve(1);
+    int64_t train_cap = 0;
+    if (guided) {
+        train_cap = std::max<int64_t>(65536, guide_config(gr->guide).record_capacity);
+        ctx->grng.reserve(np);
+        ctx->rec.reserve(np * kRecMax * kRecFloats);
+        ctx->guide_path.reserve(np);
+        ctx->guide_in.reserve(np * 16);
+        ctx->guide_out.reserve(np * kGOut);
+        ctx->train.reserve((size_t)train_cap);
+        ctx->m1.reserve((size_t)std::max<int64_t>(1, total_pixels));
+        ctx->m2.reserve((size_t)std::max<int64_t>(1, total_pixels));
+        guide_prepare_device();
+    }
     ctx->pixels.upload(plist, s);
     ctx->pix_view.upload(pview, s);
     ctx->pix_first.upload(pf, s);
     ctx->tile_first.upload(tile_first, s);
     ctx->dviews.upload(dv, s);
 
-    // per (chunk, pass) parameter blocks
+    // per (chunk, pass) parameter blocks; guided jobs run every chunk once per pass, plain jobs run
+    // a chunk's passes (one, or one per sample in sequential mode) back to back
     std::vector<Chunk> hc;
-    std::vector<int> passes_of;
-    for (size_t ci = 0; ci < cks.size(); ci++) {
+    struct Step { int chunk, pass; size_t hc; };
+    std::vector<Step> steps;
+    auto make_chunk = [&](size_t ci) {
         const ChunkInfo& ck = cks[ci];
         Chunk c{};
         c.pixels = ctx->pixels.p + ck.start;
@@ -605,21 +664,54 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         c.rr_threshold = rc.russian_roulette_threshold;
         c.film_scale = rc.film_radiance_scale;
         c.max_component = rc.max_rgb_component_value_of_film_sample;
-        const int passes = sequential ? ck.max_spp : 1;
-        passes_of.push_back(passes);
-        for (int pass = 0; pass < passes; pass++) {
-            c.pass = pass;
-            hc.push_back(c);
+        return c;
+    };
+    if (guided) {
+        for (int pass = 0; pass < n_passes; pass++)
+            for (size_t ci = 0; ci < cks.size(); ci++) {
+                Chunk c = make_chunk(ci);
+                c.guided = 1;
+                c.k0 = gr->passes[pass].first;
+                c.k1 = gr->passes[pass].second;
+                c.alpha = pass == 0 ? 0.f : guide_config(gr->guide).guide_fraction;
+                c.n_paths = (long long)c.n_pixels * (c.k1 - c.k0);
+                if (c.k0 >= c.max_spp) continue;                   // no pixel of this chunk has such samples
+                steps.push_back({(int)ci, pass, hc.size()});
+                hc.push_back(c);
+            }
+    } else {
+        for (size_t ci = 0; ci < cks.size(); ci++) {
+            const int passes = sequential ? cks[ci].max_spp : 1;
+            for (int pass = 0; pass < passes; pass++) {
+                Chunk c = make_chunk(ci);
+                c.pass = pass;
+                steps.push_back({(int)ci, pass, hc.size()});
+                hc.push_back(c);
+            }
         }
     }
     ctx->chunks.upload(hc, s);
     HIPX(hipMemsetAsync(ctx->counts.p, 0, 16 * sizeof(uint32_t), s));
     HIPX(hipMemsetAsync(ctx->totals.p, 0, 24 * sizeof(unsigned long long), s));
 
-    Paths P{ctx->ray_o.p, ctx->ray_d.p, ctx->coeff.p, ctx->L.p, ctx->state.p, ctx->dl_coeff.p, ctx->dl_light.p,
-            ctx->dl_bsdf.p, ctx->hit_tb.p, ctx->hit_ot.p, ctx->pk.p, sequential ? ctx->rng.p : nullptr, ctx->radiance.p};
-    Queues q{ctx->q_a.p, ctx->q_b.p, ctx->sh_path.p, ctx->sh_o.p, ctx->sh_d.p, ctx->sh_c.p, ctx->mi_path.p,
-             ctx->mi_o.p, ctx->mi_d.p, ctx->mi_f.p, ctx->mi_ndw.p, ctx->dl_path.p, ctx->counts.p, ctx->totals.p};
+    Paths P{};
+    P.ray_o = ctx->ray_o.p; P.ray_d = ctx->ray_d.p; P.coeff = ctx->coeff.p; P.L = ctx->L.p; P.state = ctx->state.p;
+    P.dl_coeff = ctx->dl_coeff.p; P.dl_light = ctx->dl_light.p; P.dl_bsdf = ctx->dl_bsdf.p; P.hit_tb = ctx->hit_tb.p;
+    P.hit_ot = ctx->hit_ot.p; P.pk = ctx->pk.p; P.rng = sequential ? ctx->rng.p : nullptr; P.radiance = ctx->radiance.p;
+    P.grng = guided ? ctx->grng.p : nullptr;
+    P.rec = guided ? ctx->rec.p : nullptr;
+    Queues q{};
+    q.active0 = ctx->q_a.p; q.active1 = ctx->q_b.p;
+    q.shadow_path = ctx->sh_path.p; q.shadow_o = ctx->sh_o.p; q.shadow_d = ctx->sh_d.p; q.shadow_c = ctx->sh_c.p;
+    q.mis_path = ctx->mi_path.p; q.mis_o = ctx->mi_o.p; q.mis_d = ctx->mi_d.p; q.mis_f = ctx->mi_f.p;
+    q.mis_ndw = ctx->mi_ndw.p; q.dl_path = ctx->dl_path.p; q.counts = ctx->counts.p; q.totals = ctx->totals.p;
+    if (guided) {
+        q.guide_path = ctx->guide_path.p;
+        q.guide_in = ctx->guide_in.p;
+        q.guide_out = ctx->guide_out.p;
+        q.train = ctx->train.p;
+        q.train_cap = (uint32_t)train_cap;
+    }
     FilmParams f{};
     f.out_film = ctx->out_film.p;
     f.out_var = ctx->out_var.p;
@@ -629,15 +721,25 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     f.side_info = ctx->side_info.p;
     f.side_view = ctx->side_view.p;
     f.side_capacity = (int)side_cap;
+    if (guided) {
+        f.accumulate = 1;
+        f.m1 = ctx->m1.p;
+        f.m2 = ctx->m2.p;
+    }
     LaunchCfg lc = ctx->launch_cfg();
     lc.trav_grid = ctx->cu_count * traversal_blocks_per_cu(lc);
     lc.path_grid = (int)std::max<int64_t>(1, std::min<int64_t>((max_chunk_paths + 255) / 256, (int64_t)ctx->cu_count * 8));
+    if (guided) {
+        lc.guided = true;
+        lc.net = guide_net(gr->guide);
+        lc.guide_grid = ctx->cu_count;
+    }
     const int iters = loop_iterations(rc.max_light_bounces);
-    const bool use_graph = ctx->opt.use_graph != 0;
+    const bool use_graph = ctx->opt.use_graph != 0 && !guided;
     Chunk* slot = ctx->chunk_slot.p;
 
     // polling slots: one per iteration (eager) or per (chunk, pass) completion check (graph)
-    const size_t poll_n = std::max<size_t>(hc.size() + 1, (size_t)iters + 1);
+    const size_t poll_n = std::max<size_t>(hc.size() + 1, (size_t)iters + 1) + 1;
     if (ctx->h_poll_n < poll_n) {
         if (ctx->h_poll) (void)hipHostFree(ctx->h_poll);
         ctx->h_poll = nullptr;
@@ -675,57 +777,87 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     HIPX(hipEventCreate(&e0));
     HIPX(hipEventCreate(&e1));
     HIPX(hipEventRecord(e0, s));
-    size_t hci = 0;
     bool loop_ok = true;
-    for (size_t ci = 0; ci < cks.size(); ci++) {
-        for (int pass = 0; pass < passes_of[ci]; pass++, hci++) {
-            HIPX(hipMemcpyAsync(slot, ctx->chunks.p + hci, sizeof(Chunk), hipMemcpyDeviceToDevice, s));
-            if (pass == 0) {
-                kt.begin(4);
-                launch_sample_table(hc[hci], slot, ctx->table.p, s);
-                kt.end();
+    int64_t train_records = 0;
+    double train_seconds = 0.0, train_nats = 0.0;
+    auto run_pass_loop = [&](const Step& sp) {
+        const Chunk& c = hc[sp.hc];
+        if (use_graph) {
+            HIPX(hipGraphLaunch(ctx->graph, s));
+            // the active list the last iteration wrote must be empty
+            HIPX(hipMemcpyAsync(hp + sp.hc, ctx->counts.p + (iters & 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            return;
+        }
+        kt.begin(4);
+        launch_raygen(slot, c.n_paths, P, q, s);
+        kt.end();
+        // Queue kernels take their counts from device memory; the host reads each iteration's
+        // active count asynchronously and stops enqueueing two iterations after it saw zero.
+        int it = 0;
+        for (; it < iters; it++) {
+            if (it >= 2) {
+                HIPX(hipEventSynchronize(ev[it - 2]));
+                if (hp[it - 2] == 0) break;
             }
-            if (use_graph) {
-                HIPX(hipGraphLaunch(ctx->graph, s));
-                // the active list the last iteration wrote must be empty
-                HIPX(hipMemcpyAsync(hp + hci, ctx->counts.p + (iters & 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-                continue;
+            const int parity = it & 1;
+            if (kt.on) {
+                static const int kind_of_stage[kStageCount] = {4, 0, 1, 5, 5, 2, 3, 4};
+                for (int stage = 0; stage < kStageCount; stage++) {
+                    kt.begin(kind_of_stage[stage]);
+                    launch_stage(stage, ctx->scene, slot, P, q, parity, lc);
+                    kt.end();
+                }
+            } else {
+                launch_iteration(ctx->scene, slot, P, q, parity, lc);
             }
+            HIPX(hipMemcpyAsync(hp + it, ctx->counts.p + (parity ^ 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIPX(hipEventRecord(ev[it], s));
+        }
+        if (it == iters) {
+            HIPX(hipEventSynchronize(ev[it - 1]));
+            if (hp[it - 1] != 0) loop_ok = false;
+        }
+    };
+    auto run_film = [&](const Step& sp) {
+        f.base = cks[sp.chunk].start;
+        kt.begin(4);
+        launch_film(slot, (int)cks[sp.chunk].pixels, P, f, q, s);
+        kt.end();
+    };
+    int table_chunk = -1;
+    for (size_t si = 0; si < steps.size(); si++) {
+        const Step& sp = steps[si];
+        if (guided && (si == 0 || steps[si - 1].pass != sp.pass)) {
+            // a new guided pass: keep about as many of its records as the store holds
+            int64_t pass_samples = 0;
+            const int k0 = gr->passes[sp.pass].first, k1 = gr->passes[sp.pass].second;
+            for (const Unit& u : units) pass_samples += u.count * std::max(0, std::min(k1, dv[u.part].spp) - k0);
+            q.keep_prob = (float)std::min(1.0, (double)train_cap / std::max<double>(1.0, 2.0 * pass_samples));
+        }
+        HIPX(hipMemcpyAsync(slot, ctx->chunks.p + sp.hc, sizeof(Chunk), hipMemcpyDeviceToDevice, s));
+        // a chunk's table covers all its samples; regenerate it only when another chunk used the buffer
+        if (sp.chunk != table_chunk) {
+            table_chunk = sp.chunk;
             kt.begin(4);
-            launch_raygen(slot, hc[hci].n_paths, P, q, s);
+            launch_sample_table(hc[sp.hc], slot, ctx->table.p, s);
             kt.end();
-            // Queue kernels take their counts from device memory; the host reads each iteration's
-            // active count asynchronously and stops enqueueing two iterations after it saw zero.
-            int it = 0;
-            for (; it < iters; it++) {
-                if (it >= 2) {
-                    HIPX(hipEventSynchronize(ev[it - 2]));
-                    if (hp[it - 2] == 0) break;
-                }
-                const int parity = it & 1;
-                if (kt.on) {
-                    static const int kind_of_stage[kStageCount] = {4, 0, 1, 2, 3, 4};
-                    for (int stage = 0; stage < kStageCount; stage++) {
-                        kt.begin(kind_of_stage[stage]);
-                        launch_stage(stage, ctx->scene, slot, P, q, parity, lc);
-                        kt.end();
-                    }
-                } else {
-                    launch_iteration(ctx->scene, slot, P, q, parity, lc);
-                }
-                HIPX(hipMemcpyAsync(hp + it, ctx->counts.p + (parity ^ 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-                HIPX(hipEventRecord(ev[it], s));
-            }
-            if (it == iters) {
-                HIPX(hipEventSynchronize(ev[it - 1]));
-                if (hp[it - 1] != 0) loop_ok = false;
+        }
+        run_pass_loop(sp);
+        HIPX(hipGetLastError());
+        const bool last_of_chunk = guided || si + 1 == steps.size() || steps[si + 1].chunk != sp.chunk;
+        if (last_of_chunk) run_film(sp);
+        if (guided && (si + 1 == steps.size() || steps[si + 1].pass != sp.pass)) {
+            // end of a guided pass: train the proposal on the records it produced
+            HIPX(hipMemcpyAsync(hp + poll_n - 1, ctx->counts.p + 10, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIPX(hipStreamSynchronize(s));
+            const int64_t n_rec = std::min<int64_t>(hp[poll_n - 1], train_cap);
+            train_records = hp[poll_n - 1];
+            if (sp.pass + 1 < n_passes && n_rec > 0) {
+                const auto t0 = std::chrono::steady_clock::now();
+                train_nats = guide_train_on(gr->guide, ctx->train.p, n_rec, guide_config(gr->guide).steps_per_pass, s);
+                train_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             }
         }
-        HIPX(hipGetLastError());
-        f.base = cks[ci].start;
-        kt.begin(4);
-        launch_film(slot, (int)cks[ci].pixels, P, f, q, s);
-        kt.end();
     }
     HIPX(hipEventRecord(e1, s));
     std::vector<float4> film_own(total_pixels);
@@ -744,7 +876,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         for (size_t i = 0; i < hc.size(); i++)
             if (hp[i] != 0) loop_ok = false;
     if (!loop_ok) throw HipError{"bounce loop did not terminate within max_light_bounces + 2 iterations"};
-    double ksec[5];
+    double ksec[kTimerKinds];
     kt.collect(ksec);
     const uint32_t n_side = cnt[5];
     if ((int64_t)n_side > side_cap) throw HipError{"film side list overflow"};
@@ -840,6 +972,11 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     st.shadow_seconds = ksec[2];
     st.mis_seconds = ksec[3];
     st.other_seconds = ksec[4];
+    st.guide_seconds = ksec[5];
+    st.train_seconds = train_seconds;
+    st.train_nats = train_nats;
+    st.train_records = train_records;
+    st.passes = guided ? (int64_t)gr->passes.size() : 1;
     st.render_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
 }
 
@@ -1073,6 +1210,39 @@ int yar_gpu_render_batch(yar_gpu_ctx* ctx, const int32_t* views, int32_t count, 
         return fail(ctx, YAR_ERR_HIP, "yar_gpu_render_batch: " + e.msg);
     } catch (const std::bad_alloc&) {
         return fail(ctx, YAR_ERR_OUT_OF_MEMORY, "yar_gpu_render_batch: out of host memory");
+    }
+    if (out_stats) *out_stats = st;
+    return YAR_OK;
+}
+
+int yar_gpu_render_guided(yar_gpu_ctx* ctx, yar_guide* guide, const int32_t* views, int32_t count,
+                          float* const* out_images, double* out_variances, yar_render_stats* out_stats) {
+    if (!ctx || !guide || !views || count <= 0)
+        return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_guided: invalid argument");
+    if (guide_device(guide) != ctx->device)
+        return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_guided: guide lives on another device");
+    std::vector<JobPart> parts(count);
+    for (int i = 0; i < count; i++) {
+        int32_t n = 0;
+        if (yar_film_tile_count(ctx, views[i], &n) != YAR_OK)
+            return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_guided: invalid view index");
+        parts[i].view = views[i];
+        for (int t = 0; t < n; t++) parts[i].tiles.push_back(t);
+    }
+    std::vector<PartResult> rr;
+    yar_render_stats st{};
+    GuidedRun gr;
+    gr.guide = guide;
+    try {
+        HIPX(hipSetDevice(ctx->device));
+        render_job(ctx, parts, rr, st, &gr);
+        for (int i = 0; i < count; i++)
+            merge_view(ctx->views[views[i]], rr[i], nullptr, out_images ? out_images[i] : nullptr,
+                       out_variances ? &out_variances[i] : nullptr);
+    } catch (const HipError& e) {
+        return fail(ctx, YAR_ERR_HIP, "yar_gpu_render_guided: " + e.msg);
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, YAR_ERR_OUT_OF_MEMORY, "yar_gpu_render_guided: out of host memory");
     }
     if (out_stats) *out_stats = st;
     return YAR_OK;

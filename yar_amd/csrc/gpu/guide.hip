@@ -73,9 +73,10 @@ YG_INLINE void hidden_epilogue(const mfma_cd* acc, const float* bias, uint16_t* 
 }
 
 __global__ __launch_bounds__(256) void k_guide_forward(GuideNet net, const float* __restrict__ in, int in_stride,
-                                                       long long n, float* __restrict__ out, int out_stride,
-                                                       int n_write, GuideActs acts, int store) {
+                                                       long long n, const uint32_t* n_dev, float* __restrict__ out,
+                                                       int out_stride, int n_write, GuideActs acts, int store) {
     extern __shared__ __align__(16) uint16_t lds[];
+    if (n_dev) n = *n_dev;
     uint16_t* W = lds;
     float* bias = reinterpret_cast<float*>(lds + kFwdW);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -116,7 +117,8 @@ __global__ __launch_bounds__(256) void k_guide_forward(GuideNet net, const float
                 float xi = x[0];
 #pragma unroll
                 for (int q = 1; q < kGIn; q++) xi = (i == q) ? x[q] : xi;
-                const float tt = ldexpf(xi, k);
+                const float fk = ldexpf((k & 1) ? 1.41421356f : 1.f, k >> 1);   // 2^(k/2)
+                const float tt = xi * fk;
                 const float fr = tt - floorf(tt);                    // revolutions in [0, 1)
                 const uint16_t bs = f2bf(__builtin_amdgcn_sinf(fr)), bc = f2bf(__builtin_amdgcn_cosf(fr));
                 row[kGIn + 2 * p] = bs;
@@ -164,8 +166,19 @@ void guide_forward(const GuideNet& net, const float* in, int in_stride, long lon
     if (acts) a = *acts;
     const long long tiles = (n + 63) / 64;
     const int blocks = (int)std::min<long long>(tiles, grid);
-    hipLaunchKernelGGL(k_guide_forward, dim3(blocks), dim3(256), kFwdLds, s, net, in, in_stride, n, out, out_stride,
-                       n_write, a, acts ? 1 : 0);
+    hipLaunchKernelGGL(k_guide_forward, dim3(blocks), dim3(256), kFwdLds, s, net, in, in_stride, n,
+                       (const uint32_t*)nullptr, out, out_stride, n_write, a, acts ? 1 : 0);
+}
+
+void guide_forward_dev(const GuideNet& net, const float* in, int in_stride, const uint32_t* n_dev, float* out,
+                       int out_stride, int grid, hipStream_t s) {
+    GuideActs a{};
+    hipLaunchKernelGGL(k_guide_forward, dim3(grid), dim3(256), kFwdLds, s, net, in, in_stride, 0ll, n_dev, out,
+                       out_stride, out_stride, a, 0);
+}
+
+void guide_prepare_device() {
+    (void)hipFuncSetAttribute((const void*)k_guide_forward, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFwdLds);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -252,7 +265,7 @@ __global__ __launch_bounds__(256) void k_guide_backward(GuideNet net, const Guid
     const long long n_btiles = (n + 63) / 64;
     for (long long bt = blockIdx.x; bt < n_btiles; bt += gridDim.x) {
         const long long m0 = bt * 64 + wave * 16;
-        float loss = 0.f;
+        float loss = 0.f, wsum = 0.f;
         if (g == 0) {
             const long long m = m0 + r;
             float d[kGOut];
@@ -262,6 +275,7 @@ __global__ __launch_bounds__(256) void k_guide_backward(GuideNet net, const Guid
                 float o[kGOut];
                 for (int j = 0; j < kGOut; j++) o[j] = acts.o[m * kGOutPad + j];
                 if (R.wt != 0.f) loss = loss_grad(o, v3(R.w[0], R.w[1], R.w[2]), R.wt, inv_batch, d);
+                wsum = R.wt * inv_batch;
             }
             uint16_t* d5 = acts.d[4] + ((size_t)(m / 4) * kGOutPad) * 4 + (m & 3);
             for (int j = 0; j < kGOutPad; j++) {
@@ -270,9 +284,13 @@ __global__ __launch_bounds__(256) void k_guide_backward(GuideNet net, const Guid
                 d5[(size_t)j * 4] = b;
             }
         }
-        if (loss_sum) {
-            for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
-            if (lane == 0 && loss != 0.f) atomicAdd(loss_sum, loss);
+        if (loss_sum) {                          // [0] batch loss, [1] batch mean weight
+            for (int off = 32; off > 0; off >>= 1) {
+                loss += __shfl_xor(loss, off);
+                wsum += __shfl_xor(wsum, off);
+            }
+            if (lane == 0 && loss != 0.f) atomicAdd(&loss_sum[0], loss);
+            if (lane == 0 && wsum != 0.f) atomicAdd(&loss_sum[1], wsum);
         }
         __syncthreads();
         backward_layer<3>(Da, W + kBwdOff[4], kBwdStride[4], acts.h[4], Db, acts.d[3], r, g, m0);
@@ -540,22 +558,33 @@ struct yar_guide {
     }
 
     // forward + backward + weight gradients of records rec[0..n) (n <= batch capacity)
-    void gradients(const GuideRecord* rec, long long n, float inv_batch, bool want_loss) {
+    void gradients(const GuideRecord* rec, long long n, float inv_batch, bool want_loss, hipStream_t s) {
         GuideActs a = acts(n);
         const long long np = (n + 63) / 64 * 64;
-        guide_forward(net(), reinterpret_cast<const float*>(rec), 16, n, o.p, kGOutPad, 0, &a, cu_count, stream);
-        if (want_loss) GHIP(hipMemsetAsync(loss.p, 0, sizeof(float), stream));
-        guide_backward(net(), rec, n, inv_batch, a, want_loss ? loss.p : nullptr, stream);
-        guide_weight_grads(a, np, 1024, partial.p, stream);
+        guide_forward(net(), reinterpret_cast<const float*>(rec), 16, n, o.p, kGOutPad, 0, &a, cu_count, s);
+        if (want_loss) GHIP(hipMemsetAsync(loss.p, 0, 2 * sizeof(float), s));
+        guide_backward(net(), rec, n, inv_batch, a, want_loss ? loss.p : nullptr, s);
+        guide_weight_grads(a, np, 1024, partial.p, s);
     }
 
     int slices(long long n) const { return (int)(((n + 63) / 64 * 64 + 1023) / 1024); }
 
-    void adam_step(long long n) {
+    void adam_step(long long n, hipStream_t s) {
         step++;
         const float bc1 = 1.f - std::pow(cfg.beta1, (float)step), bc2 = 1.f - std::pow(cfg.beta2, (float)step);
         guide_apply(master.p, adam_m.p, adam_v.p, partial.p, slices(n), cfg.learning_rate, cfg.beta1, cfg.beta2,
-                    cfg.epsilon, bc1, bc2, wt.p, wr.p, nullptr, stream);
+                    cfg.epsilon, bc1, bc2, wt.p, wr.p, nullptr, s);
+    }
+
+    // `steps` Adam steps on batches drawn from device records rec[0..count), on stream s
+    void train_records(const GuideRecord* rec, long long count, int steps, hipStream_t s, bool want_loss) {
+        const long long B = cfg.batch_size;
+        acts(B);
+        for (int k = 0; k < steps; k++) {
+            guide_gather(rec, count, B, cfg.seed, (uint32_t)step, batch.p, s);
+            gradients(batch.p, B, 1.f / (float)B, want_loss && k == steps - 1, s);
+            adam_step(B, s);
+        }
     }
 };
 
@@ -623,7 +652,7 @@ int yar_guide_create(int32_t device, const yar_guide_config* config, yar_guide**
         g->adam_v.reserve(kGParams);
         g->wt.reserve(kGWeights);
         g->wr.reserve(kGWeights);
-        g->loss.reserve(1);
+        g->loss.reserve(2);
         // He-uniform initialisation for the ReLU layers, a 0.1x smaller output layer so the initial
         // mixture is close to 8 broad lobes of equal weight; biases zero.
         std::vector<float> p(kGParams, 0.f);
@@ -719,7 +748,7 @@ int yar_guide_gradients(yar_guide* g, const float* x, const float* dirs, const f
         std::vector<GuideRecord> r = pack_records(x, dirs, wt, n);
         g->acts(n);
         GHIP(hipMemcpyAsync(g->batch.p, r.data(), r.size() * sizeof(GuideRecord), hipMemcpyHostToDevice, g->stream));
-        g->gradients(g->batch.p, n, 1.f / (float)n, true);
+        g->gradients(g->batch.p, n, 1.f / (float)n, true, g->stream);
         g->q_out.reserve(kGParams);
         guide_apply(g->master.p, g->adam_m.p, g->adam_v.p, g->partial.p, g->slices(n), 0.f, 0.f, 0.f, 0.f, 1.f, 1.f,
                     g->wt.p, g->wr.p, g->q_out.p, g->stream);
@@ -744,14 +773,8 @@ int yar_guide_train(yar_guide* g, const float* x, const float* dirs, const float
         std::vector<GuideRecord> r = pack_records(x, dirs, wt, n);
         g->samples.reserve((size_t)n);
         GHIP(hipMemcpyAsync(g->samples.p, r.data(), r.size() * sizeof(GuideRecord), hipMemcpyHostToDevice, g->stream));
-        const long long B = g->cfg.batch_size;
-        g->acts(B);
         float l = 0.f;
-        for (int s = 0; s < steps; s++) {
-            guide_gather(g->samples.p, n, B, g->cfg.seed, (uint32_t)g->step, g->batch.p, g->stream);
-            g->gradients(g->batch.p, B, 1.f / (float)B, s == steps - 1);
-            g->adam_step(B);
-        }
+        g->train_records(g->samples.p, n, steps, g->stream, true);
         GHIP(hipGetLastError());
         if (steps > 0) GHIP(hipMemcpyAsync(&l, g->loss.p, sizeof(float), hipMemcpyDeviceToHost, g->stream));
         GHIP(hipStreamSynchronize(g->stream));
@@ -799,3 +822,20 @@ int yar_guide_sample(yar_guide* g, const float* x, const float* u, int64_t n, fl
 }
 
 }  // extern "C"
+
+namespace yg {
+
+GuideNet guide_net(yar_guide* g) { return g->net(); }
+const yar_guide_config& guide_config(const yar_guide* g) { return g->cfg; }
+int guide_device(const yar_guide* g) { return g->device; }
+float guide_train_on(yar_guide* g, const GuideRecord* rec, long long count, int steps, hipStream_t s) {
+    g->train_records(rec, count, steps, s, true);
+    float l[2] = {0.f, 0.f};
+    GHIP(hipMemcpyAsync(l, g->loss.p, sizeof(l), hipMemcpyDeviceToHost, s));
+    GHIP(hipStreamSynchronize(s));
+    // mean log-likelihood gain over the uniform sphere density on the last batch, in nats per
+    // unit weight: (w log 4 pi - loss) / w
+    return l[1] > 0.f ? (l[1] * std::log(4.f * kPi) - l[0]) / l[1] : 0.f;
+}
+
+}  // namespace yg

@@ -2,7 +2,8 @@
 // mixture the network predicts.
 //
 // Network: 9 inputs (position in scene bounds, outgoing direction, shading normal, all in [0,1])
-// -> frequency encoding (raw inputs + sin/cos of 2*pi*frac(x*2^k), k < 16) = 297 features padded
+// -> frequency encoding (raw inputs + sin/cos of 2*pi*frac(x*f_k), f_k = 2^(k/2), k < 16, half-octave
+// steps from 1 to 181 cycles over the input range) = 297 features padded
 // to 304 -> 4 hidden layers of 64 (ReLU) -> 48 outputs, of which 40 parameterise 8 lobes
 // (logit, unnormalised mean direction, log concentration).  Weights are bf16, accumulation is fp32
 // on MFMA (v_mfma_f32_16x16x16_bf16), biases stay fp32.  The renderer has no counterpart in
@@ -10,6 +11,7 @@
 #pragma once
 
 #include "device.hpp"
+#include "../../../include/yar_gpu.h"
 
 namespace yg {
 
@@ -107,19 +109,28 @@ YG_INLINE void onb(V3 n, V3& t, V3& b) {
     b = v3(bb, s + n.y * n.y * a, -n.y);
 }
 
-// Picks a lobe with u_lobe and inverts the vMF CDF with (u1, u2).
+// Picks a lobe with u_lobe and inverts the vMF CDF with (u1, u2).  The lobe loop is unrolled with
+// selects so the mixture stays in registers.
 YG_INLINE V3 mixture_sample(const VmfLobe* L, float u_lobe, float u1, float u2) {
-    int j = 0;
-    float c = L[0].pi;
-    while (j < kGLobes - 1 && u_lobe >= c) { j++; c += L[j].pi; }
-    const float k = L[j].kappa;
+    V3 mu = L[kGLobes - 1].mu;
+    float k = L[kGLobes - 1].kappa;
+    float c = 0.f;
+    bool chosen = false;
+#pragma unroll
+    for (int j = 0; j < kGLobes - 1; j++) {
+        c += L[j].pi;
+        const bool take = !chosen && u_lobe < c;
+        mu = take ? L[j].mu : mu;
+        k = take ? L[j].kappa : k;
+        chosen = chosen || take;
+    }
     float wz = 1.f + log1pf(u1 * expm1f(-2.f * k)) / k;
     wz = fminf(fmaxf(wz, -1.f), 1.f);
     const float r = sqrtf(fmaxf(0.f, 1.f - wz * wz));
     const float phi = 2.f * kPi * u2;
     V3 t, b;
-    onb(L[j].mu, t, b);
-    return wz * L[j].mu + (r * cosf(phi)) * t + (r * sinf(phi)) * b;
+    onb(mu, t, b);
+    return wz * mu + (r * cosf(phi)) * t + (r * sinf(phi)) * b;
 }
 
 // Network inputs at a shading point.
@@ -136,6 +147,10 @@ YG_INLINE void guide_inputs(V3 p, V3 lo, V3 inv_ext, V3 wo, V3 n, float* x) {
 // a training step.
 void guide_forward(const GuideNet& net, const float* in, int in_stride, long long n, float* out, int out_stride,
                    int n_write, const GuideActs* acts, int grid, hipStream_t s);
+// Same with the query count read from device memory (graph-capturable, persistent grid).
+void guide_forward_dev(const GuideNet& net, const float* in, int in_stride, const uint32_t* n_dev, float* out,
+                       int out_stride, int grid, hipStream_t s);
+void guide_prepare_device();
 size_t guide_forward_lds();
 void guide_backward(const GuideNet& net, const GuideRecord* rec, long long n, float inv_batch, const GuideActs& acts,
                     float* loss_sum, hipStream_t s);
@@ -143,6 +158,13 @@ void guide_weight_grads(const GuideActs& acts, long long n, int slice, float* pa
 void guide_apply(float* master, float* m, float* v, const float* partial, int n_slices, float lr, float b1, float b2,
                  float eps, float bc1, float bc2, uint16_t* wt, uint16_t* wr, float* grad_out, hipStream_t s);
 void guide_sync_weights(const float* master, uint16_t* wt, uint16_t* wr, hipStream_t s);
+// Render-context access to a yar_guide (guide.hip): its network, configuration and device, and
+// `steps` Adam steps on device-resident records, launched on the caller's stream.
+GuideNet guide_net(yar_guide* g);
+const yar_guide_config& guide_config(const yar_guide* g);
+int guide_device(const yar_guide* g);
+// Returns the last batch's mean log-likelihood gain over a uniform density (nats per unit weight).
+float guide_train_on(yar_guide* g, const GuideRecord* rec, long long count, int steps, hipStream_t s);
 void guide_gather(const GuideRecord* all, long long count, long long batch, uint32_t seed, uint32_t step,
                   GuideRecord* out, hipStream_t s);
 

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "device.hpp"
+#include "guide.hpp"
 
 namespace yg {
 
@@ -34,17 +35,28 @@ struct Paths {
     uint2* pk;           // chunk pixel and sample index of the path
     uint64_t* rng;       // per-pixel fallback stream (sequential sample mode only)
     float4* radiance;    // final per-sample radiance of the chunk, at pix_first[pixel] + sample
+    // guided form only
+    uint64_t* grng;      // per-path stream: strategy choice, mixture sampling, fallback dimensions
+    float* rec;          // kRecMax training vertices per path, kRecFloats each
 };
+
+// Training vertex of a guided path: network inputs, sampled direction, throughput before the
+// continuation and the path radiance after the vertex's direct lighting.
+constexpr int kRecMax = 4;
+constexpr int kRecFloats = 18;     // x[9], w[3], beta[3], L[3]
 
 enum : uint32_t {
     kDelta = 1u << 24,        // current ray was produced by a delta bounce (path_tracing.cpp:104)
     kPendingDL = 1u << 25,    // direct-lighting halves must be added by the resolve pass
     kDone = 1u << 26,         // path finished; finalize after the resolve pass
+    kRecShift = 27,           // bits 27..29: training vertices recorded so far (guided form)
+    kRecPending = 1u << 30,   // the newest vertex still needs its post-direct-lighting radiance
 };
 
 // Work queues.  The active path list ping-pongs between two buffers; `parity` selects the input.
 // counts: [0], [1] active lists, [2] shadow, [3] mis, [4] dl, [5] film side entries,
-//         [6] [7] [8] work-fetch cursors of extend, shadow and mis.
+//         [6] [7] [8] work-fetch cursors of extend, shadow and mis, [9] guide queries,
+//         [10] training records.
 struct Queues {
     uint32_t* active0;
     uint32_t* active1;
@@ -52,6 +64,13 @@ struct Queues {
     uint32_t* mis_path; float4* mis_o; float4* mis_d; float4* mis_f; float* mis_ndw;
     uint32_t* dl_path;
     uint32_t* counts;
+    // guided form: continuation queries ([9] counts them) and training records ([10])
+    uint32_t* guide_path;
+    float* guide_in;              // 16 floats per query: inputs x[9], rec index, BSDF u[2], RR u
+    float* guide_out;             // kGOut network outputs per query
+    GuideRecord* train;
+    uint32_t train_cap;
+    float keep_prob;
     // [0..15] traversal counters (4 per kernel kind), [16] extend rays, [17] shadow rays, [18] mis rays,
     // [19] non-empty extend launches
     unsigned long long* totals;
@@ -71,6 +90,9 @@ struct Chunk {
     int pass, sequential;           // sequential: one sample per pixel per pass (RNG fallback)
     int max_bounces, rr_bounce_threshold;
     float rr_threshold, film_scale, max_component;
+    // guided form: this pass traces samples [k0, k1) of every pixel, slot = pixel * (k1 - k0) + j
+    int guided, k0, k1;
+    float alpha;                    // one-sample MIS probability of the learned mixture
 };
 
 // Film accumulation of one chunk (every sample's footprint as Film_Tile::add_sample computes it).
@@ -84,6 +106,10 @@ struct FilmParams {
     int4* side_info;          // x, y, sample, packed footprint (p0x-x+1 | (p1x-x)<<2 | (p0y-y+1)<<4 | (p1y-y)<<6)
     int* side_view;           // view of the side entry
     int side_capacity;
+    // guided passes: add the pass's samples to out_film and accumulate luminance moments
+    int accumulate;
+    double* m1;
+    double* m2;
 };
 
 struct LaunchCfg {
@@ -94,12 +120,18 @@ struct LaunchCfg {
     int trav_grid;            // persistent traversal grid (blocks)
     int path_grid;            // grid-stride grid of per-path kernels (blocks of 256)
     hipStream_t stream;
+    bool guided;              // launch the guide stages
+    GuideNet net;
+    int guide_grid;
 };
 
 void launch_sample_table(const Chunk& host_chunk, const Chunk* dev_chunk, float* table, hipStream_t s);
 void launch_raygen(const Chunk* c, long long n_paths_max, const Paths& p, Queues q, hipStream_t s);
 // Stages of one bounce-loop iteration, in launch order.
-enum IterStage { kStageReset, kStageExtend, kStageShade, kStageShadow, kStageMis, kStageResolve, kStageCount };
+enum IterStage {
+    kStageReset, kStageExtend, kStageShade, kStageGuideForward, kStageGuideSample, kStageShadow, kStageMis,
+    kStageResolve, kStageCount
+};
 void launch_stage(int stage, const DevScene& sc, const Chunk* c, const Paths& p, Queues q, int parity, const LaunchCfg& cfg);
 void launch_iteration(const DevScene& sc, const Chunk* c, const Paths& p, Queues q, int parity, const LaunchCfg& cfg);
 void launch_film(const Chunk* c, int n_pixels, const Paths& p, FilmParams f, Queues q, hipStream_t s);

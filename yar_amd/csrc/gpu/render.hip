@@ -58,6 +58,22 @@ YG_INLINE uint32_t stream_id(int2 px, uint32_t seed_offset) {     // reference_r
     return stream + seed_offset;
 }
 
+// Guided form: a pcg32 stream per (pixel, sample), disjoint from the pixel streams (whose upper
+// 32 sequence bits are zero), for strategy choices, mixture sampling and fallback dimensions.
+YG_INLINE uint64_t guide_stream(const Chunk& c, uint2 pk) {
+    const DevView& v = c.views[c.pix_view[pk.x]];
+    return ((uint64_t)(pk.y + 1) << 32) | stream_id(c.pixels[pk.x], v.seed_offset);
+}
+YG_INLINE uint64_t guide_inc(const Chunk& c, uint2 pk) { return (guide_stream(c, pk) << 1u) | 1u; }
+
+YG_INLINE float luminance(RGB c) { return 0.2126729f * c.r + 0.7151522f * c.g + 0.0721750f * c.b; }   // color.cpp:21
+
+YG_INLINE void scene_frame(const DevScene& s, V3& lo, V3& inv_ext) {
+    lo = from4(s.scene_lo);
+    const V3 e = from4(s.scene_hi) - lo;
+    inv_ext = v3(e.x > 0.f ? 1.f / e.x : 0.f, e.y > 0.f ? 1.f / e.y : 0.f, e.z > 0.f ? 1.f / e.z : 0.f);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Sample tables (Stratified_Pixel_Sampler::next_pixel, pixel_sampling.cpp:54-85;
 // generate_stratified_sequence_1d/2d, sampling.cpp:79-96; shuffle, math.h:138-144).
@@ -141,7 +157,12 @@ __global__ void k_raygen(const Chunk* __restrict__ cp, Paths P, Queues q) {
     bool active = false;
     int p = 0, k = 0;
     if (s < c.n_paths) {
-        if (c.sequential) {
+        if (c.guided) {
+            const int S = c.k1 - c.k0;
+            p = (int)(s / S);
+            k = c.k0 + (int)(s % S);
+            active = k < c.views[c.pix_view[p]].spp;
+        } else if (c.sequential) {
             p = (int)s;
             k = c.pass;
             active = k < c.views[c.pix_view[p]].spp;
@@ -176,6 +197,10 @@ __global__ void k_raygen(const Chunk* __restrict__ cp, Paths P, Queues q) {
         P.L[s] = make_float4(0.f, 0.f, 0.f, 0.f);
         P.state[s] = 0u;
         P.pk[s] = make_uint2((uint32_t)p, (uint32_t)k);
+        if (c.guided) {
+            uint64_t ginc;
+            P.grng[s] = pcg_seed_state(guide_stream(c, make_uint2((uint32_t)p, (uint32_t)k)), ginc);
+        }
         if (c.sequential && c.pass == 0) {
             // fallback stream: the pixel's RNG after next_pixel() drew every table value
             uint64_t inc;
@@ -189,9 +214,10 @@ __global__ void k_raygen(const Chunk* __restrict__ cp, Paths P, Queues q) {
     if (active) q.active0[idx] = (uint32_t)s;
 }
 
-// Clears the queue counters and cursors before a pass; the film side-list counter [5] runs on.
+// Clears the queue counters and cursors before a pass; the film side-list counter [5] and the
+// training-record counter [10] run on.
 __global__ void k_zero_counts(uint32_t* counts, int n) {
-    if ((int)threadIdx.x < n && threadIdx.x != 5) counts[threadIdx.x] = 0u;
+    if ((int)threadIdx.x < n && threadIdx.x != 5 && threadIdx.x != 10) counts[threadIdx.x] = 0u;
 }
 
 void launch_raygen(const Chunk* c, long long n_paths_max, const Paths& p, Queues q, hipStream_t s) {
@@ -314,6 +340,29 @@ __global__ void k_mis(DevScene sc, Paths P, Queues q, StackCfg cfg) {
 // ---------------------------------------------------------------------------------------------
 // Shade: one iteration of trace_path's loop body (path_tracing.cpp:22-131) for every active path
 // ---------------------------------------------------------------------------------------------
+// Guided form: one training record per recorded vertex, weighted by the luminance of the radiance
+// its continuation brought in divided by the throughput at the vertex (= f Li cos / p_sample).
+YG_INLINE void emit_records(const Paths& P, const Queues& q, uint32_t slot, int nrec, RGB L, uint64_t& rs,
+                            uint64_t rinc) {
+    for (int r = 0; r < nrec; r++) {
+        const float* R = P.rec + ((size_t)slot * kRecMax + r) * kRecFloats;
+        const RGB d = L - rgb(R[15], R[16], R[17]);
+        const RGB ratio = rgb(R[12] > 0.f ? d.r / R[12] : 0.f, R[13] > 0.f ? d.g / R[13] : 0.f,
+                              R[14] > 0.f ? d.b / R[14] : 0.f);
+        const float wt = luminance(ratio);
+        if (!(wt > 0.f) || !(wt < f_inf())) continue;
+        if (pcg_float(rs, rinc) >= q.keep_prob) continue;
+        // ring buffer over the whole render: training sees every pass's records, newest kept
+        const uint32_t idx = atomicAdd(&q.counts[10], 1u) % q.train_cap;
+        GuideRecord g;
+        for (int j = 0; j < 9; j++) g.x[j] = R[j];
+        g.wt = wt;
+        g.w[0] = R[9]; g.w[1] = R[10]; g.w[2] = R[11];
+        g.pad[0] = g.pad[1] = g.pad[2] = 0.f;
+        q.train[idx] = g;
+    }
+}
+
 YG_INLINE void finalize(const Chunk& c, const Paths& P, uint2 pk, RGB L) {          // render_tile :386-393
     float mc = fmaxf(L.r, fmaxf(L.g, L.b));
     if (mc > c.max_component) L = L * (c.max_component / mc);
@@ -331,7 +380,8 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
     for (uint32_t blk = blockIdx.x * blockDim.x; blk < n; blk += stride) {
         const uint32_t i = blk + threadIdx.x;
         const bool valid = i < n;
-        bool want_active = false, want_shadow = false, want_mis = false, want_dl = false;
+        bool want_active = false, want_shadow = false, want_mis = false, want_dl = false, want_guide = false;
+        float4 g_x0, g_x1, g_x2, g_x3;
         uint32_t slot = 0;
         V3 sh_o = v3(0, 0, 0), sh_d = v3(0, 0, 0), mi_o = v3(0, 0, 0), mi_d = v3(0, 0, 0);
         float sh_tmax = 0.f, mi_pdf = 0.f, mi_ndw = 0.f, mi_lndw = 0.f;
@@ -356,10 +406,14 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
             const float* tab = c.table + c.pix_first[p] * c.dps;
             const bool seq = c.sequential != 0;
             uint64_t rs = 0, rinc = 0;
-            if (seq) {
+            if (c.guided) {
+                rinc = guide_inc(c, pk);
+                rs = P.grng[slot];
+            } else if (seq) {
                 rinc = ((uint64_t)stream_id(c.pixels[p], v.seed_offset) << 1u) | 1u;
                 rs = P.rng[slot];
             }
+            int nrec = (int)((st >> kRecShift) & 7u);
             auto next1 = [&]() -> float {                               // pixel_sampling.cpp:159
                 if (c1 < c.d1) { float x = tab[2 * spp + c1 * spp + k]; c1++; return x; }
                 return pcg_float(rs, rinc);
@@ -501,6 +555,54 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
                     bounce++;
                     if (bounce == c.max_bounces) {
                         finished = true;
+                    } else if (c.guided) {
+                        // guided continuation: record the vertex, then either sample the BSDF here
+                        // (untrained pass) or queue a network query for the guide-sample stage
+                        float x[kGIn];
+                        V3 lo, inv_ext;
+                        scene_frame(s, lo, inv_ext);
+                        guide_inputs(sc.pos, lo, inv_ext, sc.wo, sc.n, x);
+                        int ridx = -1;
+                        if (nrec < kRecMax) {
+                            ridx = nrec++;
+                            float* R = P.rec + ((size_t)slot * kRecMax + ridx) * kRecFloats;
+                            for (int j = 0; j < kGIn; j++) R[j] = x[j];
+                            R[9] = R[10] = R[11] = 0.f;
+                            R[12] = coeff.r; R[13] = coeff.g; R[14] = coeff.b;
+                            flags |= kRecPending;
+                        }
+                        const float u_term = bounce >= c.rr_bounce_threshold ? next1() : -1.f;
+                        if (c.alpha > 0.f) {
+                            want_guide = true;
+                            g_x0 = make_float4(x[0], x[1], x[2], x[3]);
+                            g_x1 = make_float4(x[4], x[5], x[6], x[7]);
+                            g_x2 = make_float4(x[8], (float)ridx, unx, uny);
+                            g_x3 = make_float4(u_term, 0.f, 0.f, 0.f);
+                        } else {
+                            V3 wi = v3(0.f, 0.f, 0.f);
+                            float pdf = 0.f;
+                            const RGB f = bsdf_sample(m, sc, unx, uny, sc.wo, wi, pdf);
+                            if (is_black(f)) {
+                                finished = true;
+                            } else {
+                                coeff = coeff * (f * (fabsf(dot(sc.n, wi)) / pdf));
+                                P.ray_o[slot] = f4(origin_dir(sc, wi), 0.f);
+                                P.ray_d[slot] = f4(wi, 0.f);
+                                if (ridx >= 0) {
+                                    float* R = P.rec + ((size_t)slot * kRecMax + ridx) * kRecFloats;
+                                    R[9] = wi.x; R[10] = wi.y; R[11] = wi.z;
+                                }
+                                if (u_term >= 0.f) {
+                                    const float mc = fmaxf(coeff.r, fmaxf(coeff.g, coeff.b));
+                                    if (mc < c.rr_threshold) {
+                                        const float pt = fmaxf(0.05f, 1.f - mc);
+                                        if (u_term < pt) finished = true;
+                                        else coeff = coeff * (1.f / (1.f - pt));
+                                    }
+                                }
+                                if (!finished) want_active = true;
+                            }
+                        }
                     } else {
                         V3 wi = v3(0.f, 0.f, 0.f);
                         float pdf = 0.f;
@@ -526,11 +628,16 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
                 }
             }
             if (pending_dl) { flags |= kPendingDL; want_dl = true; if (finished) flags |= kDone; }
-            else if (finished) finalize(c, P, pk, L);
-            P.state[slot] = (uint32_t)bounce | ((uint32_t)c1 << 8) | ((uint32_t)c2 << 16) | flags;
+            else if (finished) {
+                if (c.guided) emit_records(P, q, slot, nrec, L, rs, rinc);
+                finalize(c, P, pk, L);
+            }
+            P.state[slot] = (uint32_t)bounce | ((uint32_t)c1 << 8) | ((uint32_t)c2 << 16) | flags |
+                            ((uint32_t)nrec << kRecShift);
             P.coeff[slot] = f4(coeff, 0.f);
             P.L[slot] = f4(L, 0.f);
-            if (seq) P.rng[slot] = rs;
+            if (c.guided) P.grng[slot] = rs;
+            else if (seq) P.rng[slot] = rs;
         }
         const uint32_t ia = wave_append(n_out, want_active);
         const uint32_t is = wave_append(&q.counts[2], want_shadow);
@@ -551,6 +658,91 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
             q.mis_ndw[im] = mi_ndw;
         }
         if (want_dl) q.dl_path[id] = slot;
+        if (c.guided) {
+            const uint32_t ig = wave_append(&q.counts[9], want_guide);
+            if (want_guide) {
+                q.guide_path[ig] = slot;
+                float4* gi = reinterpret_cast<float4*>(q.guide_in + (size_t)ig * 16);
+                gi[0] = g_x0; gi[1] = g_x1; gi[2] = g_x2; gi[3] = g_x3;
+            }
+        }
+    }
+}
+
+// Guided continuation (one-sample MIS of the learned mixture and the BSDF, Veach's balance-weight
+// one-sample model): with probability alpha the direction comes from the mixture, otherwise from
+// the BSDF with the path's own sampler dimensions; the weight is f |n.w| / (alpha q + (1-alpha) p_bsdf)
+// either way, so the estimator stays unbiased for any network output.
+__global__ void k_guide_sample(DevScene s, const Chunk* __restrict__ cp, Paths P, Queues q, int parity) {
+    const Chunk c = *cp;
+    uint32_t* out = parity ? q.active0 : q.active1;
+    uint32_t* n_out = &q.counts[parity ^ 1];
+    const uint32_t n = q.counts[9];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t blk = blockIdx.x * blockDim.x; blk < n; blk += stride) {
+        const uint32_t i = blk + threadIdx.x;
+        bool alive = false;
+        uint32_t slot = 0;
+        if (i < n) {
+            slot = q.guide_path[i];
+            const float4* gi = reinterpret_cast<const float4*>(q.guide_in + (size_t)i * 16);
+            const float4 g2 = gi[2];
+            const int ridx = (int)g2.y;
+            const float unx = g2.z, uny = g2.w, u_term = gi[3].x;
+            const uint2 pk = P.pk[slot];
+            const uint64_t rinc = guide_inc(c, pk);
+            uint64_t rs = P.grng[slot];
+            const int2 ot = P.hit_ot[slot];
+            const float4 tb = P.hit_tb[slot];
+            Shading sc;
+            init_shading(s, from4(P.ray_d[slot]), ot.x, (uint32_t)ot.y, v3(tb.y, tb.z, tb.w), sc);
+            const DevMaterial m = s.materials[sc.material];
+            VmfLobe lobes[kGLobes];
+            mixture_from_outputs(q.guide_out + (size_t)i * kGOut, lobes);
+            const float a = c.alpha;
+            V3 wi = v3(0.f, 0.f, 0.f);
+            RGB f = rgb(0.f, 0.f, 0.f);
+            float pb = 0.f;
+            if (pcg_float(rs, rinc) < a) {
+                const float u0 = pcg_float(rs, rinc), u1 = pcg_float(rs, rinc), u2 = pcg_float(rs, rinc);
+                wi = mixture_sample(lobes, u0, u1, u2);
+                const bool upper = dot(sc.n, wi) > 0.f && (m.type == 1 || to_local(sc, wi).z > 0.f);
+                if (upper) {
+                    f = bsdf_eval(m, sc, sc.wo, wi);
+                    pb = bsdf_pdf(m, sc, sc.wo, wi);
+                }
+            } else {
+                f = bsdf_sample(m, sc, unx, uny, sc.wo, wi, pb);
+                if (!(dot(sc.n, wi) > 0.f)) f = rgb(0.f, 0.f, 0.f);
+            }
+            const float pm = is_black(f) ? 0.f : a * mixture_pdf(lobes, wi) + (1.f - a) * pb;
+            RGB coeff = rgb4(P.coeff[slot]);
+            uint32_t st = P.state[slot];
+            if (pm > 0.f && pm < f_inf()) {
+                coeff = coeff * (f * (fabsf(dot(sc.n, wi)) / pm));
+                P.ray_o[slot] = f4(origin_dir(sc, wi), 0.f);
+                P.ray_d[slot] = f4(wi, 0.f);
+                if (ridx >= 0) {
+                    float* R = P.rec + ((size_t)slot * kRecMax + ridx) * kRecFloats;
+                    R[9] = wi.x; R[10] = wi.y; R[11] = wi.z;
+                }
+                alive = true;
+                if (u_term >= 0.f) {                                     // path_tracing.cpp:112-130
+                    const float mc = fmaxf(coeff.r, fmaxf(coeff.g, coeff.b));
+                    if (mc < c.rr_threshold) {
+                        const float pt = fmaxf(0.05f, 1.f - mc);
+                        if (u_term < pt) alive = false;
+                        else coeff = coeff * (1.f / (1.f - pt));
+                    }
+                }
+            }
+            if (!alive) st |= kDone;
+            P.state[slot] = st;
+            P.coeff[slot] = f4(coeff, 0.f);
+            P.grng[slot] = rs;
+        }
+        const uint32_t ia = wave_append(n_out, alive);
+        if (alive) out[ia] = slot;
     }
 }
 
@@ -564,8 +756,22 @@ __global__ void k_resolve(DevScene s, const Chunk* __restrict__ cp, Paths P, Que
         const RGB L = rgb4(P.L[slot]) + rgb4(P.dl_coeff[slot]) * dl;
         P.L[slot] = f4(L, 0.f);
         const uint32_t st = P.state[slot];
-        if (st & kDone) finalize(*cp, P, P.pk[slot], L);
-        P.state[slot] = st & ~(kPendingDL | kDone);
+        const int nrec = (int)((st >> kRecShift) & 7u);
+        if (st & kRecPending) {
+            float* R = P.rec + ((size_t)slot * kRecMax + (nrec - 1)) * kRecFloats;
+            R[15] = L.r; R[16] = L.g; R[17] = L.b;
+        }
+        if (st & kDone) {
+            const Chunk& c = *cp;
+            const uint2 pk = P.pk[slot];
+            if (c.guided) {
+                uint64_t rs = P.grng[slot];
+                emit_records(P, q, slot, nrec, L, rs, guide_inc(c, pk));
+                P.grng[slot] = rs;
+            }
+            finalize(c, P, pk, L);
+        }
+        P.state[slot] = st & ~(kPendingDL | kDone | kRecPending);
     }
 }
 
@@ -574,7 +780,7 @@ __global__ void k_iter_reset(uint32_t* counts, int parity) {
     const int t = threadIdx.x;
     if (t == 0) counts[parity ^ 1] = 0u;
     else if (t >= 2 && t <= 4) counts[t] = 0u;
-    else if (t >= 6 && t <= 8) counts[t] = 0u;
+    else if (t >= 6 && t <= 9) counts[t] = 0u;
 }
 
 static size_t stack_lds(const LaunchCfg& cfg) {
@@ -613,6 +819,13 @@ void launch_stage(int stage, const DevScene& sc, const Chunk* c, const Paths& p,
     case kStageShade: hipLaunchKernelGGL(k_shade, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q, parity); break;
     case kStageShadow: YG_LAUNCH_TRAV(k_shadow, cfg.trav_grid, sc, p, q); break;
     case kStageMis: YG_LAUNCH_TRAV(k_mis, cfg.trav_grid, sc, p, q); break;
+    case kStageGuideForward:
+        if (cfg.guided) guide_forward_dev(cfg.net, q.guide_in, 16, &q.counts[9], q.guide_out, kGOut, cfg.guide_grid, cfg.stream);
+        break;
+    case kStageGuideSample:
+        if (cfg.guided)
+            hipLaunchKernelGGL(k_guide_sample, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q, parity);
+        break;
     case kStageResolve: hipLaunchKernelGGL(k_resolve, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q); break;
     default: break;
     }
@@ -667,7 +880,8 @@ __global__ void k_film_spill(const Chunk* __restrict__ cp, Paths P, FilmParams f
     const float* tab = c.table + c.pix_first[p] * c.dps;
     const TileBox t = tile_of(v, f.tile_first, px.x, px.y);
     uint32_t bits = 0;
-    for (int k = 0; k < v.spp; k++) {
+    const int ka = c.guided ? c.k0 : 0, kb = c.guided ? min(c.k1, v.spp) : v.spp;
+    for (int k = ka; k < kb; k++) {
         const Footprint fp = footprint(px.x, px.y, tab[2 * k], tab[2 * k + 1]);
         if (fp.x0 == px.x && fp.x1 == px.x + 1 && fp.y0 == px.y && fp.y1 == px.y + 1) continue;
         bool margin = false;
@@ -698,13 +912,13 @@ YG_INLINE bool spill_bit(uint8_t packed, int dx, int dy) {       // direction fr
 }
 
 // Adds the samples of in-tile neighbour (nx, ny) whose footprint covers (x, y), in sample order.
-YG_INLINE void gather(const Chunk& c, const Paths& P, const FilmParams& f, const TileBox& t, int spp, int x, int y,
-                      int nx, int ny, float4& acc) {
+YG_INLINE void gather(const Chunk& c, const Paths& P, const FilmParams& f, const TileBox& t, int ka, int kb, int x,
+                      int y, int nx, int ny, float4& acc) {
     if (nx < t.sx0 || nx >= t.sx1 || ny < t.sy0 || ny >= t.sy1) return;
     const long long li = t.first + (long long)(ny - t.sy0) * (t.sx1 - t.sx0) + (nx - t.sx0) - f.base;   // chunk-local
     if (!spill_bit(f.spill[li], x - nx, y - ny)) return;
     const float* tab = c.table + c.pix_first[li] * c.dps;
-    for (int k = 0; k < spp; k++) {
+    for (int k = ka; k < kb; k++) {
         const Footprint fp = footprint(nx, ny, tab[2 * k], tab[2 * k + 1]);
         if (x >= fp.x0 && x < fp.x1 && y >= fp.y0 && y < fp.y1) {
             const float4 r = P.radiance[c.pix_first[li] + k];
@@ -724,24 +938,34 @@ __global__ void k_film(const Chunk* __restrict__ cp, Paths P, FilmParams f) {
     const int2 px = c.pixels[p];
     const int x = px.x, y = px.y;
     const TileBox t = tile_of(v, f.tile_first, x, y);
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    gather(c, P, f, t, spp, x, y, x - 1, y - 1, acc);
-    gather(c, P, f, t, spp, x, y, x, y - 1, acc);
-    gather(c, P, f, t, spp, x, y, x + 1, y - 1, acc);
-    gather(c, P, f, t, spp, x, y, x - 1, y, acc);
-    double lsum = 0.0, lsq = 0.0;
-    for (int k = 0; k < spp; k++) {
+    // plain renders gather all samples at once; guided passes add samples [k0, k1) to the film
+    // and to running luminance moments, and resolve the variance on the pixel's last pass
+    const int ka = c.guided ? c.k0 : 0, kb = c.guided ? min(c.k1, spp) : spp;
+    if (ka >= kb) return;
+    const bool add = f.accumulate && ka > 0;
+    float4 acc = add ? f.out_film[f.base + p] : make_float4(0.f, 0.f, 0.f, 0.f);
+    gather(c, P, f, t, ka, kb, x, y, x - 1, y - 1, acc);
+    gather(c, P, f, t, ka, kb, x, y, x, y - 1, acc);
+    gather(c, P, f, t, ka, kb, x, y, x + 1, y - 1, acc);
+    gather(c, P, f, t, ka, kb, x, y, x - 1, y, acc);
+    double lsum = add ? f.m1[f.base + p] : 0.0, lsq = add ? f.m2[f.base + p] : 0.0;
+    for (int k = ka; k < kb; k++) {
         const float4 r = P.radiance[c.pix_first[p] + k];
         acc.x += 1.f * r.x; acc.y += 1.f * r.y; acc.z += 1.f * r.z; acc.w += 1.f;
         const float lv = 0.2126729f * r.x + 0.7151522f * r.y + 0.0721750f * r.z;   // color.cpp:21
         lsum += (double)lv;
         lsq += (double)(lv * lv);
     }
-    gather(c, P, f, t, spp, x, y, x + 1, y, acc);
-    gather(c, P, f, t, spp, x, y, x - 1, y + 1, acc);
-    gather(c, P, f, t, spp, x, y, x, y + 1, acc);
-    gather(c, P, f, t, spp, x, y, x + 1, y + 1, acc);
+    gather(c, P, f, t, ka, kb, x, y, x + 1, y, acc);
+    gather(c, P, f, t, ka, kb, x, y, x - 1, y + 1, acc);
+    gather(c, P, f, t, ka, kb, x, y, x, y + 1, acc);
+    gather(c, P, f, t, ka, kb, x, y, x + 1, y + 1, acc);
     f.out_film[f.base + p] = acc;
+    if (f.accumulate) {
+        f.m1[f.base + p] = lsum;
+        f.m2[f.base + p] = lsq;
+    }
+    if (kb < spp) return;
     double pv = 0.0;
     const int n = spp;
     if (n > 1) {
