@@ -96,9 +96,15 @@ def _batch_leg():
     batched render with the bounce loop replayed from a hipGraph."""
     import yar_amd
     from yar_amd import views as yv
+    from yar_amd import autotune
     scene = _scene(1, 1.0, 1)
-    ctx = yar_amd.Context(scene, use_graph=1)
-    views = yv.batch_views(ctx.view(0), 32)
+    probe = yar_amd.Context(scene)
+    views = yv.batch_views(probe.view(0), 32)
+    probe.close()
+    # launch shape and queue capacity from the autotuner, timed on a reduced batch of the same poses
+    small = [yv.with_film(v, 256, 256, v.x_pixel_sample_count, seed_offset=v.rng_seed_offset) for v in views[:6]]
+    best, _ = autotune.tune(scene, views=small, paths=[1 << 21, 1 << 22, 1 << 23], repeats=2)
+    ctx = yar_amd.Context(scene, use_graph=1, **best)
     ctx.set_views(views)
     idx = list(range(len(views)))
     ctx.render_batch(idx)
@@ -107,6 +113,7 @@ def _batch_leg():
     dt = time.perf_counter() - t0
     ctx.close()
     return {"workload": "config 3: config-1 scene, 32 views, sides {128..1024}, spp {4,16,64}, hipGraph bounce loop",
+            "autotuned": best,
             "samples": st.samples, "ms": round(dt * 1e3, 1), "msamples_s": round(st.samples / dt / 1e6, 2),
             "mrays_s": round((st.closest_rays + st.any_rays) / dt / 1e6, 1)}
 
@@ -157,12 +164,31 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="skip the config-1, batch and guided legs")
+    ap.add_argument("--leg", choices=["config1", "batch", "guided"], help="run only this leg (profiling)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device with a torch copy kernel (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the path shards by "
               f"tiles/views without collectives, see DESIGN.md", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
+        return
+    if args.leg:
+        threads = min(16, os.cpu_count() or 1)
+        fn = {"config1": lambda: _config1_leg(threads, False), "batch": _batch_leg, "guided": _guided_leg}[args.leg]
+        print(json.dumps(fn()), flush=True)
+        return
 
     import yar_amd
     t0 = time.perf_counter()

@@ -162,9 +162,9 @@ typedef struct yar_gpu_ctx yar_gpu_ctx;
 
 typedef struct yar_gpu_options {
     int32_t device;                     /* HIP device ordinal */
-    int32_t block_size;                 /* 0 = autotuned / default */
+    int32_t block_size;                 /* traversal threads per block, 0 = 64 */
     int64_t max_paths_in_flight;        /* 0 = default (sized from free HBM) */
-    int32_t traversal_stack_depth;      /* 0 = exact depth of the built trees */
+    int32_t traversal_stack_depth;      /* short-stack entries per tree level, 0 = 8 (kd-restart beyond) */
     int32_t enable_counters;            /* count node visits / triangle tests per ray */
     int32_t use_graph;                  /* capture the bounce loop in a hipGraph */
     int32_t enable_timing;              /* per-kernel HIP-event timing in yar_render_stats */

@@ -15,12 +15,17 @@ def _hits_agree(g, o):
     return same.mean(), (rel.max() if rel.size else 0.0), (g["t"][both] == o["t"][both]).mean() if both.any() else 1.0
 
 
-@pytest.mark.parametrize("which", ["c1", "c2_small"])
-def test_trace_rays_closest_matches_oracle(which, c1_path, c2_small_path, oracle):
+@pytest.mark.parametrize("which,opts", [("c1", {}), ("c2_small", {}),
+                                        ("c2_small", {"traversal_stack_depth": 2}),
+                                        ("c2_small", {"traversal_stack_depth": 32, "stack_mode": 2})],
+                         ids=["c1", "c2_small", "c2_small-restart", "c2_small-private"])
+def test_trace_rays_closest_matches_oracle(which, opts, c1_path, c2_small_path, oracle):
+    """Short stacks restart from the root (kd-restart) when they overflow; the hit records must not
+    depend on the stack storage or depth."""
     path = c1_path if which == "c1" else c2_small_path
     scene = yar_amd.Scene(path)
     rays = raygen.random_rays(scene, 1 << 18, seed=7)
-    ctx = yar_amd.Context(scene)
+    ctx = yar_amd.Context(scene, **opts)
     g = ctx.trace_rays(rays)
     o = oracle.OracleScene(scene).trace(rays)
     same, max_rel, exact = _hits_agree(g, o)

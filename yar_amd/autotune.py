@@ -12,13 +12,16 @@ import time
 from . import Context, GpuOptions
 
 
-def candidates(full=False):
+def candidates(full=False, paths=None):
+    """Traversal block size x stack storage/depth, optionally x paths in flight (queue capacity)."""
     blocks = [64, 128, 256] if full else [64, 128]
     out = []
     for b in blocks:
-        for depth in ([4, 8, 16, 32] if full else [8, 16]):
+        for depth in ([4, 8, 16, 32] if full else [4, 8]):
             out.append(dict(block_size=b, stack_mode=1, traversal_stack_depth=depth))
-        out.append(dict(block_size=b, stack_mode=2, traversal_stack_depth=32))
+        out.append(dict(block_size=b, stack_mode=2, traversal_stack_depth=16))
+    if paths:
+        out = [dict(c, max_paths_in_flight=p) for c in out for p in paths]
     return out
 
 
@@ -33,22 +36,27 @@ def _crop_view(ctx, view_index, crop):
     return v
 
 
-def tune(scene, view_index=0, crop=256, repeats=2, full=False, extra=None, log=None):
-    """Returns (best options dict, list of (options, seconds)) for `scene`."""
+def tune(scene, view_index=0, crop=256, repeats=2, full=False, extra=None, log=None, paths=None, views=None):
+    """Returns (best options dict, list of (options, seconds)) for `scene`.  With `views` the
+    candidates are timed on a batched render of those views instead of a crop of one view."""
     results = []
-    for cand in candidates(full) + list(extra or []):
+    for cand in candidates(full, paths) + list(extra or []):
         opts = GpuOptions()
         for k, val in cand.items():
             setattr(opts, k, val)
         opts.enable_timing = 0
         ctx = Context(scene, opts)
-        v = _crop_view(ctx, view_index, crop)
-        ctx.set_views([v])
-        ctx.render(0, want_film=False)                       # warm up (allocations, code load)
+        if views:
+            ctx.set_views(views)
+            idx = list(range(len(views)))
+            run = lambda: ctx.render_batch(idx)[2]
+        else:
+            ctx.set_views([_crop_view(ctx, view_index, crop)])
+            run = lambda: ctx.render(0, want_film=False)[3]
+        run()                                                # warm up (allocations, code load)
         best = float("inf")
         for _ in range(repeats):
-            _, _, _, st = ctx.render(0, want_film=False)
-            best = min(best, st.gpu_seconds)
+            best = min(best, run().gpu_seconds)
         ctx.close()
         results.append((cand, best))
         if log:
