@@ -192,74 +192,6 @@ struct PrivStack {
     YG_INLINE void get(int slot, uint32_t& n, float& t) const { n = node[slot]; t = tmax[slot]; }
 };
 
-// One kd-tree level. `root` is the global index of the tree's root node, [t0,t1] the slab range.
-// loop_t() returns the current loop bound (closest hit t, or the fixed any-hit tmax).
-// Leaf(ids_node, first_id, count) processes a leaf and returns true to stop the whole query.
-template <bool COUNT, class Stack, class LoopT, class Leaf>
-YG_INLINE bool kd_level(const uint2* __restrict__ nodes, uint32_t root, V3 o, V3 d, V3 inv, float t0, float t1,
-                        Stack& st, int depth, LoopT loop_t, Leaf leaf, TravCounters& cnt) {
-    const int mask = depth - 1;   // depth is a power of two
-    float tmin = t0, tmax = t1;
-    uint32_t node = root;
-    int sp = 0, count = 0;
-    bool dropped = false;
-    float restart_t = -1.f;
-    // The iteration cap only guarantees that every wave terminates; real trees need far fewer.
-    for (int iter = 0; iter < (1 << 22) && loop_t() > tmin; iter++) {
-        if (COUNT) cnt.nodes++;
-        uint2 w = nodes[node];
-        if ((w.x & 3u) != 3u) {
-            int axis = (int)(w.x & 3u);
-            float oa = axis == 0 ? o.x : (axis == 1 ? o.y : o.z);
-            float da = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
-            float ia = axis == 0 ? inv.x : (axis == 1 ? inv.y : inv.z);
-            float dist = bitsf(w.y) - oa;
-            uint32_t below = node + 1, above = w.x >> 2;
-            if (dist != 0.f) {
-                uint32_t first = dist > 0.f ? below : above;
-                uint32_t second = dist > 0.f ? above : below;
-                float ts = dist * ia;
-                if (ts > tmax || ts < 0.f) {
-                    node = first;
-                } else if (ts < tmin) {
-                    node = second;
-                } else {
-                    st.put(sp & mask, second, tmax);
-                    sp++;
-                    if (count == depth) dropped = true; else count++;
-                    node = first;
-                    tmax = ts;
-                }
-            } else {
-                node = (da > 0.f) ? above : below;
-            }
-        } else {
-            uint32_t n = w.x >> 2;
-            if (COUNT) cnt.leaf_refs += n;
-            if (n > 0 && leaf(node, w.y, n)) return true;
-            if (count > 0) {
-                sp--; count--;
-                uint32_t nn; float tm;
-                st.get(sp & mask, nn, tm);
-                node = nn;
-                tmin = tmax;
-                tmax = tm;
-            } else if (dropped && tmax < t1) {
-                // restart behind the last leaf; a restart that would not advance (a ray grazing
-                // many coincident split planes) steps past the degenerate point instead
-                dropped = false;
-                node = root;
-                tmin = tmax > restart_t ? tmax : nextafterf(restart_t, f_inf());
-                restart_t = tmin;
-                tmax = t1;
-            } else {
-                break;
-            }
-        }
-    }
-    return false;
-}
-
 // Leaf id j of a leaf whose first id is in word1 of node `nd` (kdtree.h:53: ids packed inline).
 YG_INLINE uint32_t leaf_id(const uint2* __restrict__ nodes, uint32_t nd, uint32_t first, uint32_t j) {
     if (j == 0) return first;
@@ -267,74 +199,181 @@ YG_INLINE uint32_t leaf_id(const uint2* __restrict__ nodes, uint32_t nd, uint32_
     return (j & 1) ? w.x : w.y;
 }
 
+// One step of a kd-tree descent shared by both levels: interior node -> child (pushing the far
+// child when the ray straddles the split), as KdTree::intersect's inner loop (kdtree.cpp:245-290).
+template <class Stack>
+YG_INLINE void kd_interior(uint2 w, uint32_t& node, V3 o, V3 d, V3 inv, float& tmin, float& tmax, Stack& st,
+                           int mask, int depth, int& sp, int& count, bool& dropped) {
+    const int axis = (int)(w.x & 3u);
+    const float oa = axis == 0 ? o.x : (axis == 1 ? o.y : o.z);
+    const float da = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
+    const float ia = axis == 0 ? inv.x : (axis == 1 ? inv.y : inv.z);
+    const float dist = bitsf(w.y) - oa;
+    const uint32_t below = node + 1, above = w.x >> 2;
+    if (dist != 0.f) {
+        const uint32_t first = dist > 0.f ? below : above;
+        const uint32_t second = dist > 0.f ? above : below;
+        const float ts = dist * ia;
+        if (ts > tmax || ts < 0.f) {
+            node = first;
+        } else if (ts < tmin) {
+            node = second;
+        } else {
+            st.put(sp & mask, second, tmax);
+            sp++;
+            if (count == depth) dropped = true; else count++;
+            node = first;
+            tmax = ts;
+        }
+    } else {
+        node = (da > 0.f) ? above : below;
+    }
+}
+
+// After a leaf: pop the next subtree, or restart behind the last leaf when the short stack dropped
+// entries (kd-restart; a restart that would not advance steps past the degenerate point), or finish.
+// Returns false when the level's traversal is complete.
+template <class Stack>
+YG_INLINE bool kd_next(uint32_t& node, uint32_t root, float& tmin, float& tmax, float t1, Stack& st, int mask, int& sp,
+                       int& count, bool& dropped, float& restart_t) {
+    if (count > 0) {
+        sp--; count--;
+        uint32_t nn; float tm;
+        st.get(sp & mask, nn, tm);
+        node = nn;
+        tmin = tmax;
+        tmax = tm;
+        return true;
+    }
+    if (dropped && tmax < t1) {
+        dropped = false;
+        node = root;
+        tmin = tmax > restart_t ? tmax : nextafterf(restart_t, f_inf());
+        restart_t = tmin;
+        tmax = t1;
+        return true;
+    }
+    return false;
+}
+
 // Two-level closest-hit / any-hit query over the scene tree and the per-mesh trees
 // (KdTree::intersect / intersect_any with intersect_scene_geometry_data and
-// intersect_triangle_mesh_geometry_data, kdtree.cpp:12-109).
+// intersect_triangle_mesh_geometry_data, kdtree.cpp:12-109).  Both levels run in one loop: each
+// iteration advances a lane by one node of whichever tree it is in, so lanes inside a mesh tree
+// and lanes still descending the scene tree make progress together instead of the whole wave
+// waiting on the longest mesh traversal.  The visit order and hit updates are those of the nested
+// traversal, so results are identical.
 template <bool ANY, bool COUNT, class SStack, class MStack>
 YG_INLINE bool trace(const DevScene& s, V3 o, V3 d, float tmax_in, HitRec& hit, SStack& sst, int sdepth,
                      MStack& mst, int mdepth, TravCounters& cnt) {
     hit.t = tmax_in; hit.obj = -1; hit.tri = 0; hit.b = v3(0, 0, 0);
     float st0, st1;
     if (!slab(s.scene_lo, s.scene_hi, o, d, st0, st1)) return false;
-    V3 inv = v3(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    const V3 inv = v3(1.f / d.x, 1.f / d.y, 1.f / d.z);
     const float any_t = tmax_in;
-    bool any_found = false;
-    auto scene_loop_t = [&]() { return ANY ? any_t : hit.t; };
-    auto scene_leaf = [&](uint32_t nd, uint32_t first, uint32_t n) -> bool {
-        for (uint32_t j = 0; j < n; j++) {
-            uint32_t oi = leaf_id(s.scene_nodes, nd, first, j);
-            const ObjTrav ob = s.objs[oi];
-            if (COUNT) cnt.objects++;
-            M34 w2o = {ob.w2o0, ob.w2o1, ob.w2o2};
-            V3 lo = xform_point(w2o, o);
-            V3 ld = xform_vector(w2o, d);
-            float mt0, mt1;
-            if (!slab(ob.lo_node, ob.hi_tri, lo, ld, mt0, mt1)) continue;
-            V3 linv = v3(1.f / ld.x, 1.f / ld.y, 1.f / ld.z);
-            uint32_t root = fbits(ob.lo_node.w);
-            // watertight-test setup hoisted per mesh ray (intersection.cpp:55-61)
-            float ax = fabsf(ld.x), ay = fabsf(ld.y), az = fabsf(ld.z);
-            int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
-            int kx = (kz == 2 ? 0 : kz + 1);
-            int ky = (kz == 0 ? 2 : kz - 1);
-            float dx = ld[kx], dy = ld[ky], dz = ld[kz];
-            float sx = -dx / dz, sy = -dy / dz, sz = 1.f / dz;
-            float mesh_tmax = hit.t;
-            bool stop = false;
-            auto mesh_loop_t = [&]() { return ANY ? any_t : hit.t; };
-            auto mesh_leaf = [&](uint32_t mnd, uint32_t mfirst, uint32_t mn) -> bool {
-                for (uint32_t k = 0; k < mn; k++) {
-                    uint32_t ti = leaf_id(s.mesh_nodes, mnd, mfirst, k);
-                    float4 p0 = s.tri_verts[3 * (size_t)ti], p1 = s.tri_verts[3 * (size_t)ti + 1],
-                           p2 = s.tri_verts[3 * (size_t)ti + 2];
+    const int smask = sdepth - 1, mmask = mdepth - 1;
+    // scene level
+    uint32_t snode = 0;
+    float stmin = st0, stmax = st1, srestart = -1.f;
+    int ssp = 0, scount = 0;
+    bool sdropped = false;
+    // objects of the current scene leaf still to visit
+    uint32_t lnode = 0, lfirst = 0, ln = 0, lj = 0;
+    // mesh level
+    bool in_mesh = false;
+    uint32_t mroot = 0, mnode = 0, oi = 0;
+    float mtmin = 0.f, mtmax = 0.f, mt1 = 0.f, mrestart = -1.f, mesh_tmax = 0.f;
+    int msp = 0, mcount = 0;
+    bool mdropped = false;
+    V3 lo = v3(0, 0, 0), ld = v3(0, 0, 0), linv = v3(0, 0, 0);
+    int kx = 0, ky = 0, kz = 0;
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    // The iteration cap only guarantees that every wave terminates; real trees need far fewer.
+    for (int iter = 0; iter < (1 << 24); iter++) {
+        const float loop_t = ANY ? any_t : hit.t;
+        if (in_mesh) {
+            bool done = !(loop_t > mtmin);
+            if (!done) {
+                if (COUNT) cnt.nodes++;
+                const uint2 w = s.mesh_nodes[mnode];
+                if ((w.x & 3u) != 3u) {
+                    kd_interior(w, mnode, lo, ld, linv, mtmin, mtmax, mst, mmask, mdepth, msp, mcount, mdropped);
+                    continue;
+                }
+                const uint32_t n = w.x >> 2;
+                if (COUNT) cnt.leaf_refs += n;
+                for (uint32_t k = 0; k < n; k++) {
+                    const uint32_t ti = leaf_id(s.mesh_nodes, mnode, w.y, k);
+                    const float4 p0 = s.tri_verts[3 * (size_t)ti], p1 = s.tri_verts[3 * (size_t)ti + 1],
+                                 p2 = s.tri_verts[3 * (size_t)ti + 2];
                     if (COUNT) cnt.tri_tests++;
                     V3 b;
-                    float t = watertight(lo, ld, kx, ky, kz, sx, sy, sz, p0, p1, p2, b);
+                    const float t = watertight(lo, ld, kx, ky, kz, sx, sy, sz, p0, p1, p2, b);
                     if (ANY) {
                         if (t >= any_t) continue;
                         if (degenerate(p0, p1, p2)) continue;
-                        any_found = true;
                         return true;
-                    } else {
-                        if (t < hit.t) {
-                            if (degenerate(p0, p1, p2)) continue;
-                            hit.t = t; hit.tri = ti; hit.b = b;
-                        }
+                    } else if (t < hit.t) {
+                        if (degenerate(p0, p1, p2)) continue;
+                        hit.t = t; hit.tri = ti; hit.b = b;
                     }
                 }
-                return false;
-            };
-            stop = kd_level<COUNT>(s.mesh_nodes, root, lo, ld, linv, mt0, mt1, mst, mdepth, mesh_loop_t, mesh_leaf, cnt);
-            if (ANY) {
-                if (stop) return true;
-            } else if (hit.t < mesh_tmax) {
-                hit.obj = (int)oi;
+                done = !kd_next(mnode, mroot, mtmin, mtmax, mt1, mst, mmask, msp, mcount, mdropped, mrestart);
             }
+            if (done) {
+                in_mesh = false;
+                if (!ANY && hit.t < mesh_tmax) hit.obj = (int)oi;
+            }
+            continue;
         }
-        return false;
-    };
-    kd_level<COUNT>(s.scene_nodes, 0u, o, d, inv, st0, st1, sst, sdepth, scene_loop_t, scene_leaf, cnt);
-    if (ANY) return any_found;
+        if (lj < ln) {
+            // next object of the scene leaf: ray to object space, mesh bounds, watertight setup
+            oi = leaf_id(s.scene_nodes, lnode, lfirst, lj);
+            lj++;
+            const ObjTrav ob = s.objs[oi];
+            if (COUNT) cnt.objects++;
+            const M34 w2o = {ob.w2o0, ob.w2o1, ob.w2o2};
+            lo = xform_point(w2o, o);
+            ld = xform_vector(w2o, d);
+            float mt0;
+            if (!slab(ob.lo_node, ob.hi_tri, lo, ld, mt0, mt1)) continue;
+            linv = v3(1.f / ld.x, 1.f / ld.y, 1.f / ld.z);
+            mroot = fbits(ob.lo_node.w);
+            // watertight-test setup hoisted per mesh ray (intersection.cpp:55-61)
+            const float ax = fabsf(ld.x), ay = fabsf(ld.y), az = fabsf(ld.z);
+            kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+            kx = (kz == 2 ? 0 : kz + 1);
+            ky = (kz == 0 ? 2 : kz - 1);
+            const float dx = ld[kx], dy = ld[ky], dz = ld[kz];
+            sx = -dx / dz; sy = -dy / dz; sz = 1.f / dz;
+            mesh_tmax = hit.t;
+            mnode = mroot; mtmin = mt0; mtmax = mt1; mrestart = -1.f;
+            msp = 0; mcount = 0; mdropped = false;
+            in_mesh = true;
+            continue;
+        }
+        if (ln > 0) {
+            // the leaf's objects are done: continue the scene traversal behind it
+            ln = 0;
+            if (!kd_next(snode, 0u, stmin, stmax, st1, sst, smask, ssp, scount, sdropped, srestart)) break;
+            continue;
+        }
+        if (!(loop_t > stmin)) break;
+        if (COUNT) cnt.nodes++;
+        const uint2 w = s.scene_nodes[snode];
+        if ((w.x & 3u) != 3u) {
+            kd_interior(w, snode, o, d, inv, stmin, stmax, sst, smask, sdepth, ssp, scount, sdropped);
+            continue;
+        }
+        const uint32_t n = w.x >> 2;
+        if (COUNT) cnt.leaf_refs += n;
+        if (n > 0) {
+            lnode = snode; lfirst = w.y; ln = n; lj = 0;
+        } else if (!kd_next(snode, 0u, stmin, stmax, st1, sst, smask, ssp, scount, sdropped, srestart)) {
+            break;
+        }
+    }
+    if (ANY) return false;
     return hit.t < tmax_in;
 }
 

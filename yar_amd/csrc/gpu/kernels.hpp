@@ -72,7 +72,7 @@ struct Queues {
     uint32_t train_cap;
     float keep_prob;
     // [0..15] traversal counters (4 per kernel kind), [16] extend rays, [17] shadow rays, [18] mis rays,
-    // [19] non-empty extend launches
+    // [19] extend launches
     unsigned long long* totals;
 };
 

@@ -262,7 +262,7 @@ template <bool COUNT, bool PRIV>
 __global__ void k_extend(DevScene sc, Paths P, Queues q, int parity, StackCfg cfg) {
     const uint32_t* in = parity ? q.active1 : q.active0;
     const uint32_t n = q.counts[parity];
-    if (blockIdx.x == 0 && threadIdx.x == 0 && n) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         atomicAdd(&q.totals[16], (unsigned long long)n);
         atomicAdd(&q.totals[19], 1ull);
     }
