@@ -103,7 +103,7 @@ def _batch_leg():
     probe.close()
     # launch shape and queue capacity from the autotuner, timed on a reduced batch of the same poses
     small = [yv.with_film(v, 256, 256, v.x_pixel_sample_count, seed_offset=v.rng_seed_offset) for v in views[:6]]
-    best, _ = autotune.tune(scene, views=small, paths=[1 << 21, 1 << 22, 1 << 23], repeats=2)
+    best, _ = autotune.tune(scene, views=small, paths=[1 << 22, 1 << 23], repeats=2)
     ctx = yar_amd.Context(scene, use_graph=1, **best)
     ctx.set_views(views)
     idx = list(range(len(views)))

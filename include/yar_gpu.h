@@ -169,7 +169,8 @@ typedef struct yar_gpu_options {
     int32_t use_graph;                  /* capture the bounce loop in a hipGraph */
     int32_t enable_timing;              /* per-kernel HIP-event timing in yar_render_stats */
     int32_t stack_mode;                 /* traversal stack storage: 0 = default, 1 = LDS, 2 = private */
-    int32_t reserved[3];
+    int32_t lane_refill;                /* 1 = traversal lanes take new rays individually, 0/2 = batches of 64 */
+    int32_t reserved[2];
 } yar_gpu_options;
 
 typedef struct yar_view {               /* Scene_Overrides analog, reference_renderer.h:23 */

@@ -13,13 +13,14 @@ from . import Context, GpuOptions
 
 
 def candidates(full=False, paths=None):
-    """Traversal block size x stack storage/depth, optionally x paths in flight (queue capacity)."""
+    """Traversal block size x stack storage/depth x lane refill, optionally x paths in flight."""
     blocks = [64, 128, 256] if full else [64, 128]
     out = []
     for b in blocks:
         for depth in ([4, 8, 16, 32] if full else [4, 8]):
             out.append(dict(block_size=b, stack_mode=1, traversal_stack_depth=depth))
         out.append(dict(block_size=b, stack_mode=2, traversal_stack_depth=16))
+    out = [dict(c, lane_refill=r) for c in out for r in (1, 2)]
     if paths:
         out = [dict(c, max_paths_in_flight=p) for c in out for p in paths]
     return out

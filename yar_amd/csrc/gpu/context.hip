@@ -192,6 +192,7 @@ struct yar_gpu_ctx {
         lc.scene_stack = info.stack_depth >> 16;
         lc.mesh_stack = info.stack_depth & 0xffff;
         lc.priv_stack = opt.stack_mode == 2;
+        lc.refill = opt.lane_refill == 1;
         lc.counters = opt.enable_counters != 0;
         lc.trav_grid = 1;
         lc.path_grid = 1;

@@ -258,38 +258,60 @@ YG_INLINE bool kd_next(uint32_t& node, uint32_t root, float& tmin, float& tmax, 
 
 // Two-level closest-hit / any-hit query over the scene tree and the per-mesh trees
 // (KdTree::intersect / intersect_any with intersect_scene_geometry_data and
-// intersect_triangle_mesh_geometry_data, kdtree.cpp:12-109).  Both levels run in one loop: each
-// iteration advances a lane by one node of whichever tree it is in, so lanes inside a mesh tree
-// and lanes still descending the scene tree make progress together instead of the whole wave
-// waiting on the longest mesh traversal.  The visit order and hit updates are those of the nested
-// traversal, so results are identical.
-template <bool ANY, bool COUNT, class SStack, class MStack>
-YG_INLINE bool trace(const DevScene& s, V3 o, V3 d, float tmax_in, HitRec& hit, SStack& sst, int sdepth,
-                     MStack& mst, int mdepth, TravCounters& cnt) {
-    hit.t = tmax_in; hit.obj = -1; hit.tri = 0; hit.b = v3(0, 0, 0);
-    float st0, st1;
-    if (!slab(s.scene_lo, s.scene_hi, o, d, st0, st1)) return false;
-    const V3 inv = v3(1.f / d.x, 1.f / d.y, 1.f / d.z);
-    const float any_t = tmax_in;
-    const int smask = sdepth - 1, mmask = mdepth - 1;
+// intersect_triangle_mesh_geometry_data, kdtree.cpp:12-109), as a resumable per-lane state
+// machine.  Both levels run in one loop: each step() advances a lane by one node of whichever tree
+// it is in (scene tree, or the mesh tree of the object being visited), so lanes inside a mesh tree
+// and lanes still descending the scene tree make progress together, and a persistent kernel can
+// hand a lane a new ray as soon as its query ends.  The visit order and hit updates are those of
+// the nested traversal, so results are identical.
+template <bool ANY, bool COUNT>
+struct Tracer {
+    V3 o, d, inv;
+    float any_t, tmax_in, st1;
+    HitRec hit;
+    bool found_any;
     // scene level
-    uint32_t snode = 0;
-    float stmin = st0, stmax = st1, srestart = -1.f;
-    int ssp = 0, scount = 0;
-    bool sdropped = false;
+    uint32_t snode;
+    float stmin, stmax, srestart;
+    int ssp, scount;
+    bool sdropped;
     // objects of the current scene leaf still to visit
-    uint32_t lnode = 0, lfirst = 0, ln = 0, lj = 0;
+    uint32_t lnode, lfirst, ln, lj;
     // mesh level
-    bool in_mesh = false;
-    uint32_t mroot = 0, mnode = 0, oi = 0;
-    float mtmin = 0.f, mtmax = 0.f, mt1 = 0.f, mrestart = -1.f, mesh_tmax = 0.f;
-    int msp = 0, mcount = 0;
-    bool mdropped = false;
-    V3 lo = v3(0, 0, 0), ld = v3(0, 0, 0), linv = v3(0, 0, 0);
-    int kx = 0, ky = 0, kz = 0;
-    float sx = 0.f, sy = 0.f, sz = 0.f;
-    // The iteration cap only guarantees that every wave terminates; real trees need far fewer.
-    for (int iter = 0; iter < (1 << 24); iter++) {
+    bool in_mesh;
+    uint32_t mroot, mnode, oi;
+    float mtmin, mtmax, mt1, mrestart, mesh_tmax;
+    int msp, mcount;
+    bool mdropped;
+    V3 lo, ld, linv;
+    int kx, ky, kz;
+    float sx, sy, sz;
+    int iter;
+
+    // Returns false when the ray misses the scene bounds (the query is then already complete).
+    YG_INLINE bool start(const DevScene& s, V3 o_, V3 d_, float tmax_) {
+        o = o_; d = d_; tmax_in = tmax_; any_t = tmax_;
+        hit.t = tmax_; hit.obj = -1; hit.tri = 0; hit.b = v3(0, 0, 0);
+        found_any = false;
+        float t0, t1;
+        if (!slab(s.scene_lo, s.scene_hi, o, d, t0, t1)) return false;
+        inv = v3(1.f / d.x, 1.f / d.y, 1.f / d.z);
+        st1 = t1;
+        snode = 0; stmin = t0; stmax = t1; srestart = -1.f; ssp = 0; scount = 0; sdropped = false;
+        lnode = lfirst = ln = lj = 0;
+        in_mesh = false;
+        iter = 0;
+        return true;
+    }
+
+    // closest hit: hit.t < tmax_in; any hit: found_any
+    YG_INLINE bool result() const { return ANY ? found_any : hit.t < tmax_in; }
+
+    // One traversal step; returns true when the query is complete.
+    template <class SStack, class MStack>
+    YG_INLINE bool step(const DevScene& s, SStack& sst, int sdepth, MStack& mst, int mdepth, TravCounters& cnt) {
+        // The iteration cap only guarantees that every wave terminates; real trees need far fewer.
+        if (++iter > (1 << 24)) return true;
         const float loop_t = ANY ? any_t : hit.t;
         if (in_mesh) {
             bool done = !(loop_t > mtmin);
@@ -297,8 +319,8 @@ YG_INLINE bool trace(const DevScene& s, V3 o, V3 d, float tmax_in, HitRec& hit, 
                 if (COUNT) cnt.nodes++;
                 const uint2 w = s.mesh_nodes[mnode];
                 if ((w.x & 3u) != 3u) {
-                    kd_interior(w, mnode, lo, ld, linv, mtmin, mtmax, mst, mmask, mdepth, msp, mcount, mdropped);
-                    continue;
+                    kd_interior(w, mnode, lo, ld, linv, mtmin, mtmax, mst, mdepth - 1, mdepth, msp, mcount, mdropped);
+                    return false;
                 }
                 const uint32_t n = w.x >> 2;
                 if (COUNT) cnt.leaf_refs += n;
@@ -312,19 +334,20 @@ YG_INLINE bool trace(const DevScene& s, V3 o, V3 d, float tmax_in, HitRec& hit, 
                     if (ANY) {
                         if (t >= any_t) continue;
                         if (degenerate(p0, p1, p2)) continue;
+                        found_any = true;
                         return true;
                     } else if (t < hit.t) {
                         if (degenerate(p0, p1, p2)) continue;
                         hit.t = t; hit.tri = ti; hit.b = b;
                     }
                 }
-                done = !kd_next(mnode, mroot, mtmin, mtmax, mt1, mst, mmask, msp, mcount, mdropped, mrestart);
+                done = !kd_next(mnode, mroot, mtmin, mtmax, mt1, mst, mdepth - 1, msp, mcount, mdropped, mrestart);
             }
             if (done) {
                 in_mesh = false;
                 if (!ANY && hit.t < mesh_tmax) hit.obj = (int)oi;
             }
-            continue;
+            return false;
         }
         if (lj < ln) {
             // next object of the scene leaf: ray to object space, mesh bounds, watertight setup
@@ -336,7 +359,7 @@ YG_INLINE bool trace(const DevScene& s, V3 o, V3 d, float tmax_in, HitRec& hit, 
             lo = xform_point(w2o, o);
             ld = xform_vector(w2o, d);
             float mt0;
-            if (!slab(ob.lo_node, ob.hi_tri, lo, ld, mt0, mt1)) continue;
+            if (!slab(ob.lo_node, ob.hi_tri, lo, ld, mt0, mt1)) return false;
             linv = v3(1.f / ld.x, 1.f / ld.y, 1.f / ld.z);
             mroot = fbits(ob.lo_node.w);
             // watertight-test setup hoisted per mesh ray (intersection.cpp:55-61)
@@ -350,31 +373,39 @@ YG_INLINE bool trace(const DevScene& s, V3 o, V3 d, float tmax_in, HitRec& hit, 
             mnode = mroot; mtmin = mt0; mtmax = mt1; mrestart = -1.f;
             msp = 0; mcount = 0; mdropped = false;
             in_mesh = true;
-            continue;
+            return false;
         }
         if (ln > 0) {
             // the leaf's objects are done: continue the scene traversal behind it
             ln = 0;
-            if (!kd_next(snode, 0u, stmin, stmax, st1, sst, smask, ssp, scount, sdropped, srestart)) break;
-            continue;
+            return !kd_next(snode, 0u, stmin, stmax, st1, sst, sdepth - 1, ssp, scount, sdropped, srestart);
         }
-        if (!(loop_t > stmin)) break;
+        if (!(loop_t > stmin)) return true;
         if (COUNT) cnt.nodes++;
         const uint2 w = s.scene_nodes[snode];
         if ((w.x & 3u) != 3u) {
-            kd_interior(w, snode, o, d, inv, stmin, stmax, sst, smask, sdepth, ssp, scount, sdropped);
-            continue;
+            kd_interior(w, snode, o, d, inv, stmin, stmax, sst, sdepth - 1, sdepth, ssp, scount, sdropped);
+            return false;
         }
         const uint32_t n = w.x >> 2;
         if (COUNT) cnt.leaf_refs += n;
         if (n > 0) {
             lnode = snode; lfirst = w.y; ln = n; lj = 0;
-        } else if (!kd_next(snode, 0u, stmin, stmax, st1, sst, smask, ssp, scount, sdropped, srestart)) {
-            break;
+            return false;
         }
+        return !kd_next(snode, 0u, stmin, stmax, st1, sst, sdepth - 1, ssp, scount, sdropped, srestart);
     }
-    if (ANY) return false;
-    return hit.t < tmax_in;
+};
+
+// Runs one query to completion.
+template <bool ANY, bool COUNT, class SStack, class MStack>
+YG_INLINE bool trace(const DevScene& s, V3 o, V3 d, float tmax_in, HitRec& hit, SStack& sst, int sdepth,
+                     MStack& mst, int mdepth, TravCounters& cnt) {
+    Tracer<ANY, COUNT> tr;
+    if (tr.start(s, o, d, tmax_in))
+        while (!tr.step(s, sst, sdepth, mst, mdepth, cnt)) {}
+    hit = tr.hit;
+    return tr.result();
 }
 
 // ---- PCG32 (third-party/pcg/pcg_basic.c:46-75) ------------------------------------------

@@ -116,6 +116,7 @@ struct LaunchCfg {
     int block;                // threads per block for traversal kernels
     int scene_stack, mesh_stack;
     bool priv_stack;          // stacks in private (scratch) memory instead of LDS
+    bool refill;              // per-lane refill in the persistent traversal kernels
     bool counters;
     int trav_grid;            // persistent traversal grid (blocks)
     int path_grid;            // grid-stride grid of per-path kernels (blocks of 256)

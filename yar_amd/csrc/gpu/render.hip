@@ -230,7 +230,7 @@ void launch_raygen(const Chunk* c, long long n_paths_max, const Paths& p, Queues
 // BSDF-sampled light rays, direct_lighting.cpp:163-191).  Persistent: each wave fetches 64 rays
 // at a time from a device cursor until the queue is drained.
 // ---------------------------------------------------------------------------------------------
-struct StackCfg { int sd, md; };
+struct StackCfg { int sd, md, refill; };
 
 YG_INLINE void lds_stacks(uint32_t* lds_base, StackCfg sc, LdsStack& sst, LdsStack& mst) {
     const int T = blockDim.x, t = threadIdx.x;
@@ -258,6 +258,63 @@ YG_INLINE void with_stacks(StackCfg cfg, Body body) {
     }
 }
 
+// Persistent traversal over a device queue.  With `refill` a lane whose query ended takes the next
+// queue entry as soon as at least 16 lanes of its wave are idle, so a wave is not held by its
+// longest ray (pays on small scenes with short, uneven queries); otherwise waves take batches of 64.
+// setup(index, o, d, tmax) returns the payload finish(payload, tracer) receives.
+template <bool ANY, bool COUNT, class SStack, class MStack, class Setup, class Finish>
+YG_INLINE void trace_queue(const DevScene& sc, uint32_t n, uint32_t* cursor, SStack& sst, int sd, MStack& mst, int md,
+                           TravCounters& cnt, bool refill, Setup setup, Finish finish) {
+    Tracer<ANY, COUNT> tr;
+    if (!refill) {
+        // batches of 64: every lane of the wave runs its query to completion before the next fetch
+        for (uint32_t base = wave_fetch(cursor); base < n; base = wave_fetch(cursor)) {
+            const uint32_t i = base + __lane_id();
+            if (i < n) {
+                V3 o, d;
+                float tmax;
+                const uint32_t payload = setup(i, o, d, tmax);
+                if (tr.start(sc, o, d, tmax))
+                    while (!tr.step(sc, sst, sd, mst, md, cnt)) {}
+                finish(payload, tr);
+            }
+        }
+        return;
+    }
+    bool busy = false, exhausted = false;
+    uint32_t payload = 0;
+    const uint32_t lane = __lane_id();
+    for (;;) {
+        const uint64_t idle = __ballot(!busy);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (!exhausted && (n_idle >= 16 || n_idle == 64)) {
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(cursor, n_idle);
+            base = __shfl(base, leader);
+            exhausted = base + n_idle >= n;
+            if (!busy) {
+                const uint32_t idx = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (idx < n) {
+                    V3 o, d;
+                    float tmax;
+                    payload = setup(idx, o, d, tmax);
+                    busy = tr.start(sc, o, d, tmax);
+                    if (!busy) finish(payload, tr);                  // missed the scene bounds
+                }
+            }
+        }
+        if (__ballot(busy) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (busy && tr.step(sc, sst, sd, mst, md, cnt)) {
+            finish(payload, tr);
+            busy = false;
+        }
+    }
+}
+
 template <bool COUNT, bool PRIV>
 __global__ void k_extend(DevScene sc, Paths P, Queues q, int parity, StackCfg cfg) {
     const uint32_t* in = parity ? q.active1 : q.active0;
@@ -268,17 +325,19 @@ __global__ void k_extend(DevScene sc, Paths P, Queues q, int parity, StackCfg cf
     }
     TravCounters cnt = {0, 0, 0, 0};
     with_stacks<PRIV>(cfg, [&](auto& sst, int sd, auto& mst, int md) {
-        for (uint32_t base = wave_fetch(&q.counts[6]); base < n; base = wave_fetch(&q.counts[6])) {
-            const uint32_t i = base + __lane_id();
-            if (i < n) {
+        trace_queue<false, COUNT>(sc, n, &q.counts[6], sst, sd, mst, md, cnt, cfg.refill != 0,
+            [&](uint32_t i, V3& o, V3& d, float& tmax) -> uint32_t {
                 const uint32_t slot = in[i];
-                const float4 o4 = P.ray_o[slot], d4 = P.ray_d[slot];
-                HitRec h;
-                trace<false, COUNT>(sc, from4(o4), from4(d4), f_inf(), h, sst, sd, mst, md, cnt);
+                o = from4(P.ray_o[slot]);
+                d = from4(P.ray_d[slot]);
+                tmax = f_inf();
+                return slot;
+            },
+            [&](uint32_t slot, const Tracer<false, COUNT>& tr) {
+                const HitRec& h = tr.hit;
                 P.hit_tb[slot] = make_float4(h.t, h.b.x, h.b.y, h.b.z);
                 P.hit_ot[slot] = make_int2(h.obj, (int)h.tri);
-            }
-        }
+            });
     });
     flush_counters<COUNT>(q.totals, 0, cnt);
 }
@@ -289,15 +348,17 @@ __global__ void k_shadow(DevScene sc, Paths P, Queues q, StackCfg cfg) {
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&q.totals[17], (unsigned long long)n);
     TravCounters cnt = {0, 0, 0, 0};
     with_stacks<PRIV>(cfg, [&](auto& sst, int sd, auto& mst, int md) {
-        for (uint32_t base = wave_fetch(&q.counts[7]); base < n; base = wave_fetch(&q.counts[7])) {
-            const uint32_t i = base + __lane_id();
-            if (i < n) {
-                const float4 o4 = q.shadow_o[i], d4 = q.shadow_d[i];
-                HitRec h;
-                bool occluded = trace<true, COUNT>(sc, from4(o4), from4(d4), o4.w, h, sst, sd, mst, md, cnt);
-                if (!occluded) P.dl_light[q.shadow_path[i]] = q.shadow_c[i];
-            }
-        }
+        trace_queue<true, COUNT>(sc, n, &q.counts[7], sst, sd, mst, md, cnt, cfg.refill != 0,
+            [&](uint32_t i, V3& o, V3& d, float& tmax) -> uint32_t {
+                const float4 o4 = q.shadow_o[i];
+                o = from4(o4);
+                d = from4(q.shadow_d[i]);
+                tmax = o4.w;
+                return i;
+            },
+            [&](uint32_t i, const Tracer<true, COUNT>& tr) {
+                if (!tr.result()) P.dl_light[q.shadow_path[i]] = q.shadow_c[i];
+            });
     });
     flush_counters<COUNT>(q.totals, 1, cnt);
 }
@@ -308,21 +369,24 @@ __global__ void k_mis(DevScene sc, Paths P, Queues q, StackCfg cfg) {
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&q.totals[18], (unsigned long long)n);
     TravCounters cnt = {0, 0, 0, 0};
     with_stacks<PRIV>(cfg, [&](auto& sst, int sd, auto& mst, int md) {
-        for (uint32_t base = wave_fetch(&q.counts[8]); base < n; base = wave_fetch(&q.counts[8])) {
-            const uint32_t i = base + __lane_id();
-            if (i < n) {
+        trace_queue<false, COUNT>(sc, n, &q.counts[8], sst, sd, mst, md, cnt, cfg.refill != 0,
+            [&](uint32_t i, V3& o, V3& d, float& tmax) -> uint32_t {
+                o = from4(q.mis_o[i]);
+                d = from4(q.mis_d[i]);
+                tmax = f_inf();
+                return i;
+            },
+            [&](uint32_t i, const Tracer<false, COUNT>& tr) {
+                const HitRec& h = tr.hit;
                 const float4 o4 = q.mis_o[i], d4 = q.mis_d[i];
-                const V3 o = from4(o4);
-                HitRec h;
-                const bool found = trace<false, COUNT>(sc, o, from4(d4), f_inf(), h, sst, sd, mst, md, cnt);
                 const int light = (int)fbits(o4.w);
-                if (found && sc.obj_info[h.obj].y == light) {
+                if (tr.result() && sc.obj_info[h.obj].y == light) {
                     const DevRect L = sc.rects[light];
                     const float4 v0 = sc.tri_verts[3 * (size_t)h.tri], v1 = sc.tri_verts[3 * (size_t)h.tri + 1],
                                  v2 = sc.tri_verts[3 * (size_t)h.tri + 2];
                     // mesh-space hit point, as Triangle_Mesh::get_position (direct_lighting.cpp:181)
                     const V3 p = h.b.x * from4(v0) + h.b.y * from4(v1) + h.b.z * from4(v2);
-                    const float dd = length(p - o);
+                    const float dd = length(p - from4(o4));
                     const float lndw = d4.w;
                     const float lpdf = (dd * dd) / (L.size.x * L.size.y * lndw);
                     const float4 f4v = q.mis_f[i];
@@ -331,8 +395,7 @@ __global__ void k_mis(DevScene sc, Paths P, Queues q, StackCfg cfg) {
                     const RGB contrib = (rgb4(L.L) * rgb4(f4v)) * (w * fabsf(q.mis_ndw[i]) / bpdf);
                     P.dl_bsdf[q.mis_path[i]] = f4(contrib, 0.f);
                 }
-            }
-        }
+            });
     });
     flush_counters<COUNT>(q.totals, 2, cnt);
 }
@@ -790,7 +853,7 @@ static size_t stack_lds(const LaunchCfg& cfg) {
 
 #define YG_LAUNCH_TRAV(KERNEL, GRID, ...)                                                                  \
     do {                                                                                                   \
-        StackCfg st{cfg.scene_stack, cfg.mesh_stack};                                                      \
+        StackCfg st{cfg.scene_stack, cfg.mesh_stack, cfg.refill ? 1 : 0};                                  \
         dim3 grid_((unsigned)(GRID)), block_(cfg.block);                                                   \
         size_t shm = stack_lds(cfg);                                                                       \
         if (cfg.counters) {                                                                                \
