@@ -1399,4 +1399,55 @@ int32_t oref_render(void* sp, const yar_view* view, yar_film_pixel* film_out, fl
     return 0;
 }
 
+// BSDF probes in a local frame (T = x, B = y, n = z) for the tests' mathematical pins of the
+// oracle's BSDF code (white furnace, Fresnel closed forms, pdf normalisation, sample/pdf
+// consistency, reciprocity).  type: YAR_MATERIAL_DIFFUSE or YAR_MATERIAL_METAL.
+static Scene probe_scene(int type, const float* refl, const float* eta, const float* k, float eta_i, float ax, float ay) {
+    Scene s;
+    Material m{};
+    m.type = type;
+    m.refl = RGB{refl[0], refl[1], refl[2]};
+    m.eta = RGB{eta[0], eta[1], eta[2]};
+    m.k = RGB{k[0], k[1], k[2]};
+    m.eta_i = eta_i; m.alpha_x = ax; m.alpha_y = ay;
+    s.materials.push_back(m);
+    return s;
+}
+static Shading probe_frame() {
+    Shading sc;
+    sc.material = 0;
+    sc.T = V3(1, 0, 0); sc.B = V3(0, 1, 0); sc.n = V3(0, 0, 1); sc.ng = V3(0, 0, 1);
+    return sc;
+}
+
+void oref_bsdf_eval_local(int32_t type, const float* refl, const float* eta, const float* k, float eta_i, float ax,
+                          float ay, const float* wo, const float* wi, float* f_out, float* pdf_out) {
+    Scene s = probe_scene(type, refl, eta, k, eta_i, ax, ay);
+    Shading sc = probe_frame();
+    V3 o(wo[0], wo[1], wo[2]), i(wi[0], wi[1], wi[2]);
+    RGB f = bsdf_eval(s, sc, o, i);
+    f_out[0] = f.r; f_out[1] = f.g; f_out[2] = f.b;
+    *pdf_out = bsdf_pdf(s, sc, o, i);
+}
+
+int32_t oref_bsdf_sample_local(int32_t type, const float* refl, const float* eta, const float* k, float eta_i, float ax,
+                               float ay, const float* wo, const float* u, float* wi_out, float* f_out, float* pdf_out) {
+    Scene s = probe_scene(type, refl, eta, k, eta_i, ax, ay);
+    Shading sc = probe_frame();
+    V3 wi;
+    float pdf = 0.f;
+    RGB f = bsdf_sample(s, sc, V2{u[0], u[1]}, V3(wo[0], wo[1], wo[2]), &wi, &pdf);
+    wi_out[0] = wi.x; wi_out[1] = wi.y; wi_out[2] = wi.z;
+    f_out[0] = f.r; f_out[1] = f.g; f_out[2] = f.b;
+    *pdf_out = pdf;
+    return (f.r != 0.f || f.g != 0.f || f.b != 0.f) ? 1 : 0;
+}
+
+float oref_dielectric_fresnel(float cos_i, float eta) { return dielectric_fresnel(cos_i, eta); }
+
+void oref_conductor_fresnel(float cos_i, float eta_i, const float* eta, const float* k, float* out) {
+    RGB r = conductor_fresnel(cos_i, eta_i, RGB{eta[0], eta[1], eta[2]}, RGB{k[0], k[1], k[2]});
+    out[0] = r.r; out[1] = r.g; out[2] = r.b;
+}
+
 }  // extern "C"

@@ -182,3 +182,21 @@ def indirect_room_scene(res=64, spp_side=4, maxdepth=5, albedo=0.7, Le=50.0):
         camera={"c2w": c2w, "fov": 70.0},
         config={"film_width": res, "film_height": res, "region": (0, 0, res, res), "nx": spp_side, "ny": spp_side,
                 "maxdepth": maxdepth})
+
+
+def point_light_scene(albedo=0.5, intensity=2.0, h=1.0, spp_side=4, res=16, fov=2.0, maxdepth=1):
+    """Diffuse plane y=0 under an isotropic point light at (0, h, 0); the camera looks straight down
+    at the origin from y = h/2 with a narrow field of view.  With maxdepth 1 a point at distance r
+    from the axis has radiance albedo/pi * I * h / (h^2 + r^2)^(3/2) (direct_lighting.cpp:26-47)."""
+    plane = {"vertices": [[-50, 0, -50], [-50, 0, 50], [50, 0, 50], [50, 0, -50]], "indices": [0, 1, 2, 0, 2, 3]}
+    c2w = [1, 0, 0, 0,
+           0, 0, 1, h / 2,
+           0, -1, 0, 0]
+    return DescScene(
+        meshes=[plane],
+        materials=[{"type": _abi.YAR_MATERIAL_DIFFUSE, "reflectance": [albedo] * 3}],
+        objects=[{"mesh": 0, "material": 0}],
+        points=[{"position": [0.0, h, 0.0], "intensity": [intensity] * 3}],
+        camera={"c2w": c2w, "fov": fov},
+        config={"film_width": res, "film_height": res, "region": (0, 0, res, res), "nx": spp_side, "ny": spp_side,
+                "maxdepth": maxdepth})
