@@ -171,7 +171,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the path shards by "
-              f"tiles/views without collectives, see DESIGN.md", file=sys.stderr)
+              f"film tiles or views with no collectives (one context per GPU, Film_Tiles merged in tile order)",
+              file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calibration:
