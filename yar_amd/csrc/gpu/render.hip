@@ -433,6 +433,9 @@ YG_INLINE void finalize(const Chunk& c, const Paths& P, uint2 pk, RGB L) {      
     P.radiance[c.pix_first[pk.x] + pk.y] = f4(L, 0.f);
 }
 
+// GUIDED selects the guided form's variant at compile time, so the plain kernel keeps its own
+// (smaller) register allocation.
+template <bool GUIDED>
 __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queues q, int parity) {
     const Chunk c = *cp;
     const uint32_t* in = parity ? q.active1 : q.active0;
@@ -469,7 +472,7 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
             const float* tab = c.table + c.pix_first[p] * c.dps;
             const bool seq = c.sequential != 0;
             uint64_t rs = 0, rinc = 0;
-            if (c.guided) {
+            if (GUIDED && c.guided) {
                 rinc = guide_inc(c, pk);
                 rs = P.grng[slot];
             } else if (seq) {
@@ -618,7 +621,7 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
                     bounce++;
                     if (bounce == c.max_bounces) {
                         finished = true;
-                    } else if (c.guided) {
+                    } else if (GUIDED && c.guided) {
                         // guided continuation: record the vertex, then either sample the BSDF here
                         // (untrained pass) or queue a network query for the guide-sample stage
                         float x[kGIn];
@@ -692,14 +695,14 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
             }
             if (pending_dl) { flags |= kPendingDL; want_dl = true; if (finished) flags |= kDone; }
             else if (finished) {
-                if (c.guided) emit_records(P, q, slot, nrec, L, rs, rinc);
+                if (GUIDED && c.guided) emit_records(P, q, slot, nrec, L, rs, rinc);
                 finalize(c, P, pk, L);
             }
             P.state[slot] = (uint32_t)bounce | ((uint32_t)c1 << 8) | ((uint32_t)c2 << 16) | flags |
                             ((uint32_t)nrec << kRecShift);
             P.coeff[slot] = f4(coeff, 0.f);
             P.L[slot] = f4(L, 0.f);
-            if (c.guided) P.grng[slot] = rs;
+            if (GUIDED && c.guided) P.grng[slot] = rs;
             else if (seq) P.rng[slot] = rs;
         }
         const uint32_t ia = wave_append(n_out, want_active);
@@ -721,7 +724,7 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
             q.mis_ndw[im] = mi_ndw;
         }
         if (want_dl) q.dl_path[id] = slot;
-        if (c.guided) {
+        if (GUIDED && c.guided) {
             const uint32_t ig = wave_append(&q.counts[9], want_guide);
             if (want_guide) {
                 q.guide_path[ig] = slot;
@@ -879,7 +882,10 @@ void launch_stage(int stage, const DevScene& sc, const Chunk* c, const Paths& p,
     switch (stage) {
     case kStageReset: hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(64), 0, cfg.stream, q.counts, parity); break;
     case kStageExtend: YG_LAUNCH_TRAV(k_extend, cfg.trav_grid, sc, p, q, parity); break;
-    case kStageShade: hipLaunchKernelGGL(k_shade, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q, parity); break;
+    case kStageShade:
+        if (cfg.guided) hipLaunchKernelGGL(k_shade<true>, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q, parity);
+        else hipLaunchKernelGGL(k_shade<false>, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q, parity);
+        break;
     case kStageShadow: YG_LAUNCH_TRAV(k_shadow, cfg.trav_grid, sc, p, q); break;
     case kStageMis: YG_LAUNCH_TRAV(k_mis, cfg.trav_grid, sc, p, q); break;
     case kStageGuideForward:
