@@ -316,11 +316,13 @@ struct Tracer {
         if (in_mesh) {
             bool done = !(loop_t > mtmin);
             if (!done) {
+                // descend to the next leaf: neither the loop bound nor tmin changes on the way down
                 if (COUNT) cnt.nodes++;
-                const uint2 w = s.mesh_nodes[mnode];
-                if ((w.x & 3u) != 3u) {
+                uint2 w = s.mesh_nodes[mnode];
+                while ((w.x & 3u) != 3u) {
                     kd_interior(w, mnode, lo, ld, linv, mtmin, mtmax, mst, mdepth - 1, mdepth, msp, mcount, mdropped);
-                    return false;
+                    if (COUNT) cnt.nodes++;
+                    w = s.mesh_nodes[mnode];
                 }
                 const uint32_t n = w.x >> 2;
                 if (COUNT) cnt.leaf_refs += n;
@@ -382,10 +384,11 @@ struct Tracer {
         }
         if (!(loop_t > stmin)) return true;
         if (COUNT) cnt.nodes++;
-        const uint2 w = s.scene_nodes[snode];
-        if ((w.x & 3u) != 3u) {
+        uint2 w = s.scene_nodes[snode];
+        while ((w.x & 3u) != 3u) {
             kd_interior(w, snode, o, d, inv, stmin, stmax, sst, sdepth - 1, sdepth, ssp, scount, sdropped);
-            return false;
+            if (COUNT) cnt.nodes++;
+            w = s.scene_nodes[snode];
         }
         const uint32_t n = w.x >> 2;
         if (COUNT) cnt.leaf_refs += n;
