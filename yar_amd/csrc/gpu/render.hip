@@ -82,68 +82,119 @@ YG_INLINE void scene_frame(const DevScene& s, V3& lo, V3& inv_ext) {
 // on the item's own output run (a few hundred bytes that stay in L1/L2).
 // Table layout per pixel: image plane [2*spp], 1D dims [d1][spp], 2D dims [d2][spp][2].
 // ---------------------------------------------------------------------------------------------
-__global__ void k_sample_table(const Chunk* __restrict__ cp, float* __restrict__ table, int n_dims) {
-    const Chunk c = *cp;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= (long long)c.n_pixels * n_dims) return;
-    const int p = (int)(g / n_dims), dim = (int)(g % n_dims);
-    const DevView& v = c.views[c.pix_view[p]];
-    const int spp = v.spp, nx = v.nx, ny = v.ny;
-    uint64_t inc;
-    uint64_t st = pcg_seed_state(stream_id(c.pixels[p], v.seed_offset), inc);
-    float* out = table + c.pix_first[p] * c.dps;
-    if (dim == 0) {
-        const float dx = 1.f / float(nx), dy = 1.f / float(ny);
-        int k = 0;
-        for (int y = 0; y < ny; y++)
-            for (int x = 0; x < nx; x++, k++) {
-                const float vx = fminf((float(x) + pcg_float(st, inc)) * dx, kOneMinusEps);
-                const float vy = fminf((float(y) + pcg_float(st, inc)) * dy, kOneMinusEps);
-                out[2 * k] = vx;
-                out[2 * k + 1] = vy;
-            }
-        return;
-    }
-    if (dim <= c.d1) {
-        const int i = dim - 1;
-        st = pcg_advance(st, inc, (uint64_t)(2 * spp) + (uint64_t)i * (uint64_t)(2 * spp - 1));
-        float* X = out + 2 * spp + (long long)i * spp;
-        const float dx = 1.f / float(spp);
-        for (int x = 0; x < spp; x++) X[x] = fminf((float(x) + pcg_float(st, inc)) * dx, kOneMinusEps);
-        for (int n = spp; n > 1; n--) {
-            const uint32_t k = (uint32_t)(((uint64_t)pcg_next(st, inc) * (uint64_t)n) >> 32);
-            const float a = X[n - 1];
-            X[n - 1] = X[k];
-            X[k] = a;
-        }
-        return;
-    }
-    const int i = dim - 1 - c.d1;
-    st = pcg_advance(st, inc, (uint64_t)(2 * spp) + (uint64_t)c.d1 * (uint64_t)(2 * spp - 1) +
-                                  (uint64_t)i * (uint64_t)(3 * spp - 1));
-    float2* XY = reinterpret_cast<float2*>(out + 2 * spp + (long long)c.d1 * spp + 2 * (long long)i * spp);
-    const float dx = 1.f / float(nx), dy = 1.f / float(ny);
-    int k = 0;
-    for (int y = 0; y < ny; y++)
-        for (int x = 0; x < nx; x++, k++) {
-            const float vx = fminf((float(x) + pcg_float(st, inc)) * dx, kOneMinusEps);
-            const float vy = fminf((float(y) + pcg_float(st, inc)) * dy, kOneMinusEps);
-            XY[k] = make_float2(vx, vy);
-        }
-    for (int n = spp; n > 1; n--) {
-        const uint32_t kk = (uint32_t)(((uint64_t)pcg_next(st, inc) * (uint64_t)n) >> 32);
-        const float2 a = XY[n - 1];
-        XY[n - 1] = XY[kk];
-        XY[kk] = a;
-    }
+// Offset of (pixel p, dimension dim) in the chunk's table and the length of its run, in floats.
+YG_INLINE long long table_run(const Chunk& c, int p, int dim, int spp, int& len) {
+    long long off = c.pix_first[p] * c.dps;
+    if (dim == 0) { len = 2 * spp; return off; }
+    if (dim <= c.d1) { len = spp; return off + 2 * spp + (long long)(dim - 1) * spp; }
+    len = 2 * spp;
+    return off + 2 * spp + (long long)c.d1 * spp + 2 * (long long)(dim - 1 - c.d1) * spp;
 }
+
+// Stage index with one pad float per 32 so lanes writing runs 64 floats apart spread over banks.
+YG_INLINE int stage_index(int i) { return i + (i >> 5); }
+
+// STAGED: the block's items (consecutive (pixel, dimension) pairs, whose runs are adjacent in the
+// table) are generated and shuffled in LDS and then written out as one contiguous, coalesced range.
+template <bool STAGED>
+__global__ void k_sample_table(const Chunk* __restrict__ cp, float* __restrict__ table, int n_dims) {
+    extern __shared__ float stage[];
+    const Chunk c = *cp;
+    const long long total = (long long)c.n_pixels * n_dims;
+    const long long g0 = (long long)blockIdx.x * blockDim.x;
+    const long long g = g0 + threadIdx.x;
+    const bool valid = g < total;
+    long long gbase = 0;
+    if (STAGED) {
+        const int p0 = (int)(g0 / n_dims), d0 = (int)(g0 % n_dims);
+        int len0;
+        gbase = table_run(c, p0, d0, c.views[c.pix_view[p0]].spp, len0);
+    }
+    if (valid) {
+        const int p = (int)(g / n_dims), dim = (int)(g % n_dims);
+        const DevView& v = c.views[c.pix_view[p]];
+        const int spp = v.spp, nx = v.nx, ny = v.ny;
+        int len;
+        const long long off = table_run(c, p, dim, spp, len);
+        uint64_t inc;
+        uint64_t st = pcg_seed_state(stream_id(c.pixels[p], v.seed_offset), inc);
+        float* gout = table + off;
+        const int rel = (int)(off - gbase);
+        auto put = [&](int k, float x) {
+            if (STAGED) stage[stage_index(rel + k)] = x; else gout[k] = x;
+        };
+        auto get = [&](int k) -> float { return STAGED ? stage[stage_index(rel + k)] : gout[k]; };
+        if (dim == 0) {
+            const float dx = 1.f / float(nx), dy = 1.f / float(ny);
+            int k = 0;
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++, k++) {
+                    const float vx = fminf((float(x) + pcg_float(st, inc)) * dx, kOneMinusEps);
+                    const float vy = fminf((float(y) + pcg_float(st, inc)) * dy, kOneMinusEps);
+                    put(2 * k, vx);
+                    put(2 * k + 1, vy);
+                }
+        } else if (dim <= c.d1) {
+            const int i = dim - 1;
+            st = pcg_advance(st, inc, (uint64_t)(2 * spp) + (uint64_t)i * (uint64_t)(2 * spp - 1));
+            const float dx = 1.f / float(spp);
+            for (int x = 0; x < spp; x++) put(x, fminf((float(x) + pcg_float(st, inc)) * dx, kOneMinusEps));
+            for (int n = spp; n > 1; n--) {
+                const int k = (int)(((uint64_t)pcg_next(st, inc) * (uint64_t)n) >> 32);
+                const float a = get(n - 1);
+                put(n - 1, get(k));
+                put(k, a);
+            }
+        } else {
+            const int i = dim - 1 - c.d1;
+            st = pcg_advance(st, inc, (uint64_t)(2 * spp) + (uint64_t)c.d1 * (uint64_t)(2 * spp - 1) +
+                                          (uint64_t)i * (uint64_t)(3 * spp - 1));
+            const float dx = 1.f / float(nx), dy = 1.f / float(ny);
+            int k = 0;
+            for (int y = 0; y < ny; y++)
+                for (int x = 0; x < nx; x++, k++) {
+                    const float vx = fminf((float(x) + pcg_float(st, inc)) * dx, kOneMinusEps);
+                    const float vy = fminf((float(y) + pcg_float(st, inc)) * dy, kOneMinusEps);
+                    put(2 * k, vx);
+                    put(2 * k + 1, vy);
+                }
+            for (int n = spp; n > 1; n--) {
+                const int kk = (int)(((uint64_t)pcg_next(st, inc) * (uint64_t)n) >> 32);
+                const float ax = get(2 * (n - 1)), ay = get(2 * (n - 1) + 1);
+                put(2 * (n - 1), get(2 * kk));
+                put(2 * (n - 1) + 1, get(2 * kk + 1));
+                put(2 * kk, ax);
+                put(2 * kk + 1, ay);
+            }
+        }
+    }
+    if (!STAGED) return;
+    __syncthreads();
+    // the block's runs cover [gbase, end of its last item) without gaps
+    const long long glast = (total < g0 + blockDim.x ? total : g0 + blockDim.x) - 1;
+    const int pl = (int)(glast / n_dims), dl = (int)(glast % n_dims);
+    int lenl;
+    const long long gend = table_run(c, pl, dl, c.views[c.pix_view[pl]].spp, lenl) + lenl;
+    const int n_out = (int)(gend - gbase);
+    for (int j = threadIdx.x; j < n_out; j += blockDim.x) table[gbase + j] = stage[stage_index(j)];
+}
+
+constexpr int kTableBlock = 64;
 
 void launch_sample_table(const Chunk& hc, const Chunk* dc, float* table, hipStream_t s) {
     const int n_dims = 1 + hc.d1 + hc.d2;
-    const int T = 256;
     const long long items = (long long)hc.n_pixels * n_dims;
+    if (hc.max_spp <= 64) {
+        // at most kTableBlock runs of 2 * max_spp floats per block, plus the bank padding
+        const int floats = kTableBlock * 2 * hc.max_spp;
+        const size_t shm = (size_t)(floats + floats / 32 + 1) * sizeof(float);
+        hipLaunchKernelGGL(k_sample_table<true>, dim3((unsigned)((items + kTableBlock - 1) / kTableBlock)),
+                           dim3(kTableBlock), shm, s, dc, table, n_dims);
+        return;
+    }
+    const int T = 256;
     const long long blocks = (items + T - 1) / T;
-    hipLaunchKernelGGL(k_sample_table, dim3((unsigned)blocks), dim3(T), 0, s, dc, table, n_dims);
+    hipLaunchKernelGGL(k_sample_table<false>, dim3((unsigned)blocks), dim3(T), 0, s, dc, table, n_dims);
 }
 
 // ---------------------------------------------------------------------------------------------
