@@ -563,22 +563,23 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     // chunks of whole tiles, bounded by paths in flight, sample-table and radiance sizes
     const int64_t max_paths = ctx->opt.max_paths_in_flight > 0 ? ctx->opt.max_paths_in_flight : (int64_t)1 << 22;
     const int64_t max_samples = std::min<int64_t>(((int64_t)1 << 29) / dps, (int64_t)1 << 27);
-    struct ChunkInfo { int64_t start, pixels, samples, paths; int max_spp; };
+    struct ChunkInfo { int64_t start, pixels, samples, paths; int max_spp, min_spp; };
     std::vector<ChunkInfo> cks;
     {
-        ChunkInfo cur{0, 0, 0, 0, 1};
+        ChunkInfo cur{0, 0, 0, 0, 1, 1 << 30};
         for (const Unit& u : units) {
             const int spp = dv[u.part].spp;
             const int64_t us = u.count * spp;
             const int64_t up = guided ? u.count * pass_max : sequential ? u.count : us;
             if (cur.pixels > 0 && (cur.paths + up > max_paths || cur.samples + us > max_samples)) {
                 cks.push_back(cur);
-                cur = ChunkInfo{u.start, 0, 0, 0, 1};
+                cur = ChunkInfo{u.start, 0, 0, 0, 1, 1 << 30};
             }
             cur.pixels += u.count;
             cur.samples += us;
             cur.paths += up;
             cur.max_spp = std::max(cur.max_spp, spp);
+            cur.min_spp = std::min(cur.min_spp, spp);
         }
         if (cur.pixels > 0) cks.push_back(cur);
     }
@@ -661,6 +662,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         c.table = ctx->table.p;
         c.n_pixels = (int)ck.pixels;
         c.max_spp = ck.max_spp;
+        c.uniform_spp = ck.min_spp == ck.max_spp ? ck.max_spp : 0;
         c.d1 = d1; c.d2 = d2; c.dps = dps;
         c.sequential = sequential ? 1 : 0;
         c.n_paths = ck.paths;

@@ -93,6 +93,7 @@ struct Chunk {
     // guided form: this pass traces samples [k0, k1) of every pixel, slot = pixel * (k1 - k0) + j
     int guided, k0, k1;
     float alpha;                    // one-sample MIS probability of the learned mixture
+    int uniform_spp;                // > 0 when every pixel of the chunk has this many samples
 };
 
 // Film accumulation of one chunk (every sample's footprint as Film_Tile::add_sample computes it).

@@ -217,6 +217,10 @@ __global__ void k_raygen(const Chunk* __restrict__ cp, Paths P, Queues q) {
             p = (int)s;
             k = c.pass;
             active = k < c.views[c.pix_view[p]].spp;
+        } else if (c.uniform_spp > 0) {
+            p = (int)(s / c.uniform_spp);
+            k = (int)(s % c.uniform_spp);
+            active = true;
         } else {
             int lo = 0, hi = c.n_pixels - 1;            // pixel owning slot s: pix_first[p] <= s
             while (lo < hi) {
