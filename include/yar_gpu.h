@@ -295,7 +295,7 @@ typedef struct yar_guide_config {
     float    beta1, beta2, epsilon;     /* 0 = 0.9, 0.99, 1e-8 */
     int32_t  steps_per_pass;            /* Adam steps after each guided render pass (0 = 32) */
     float    guide_fraction;            /* one-sample MIS probability of the learned mixture (0 = 0.5) */
-    int32_t  first_pass_spp;            /* samples per pixel of the first (untrained) pass (0 = 1) */
+    int32_t  first_pass_spp;            /* samples per pixel of the first (untrained) pass (0 = 4) */
     int32_t  record_capacity;           /* training records kept per pass (0 = 2^22) */
     int32_t  reserved[6];
 } yar_guide_config;

@@ -33,7 +33,7 @@ def test_guided_mean_matches_plain_c1(c1_path):
     print(f"guided mean {_lum(img).mean():.6f} plain {_lum(ref).mean():.6f} diff {diff:.3g} sigma {sigma:.3g} "
           f"passes {st.passes} records {st.train_records} nats {st.train_nats:.3f}")
     assert np.isfinite(img).all()
-    assert st.passes == 5 and st.train_records > 0
+    assert st.passes == 3 and st.train_records > 0          # 16 spp in passes of 4, 4, 8
     assert abs(diff) < 5 * sigma
 
 
