@@ -561,7 +561,10 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         for (auto& pr : gr->passes) pass_max = std::max(pass_max, pr.second - pr.first);
 
     // chunks of whole tiles, bounded by paths in flight, sample-table and radiance sizes
-    const int64_t max_paths = ctx->opt.max_paths_in_flight > 0 ? ctx->opt.max_paths_in_flight : (int64_t)1 << 22;
+    // guided jobs revisit every chunk once per pass, so by default they keep up to 16M paths (about
+    // 4 GB of path state) in one chunk to generate each sample table once
+    const int64_t max_paths = ctx->opt.max_paths_in_flight > 0 ? ctx->opt.max_paths_in_flight
+                                                               : (int64_t)1 << (guided ? 24 : 22);
     const int64_t max_samples = std::min<int64_t>(((int64_t)1 << 29) / dps, (int64_t)1 << 27);
     struct ChunkInfo { int64_t start, pixels, samples, paths; int max_spp, min_spp; };
     std::vector<ChunkInfo> cks;
