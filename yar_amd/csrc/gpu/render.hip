@@ -58,6 +58,22 @@ YG_INLINE uint32_t stream_id(int2 px, uint32_t seed_offset) {     // reference_r
     return stream + seed_offset;
 }
 
+// Conservative ray / light-rectangle test: false only when the ray clearly misses the rectangle
+// (by more than 2% of its size plus 1e-4 per unit distance), where the closest-hit query of the
+// MIS half could not end on the light's triangles anyway; such rays need no traversal.
+YG_INLINE bool ray_may_hit_rect(const DevRect& R, V3 o, V3 d) {
+    const V3 c = v3(R.l0.w, R.l1.w, R.l2.w);
+    const V3 ex = v3(R.l0.x, R.l1.x, R.l2.x), ey = v3(R.l0.y, R.l1.y, R.l2.y), ln = v3(R.l0.z, R.l1.z, R.l2.z);
+    const float denom = dot(d, ln);
+    if (!(denom < 0.f)) return true;
+    const float t = dot(c - o, ln) / denom;
+    if (t != t) return true;
+    if (!(t > 1e-4f)) return t > -1e-4f;          // plane behind the origin: miss; at the origin: may hit
+    const V3 q = (o + t * d) - c;
+    const float mu = 0.51f * R.size.x + 1e-4f * (1.f + t), mv = 0.51f * R.size.y + 1e-4f * (1.f + t);
+    return fabsf(dot(q, ex)) <= mu && fabsf(dot(q, ey)) <= mv;
+}
+
 // Guided form: a pcg32 stream per (pixel, sample), disjoint from the pixel streams (whose upper
 // 32 sequence bits are zero), for strategy choices, mixture sampling and fallback dimensions.
 YG_INLINE uint64_t guide_stream(const Chunk& c, uint2 pk) {
@@ -665,7 +681,8 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
                                 if (lndw > 1e-4f) {
                                     mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
                                     mi_ndw = dot(sc.n, wi); mi_lndw = lndw; mi_light = ri;
-                                    want_mis = true;
+                                    // only a ray that can reach the light's rectangle can contribute
+                                    want_mis = ray_may_hit_rect(R, mi_o, wi);
                                 }
                             }
                         }
