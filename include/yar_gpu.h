@@ -41,6 +41,7 @@ extern "C" {
 #define YAR_ERR_HIP              5
 #define YAR_ERR_OUT_OF_MEMORY    6
 #define YAR_ERR_CAPACITY         7
+#define YAR_ERR_INTERNAL         8       /* any other failure inside the library (never an abort) */
 
 /* ---- scene description (host memory) ---------------------------------------------------
  * Mirrors the parts of the reference Scene (src/lib/scene.h:29) the RAY path tracer reads.
@@ -170,7 +171,8 @@ typedef struct yar_gpu_options {
     int32_t enable_timing;              /* per-kernel HIP-event timing in yar_render_stats */
     int32_t stack_mode;                 /* traversal stack storage: 0 = default, 1 = LDS, 2 = private */
     int32_t lane_refill;                /* 1 = traversal lanes take new rays individually, 0/2 = batches of 64 */
-    int32_t reserved[2];
+    int32_t disable_mis_cull;           /* 1 = trace every MIS BSDF ray (no analytic light-rectangle test) */
+    int32_t reserved[1];
 } yar_gpu_options;
 
 typedef struct yar_view {               /* Scene_Overrides analog, reference_renderer.h:23 */

@@ -73,17 +73,78 @@ def _leaf_lists(words):
     return deepest, leaves
 
 
-def test_builder_structure_invariants(c1_path):
-    scene = yar_amd.Scene(c1_path)
-    for v, idx in _meshes(scene):
+def _leaf_boxes(words, lo, hi):
+    """Leaf id lists with the box of each leaf (the root box split at every interior node's plane)."""
+    out = []
+    stack = [(0, np.array(lo, np.float64), np.array(hi, np.float64))]
+    while stack:
+        n, blo, bhi = stack.pop()
+        w0, w1 = int(words[2 * n]), int(words[2 * n + 1])
+        if (w0 & 3) == 3:
+            out.append(([int(words[2 * n + 1 + i]) for i in range(w0 >> 2)], blo, bhi))
+            continue
+        axis = w0 & 3
+        split = float(np.array([w1], np.uint32).view(np.float32)[0])
+        below_hi, above_lo = bhi.copy(), blo.copy()
+        below_hi[axis] = split
+        above_lo[axis] = split
+        stack.append((n + 1, blo, below_hi))
+        stack.append((w0 >> 2, above_lo, bhi))
+    return out
+
+
+def _tris_overlap_open_box(tri, lo, hi):
+    """Separating-axis test (13 axes) of triangles (n, 3, 3) against the open box (lo, hi), float64."""
+    c, h = 0.5 * (lo + hi), 0.5 * (hi - lo)
+    v = tri - c
+    ok = (v.min(1) < h).all(1) & (v.max(1) > -h).all(1)
+    edges = [v[:, 1] - v[:, 0], v[:, 2] - v[:, 1], v[:, 0] - v[:, 2]]
+    nrm = np.cross(edges[0], edges[1])
+    ok &= np.abs((nrm * v[:, 0]).sum(1)) < (np.abs(nrm) * h).sum(1)
+    for e in edges:
+        for k in range(3):
+            axis = np.cross(e, np.eye(3)[k])
+            p = np.einsum("nij,nj->ni", v, axis)
+            r = (np.abs(axis) * h).sum(1)
+            ok &= (p.min(1) < r) & (p.max(1) > -r)
+    return ok
+
+
+@pytest.mark.parametrize("which", ["c1", "c2_small"])
+def test_builder_structure_invariants(which, c1_path, c2_small_path):
+    # depth <= get_max_depth_limit (kdtree.cpp:199-203); every triangle that overlaps the interior of
+    # a leaf's box is listed in that leaf (the builder clips triangles to node boxes and keeps every
+    # primitive whose clipped bounds reach a child, kdtree_builder.cpp:207-301)
+    scene = yar_amd.Scene(c1_path if which == "c1" else c2_small_path)
+    checked = 0
+    for mi, (v, idx) in enumerate(_meshes(scene)):
+        if which == "c2_small" and mi > 3:
+            break
         t = yar_amd.build_kdtree(v, idx)
         deepest, leaves = _leaf_lists(t.words)
         ntri = idx.size // 3
-        limit = min(int(round(8 + 1.3 * np.floor(np.log2(ntri)))), 40)
-        assert deepest <= limit == deepest or deepest <= limit
+        limit = min(int(np.round(8 + 1.3 * np.floor(np.log2(ntri)))), 40)
+        assert deepest <= limit
         # every triangle is referenced by some leaf
         seen = set(i for lst in leaves for i in lst)
         assert seen == set(range(ntri))
+        tri = v.reshape(-1, 3).astype(np.float64)[idx.reshape(-1, 3)]
+        tlo, thi = tri.min(1), tri.max(1)
+        b = t.bounds.astype(np.float64)
+        # float32 clipping decides overlaps to a few ulps of the coordinates: shrink every leaf box by
+        # 1e-4 of its extent plus 1e-6 of the largest coordinate before demanding the listing
+        eps = 1e-6 * np.abs(b).max()
+        for ids, lo, hi in _leaf_boxes(t.words, b[:3], b[3:]):
+            ext = hi - lo
+            slo, shi = lo + 1e-4 * ext + eps, hi - 1e-4 * ext - eps
+            cand = np.nonzero((tlo < shi).all(1) & (thi > slo).all(1))[0]
+            if cand.size == 0 or np.any(shi <= slo):
+                continue
+            inside = cand[_tris_overlap_open_box(tri[cand], slo, shi)]
+            missing = set(inside.tolist()) - set(ids)
+            assert not missing, (mi, sorted(missing)[:5])
+            checked += inside.size
+    assert checked > 0
 
 
 def test_kdtree_save_layout(tmp_path):

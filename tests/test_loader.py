@@ -138,3 +138,45 @@ def test_point_light(tmp_path):
     assert d.point_light_count == 1
     assert list(d.point_lights[0].position) == [1, 2, 3]
     assert list(d.point_lights[0].intensity) == [2, 4, 6]
+
+
+def test_shape_transform_is_baked_into_world_space(tmp_path):
+    """pbrt-parser bakes the CTM into P and N (impl/semantic/Geometry.cpp:248-252), so a transformed
+    area light is detected as a rectangle in world space and every object transform is the identity."""
+    body = '''AttributeBegin
+Translate 0 3 0
+Rotate 90 0 0 1
+AreaLightSource "diffuse" "rgb L" [ 1 1 1 ]
+Shape "trianglemesh" "integer indices" [ 0 1 2 0 2 3 ] "point P" [ -1 0 -0.5  1 0 -0.5  1 0 0.5  -1 0 0.5 ]
+AttributeEnd
+AttributeBegin
+Translate 10 0 0
+Scale 2 1 1
+Material "matte"
+Shape "trianglemesh" "integer indices" [ 0 1 2 ] "point P" [ 0 0 0  1 0 0  0 1 0 ] "normal N" [ 1 1 0  1 1 0  1 1 0 ]
+AttributeEnd
+'''
+    s = _load(tmp_path, body)
+    d = s.desc
+    assert d.rect_light_count == 1 and d.object_count == 2
+    # Rotate 90 about z maps x -> y; the quad spans y in [2, 4] and z in [-0.5, 0.5] at x = 0
+    P = _arr(d.meshes[0].vertices, 12).reshape(4, 3)
+    assert np.allclose(P, [[0, 2, -0.5], [0, 4, -0.5], [0, 4, 0.5], [0, 2, 0.5]], atol=1e-6)
+    L = d.rect_lights[0]
+    l2w = np.array(L.light_to_world[:]).reshape(3, 4)
+    assert list(L.size) == pytest.approx([2.0, 1.0])
+    assert np.allclose(l2w[:, 3], [0, 3, 0], atol=1e-6)          # world-space mid point
+    assert np.allclose(l2w[:, 0], [0, 1, 0], atol=1e-6)          # first edge p0->p1, now along +y
+    assert np.allclose(l2w[:, 1], [0, 0, 1], atol=1e-6)
+    # the emitter's triangles lie inside the world-space rectangle the light samples
+    c, ex, ey = l2w[:, 3], l2w[:, 0], l2w[:, 1]
+    q = P - c
+    assert np.all(np.abs(q @ ex) <= 0.5 * L.size[0] + 1e-6) and np.all(np.abs(q @ ey) <= 0.5 * L.size[1] + 1e-6)
+    for o in (d.objects[0], d.objects[1]):
+        assert np.allclose(np.array(o.object_to_world[:]).reshape(3, 4), np.eye(3, 4))
+        assert np.allclose(np.array(o.world_to_object[:]).reshape(3, 4), np.eye(3, 4))
+    # Translate 10 0 0 * Scale 2 1 1: points scale then move; normals use the inverse transpose
+    P2 = _arr(d.meshes[1].vertices, 9).reshape(3, 3)
+    assert np.allclose(P2, [[10, 0, 0], [12, 0, 0], [10, 1, 0]])
+    N2 = _arr(d.meshes[1].normals, 9).reshape(3, 3)
+    assert np.allclose(N2, [[0.5, 1, 0]] * 3)

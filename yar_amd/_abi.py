@@ -64,7 +64,7 @@ class GpuOptions(C.Structure):
     _fields_ = [("device", C.c_int32), ("block_size", C.c_int32), ("max_paths_in_flight", C.c_int64),
                 ("traversal_stack_depth", C.c_int32), ("enable_counters", C.c_int32),
                 ("use_graph", C.c_int32), ("enable_timing", C.c_int32), ("stack_mode", C.c_int32),
-                ("lane_refill", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("lane_refill", C.c_int32), ("disable_mis_cull", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class View(C.Structure):

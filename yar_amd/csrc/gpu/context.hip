@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <memory>
 #include <cstdio>
 #include <string>
 #include <thread>
@@ -21,6 +22,7 @@ using namespace yg;
 
 struct HipError {
     std::string msg;
+    int code = YAR_ERR_HIP;
 };
 
 #define HIPX(expr)                                                                                   \
@@ -212,6 +214,24 @@ static int fail(yar_gpu_ctx* ctx, int code, const std::string& msg) {
     return code;
 }
 
+// Every extern "C" entry point runs its body through guarded(): no C++ exception crosses the C ABI;
+// each becomes a status code and the context's (or the thread's) last-error text.
+template <class Body>
+static int guarded(yar_gpu_ctx* ctx, const char* name, Body&& body) {
+    try {
+        body();
+        return YAR_OK;
+    } catch (const HipError& e) {
+        return fail(ctx, e.code, std::string(name) + ": " + e.msg);
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, YAR_ERR_OUT_OF_MEMORY, std::string(name) + ": out of host memory");
+    } catch (const std::exception& e) {
+        return fail(ctx, YAR_ERR_INTERNAL, std::string(name) + ": " + e.what());
+    } catch (...) {
+        return fail(ctx, YAR_ERR_INTERNAL, std::string(name) + ": unknown exception");
+    }
+}
+
 static void m34_rows(const float* m, float4& r0, float4& r1, float4& r2) {
     r0 = make_float4(m[0], m[1], m[2], m[3]);
     r1 = make_float4(m[4], m[5], m[6], m[7]);
@@ -261,7 +281,7 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
         mnode[i + 1] = mnode[i] + (int64_t)(trees[i].words.size() / 2);
     }
     if (mnode[d->mesh_count] >= (int64_t)0x40000000 || mtri[d->mesh_count] >= (int64_t)0x7fffffff)
-        throw HipError{"scene exceeds 2^30 kd nodes or 2^31 triangles"};
+        throw HipError{"scene exceeds 2^30 kd nodes or 2^31 triangles", YAR_ERR_CAPACITY};
     std::vector<float4> tv(3 * (size_t)mtri[d->mesh_count]);
     std::vector<int4> tvi(mtri[d->mesh_count]);
     std::vector<float4> vn(std::max<int64_t>(1, mvtx[d->mesh_count]), make_float4(0, 0, 0, 0));
@@ -348,15 +368,34 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
         g.trans = make_float4(m.transmittance[0], m.transmittance[1], m.transmittance[2], m.index_of_refraction);
         g.eta = make_float4(m.eta[0], m.eta[1], m.eta[2], 0.f);
         g.k = make_float4(m.k[0], m.k[1], m.k[2], 0.f);
-        if (m.type == YAR_MATERIAL_GLASS && m.u_roughness > 0.f) throw HipError{"rough glass is not supported"};
-        if (m.type < YAR_MATERIAL_DIFFUSE || m.type > YAR_MATERIAL_GLASS) throw HipError{"unsupported material type"};
+        if (m.type == YAR_MATERIAL_GLASS && m.u_roughness > 0.f) throw HipError{"rough glass is not supported", YAR_ERR_UNSUPPORTED};
+        if (m.type < YAR_MATERIAL_DIFFUSE || m.type > YAR_MATERIAL_GLASS) throw HipError{"unsupported material type", YAR_ERR_UNSUPPORTED};
     }
     std::vector<DevRect> rl(std::max(1, d->rect_light_count));
     for (int i = 0; i < d->rect_light_count; i++) {
         const yar_rect_light_desc& l = d->rect_lights[i];
         m34_rows(l.light_to_world, rl[i].l0, rl[i].l1, rl[i].l2);
         rl[i].L = make_float4(l.emitted_radiance[0], l.emitted_radiance[1], l.emitted_radiance[2], 0.f);
-        rl[i].size = make_float4(l.size[0], l.size[1], 0.f, 0.f);
+        rl[i].size = make_float4(l.size[0], l.size[1], ctx->opt.disable_mis_cull ? 1.f : 0.f, 0.f);
+    }
+    // The analytic MIS-ray test assumes every emitter triangle lies inside its light's rectangle; lights
+    // whose world-space emitter vertices fall outside it (by more than 1e-4 of the size) trace every ray.
+    for (int i = 0; i < d->object_count; i++) {
+        const yar_object_desc& o = d->objects[i];
+        if (o.area_light < 0) continue;
+        DevRect& R = rl[o.area_light];
+        yar::M34 m;
+        std::memcpy(m.a, o.object_to_world, sizeof(m.a));
+        const F3 c(R.l0.w, R.l1.w, R.l2.w), ex(R.l0.x, R.l1.x, R.l2.x), ey(R.l0.y, R.l1.y, R.l2.y),
+            ez(R.l0.z, R.l1.z, R.l2.z);
+        const float hx = 0.5f * R.size.x * (1.f + 1e-4f) + 1e-6f, hy = 0.5f * R.size.y * (1.f + 1e-4f) + 1e-6f;
+        const float hz = 1e-4f * std::max(R.size.x, R.size.y) + 1e-6f;
+        const yar_mesh_desc& mesh = d->meshes[o.mesh];
+        for (int64_t k = 0; k < 3 * (int64_t)mesh.triangle_count && R.size.z == 0.f; k++) {
+            const float* v = mesh.vertices + 3 * (size_t)mesh.indices[k];
+            const F3 q = yar::transform_point(m, F3(v[0], v[1], v[2])) - c;
+            if (std::fabs(dot(q, ex)) > hx || std::fabs(dot(q, ey)) > hy || std::fabs(dot(q, ez)) > hz) R.size.z = 1.f;
+        }
     }
     std::vector<DevPoint> pl(std::max(1, d->point_light_count));
     for (int i = 0; i < d->point_light_count; i++) {
@@ -514,7 +553,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
                        yar_render_stats& st, GuidedRun* gr = nullptr) {
     auto t_start = std::chrono::steady_clock::now();
     const yar_render_config& rc = ctx->cfg;
-    if (rc.pixel_filter_radius != 0.5f) throw HipError{"only the box filter of radius 0.5 is supported"};
+    if (rc.pixel_filter_radius != 0.5f) throw HipError{"only the box filter of radius 0.5 is supported", YAR_ERR_UNSUPPORTED};
     const bool guided = gr != nullptr;
     const int mdim = std::min(10, rc.max_light_bounces);
     const int d1 = 4 * std::max(0, mdim), d2 = 3 * std::max(0, mdim);          // reference_renderer.cpp:87-99
@@ -534,7 +573,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     for (size_t pi = 0; pi < parts.size(); pi++) {
         const yar_view& v = ctx->views[parts[pi].view];
         const int spp = v.x_pixel_sample_count * v.y_pixel_sample_count;
-        if (spp <= 0 || spp > 65536) throw HipError{"samples per pixel out of range"};
+        if (spp <= 0 || spp > 65536) throw HipError{"samples per pixel out of range", YAR_ERR_INVALID_ARGUMENT};
         job_max_spp = std::max(job_max_spp, spp);
         int tgx, tgy, sx0, sy0, sx1, sy1;
         film_layout(v, &tgx, &tgy, &sx0, &sy0, &sx1, &sy1);
@@ -544,7 +583,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         for (int t : parts[pi].tiles) {
             const TileGeom g = tile_geom(v, t);
             if (plist.size() + (size_t)(g.x1 - g.x0) * (g.y1 - g.y0) > (size_t)0x7fffffff)
-                throw HipError{"more than 2^31 pixels in one job"};
+                throw HipError{"more than 2^31 pixels in one job", YAR_ERR_CAPACITY};
             tile_first[base + t] = (int)plist.size();
             units.push_back({(int)pi, t, (int64_t)plist.size(), (int64_t)(g.x1 - g.x0) * (g.y1 - g.y0)});
             for (int y = g.y0; y < g.y1; y++)
@@ -560,11 +599,19 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     if (guided)
         for (auto& pr : gr->passes) pass_max = std::max(pass_max, pr.second - pr.first);
 
-    // chunks of whole tiles, bounded by paths in flight, sample-table and radiance sizes
-    // guided jobs revisit every chunk once per pass, so by default they keep up to 16M paths (about
-    // 4 GB of path state) in one chunk to generate each sample table once
-    const int64_t max_paths = ctx->opt.max_paths_in_flight > 0 ? ctx->opt.max_paths_in_flight
-                                                               : (int64_t)1 << (guided ? 24 : 22);
+    // chunks of whole tiles, bounded by paths in flight, sample-table and radiance sizes.
+    // Guided jobs revisit every chunk once per pass, so by default they keep up to 16M paths in one
+    // chunk to generate each sample table once.  Path state is about 280 B per path (plain: 14
+    // float4 rays / throughputs / direct-lighting halves / queue payloads, hit, state and 5 queue
+    // slots) and about 800 B guided (+ training vertices, network inputs and outputs): 16M guided
+    // paths hold about 13 GB.  The default never takes more than a quarter of the free device memory.
+    int64_t max_paths = ctx->opt.max_paths_in_flight > 0 ? ctx->opt.max_paths_in_flight : (int64_t)1 << (guided ? 24 : 22);
+    if (ctx->opt.max_paths_in_flight <= 0) {
+        size_t free_b = 0, total_b = 0;
+        const int64_t per_path = guided ? 800 : 280;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
+            max_paths = std::max<int64_t>((int64_t)1 << 16, std::min<int64_t>(max_paths, (int64_t)(free_b / 4) / per_path));
+    }
     const int64_t max_samples = std::min<int64_t>(((int64_t)1 << 29) / dps, (int64_t)1 << 27);
     struct ChunkInfo { int64_t start, pixels, samples, paths; int max_spp, min_spp; };
     std::vector<ChunkInfo> cks;
@@ -889,7 +936,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     double ksec[kTimerKinds];
     kt.collect(ksec);
     const uint32_t n_side = cnt[5];
-    if ((int64_t)n_side > side_cap) throw HipError{"film side list overflow"};
+    if ((int64_t)n_side > side_cap) throw HipError{"film side list overflow", YAR_ERR_CAPACITY};
     std::vector<float4> side_rad(n_side);
     std::vector<int4> side_info(n_side);
     std::vector<int> side_view(n_side);
@@ -1066,44 +1113,37 @@ int yar_gpu_create(const yar_scene_desc* scene, const yar_gpu_options* options, 
         for (int64_t k = 0; k < 3 * (int64_t)m.triangle_count; k++)
             if (m.indices[k] < 0 || m.indices[k] >= m.vertex_count) return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "mesh index out of range");
     }
-    auto* ctx = new yar_gpu_ctx;
-    if (options) ctx->opt = *options;
-    ctx->device = ctx->opt.device;
-    ctx->cfg = scene->config;
-    try {
+    std::unique_ptr<yar_gpu_ctx> ctx;
+    const int rc = guarded(nullptr, "yar_gpu_create", [&] {
+        ctx.reset(new yar_gpu_ctx);
+        if (options) ctx->opt = *options;
+        ctx->device = ctx->opt.device;
+        ctx->cfg = scene->config;
         int ndev = 0;
         HIPX(hipGetDeviceCount(&ndev));
-        if (ctx->device < 0 || ctx->device >= ndev) throw HipError{"no such HIP device"};
+        if (ctx->device < 0 || ctx->device >= ndev) throw HipError{"no such HIP device", YAR_ERR_INVALID_ARGUMENT};
         HIPX(hipSetDevice(ctx->device));
         hipDeviceProp_t prop;
         HIPX(hipGetDeviceProperties(&prop, ctx->device));
         std::snprintf(ctx->info.device_name, sizeof(ctx->info.device_name), "%s", prop.gcnArchName);
         ctx->cu_count = std::max(1, prop.multiProcessorCount);
         HIPX(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        upload_scene(ctx, scene);
-    } catch (const HipError& e) {
-        std::string msg = e.msg;
-        int code = msg.find("not supported") != std::string::npos ? YAR_ERR_UNSUPPORTED : YAR_ERR_HIP;
-        delete ctx;
-        return fail(nullptr, code, "yar_gpu_create: " + msg);
-    } catch (const std::bad_alloc&) {
-        delete ctx;
-        return fail(nullptr, YAR_ERR_OUT_OF_MEMORY, "yar_gpu_create: out of host memory");
-    }
-    yar_view v{};
-    std::memcpy(v.camera_to_world, scene->camera.camera_to_world, sizeof(v.camera_to_world));
-    v.fov_y_degrees = scene->camera.fov_y_degrees;
-    v.z_is_up = scene->camera.z_is_up;
-    v.film_width = scene->config.film_width;
-    v.film_height = scene->config.film_height;
-    v.region_x0 = scene->config.region_x0; v.region_y0 = scene->config.region_y0;
-    v.region_x1 = scene->config.region_x1; v.region_y1 = scene->config.region_y1;
-    v.x_pixel_sample_count = scene->config.x_pixel_sample_count;
-    v.y_pixel_sample_count = scene->config.y_pixel_sample_count;
-    v.rng_seed_offset = scene->config.rng_seed_offset;
-    ctx->views.assign(1, v);
-    *out_ctx = ctx;
-    return YAR_OK;
+        upload_scene(ctx.get(), scene);
+        yar_view v{};
+        std::memcpy(v.camera_to_world, scene->camera.camera_to_world, sizeof(v.camera_to_world));
+        v.fov_y_degrees = scene->camera.fov_y_degrees;
+        v.z_is_up = scene->camera.z_is_up;
+        v.film_width = scene->config.film_width;
+        v.film_height = scene->config.film_height;
+        v.region_x0 = scene->config.region_x0; v.region_y0 = scene->config.region_y0;
+        v.region_x1 = scene->config.region_x1; v.region_y1 = scene->config.region_y1;
+        v.x_pixel_sample_count = scene->config.x_pixel_sample_count;
+        v.y_pixel_sample_count = scene->config.y_pixel_sample_count;
+        v.rng_seed_offset = scene->config.rng_seed_offset;
+        ctx->views.assign(1, v);
+    });
+    if (rc == YAR_OK) *out_ctx = ctx.release();
+    return rc;
 }
 
 void yar_gpu_destroy(yar_gpu_ctx* ctx) { delete ctx; }
@@ -1115,8 +1155,7 @@ int yar_gpu_set_views(yar_gpu_ctx* ctx, const yar_view* views, int32_t count) {
     std::string why;
     for (int i = 0; i < count; i++)
         if (!valid_view(views[i], &why)) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_set_views: view " + std::to_string(i) + ": " + why);
-    ctx->views.assign(views, views + count);
-    return YAR_OK;
+    return guarded(ctx, "yar_gpu_set_views", [&] { ctx->views.assign(views, views + count); });
 }
 
 int yar_gpu_get_view(const yar_gpu_ctx* ctx, int32_t index, yar_view* out_view) {
@@ -1148,81 +1187,71 @@ int yar_gpu_render_tiles(yar_gpu_ctx* ctx, int32_t view, const int32_t* tile_ind
     int32_t n = 0;
     if (!ctx || !tile_indices || count <= 0 || !out_tiles || yar_film_tile_count(ctx, view, &n) != YAR_OK)
         return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_tiles: invalid argument");
-    std::vector<JobPart> parts(1);
-    parts[0].view = view;
-    parts[0].tiles.assign(tile_indices, tile_indices + count);
-    std::vector<char> seen(n, 0);
-    for (int t : parts[0].tiles) {
-        if (t < 0 || t >= n) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_tiles: tile index out of range");
-        if (seen[t]) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_tiles: tile listed twice");
-        seen[t] = 1;
+    std::vector<JobPart> parts;
+    {
+        const int rc = guarded(ctx, "yar_gpu_render_tiles", [&] {
+            parts.resize(1);
+            parts[0].view = view;
+            parts[0].tiles.assign(tile_indices, tile_indices + count);
+            std::vector<char> seen(n, 0);
+            for (int t : parts[0].tiles) {
+                if (t < 0 || t >= n) throw HipError{"tile index out of range", YAR_ERR_INVALID_ARGUMENT};
+                if (seen[t]) throw HipError{"tile listed twice", YAR_ERR_INVALID_ARGUMENT};
+                seen[t] = 1;
+            }
+        });
+        if (rc != YAR_OK) return rc;
     }
-    std::vector<PartResult> rr;
-    yar_render_stats st{};
-    try {
+    return guarded(ctx, "yar_gpu_render_tiles", [&] {
+        std::vector<PartResult> rr;
+        yar_render_stats st{};
         HIPX(hipSetDevice(ctx->device));
         render_job(ctx, parts, rr, st);
-    } catch (const HipError& e) {
-        return fail(ctx, YAR_ERR_HIP, "yar_gpu_render_tiles: " + e.msg);
-    } catch (const std::bad_alloc&) {
-        return fail(ctx, YAR_ERR_OUT_OF_MEMORY, "yar_gpu_render_tiles: out of host memory");
-    }
-    for (int i = 0; i < count; i++) {
-        if (out_tiles[i]) std::memcpy(out_tiles[i], rr[0].tiles[i].data(), rr[0].tiles[i].size() * sizeof(yar_film_pixel));
-        if (out_tile_variance) out_tile_variance[i] = rr[0].tile_var[i];
-    }
-    if (out_stats) *out_stats = st;
-    return YAR_OK;
+        for (int i = 0; i < count; i++) {
+            if (out_tiles[i]) std::memcpy(out_tiles[i], rr[0].tiles[i].data(), rr[0].tiles[i].size() * sizeof(yar_film_pixel));
+            if (out_tile_variance) out_tile_variance[i] = rr[0].tile_var[i];
+        }
+        if (out_stats) *out_stats = st;
+    });
 }
 
 int yar_gpu_render(yar_gpu_ctx* ctx, int32_t view, yar_film_pixel* out_film, float* out_image, double* out_variance,
                    yar_render_stats* out_stats) {
     int32_t n = 0;
     if (!ctx || yar_film_tile_count(ctx, view, &n) != YAR_OK) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render: invalid view");
-    std::vector<JobPart> parts(1);
-    parts[0].view = view;
-    for (int i = 0; i < n; i++) parts[0].tiles.push_back(i);
-    std::vector<PartResult> rr;
-    yar_render_stats st{};
-    try {
+    return guarded(ctx, "yar_gpu_render", [&] {
+        std::vector<JobPart> parts(1);
+        parts[0].view = view;
+        for (int i = 0; i < n; i++) parts[0].tiles.push_back(i);
+        std::vector<PartResult> rr;
+        yar_render_stats st{};
         HIPX(hipSetDevice(ctx->device));
         render_job(ctx, parts, rr, st);
         merge_view(ctx->views[view], rr[0], out_film, out_image, out_variance);
-    } catch (const HipError& e) {
-        return fail(ctx, YAR_ERR_HIP, "yar_gpu_render: " + e.msg);
-    } catch (const std::bad_alloc&) {
-        return fail(ctx, YAR_ERR_OUT_OF_MEMORY, "yar_gpu_render: out of host memory");
-    }
-    if (out_stats) *out_stats = st;
-    return YAR_OK;
+        if (out_stats) *out_stats = st;
+    });
 }
 
 int yar_gpu_render_batch(yar_gpu_ctx* ctx, const int32_t* views, int32_t count, float* const* out_images,
                          double* out_variances, yar_render_stats* out_stats) {
     if (!ctx || !views || count <= 0) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_batch: invalid argument");
-    std::vector<JobPart> parts(count);
-    for (int i = 0; i < count; i++) {
-        int32_t n = 0;
-        if (yar_film_tile_count(ctx, views[i], &n) != YAR_OK)
-            return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_batch: invalid view index");
-        parts[i].view = views[i];
-        for (int t = 0; t < n; t++) parts[i].tiles.push_back(t);
-    }
-    std::vector<PartResult> rr;
-    yar_render_stats st{};
-    try {
+    return guarded(ctx, "yar_gpu_render_batch", [&] {
+        std::vector<JobPart> parts(count);
+        for (int i = 0; i < count; i++) {
+            int32_t n = 0;
+            if (yar_film_tile_count(ctx, views[i], &n) != YAR_OK) throw HipError{"invalid view index", YAR_ERR_INVALID_ARGUMENT};
+            parts[i].view = views[i];
+            for (int t = 0; t < n; t++) parts[i].tiles.push_back(t);
+        }
+        std::vector<PartResult> rr;
+        yar_render_stats st{};
         HIPX(hipSetDevice(ctx->device));
         render_job(ctx, parts, rr, st);
         for (int i = 0; i < count; i++)
             merge_view(ctx->views[views[i]], rr[i], nullptr, out_images ? out_images[i] : nullptr,
                        out_variances ? &out_variances[i] : nullptr);
-    } catch (const HipError& e) {
-        return fail(ctx, YAR_ERR_HIP, "yar_gpu_render_batch: " + e.msg);
-    } catch (const std::bad_alloc&) {
-        return fail(ctx, YAR_ERR_OUT_OF_MEMORY, "yar_gpu_render_batch: out of host memory");
-    }
-    if (out_stats) *out_stats = st;
-    return YAR_OK;
+        if (out_stats) *out_stats = st;
+    });
 }
 
 int yar_gpu_render_guided(yar_gpu_ctx* ctx, yar_guide* guide, const int32_t* views, int32_t count,
@@ -1231,31 +1260,25 @@ int yar_gpu_render_guided(yar_gpu_ctx* ctx, yar_guide* guide, const int32_t* vie
         return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_guided: invalid argument");
     if (guide_device(guide) != ctx->device)
         return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_guided: guide lives on another device");
-    std::vector<JobPart> parts(count);
-    for (int i = 0; i < count; i++) {
-        int32_t n = 0;
-        if (yar_film_tile_count(ctx, views[i], &n) != YAR_OK)
-            return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_guided: invalid view index");
-        parts[i].view = views[i];
-        for (int t = 0; t < n; t++) parts[i].tiles.push_back(t);
-    }
-    std::vector<PartResult> rr;
-    yar_render_stats st{};
-    GuidedRun gr;
-    gr.guide = guide;
-    try {
+    return guarded(ctx, "yar_gpu_render_guided", [&] {
+        std::vector<JobPart> parts(count);
+        for (int i = 0; i < count; i++) {
+            int32_t n = 0;
+            if (yar_film_tile_count(ctx, views[i], &n) != YAR_OK) throw HipError{"invalid view index", YAR_ERR_INVALID_ARGUMENT};
+            parts[i].view = views[i];
+            for (int t = 0; t < n; t++) parts[i].tiles.push_back(t);
+        }
+        std::vector<PartResult> rr;
+        yar_render_stats st{};
+        GuidedRun gr;
+        gr.guide = guide;
         HIPX(hipSetDevice(ctx->device));
         render_job(ctx, parts, rr, st, &gr);
         for (int i = 0; i < count; i++)
             merge_view(ctx->views[views[i]], rr[i], nullptr, out_images ? out_images[i] : nullptr,
                        out_variances ? &out_variances[i] : nullptr);
-    } catch (const HipError& e) {
-        return fail(ctx, YAR_ERR_HIP, "yar_gpu_render_guided: " + e.msg);
-    } catch (const std::bad_alloc&) {
-        return fail(ctx, YAR_ERR_OUT_OF_MEMORY, "yar_gpu_render_guided: out of host memory");
-    }
-    if (out_stats) *out_stats = st;
-    return YAR_OK;
+        if (out_stats) *out_stats = st;
+    });
 }
 
 int yar_gpu_trace_rays_device(yar_gpu_ctx* ctx, const yar_ray* d_rays, int64_t count, int32_t any_hit, yar_hit* d_hits,
@@ -1265,7 +1288,7 @@ int yar_gpu_trace_rays(yar_gpu_ctx* ctx, const yar_ray* rays, int64_t count, int
     if (!ctx || (!rays && count > 0) || (!out_hits && count > 0) || count < 0)
         return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_trace_rays: invalid argument");
     if (count == 0) return YAR_OK;
-    try {
+    return guarded(ctx, "yar_gpu_trace_rays", [&] {
         HIPX(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
         ctx->tr_rays.reserve(2 * (size_t)count);
@@ -1297,10 +1320,7 @@ int yar_gpu_trace_rays(yar_gpu_ctx* ctx, const yar_ray* rays, int64_t count, int
         ctx->last_trace.triangle_tests = (int64_t)tot[14];
         ctx->last_trace.object_visits = (int64_t)tot[15];
         if (any_hit) ctx->last_trace.any_rays = count; else ctx->last_trace.closest_rays = count;
-    } catch (const HipError& e) {
-        return fail(ctx, YAR_ERR_HIP, "yar_gpu_trace_rays: " + e.msg);
-    }
-    return YAR_OK;
+    });
 }
 
 }  // extern "C"
@@ -1323,7 +1343,7 @@ extern "C" int yar_gpu_trace_rays_device(yar_gpu_ctx* ctx, const yar_ray* d_rays
     if (!ctx || count < 0 || (count > 0 && (!d_rays || !d_hits)))
         return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_trace_rays_device: invalid argument");
     if (count == 0) return YAR_OK;
-    try {
+    return guarded(ctx, "yar_gpu_trace_rays_device", [&] {
         HIPX(hipSetDevice(ctx->device));
         ctx->tr_tb.reserve((size_t)count);
         ctx->tr_ot.reserve((size_t)count);
@@ -1336,10 +1356,7 @@ extern "C" int yar_gpu_trace_rays_device(yar_gpu_ctx* ctx, const yar_ray* d_rays
         hipLaunchKernelGGL(k_pack_hits, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, lc.stream, ctx->tr_tb.p,
                            ctx->tr_ot.p, (long long)count, d_hits);
         HIPX(hipGetLastError());
-    } catch (const HipError& e) {
-        return fail(ctx, YAR_ERR_HIP, "yar_gpu_trace_rays_device: " + e.msg);
-    }
-    return YAR_OK;
+    });
 }
 
 extern "C" int yar_gpu_last_trace_counters(yar_gpu_ctx* ctx, yar_render_stats* out_stats) {

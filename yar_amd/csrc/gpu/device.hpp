@@ -81,7 +81,9 @@ struct DevMaterial {      // 64 bytes
     float4 eta;           // metal eta
     float4 k;             // metal k
 };
-struct DevRect { float4 l0, l1, l2; float4 L; float4 size; };   // light_to_world rows, radiance, size (xy)
+// light_to_world rows, radiance, size (xy); size.z != 0 when the analytic MIS-ray test may not be used
+// for this light (its emitter geometry is not known to lie inside the rectangle)
+struct DevRect { float4 l0, l1, l2; float4 L; float4 size; };
 struct DevPoint { float4 p; float4 I; };
 
 struct DevScene {

@@ -6,6 +6,8 @@
 // to the fp32 master copy that Adam updates.  A block stages all weights it needs in LDS (forward
 // 73 KB, backward 34 KB) and runs 4 waves in lockstep, each wave on a 16-row tile.
 #include "guide.hpp"
+
+#include <new>
 #include "../../../include/yar_gpu.h"
 #include "../host/common.hpp"
 
@@ -630,7 +632,8 @@ int64_t yar_guide_param_count(void) { return kGParams; }
 int yar_guide_create(int32_t device, const yar_guide_config* config, yar_guide** out) {
     if (!out) return gfail(nullptr, YAR_ERR_INVALID_ARGUMENT, "yar_guide_create: null argument");
     *out = nullptr;
-    auto* g = new yar_guide;
+    auto* g = new (std::nothrow) yar_guide;
+    if (!g) return gfail(nullptr, YAR_ERR_OUT_OF_MEMORY, "yar_guide_create: out of host memory");
     g->device = device;
     if (config) g->cfg = *config;
     fill_defaults(g->cfg);
@@ -676,6 +679,15 @@ int yar_guide_create(int32_t device, const yar_guide_config* config, yar_guide**
     } catch (const GuideErr& e) {
         delete g;
         return gfail(nullptr, YAR_ERR_HIP, "yar_guide_create: " + e.msg);
+    } catch (const std::bad_alloc&) {
+        delete g;
+        return gfail(nullptr, YAR_ERR_OUT_OF_MEMORY, "yar_guide_create: " + std::string("out of host memory"));
+    } catch (const std::exception& e) {
+        delete g;
+        return gfail(nullptr, YAR_ERR_INTERNAL, "yar_guide_create: " + std::string(e.what()));
+    } catch (...) {
+        delete g;
+        return gfail(nullptr, YAR_ERR_INTERNAL, "yar_guide_create: " + std::string("unknown exception"));
     }
     *out = g;
     return YAR_OK;
@@ -693,6 +705,12 @@ int yar_guide_get_params(yar_guide* g, float* out) {
         GHIP(hipStreamSynchronize(g->stream));
     } catch (const GuideErr& e) {
         return gfail(g, YAR_ERR_HIP, "yar_guide_get_params: " + e.msg);
+    } catch (const std::bad_alloc&) {
+        return gfail(g, YAR_ERR_OUT_OF_MEMORY, "yar_guide_get_params: " + std::string("out of host memory"));
+    } catch (const std::exception& e) {
+        return gfail(g, YAR_ERR_INTERNAL, "yar_guide_get_params: " + std::string(e.what()));
+    } catch (...) {
+        return gfail(g, YAR_ERR_INTERNAL, "yar_guide_get_params: " + std::string("unknown exception"));
     }
     return YAR_OK;
 }
@@ -716,6 +734,12 @@ int yar_guide_set_params(yar_guide* g, const float* params) {
         g->step = 0;
     } catch (const GuideErr& e) {
         return gfail(g, YAR_ERR_HIP, "yar_guide_set_params: " + e.msg);
+    } catch (const std::bad_alloc&) {
+        return gfail(g, YAR_ERR_OUT_OF_MEMORY, "yar_guide_set_params: " + std::string("out of host memory"));
+    } catch (const std::exception& e) {
+        return gfail(g, YAR_ERR_INTERNAL, "yar_guide_set_params: " + std::string(e.what()));
+    } catch (...) {
+        return gfail(g, YAR_ERR_INTERNAL, "yar_guide_set_params: " + std::string("unknown exception"));
     }
     return YAR_OK;
 }
@@ -734,6 +758,12 @@ int yar_guide_forward(yar_guide* g, const float* x, int64_t n, float* out) {
         GHIP(hipStreamSynchronize(g->stream));
     } catch (const GuideErr& e) {
         return gfail(g, YAR_ERR_HIP, "yar_guide_forward: " + e.msg);
+    } catch (const std::bad_alloc&) {
+        return gfail(g, YAR_ERR_OUT_OF_MEMORY, "yar_guide_forward: " + std::string("out of host memory"));
+    } catch (const std::exception& e) {
+        return gfail(g, YAR_ERR_INTERNAL, "yar_guide_forward: " + std::string(e.what()));
+    } catch (...) {
+        return gfail(g, YAR_ERR_INTERNAL, "yar_guide_forward: " + std::string("unknown exception"));
     }
     return YAR_OK;
 }
@@ -760,6 +790,12 @@ int yar_guide_gradients(yar_guide* g, const float* x, const float* dirs, const f
         if (out_loss) *out_loss = l;
     } catch (const GuideErr& e) {
         return gfail(g, YAR_ERR_HIP, "yar_guide_gradients: " + e.msg);
+    } catch (const std::bad_alloc&) {
+        return gfail(g, YAR_ERR_OUT_OF_MEMORY, "yar_guide_gradients: " + std::string("out of host memory"));
+    } catch (const std::exception& e) {
+        return gfail(g, YAR_ERR_INTERNAL, "yar_guide_gradients: " + std::string(e.what()));
+    } catch (...) {
+        return gfail(g, YAR_ERR_INTERNAL, "yar_guide_gradients: " + std::string("unknown exception"));
     }
     return YAR_OK;
 }
@@ -781,6 +817,12 @@ int yar_guide_train(yar_guide* g, const float* x, const float* dirs, const float
         if (out_loss) *out_loss = l;
     } catch (const GuideErr& e) {
         return gfail(g, YAR_ERR_HIP, "yar_guide_train: " + e.msg);
+    } catch (const std::bad_alloc&) {
+        return gfail(g, YAR_ERR_OUT_OF_MEMORY, "yar_guide_train: " + std::string("out of host memory"));
+    } catch (const std::exception& e) {
+        return gfail(g, YAR_ERR_INTERNAL, "yar_guide_train: " + std::string(e.what()));
+    } catch (...) {
+        return gfail(g, YAR_ERR_INTERNAL, "yar_guide_train: " + std::string("unknown exception"));
     }
     return YAR_OK;
 }
@@ -809,6 +851,12 @@ static int guide_mixture_call(yar_guide* g, const float* x, const float* dirs, c
         GHIP(hipStreamSynchronize(g->stream));
     } catch (const GuideErr& e) {
         return gfail(g, YAR_ERR_HIP, std::string(name) + ": " + e.msg);
+    } catch (const std::bad_alloc&) {
+        return gfail(g, YAR_ERR_OUT_OF_MEMORY, std::string(name) + ": " + std::string("out of host memory"));
+    } catch (const std::exception& e) {
+        return gfail(g, YAR_ERR_INTERNAL, std::string(name) + ": " + std::string(e.what()));
+    } catch (...) {
+        return gfail(g, YAR_ERR_INTERNAL, std::string(name) + ": " + std::string("unknown exception"));
     }
     return YAR_OK;
 }

@@ -59,9 +59,12 @@ YG_INLINE uint32_t stream_id(int2 px, uint32_t seed_offset) {     // reference_r
 }
 
 // Conservative ray / light-rectangle test: false only when the ray clearly misses the rectangle
-// (by more than 2% of its size plus 1e-4 per unit distance), where the closest-hit query of the
-// MIS half could not end on the light's triangles anyway; such rays need no traversal.
+// (by more than 2% of its size plus 1e-4 per unit distance plus 1e-5 of the coordinate magnitude),
+// where the closest-hit query of the MIS half could not end on the light's triangles anyway; such
+// rays need no traversal.  Only used for lights whose emitter triangles were verified at scene
+// creation to lie inside their rectangle (size.z == 0).
 YG_INLINE bool ray_may_hit_rect(const DevRect& R, V3 o, V3 d) {
+    if (R.size.z != 0.f) return true;
     const V3 c = v3(R.l0.w, R.l1.w, R.l2.w);
     const V3 ex = v3(R.l0.x, R.l1.x, R.l2.x), ey = v3(R.l0.y, R.l1.y, R.l2.y), ln = v3(R.l0.z, R.l1.z, R.l2.z);
     const float denom = dot(d, ln);
@@ -70,7 +73,9 @@ YG_INLINE bool ray_may_hit_rect(const DevRect& R, V3 o, V3 d) {
     if (t != t) return true;
     if (!(t > 1e-4f)) return t > -1e-4f;          // plane behind the origin: miss; at the origin: may hit
     const V3 q = (o + t * d) - c;
-    const float mu = 0.51f * R.size.x + 1e-4f * (1.f + t), mv = 0.51f * R.size.y + 1e-4f * (1.f + t);
+    const float mag = fmaxf(fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fmaxf(fabsf(o.z), fabsf(c.x))), fmaxf(fabsf(c.y), fabsf(c.z)));
+    const float tol = 1e-4f * (1.f + t) + 1e-5f * mag;
+    const float mu = 0.51f * R.size.x + tol, mv = 0.51f * R.size.y + tol;
     return fabsf(dot(q, ex)) <= mu && fabsf(dot(q, ey)) <= mv;
 }
 

@@ -8,7 +8,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <exception>
 #include <limits>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -25,6 +27,28 @@ struct Error {
     int code;
     std::string msg;
 };
+
+// Runs the body of a context-free extern "C" entry point: every C++ exception becomes a status code
+// and the thread's last-error text ("name: message"), so none crosses the C ABI.
+template <class Body>
+int abi_guard(const std::string& name, Body&& body) {
+    try {
+        body();
+        return YAR_OK;
+    } catch (const Error& e) {
+        set_last_error(name + ": " + e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error(name + ": out of host memory");
+        return YAR_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(name + ": " + e.what());
+        return YAR_ERR_INTERNAL;
+    } catch (...) {
+        set_last_error(name + ": unknown exception");
+        return YAR_ERR_INTERNAL;
+    }
+}
 
 struct F3 {
     float x = 0, y = 0, z = 0;

@@ -13,6 +13,8 @@
 // meshes are built on worker threads.
 #include "kdtree_builder.hpp"
 
+#include <memory>
+
 #include <algorithm>
 #include <atomic>
 #include <thread>
@@ -294,10 +296,11 @@ int yar_kdtree_build_mesh(const float* vertices, const int32_t* indices, int32_t
         yar::set_last_error("yar_kdtree_build_mesh: more than 2^30-1 primitives");
         return YAR_ERR_CAPACITY;
     }
-    auto* t = new yar_kdtree;
-    t->r = yar::build_mesh_kdtree(vertices, indices, triangle_count);
-    *out_tree = t;
-    return YAR_OK;
+    return yar::abi_guard("yar_kdtree_build_mesh", [&] {
+        std::unique_ptr<yar_kdtree> t(new yar_kdtree);
+        t->r = yar::build_mesh_kdtree(vertices, indices, triangle_count);
+        *out_tree = t.release();
+    });
 }
 
 int yar_kdtree_build_boxes(const float* boxes, int32_t count, yar_kdtree** out_tree) {
@@ -305,15 +308,17 @@ int yar_kdtree_build_boxes(const float* boxes, int32_t count, yar_kdtree** out_t
         yar::set_last_error("yar_kdtree_build_boxes: invalid argument");
         return YAR_ERR_INVALID_ARGUMENT;
     }
-    auto* t = new yar_kdtree;
-    t->r = yar::build_box_kdtree(boxes, count);
-    *out_tree = t;
-    return YAR_OK;
+    return yar::abi_guard("yar_kdtree_build_boxes", [&] {
+        std::unique_ptr<yar_kdtree> t(new yar_kdtree);
+        t->r = yar::build_box_kdtree(boxes, count);
+        *out_tree = t.release();
+    });
 }
 
 uint32_t yar_kdtree_node_count(const yar_kdtree* tree) { return tree ? (uint32_t)(tree->r.words.size() / 2) : 0; }
 const uint32_t* yar_kdtree_nodes(const yar_kdtree* tree) { return tree ? tree->r.words.data() : nullptr; }
 void yar_kdtree_bounds(const yar_kdtree* tree, float out_bounds[6]) {
+    if (!tree || !out_bounds) return;
     for (int a = 0; a < 3; a++) { out_bounds[a] = tree->r.bounds.lo[a]; out_bounds[3 + a] = tree->r.bounds.hi[a]; }
 }
 int32_t yar_kdtree_max_depth(const yar_kdtree* tree) { return tree ? tree->r.depth : -1; }

@@ -180,6 +180,37 @@ struct GraphicsState {
     bool reverse_orientation = false;
 };
 
+// pbrt-parser bakes a shape's current transform into its mesh (third-party/pbrt-parser/impl/semantic/
+// Geometry.cpp:248-252): points by xfmPoint = l * p + t, normals by xfmNormal = inverse_transpose(l) * n
+// with inverse_transpose(l) = adjoint_transpose(l) * (1 / det l), where l * v sums the columns of l
+// weighted by v.x, v.y, v.z in that order (include/pbrtParser/math.h:146-168).  Normals stay unnormalised.
+static void bake_transform(const M34& m, std::vector<float>& P, std::vector<float>& N) {
+    for (size_t i = 0; i + 2 < P.size(); i += 3) {
+        const F3 r = transform_point(m, F3(P[i], P[i + 1], P[i + 2]));
+        P[i] = r.x; P[i + 1] = r.y; P[i + 2] = r.z;
+    }
+    if (N.empty()) return;
+    const F3 vx = m.column(0), vy = m.column(1), vz = m.column(2);
+    const float inv_det = 1.f / dot(vx, cross(vy, vz));
+    const F3 c0 = cross(vy, vz) * inv_det, c1 = cross(vz, vx) * inv_det, c2 = cross(vx, vy) * inv_det;
+    for (size_t i = 0; i + 2 < N.size(); i += 3) {
+        const F3 n = (c0 * N[i] + c1 * N[i + 1]) + c2 * N[i + 2];
+        N[i] = n.x; N[i + 1] = n.y; N[i + 2] = n.z;
+    }
+}
+
+// pbrt-parser's affine3f::rotate (include/pbrtParser/math.h:170-177), angle in radians.
+static M34 rotation(F3 axis, float r) {
+    const F3 u = normalized(axis);
+    const float s = std::sin(r), c = std::cos(r);
+    const F3 vx(u.x * u.x + (1 - u.x * u.x) * c, u.x * u.y * (1 - c) + u.z * s, u.x * u.z * (1 - c) - u.y * s);
+    const F3 vy(u.x * u.y * (1 - c) - u.z * s, u.y * u.y + (1 - u.y * u.y) * c, u.y * u.z * (1 - c) + u.x * s);
+    const F3 vz(u.x * u.z * (1 - c) + u.y * s, u.y * u.z * (1 - c) - u.x * s, u.z * u.z + (1 - u.z * u.z) * c);
+    M34 m{};
+    m.set_column(0, vx); m.set_column(1, vy); m.set_column(2, vz); m.set_column(3, F3(0, 0, 0));
+    return m;
+}
+
 // check_if_mesh_is_rectangle, src/lib/pbrt_scene.cpp:45-121
 static bool mesh_is_rectangle(const MeshData& mesh, float size[2], M34& xf) {
     size_t nv = mesh.vertices.size() / 3;
@@ -303,6 +334,7 @@ struct Loader {
             if (N->nums.size() != mesh.vertices.size()) throw ParseError{"N count differs from P count"};
             mesh.normals = N->fnums;
         }
+        bake_transform(gs.ctm, mesh.vertices, mesh.normals);
         const Param* UV = find(ps, "uv");
         if (!UV) UV = find(ps, "st");
         if (UV) {
@@ -334,7 +366,9 @@ struct Loader {
         obj.mesh = (int32_t)sd.meshes.size();
         obj.material = -1;
         obj.area_light = -1;
-        M34 o2w = gs.ctm;
+        // the shape's transform is baked into its vertices: object space is world space
+        // (pbrt_scene.cpp:1084, instance transform and shape transform both identity)
+        M34 o2w = M34::identity();
         M34 w2o = inverse_transform(o2w);
         std::memcpy(obj.object_to_world, o2w.a, sizeof(o2w.a));
         std::memcpy(obj.world_to_object, w2o.a, sizeof(w2o.a));
@@ -385,6 +419,11 @@ struct Loader {
                 M34 x = M34::identity();
                 for (int i = 0; i < 3; i++) lx.number2(&x.a[i][3]);
                 transform(x);
+            } else if (d == "Rotate") {                                      // Parser.inl:335-339
+                float angle, ax[3];
+                lx.number2(&angle);
+                for (float& x : ax) lx.number2(&x);
+                transform(rotation(F3(ax[0], ax[1], ax[2]), angle * kPi / 180.f));
             } else if (d == "Scale") {
                 M34 x = M34::identity();
                 for (int i = 0; i < 3; i++) lx.number2(&x.a[i][i]);
@@ -558,26 +597,21 @@ extern "C" int yar_scene_load_pbrt(const char* path, yar_scene** out_scene) {
     using namespace yar;
     if (!path || !out_scene) { set_last_error("yar_scene_load_pbrt: null argument"); return YAR_ERR_INVALID_ARGUMENT; }
     *out_scene = nullptr;
-    std::ifstream f(path, std::ios::binary);
-    if (!f) { set_last_error(std::string("cannot open ") + path); return YAR_ERR_IO; }
-    std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
-    auto* scene = new yar_scene;
-    try {
-        Lexer lx{text.data(), text.data() + text.size()};
-        Loader ld(scene->data);
-        ld.run(lx);
-        ld.finish();
-    } catch (const ParseError& e) {
-        delete scene;
-        set_last_error(std::string(path) + ": " + e.msg);
-        return YAR_ERR_PARSE;
-    } catch (const std::bad_alloc&) {
-        delete scene;
-        set_last_error("out of host memory while loading scene");
-        return YAR_ERR_OUT_OF_MEMORY;
-    }
-    *out_scene = scene;
-    return YAR_OK;
+    return abi_guard(path, [&] {
+        std::ifstream f(path, std::ios::binary);
+        if (!f) throw Error{YAR_ERR_IO, "cannot open file"};
+        std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+        std::unique_ptr<yar_scene> scene(new yar_scene);
+        try {
+            Lexer lx{text.data(), text.data() + text.size()};
+            Loader ld(scene->data);
+            ld.run(lx);
+            ld.finish();
+        } catch (const ParseError& e) {
+            throw Error{YAR_ERR_PARSE, e.msg};
+        }
+        *out_scene = scene.release();
+    });
 }
 
 extern "C" const yar_scene_desc* yar_scene_get_desc(const yar_scene* scene) {

@@ -462,18 +462,22 @@ extern "C" int yar_scene_generate(int32_t config, uint64_t seed, double scale, c
         set_last_error("yar_scene_generate: invalid argument");
         return YAR_ERR_INVALID_ARGUMENT;
     }
-    std::string text;
-    switch (config) {
-        case 1: case 3: text = config1(seed, scale); break;
-        case 2: text = config2(seed, scale); break;
-        case 4: text = config4(seed, scale); break;
-        case 5: text = config5(seed, scale); break;
-        default: set_last_error("yar_scene_generate: config must be 1..5"); return YAR_ERR_INVALID_ARGUMENT;
+    if (config < 1 || config > 5) {
+        set_last_error("yar_scene_generate: config must be 1..5");
+        return YAR_ERR_INVALID_ARGUMENT;
     }
-    FILE* f = std::fopen(pbrt_path, "wb");
-    if (!f) { set_last_error(std::string("cannot write ") + pbrt_path); return YAR_ERR_IO; }
-    bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
-    ok = (std::fclose(f) == 0) && ok;
-    if (!ok) { set_last_error(std::string("write failed: ") + pbrt_path); return YAR_ERR_IO; }
-    return YAR_OK;
+    return abi_guard("yar_scene_generate", [&] {
+        std::string text;
+        switch (config) {
+            case 1: case 3: text = config1(seed, scale); break;
+            case 2: text = config2(seed, scale); break;
+            case 4: text = config4(seed, scale); break;
+            default: text = config5(seed, scale); break;
+        }
+        FILE* f = std::fopen(pbrt_path, "wb");
+        if (!f) throw Error{YAR_ERR_IO, std::string("cannot write ") + pbrt_path};
+        bool ok = std::fwrite(text.data(), 1, text.size(), f) == text.size();
+        ok = (std::fclose(f) == 0) && ok;
+        if (!ok) throw Error{YAR_ERR_IO, std::string("write failed: ") + pbrt_path};
+    });
 }
