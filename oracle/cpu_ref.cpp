@@ -920,6 +920,30 @@ static RGB direct_single(const Scene& s, const Shading& sc, float u_sel, V2 ul, 
     return (float)s.total_lights * light_rect(s, sc, li, ul, ub, us, pc, tc);
 }
 
+// Smooth glass: get_glass_info (delta_scattering.cpp:53-100), enter/exit judged by the original
+// shading-normal orientation (no nested-dielectric tracking for pbrt scenes).  Fresnel-chosen
+// reflection (always under total internal reflection) or refraction (refract, math.h:115) with the
+// transmitted radiance scaled by eta^2, eta = eta_incident / eta_transmitted.  Returns true on reflection.
+static bool glass_scatter(const Shading& sc, const Material& m, float u_scat, RGB* att, V3* dir) {
+    bool enter = !sc.flipped;
+    float etaT_over_etaI = enter ? m.ior : 1.f / m.ior;
+    float cos_i = dot(sc.n, sc.wo);
+    float F = dielectric_fresnel(cos_i, etaT_over_etaI);
+    if (u_scat < F) {
+        *att = m.refl;
+        *dir = reflect(sc.wo, sc.n);
+        return true;
+    }
+    float eta = 1.f / etaT_over_etaI;
+    *att = m.trans;
+    *att = *att * (eta * eta);
+    float ci = dot(sc.wo, sc.n);
+    float st2 = eta * eta * std::max(0.f, 1.f - ci * ci);
+    float ct = std::sqrt(1.f - st2);
+    *dir = (-eta) * sc.wo + (eta * ci - ct) * sc.n;
+    return false;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Path tracer (src/ref/path_tracing.cpp:11-133)
 // ---------------------------------------------------------------------------------------------
@@ -965,24 +989,8 @@ static RGB trace_path(const Scene& s, Ray ray, PixelSampler& ps, PathCounters& p
             ray.o = sc.origin_dir(wi);
             ray.d = wi;
         } else {
-            // get_glass_info (delta_scattering.cpp:53-100) with ad-hoc enter/exit tracking
-            bool enter = !sc.flipped;
-            float etaT_over_etaI = enter ? m.ior : 1.f / m.ior;
-            float cos_i = dot(sc.n, sc.wo);
-            float F = dielectric_fresnel(cos_i, etaT_over_etaI);
             RGB att; V3 dir;
-            if (u_scat < F) {
-                att = m.refl;
-                dir = reflect(sc.wo, sc.n);
-            } else {
-                float eta = 1.f / etaT_over_etaI;
-                att = m.trans;
-                att = att * (eta * eta);
-                float ci = dot(sc.wo, sc.n);                                          // math.h:115 refract
-                float st2 = eta * eta * std::max(0.f, 1.f - ci * ci);
-                float ct = std::sqrt(1.f - st2);
-                dir = (-eta) * sc.wo + (eta * ci - ct) * sc.n;
-            }
+            glass_scatter(sc, m, u_scat, &att, &dir);
             Ray dray{sc.origin_dir(dir), dir};
             hit = Hit{Inf, -1, 0, {}};
             pc.closest++;
@@ -1045,6 +1053,15 @@ struct TileResult {
 
 struct RenderOut { int64_t closest = 0, any = 0; Counters trav; };
 
+// Pixels [x0, x1) x [y0, y1) a sample at film position (fx, fy) reaches (Film_Tile::add_sample,
+// film.cpp:27-50): pixel centres within the filter radius, before clipping to the tile.
+static void film_footprint(float fx, float fy, float rad, int* x0, int* y0, int* x1, int* y1) {
+    *x0 = (int)std::ceil(fx - rad - 0.5f);
+    *y0 = (int)std::ceil(fy - rad - 0.5f);
+    *x1 = (int)std::floor(fx + rad - 0.5f) + 1;
+    *y1 = (int)std::floor(fy + rad - 0.5f) + 1;
+}
+
 // render_tile (reference_renderer.cpp:331-429) for one 64x64 sample tile
 static void render_tile(const Scene& s, const yar_view& view, const Camera& cam, const Region& region,
                         const Region& sb, TileResult& tr, PathCounters& pc, Counters* tc, float* per_sample) {
@@ -1083,8 +1100,8 @@ static void render_tile(const Scene& s, const yar_view& view, const Camera& cam,
                     per_sample[o] = rad_.r; per_sample[o + 1] = rad_.g; per_sample[o + 2] = rad_.b;
                 }
                 // Film_Tile::add_sample (film.cpp:27-50), box filter weight 1
-                int px0 = (int)std::ceil(fx - rad - 0.5f), py0 = (int)std::ceil(fy - rad - 0.5f);
-                int px1 = (int)std::floor(fx + rad - 0.5f) + 1, py1 = (int)std::floor(fy + rad - 0.5f) + 1;
+                int px0, py0, px1, py1;
+                film_footprint(fx, fy, rad, &px0, &py0, &px1, &py1);
                 px0 = std::max(px0, tr.pix.x0); py0 = std::max(py0, tr.pix.y0);
                 px1 = std::min(px1, tr.pix.x1); py1 = std::min(py1, tr.pix.y1);
                 for (int yy = py0; yy < py1; yy++)
@@ -1444,6 +1461,48 @@ int32_t oref_bsdf_sample_local(int32_t type, const float* refl, const float* eta
 }
 
 float oref_dielectric_fresnel(float cos_i, float eta) { return dielectric_fresnel(cos_i, eta); }
+
+// Smooth-glass event at a surface with shading normal n (facing wo), original orientation flipped or not.
+int32_t oref_glass_scatter(const float* n, const float* wo, int32_t flipped, float ior, const float* kr, const float* kt,
+                           float u_scat, float* dir_out, float* att_out) {
+    Shading sc;
+    sc.n = V3(n[0], n[1], n[2]);
+    sc.wo = V3(wo[0], wo[1], wo[2]);
+    sc.flipped = flipped != 0;
+    Material m{};
+    m.type = YAR_MATERIAL_GLASS;
+    m.ior = ior;
+    m.refl = RGB{kr[0], kr[1], kr[2]};
+    m.trans = RGB{kt[0], kt[1], kt[2]};
+    RGB att; V3 dir;
+    const bool refl = glass_scatter(sc, m, u_scat, &att, &dir);
+    dir_out[0] = dir.x; dir_out[1] = dir.y; dir_out[2] = dir.z;
+    att_out[0] = att.r; att_out[1] = att.g; att_out[2] = att.b;
+    return refl ? 1 : 0;
+}
+
+// adjust_shading_normal (shading_context.cpp:17-63): returns 1 when the normal was changed.
+int32_t oref_adjust_normal(const float* wo, const float* ng, const float* n, float* out) {
+    V3 nn(n[0], n[1], n[2]);
+    const bool changed = adjust_normal(V3(wo[0], wo[1], wo[2]), V3(ng[0], ng[1], ng[2]), &nn);
+    out[0] = nn.x; out[1] = nn.y; out[2] = nn.z;
+    return changed ? 1 : 0;
+}
+
+// Camera::generate_ray (camera.cpp:16-32) of a view at film position (fx, fy).
+void oref_camera_ray(const yar_view* view, float fx, float fy, float* o_out, float* d_out) {
+    Camera cam;
+    cam.init(*view);
+    const Ray r = cam.gen(fx, fy);
+    o_out[0] = r.o.x; o_out[1] = r.o.y; o_out[2] = r.o.z;
+    d_out[0] = r.d.x; d_out[1] = r.d.y; d_out[2] = r.d.z;
+}
+
+// Pixels [x0, x1) x [y0, y1) a film sample at pixel (x, y) + jitter reaches (Film_Tile::add_sample,
+// film.cpp:27-50, box filter of the given radius), before clipping to the tile.
+void oref_film_footprint(int32_t x, int32_t y, float jx, float jy, float radius, int32_t* out4) {
+    film_footprint((float)x + jx, (float)y + jy, radius, &out4[0], &out4[1], &out4[2], &out4[3]);
+}
 
 void oref_conductor_fresnel(float cos_i, float eta_i, const float* eta, const float* k, float* out) {
     RGB r = conductor_fresnel(cos_i, eta_i, RGB{eta[0], eta[1], eta[2]}, RGB{k[0], k[1], k[2]});

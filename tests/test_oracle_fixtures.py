@@ -92,51 +92,6 @@ def test_sampler_stratification_and_permutation():
         assert sorted(cells.tolist()) == list(range(spp))
 
 
-def test_sampler_matches_numpy_restatement():
-    # Independent restatement of next_pixel's draw order with the pcg32 stream itself.
-    nx, ny, d1, d2 = 2, 2, 3, 2
-    spp = nx * ny
-    x, y, off = 5, 9, 7
-    stream = ((x & 0xFFFF) | (y << 16)) + off
-    draws = O.pcg32(0, stream, 2 * spp + d1 * (2 * spp - 1) + d2 * (3 * spp - 1) + 4)
-    pos = 0
-
-    def f():
-        nonlocal pos
-        u = draws[pos]
-        pos += 1
-        return np.frombuffer(np.uint32((u >> 9) | 0x3F800000).tobytes(), np.float32)[0] - np.float32(1)
-
-    def bounded(n):
-        nonlocal pos
-        u = draws[pos]
-        pos += 1
-        return int((int(u) * n) >> 32)
-
-    ome = np.float32(float.fromhex('0x1.fffffep-1'))
-    want_img = [(min((np.float32(i) + f()) * np.float32(1 / nx), ome), min((np.float32(j) + f()) * np.float32(1 / ny), ome))
-                for j in range(ny) for i in range(nx)]
-    want1 = []
-    for _ in range(d1):
-        v = [min((np.float32(i) + f()) * (np.float32(1) / np.float32(spp)), ome) for i in range(spp)]
-        for n in range(spp, 1, -1):
-            k = bounded(n)
-            v[n - 1], v[k] = v[k], v[n - 1]
-        want1.append(v)
-    want2 = []
-    for _ in range(d2):
-        v = [(min((np.float32(i) + f()) * np.float32(1 / nx), ome), min((np.float32(j) + f()) * np.float32(1 / ny), ome))
-             for j in range(ny) for i in range(nx)]
-        for n in range(spp, 1, -1):
-            k = bounded(n)
-            v[n - 1], v[k] = v[k], v[n - 1]
-        want2.append(v)
-    img, s1, s2, st = O.pixel_samples(x, y, off, nx, ny, d1, d2)
-    assert np.array_equal(img, np.array(want_img, np.float32))
-    assert np.array_equal(s1, np.array(want1, np.float32))
-    assert np.array_equal(s2, np.array(want2, np.float32))
-
-
 def test_form_factor_closed_form_sanity():
     from tests.scenes import parallel_rect_form_factor
     # an infinite parallel plane has form factor 1

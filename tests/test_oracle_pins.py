@@ -1,0 +1,394 @@
+"""Pins of oracle functions that no reference fixture covers, by what the mathematics fixes:
+Snell's law, total internal reflection and Fresnel frequencies for smooth glass
+(delta_scattering.cpp:53-100), the eta^2 radiance scaling through an equilibrium (furnace) argument,
+adjust_shading_normal (shading_context.cpp:17-63) against its invariants and a closed-form planar case,
+the pinhole camera (camera.cpp:16-32) against its geometry, the box-filter footprint (film.cpp:27-50)
+against the inclusive-support definition, and the SAH split choice (kdtree_builder.cpp:322-423)
+against brute force over every candidate plane.  CPU only."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import yar_amd
+from tests import oracle_lib as O
+
+F3 = C.c_float * 3
+
+
+def _L():
+    L = O.lib()
+    P = C.c_void_p
+    L.oref_glass_scatter.argtypes = [P, P, C.c_int32, C.c_float, P, P, C.c_float, P, P]
+    L.oref_glass_scatter.restype = C.c_int32
+    L.oref_adjust_normal.argtypes = [P, P, P, P]
+    L.oref_adjust_normal.restype = C.c_int32
+    L.oref_camera_ray.argtypes = [C.POINTER(yar_amd.View), C.c_float, C.c_float, P, P]
+    L.oref_film_footprint.argtypes = [C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float, P]
+    return L
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def glass(n, wo, flipped, ior, u, kr=(1, 1, 1), kt=(1, 1, 1)):
+    n, wo, kr, kt = _f32(n), _f32(wo), _f32(kr), _f32(kt)
+    d = np.zeros(3, np.float32)
+    att = np.zeros(3, np.float32)
+    r = _L().oref_glass_scatter(n.ctypes.data, wo.ctypes.data, flipped, ior, kr.ctypes.data, kt.ctypes.data, u,
+                                d.ctypes.data, att.ctypes.data)
+    return bool(r), d.astype(np.float64), att.astype(np.float64)
+
+
+def _unit(rng, n):
+    v = rng.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def _textbook_fresnel(cos_i, n1, n2):
+    """Unpolarised Fresnel reflectance for light going from index n1 into n2 (float64)."""
+    sin_t = n1 / n2 * np.sqrt(np.maximum(0.0, 1 - cos_i ** 2))
+    if sin_t >= 1:
+        return 1.0
+    cos_t = np.sqrt(1 - sin_t ** 2)
+    rs = (n1 * cos_i - n2 * cos_t) / (n1 * cos_i + n2 * cos_t)
+    rp = (n2 * cos_i - n1 * cos_t) / (n2 * cos_i + n1 * cos_t)
+    return 0.5 * (rs * rs + rp * rp)
+
+
+@pytest.mark.parametrize("ior", [1.5, 1.33])
+def test_glass_refraction_obeys_snell(ior):
+    rng = np.random.default_rng(5)
+    ns, wos = _unit(rng, 400), _unit(rng, 400)
+    checked = 0
+    for n, wo in zip(ns, wos):
+        if wo @ n < 0:
+            wo = -wo
+        for flipped in (0, 1):
+            n1, n2 = (1.0, ior) if not flipped else (ior, 1.0)        # wo's side first
+            sin_i = np.linalg.norm(np.cross(wo, n))
+            if n1 / n2 * sin_i >= 0.999:                               # total internal reflection
+                continue
+            refl, d, att = glass(n, wo, flipped, ior, 0.9999999, kt=(0.5, 0.7, 0.9))
+            assert not refl
+            assert abs(np.linalg.norm(d) - 1) < 2e-6                   # unit direction
+            assert d @ n < 0                                           # transmitted to the far side
+            assert abs(d @ np.cross(wo, n)) < 2e-6                     # plane of incidence
+            sin_t = np.linalg.norm(np.cross(d, n))
+            assert abs(n1 * sin_i - n2 * sin_t) < 5e-6                 # Snell
+            # radiance crossing into index n2 (towards the camera: from n2 into n1) scales by (n1/n2)^2
+            assert np.allclose(att, np.array([0.5, 0.7, 0.9]) * (n1 / n2) ** 2, rtol=1e-6)
+            checked += 1
+    assert checked > 500
+
+
+def test_glass_total_internal_reflection():
+    rng = np.random.default_rng(6)
+    ior = 1.5
+    crit = np.arcsin(1 / ior)
+    for _ in range(300):
+        n = _unit(rng, 1)[0]
+        theta = rng.uniform(crit + 1e-3, np.pi / 2 - 1e-3)
+        t = np.cross(n, _unit(rng, 1)[0])
+        t /= np.linalg.norm(t)
+        wo = np.cos(theta) * n + np.sin(theta) * t
+        for u in (0.0, 0.5, 0.9999999):
+            refl, d, att = glass(n, wo, 1, ior, u, kr=(0.3, 0.6, 0.9))
+            assert refl
+            mirror = 2 * (wo @ n) * n - wo
+            assert np.allclose(d, mirror, atol=2e-6)
+            assert np.allclose(att, [0.3, 0.6, 0.9])
+
+
+@pytest.mark.parametrize("flipped", [0, 1])
+def test_glass_reflection_frequency_is_fresnel(flipped):
+    ior = 1.5
+    n = np.array([0.0, 0.0, 1.0])
+    rng = np.random.default_rng(7 + flipped)
+    for theta in (0.0, 0.4, 0.7, 1.0, 1.3, 1.5):
+        wo = np.array([np.sin(theta), 0.0, np.cos(theta)])
+        n1, n2 = (1.0, ior) if not flipped else (ior, 1.0)
+        F = _textbook_fresnel(np.cos(theta), n1, n2)
+        us = rng.uniform(size=4000).astype(np.float32)
+        frac = np.mean([glass(n, wo, flipped, ior, float(u))[0] for u in us])
+        sigma = np.sqrt(max(F * (1 - F), 1e-4) / us.size)
+        assert abs(frac - F) < 5 * sigma + 1e-3, (theta, frac, F)
+
+
+def _icosphere(level):
+    t = (1 + 5 ** 0.5) / 2
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    v = [np.array(p, float) / np.linalg.norm(p) for p in v]
+    for _ in range(level):
+        cache, nf = {}, []
+
+        def mid(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in cache:
+                m = v[a] + v[b]
+                v.append(m / np.linalg.norm(m))
+                cache[k] = len(v) - 1
+            return cache[k]
+        for a, b, c in f:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        f = nf
+    return np.array(v), np.array(f)
+
+
+def _furnace_scene(tmp_path, eye, look, name):
+    """A closed glass icosphere (radius 1, outward winding, Kr = Kt = 1) inside a cube of six
+    rectangular emitters of radiance 1 (half size 3)."""
+    v, f = _icosphere(3)
+    P = " ".join(f"{x:.8f}" for x in v.ravel())
+    I = " ".join(str(i) for i in f.ravel())
+    s = 3.0
+    quads = []
+    for axis in range(3):
+        for sign in (-1, 1):
+            c = np.zeros(3)
+            c[axis] = sign * s
+            a, b = np.eye(3)[(axis + 1) % 3] * s, np.eye(3)[(axis + 2) % 3] * s
+            pts = [c - a - b, c + a - b, c + a + b, c - a + b]
+            if sign > 0:
+                pts = pts[::-1]                                     # emit towards the centre
+            quads.append(pts)
+    body = f'LookAt {eye[0]} {eye[1]} {eye[2]}  {look[0]} {look[1]} {look[2]}  0 1 0\n'
+    body += 'Camera "perspective" "float fov" [ 50 ]\nFilm "image" "integer xresolution" [ 24 ] "integer yresolution" [ 24 ]\n'
+    body += 'Sampler "stratified" "integer xsamples" [ 2 ] "integer ysamples" [ 2 ]\n'
+    body += 'Integrator "path" "integer maxdepth" [ 16 ]\nWorldBegin\n'
+    body += f'AttributeBegin\nMaterial "glass" "float index" [ 1.5 ]\nShape "trianglemesh" "integer indices" [ {I} ] "point P" [ {P} ]\nAttributeEnd\n'
+    for q in quads:
+        pts = " ".join(f"{x:.6f}" for p in q for x in p)
+        body += ('AttributeBegin\nAreaLightSource "diffuse" "rgb L" [ 1 1 1 ]\n'
+                 f'Shape "trianglemesh" "integer indices" [ 0 1 2 0 2 3 ] "point P" [ {pts} ]\nAttributeEnd\n')
+    body += "WorldEnd\n"
+    p = tmp_path / name
+    p.write_text(body)
+    return yar_amd.Scene(str(p))
+
+
+def test_glass_furnace_radiance_scales_with_eta_squared(tmp_path):
+    # Equilibrium radiance inside a lossless dielectric of index eta surrounded by a uniform emitter L
+    # is eta^2 L (radiance / eta^2 is invariant), and back outside it is L again.
+    inside = _furnace_scene(tmp_path, (0, 0, 0), (0.3, 0.1, -1), "in.pbrt")
+    ctx = O.OracleScene(inside, threads=4)
+    v = yar_amd.View()
+    d = inside.desc
+    for name, _ in yar_amd.View._fields_:
+        pass
+    v.camera_to_world[:] = d.camera.camera_to_world[:]
+    v.fov_y_degrees = d.camera.fov_y_degrees
+    v.z_is_up = d.camera.z_is_up
+    v.film_width = v.film_height = 24
+    v.region_x1 = v.region_y1 = 24
+    v.x_pixel_sample_count = v.y_pixel_sample_count = 2
+    img, _, _, _, _ = ctx.render(v)
+    # every path refracts out exactly once (reflections keep throughput 1): each pixel is 2.25 L
+    assert np.allclose(img, 2.25, rtol=1e-5), (img.min(), img.max())
+
+    outside = _furnace_scene(tmp_path, (0, 0, 2.5), (0, 0, 0), "out.pbrt")
+    ctx = O.OracleScene(outside, threads=4)
+    v.camera_to_world[:] = outside.desc.camera.camera_to_world[:]
+    v.x_pixel_sample_count = v.y_pixel_sample_count = 8
+    img, _, _, ps, _ = ctx.render(v, per_sample=True)
+    # enter (x 1/2.25) and leave (x 2.25): radiance L; Russian roulette keeps the mean
+    assert np.median(ps) == pytest.approx(1.0, rel=1e-5)
+    assert abs(ps.mean() - 1.0) < 0.02, ps.mean()
+
+
+def adjust(wo, ng, n):
+    wo, ng, n = _f32(wo), _f32(ng), _f32(n)
+    out = np.zeros(3, np.float32)
+    ch = _L().oref_adjust_normal(wo.ctypes.data, ng.ctypes.data, n.ctypes.data, out.ctypes.data)
+    return bool(ch), out.astype(np.float64)
+
+
+def test_adjust_shading_normal_identity_and_grazing():
+    rng = np.random.default_rng(8)
+    ng = np.array([0.0, 0.0, 1.0])
+    unchanged = 0
+    for _ in range(500):
+        wo = _unit(rng, 1)[0]
+        wo[2] = abs(wo[2]) + 0.05
+        wo /= np.linalg.norm(wo)
+        n = ng + 0.2 * rng.normal(size=3)
+        n[2] = abs(n[2])
+        n /= np.linalg.norm(n)
+        r = 2 * (wo @ n) * n - wo
+        if r @ ng >= 1e-4:
+            ch, out = adjust(wo, ng, n)
+            assert not ch and np.array_equal(out.astype(np.float32), n.astype(np.float32))
+            unchanged += 1
+    assert unchanged > 100
+    # below the grazing threshold (wo . ng < 0.0017453) the geometric normal is used
+    wo = np.array([np.cos(0.001), 0.0, np.sin(0.001)])
+    n = np.array([-0.6, 0.0, 0.8])
+    ch, out = adjust(wo, ng, n)
+    assert ch and np.array_equal(out, ng)
+
+
+def test_adjust_shading_normal_closed_form_plane():
+    # planar case: ng = z, wo at angle theta from z towards +x, n tilted by alpha towards -x so that the
+    # mirror of wo about n (at angle -(theta + 2 alpha)) falls below the horizon.  The adjusted normal is
+    # the half vector of wo and the horizon direction -x lifted by 1e-4 ng.
+    ng = np.array([0.0, 0.0, 1.0])
+    for theta in (0.2, 0.5, 0.9, 1.3):
+        for alpha in (0.5, 0.7, 1.0):
+            if theta + 2 * alpha <= np.pi / 2 + 1e-3:
+                continue
+            wo = np.array([np.sin(theta), 0.0, np.cos(theta)])
+            n = np.array([-np.sin(alpha), 0.0, np.cos(alpha)])
+            ch, out = adjust(wo, ng, n)
+            h = wo + np.array([-1.0, 0.0, 1e-4])
+            assert ch and np.allclose(out, h / np.linalg.norm(h), atol=2e-6), (theta, alpha, out)
+
+
+def test_adjust_shading_normal_keeps_reflection_above_surface():
+    rng = np.random.default_rng(9)
+    changed = 0
+    for _ in range(2000):
+        ng = _unit(rng, 1)[0]
+        wo = _unit(rng, 1)[0]
+        if wo @ ng < 0.01:
+            wo = wo - 2 * (wo @ ng) * ng + 0.02 * ng
+            wo /= np.linalg.norm(wo)
+        n = ng + 0.8 * rng.normal(size=3)
+        if n @ ng < 0.05:
+            n = n - 2 * (n @ ng) * ng + 0.05 * ng
+        n /= np.linalg.norm(n)
+        ch, out = adjust(wo, ng, n)
+        assert abs(np.linalg.norm(out) - 1) < 1e-5
+        r = 2 * (wo @ out) * out - wo
+        assert r @ ng > -1e-5
+        changed += ch
+    assert changed > 200
+
+
+def _view(c2w, fov, w, h, zup=0):
+    v = yar_amd.View()
+    v.camera_to_world[:] = list(np.asarray(c2w, np.float32).ravel())
+    v.fov_y_degrees = fov
+    v.z_is_up = zup
+    v.film_width, v.film_height = w, h
+    v.region_x1, v.region_y1 = w, h
+    v.x_pixel_sample_count = v.y_pixel_sample_count = 1
+    return v
+
+
+def camray(v, fx, fy):
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    _L().oref_camera_ray(C.byref(v), fx, fy, o.ctypes.data, d.ctypes.data)
+    return o.astype(np.float64), d.astype(np.float64)
+
+
+@pytest.mark.parametrize("zup", [0, 1])
+def test_pinhole_camera_geometry(zup):
+    rng = np.random.default_rng(10)
+    a = rng.normal(size=(3, 3))
+    q, _ = np.linalg.qr(a)
+    q *= np.sign(np.linalg.det(q))                                  # rotation
+    pos = np.array([1.5, -2.0, 3.0])
+    c2w = np.concatenate([q, pos[:, None]], 1)
+    fov, w, h = 60.0, 200, 100
+    v = _view(c2w, fov, w, h, zup)
+    fwd = q @ (np.array([0, 1, 0]) if zup else np.array([0, 0, -1]))
+    up = q @ (np.array([0, 0, 1]) if zup else np.array([0, 1, 0]))
+    right = np.cross(fwd, up)
+    o, d = camray(v, w / 2, h / 2)
+    assert np.allclose(o, pos, atol=1e-6) and np.allclose(d, fwd, atol=1e-6)
+    t = np.tan(np.radians(fov / 2))
+    # top edge centre: the vertical half field of view; right edge centre: atan(aspect tan(fov/2))
+    _, d = camray(v, w / 2, 0)
+    assert np.degrees(np.arccos(d @ fwd)) == pytest.approx(fov / 2, abs=1e-4) and d @ up > 0
+    _, d = camray(v, w, h / 2)
+    assert np.arccos(d @ fwd) == pytest.approx(np.arctan(t * w / h), abs=2e-6) and d @ right > 0
+    # any film point: direction proportional to fwd + (2 fx/w - 1) t aspect right - (2 fy/h - 1) t up
+    for fx, fy in rng.uniform([0, 0], [w, h], size=(50, 2)):
+        _, d = camray(v, fx, fy)
+        e = fwd + (2 * fx / w - 1) * t * (w / h) * right - (2 * fy / h - 1) * t * up
+        assert np.allclose(d, e / np.linalg.norm(e), atol=2e-6)
+
+
+def footprint(x, y, jx, jy, r=0.5):
+    out = np.zeros(4, np.int32)
+    _L().oref_film_footprint(x, y, jx, jy, r, out.ctypes.data)
+    return out
+
+
+def test_film_footprint_is_inclusive_box_support():
+    rng = np.random.default_rng(11)
+    one_minus = np.float32(1) - np.float32(2 ** -24)
+    cases = [(10, 20, 0.0, 0.0), (10, 20, 0.5, 0.5), (0, 0, 0.0, 0.25), (7, 3, float(one_minus), 0.5),
+             ((1 << 20) + 3, 5, float(one_minus), 0.0)]
+    cases += [(int(x), int(y), float(jx), float(jy)) for x, y, jx, jy in
+              zip(rng.integers(0, 4096, 300), rng.integers(0, 4096, 300),
+                  rng.uniform(0, 1, 300).astype(np.float32), rng.uniform(0, 1, 300).astype(np.float32))]
+    for x, y, jx, jy in cases:
+        fx = float(np.float32(x) + np.float32(jx))                    # film_pos = pixel + jitter in float32
+        fy = float(np.float32(y) + np.float32(jy))
+        px = [p for p in range(x - 2, x + 3) if abs(p + 0.5 - fx) <= 0.5]
+        py = [p for p in range(y - 2, y + 3) if abs(p + 0.5 - fy) <= 0.5]
+        got = footprint(x, y, jx, jy)
+        assert (got[0], got[2], got[1], got[3]) == (px[0], px[-1] + 1, py[0], py[-1] + 1), (x, y, jx, jy, got)
+    # a zero jitter puts the sample on the corner of four pixel squares: it reaches the left/upper pixels
+    assert list(footprint(10, 20, 0.0, 0.0)) == [9, 19, 11, 21]
+    # a jitter that rounds the float sum up to the next integer reaches the pixel to the right only
+    assert list(footprint((1 << 20) + 3, 5, float(one_minus), 0.5))[:3:2] == [(1 << 20) + 3, (1 << 20) + 5]
+
+
+def _brute_root_split(lo, hi):
+    """Best SAH plane over all candidate positions (float64 costs) and its margin to the runner-up."""
+    n = lo.shape[0]
+    blo, bhi = lo.min(0), hi.max(0)
+    diag = bhi - blo
+    total = 2 * (diag[0] * diag[1] + diag[0] * diag[2] + diag[1] * diag[2])
+    cands = []
+    for axis in range(3):
+        o0, o1 = [a for a in range(3) if a != axis]
+        for t in np.unique(np.concatenate([lo[:, axis], hi[:, axis]])):
+            if not (blo[axis] < t < bhi[axis]):
+                continue
+            nb = int((lo[:, axis] < t).sum())                       # reach below the plane
+            na = int((hi[:, axis] > t).sum())                       # reach above the plane
+            ab = 2 * diag[o0] * diag[o1] + 2 * (diag[o0] + diag[o1]) * (t - blo[axis])
+            aa = 2 * diag[o0] * diag[o1] + 2 * (diag[o0] + diag[o1]) * (bhi[axis] - t)
+            bonus = 0.3 if (nb == 0 or na == 0) else 0.0
+            cands.append(((1 - bonus) * (ab / total * nb + aa / total * na), axis, float(t)))
+    cands.sort(key=lambda c: c[0])
+    return cands, n
+
+
+def test_sah_root_split_matches_brute_force():
+    rng = np.random.default_rng(12)
+    checked = leaves = 0
+    for trial in range(300):
+        n = int(rng.integers(3, 14))
+        lo = rng.uniform(0, 10, (n, 3)).astype(np.float32)
+        hi = (lo + rng.uniform(0, 3, (n, 3))).astype(np.float32)
+        if trial % 3 == 0:
+            k = rng.integers(0, 3)
+            hi[::2, k] = lo[::2, k]                                   # some planar boxes
+        words, _, _ = O.kdtree_boxes(np.concatenate([lo, hi], 1))
+        prod = yar_amd.build_box_kdtree(np.concatenate([lo, hi], 1)).words
+        assert np.array_equal(words, prod)
+        cands, n = _brute_root_split(lo.astype(np.float64), hi.astype(np.float64))
+        w0, w1 = int(words[0]), int(words[1])
+        best = cands[0] if cands else None
+        if best is None or best[0] >= n * (1 - 1e-6):
+            if best is None or best[0] > n * (1 + 1e-6):
+                assert (w0 & 3) == 3                                   # no plane beats the leaf cost N
+                leaves += 1
+            continue
+        if len(cands) > 1 and cands[1][0] - best[0] < 1e-5 * best[0]:
+            continue                                                  # near tie: float order decides
+        assert (w0 & 3) == best[1], (trial, best, w0 & 3)
+        assert np.array([w1], np.uint32).view(np.float32)[0] == np.float32(best[2])
+        checked += 1
+    assert checked > 150 and leaves >= 0
