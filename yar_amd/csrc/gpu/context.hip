@@ -122,7 +122,8 @@ struct yar_gpu_ctx {
     DevBuf<long long> pix_first;
     DevBuf<DevView> dviews;
     DevBuf<Chunk> chunks, chunk_slot;
-    DevBuf<float> table;
+    DevBuf<uint8_t> perm;                 // sampler shuffle permutations of the current chunk
+    DevBuf<ulonglong2> jump_lo, jump_hi;  // pcg32 jump-ahead tables
     DevBuf<float4> ray_o, ray_d, coeff, L, dl_coeff, dl_light, dl_bsdf, hit_tb, radiance;
     DevBuf<int2> hit_ot;
     DevBuf<uint2> pk;
@@ -178,7 +179,7 @@ struct yar_gpu_ctx {
         for (auto* b : bi) b->release();
         vtx_uv.release(); materials.release(); rects.release(); points.release();
         pixels.release(); pix_first.release(); dviews.release(); chunks.release(); chunk_slot.release();
-        table.release(); hit_ot.release(); tr_ot.release(); state.release(); rng.release();
+        perm.release(); jump_lo.release(); jump_hi.release(); hit_ot.release(); tr_ot.release(); state.release(); rng.release();
         DevBuf<uint32_t>* bu[] = {&q_a, &q_b, &sh_path, &mi_path, &dl_path, &counts};
         for (auto* b : bu) b->release();
         mi_ndw.release(); totals.release(); out_var.release(); spill.release();
@@ -463,6 +464,28 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
     inf.upload_seconds = std::chrono::duration<double>(t2 - t1).count();
 }
 
+// pcg32 jump-ahead tables (see pcg_jump): lo[r] = (M^r, G_r) for r < kJumpLo and hi[q] = (M^qL, G_qL)
+// with L = kJumpLo, where G_(n+1) = M G_n + 1 and G_0 = 0.
+static void upload_jump_tables(yar_gpu_ctx* ctx) {
+    std::vector<ulonglong2> lo(kJumpLo), hi(kJumpHi);
+    uint64_t A = 1, G = 0;
+    for (uint32_t r = 0; r < kJumpLo; r++) {
+        lo[r] = make_ulonglong2(A, G);
+        G = kPcgMult * G + 1;
+        A = kPcgMult * A;
+    }
+    const uint64_t AL = A, GL = G;
+    uint64_t a = 1, g = 0;
+    for (uint32_t q = 0; q < kJumpHi; q++) {
+        hi[q] = make_ulonglong2(a, g);
+        g = AL * g + GL;
+        a = AL * a;
+    }
+    ctx->jump_lo.upload(lo, ctx->stream);
+    ctx->jump_hi.upload(hi, ctx->stream);
+    HIPX(hipStreamSynchronize(ctx->stream));
+}
+
 // Film layout of a view (Film::Film, film.cpp:52-71; box filter radius 0.5 -> sample region ==
 // render region) and the bounds of one tile (Film::get_tile_bounds, film.cpp:73-86).
 struct TileGeom {
@@ -557,7 +580,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     const bool guided = gr != nullptr;
     const int mdim = std::min(10, rc.max_light_bounces);
     const int d1 = 4 * std::max(0, mdim), d2 = 3 * std::max(0, mdim);          // reference_renderer.cpp:87-99
-    const int dps = 2 + d1 + 2 * d2;
+    const int n_dims = d1 + d2;                                               // shuffled sampler runs per pixel
     // RNG fallback past 10 bounces couples a pixel's samples; guided passes draw those dimensions
     // from each path's own stream instead
     const bool sequential = rc.max_light_bounces > 10 && !guided;
@@ -612,7 +635,10 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
             max_paths = std::max<int64_t>((int64_t)1 << 16, std::min<int64_t>(max_paths, (int64_t)(free_b / 4) / per_path));
     }
-    const int64_t max_samples = std::min<int64_t>(((int64_t)1 << 29) / dps, (int64_t)1 << 27);
+    const int perm_bytes = job_max_spp <= 256 ? 1 : 2;
+    const int64_t draws = 2 * (int64_t)job_max_spp + d1 * (2 * (int64_t)job_max_spp - 1) + d2 * (3 * (int64_t)job_max_spp - 1);
+    if (draws >= (int64_t)kJumpLo * kJumpHi) throw HipError{"samples per pixel too large for the sampler's jump tables", YAR_ERR_CAPACITY};
+    const int64_t max_samples = std::min<int64_t>(((int64_t)1 << 31) / std::max(1, n_dims * perm_bytes), (int64_t)1 << 27);
     struct ChunkInfo { int64_t start, pixels, samples, paths; int max_spp, min_spp; };
     std::vector<ChunkInfo> cks;
     {
@@ -657,7 +683,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     // buffers
     hipStream_t s = ctx->stream;
     const size_t np = (size_t)max_chunk_paths;
-    ctx->table.reserve((size_t)(max_chunk_samples * dps));
+    ctx->perm.reserve((size_t)(max_chunk_samples * n_dims * perm_bytes));
     ctx->radiance.reserve((size_t)max_chunk_samples);
     ctx->spill.reserve((size_t)max_chunk_pixels);
     DevBuf<float4>* pb[] = {&ctx->ray_o, &ctx->ray_d, &ctx->coeff, &ctx->L, &ctx->dl_coeff, &ctx->dl_light, &ctx->dl_bsdf,
@@ -709,11 +735,12 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         c.pix_view = ctx->pix_view.p + ck.start;
         c.pix_first = ctx->pix_first.p + pf_off[ci];
         c.views = ctx->dviews.p;
-        c.table = ctx->table.p;
+        c.perm = ctx->perm.p;
+        c.jt = JumpTables{ctx->jump_lo.p, ctx->jump_hi.p};
         c.n_pixels = (int)ck.pixels;
         c.max_spp = ck.max_spp;
         c.uniform_spp = ck.min_spp == ck.max_spp ? ck.max_spp : 0;
-        c.d1 = d1; c.d2 = d2; c.dps = dps;
+        c.d1 = d1; c.d2 = d2; c.perm_bytes = perm_bytes;
         c.sequential = sequential ? 1 : 0;
         c.n_paths = ck.paths;
         c.max_bounces = rc.max_light_bounces;
@@ -896,7 +923,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         if (sp.chunk != table_chunk) {
             table_chunk = sp.chunk;
             kt.begin(4);
-            launch_sample_table(hc[sp.hc], slot, ctx->table.p, s);
+            launch_sample_perms(hc[sp.hc], slot, s);
             kt.end();
         }
         run_pass_loop(sp);
@@ -1129,6 +1156,7 @@ int yar_gpu_create(const yar_scene_desc* scene, const yar_gpu_options* options, 
         ctx->cu_count = std::max(1, prop.multiProcessorCount);
         HIPX(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         upload_scene(ctx.get(), scene);
+        upload_jump_tables(ctx.get());
         yar_view v{};
         std::memcpy(v.camera_to_world, scene->camera.camera_to_world, sizeof(v.camera_to_world));
         v.fov_y_degrees = scene->camera.fov_y_degrees;
@@ -1370,3 +1398,4 @@ extern "C" int yar_gpu_get_info(const yar_gpu_ctx* ctx, yar_gpu_info* out_info) 
     *out_info = ctx->info;
     return YAR_OK;
 }
+

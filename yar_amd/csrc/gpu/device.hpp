@@ -439,6 +439,20 @@ YG_INLINE uint64_t pcg_advance(uint64_t state, uint64_t inc, uint64_t delta) {
     }
     return acc_mult * state + acc_plus;
 }
+// Table-driven jump-ahead: the state n steps after s is A_n * s + G_n * inc (mod 2^64) with
+// A_n = M^n and G_n = M^(n-1) + ... + M + 1.  n = hi * kJumpLo + lo is looked up in two tables
+// (lo[r] = (A_r, G_r), hi[q] = (A_qL, G_qL)), so any position of a stream costs two affine steps.
+constexpr int kJumpLoBits = 12;
+constexpr uint32_t kJumpLo = 1u << kJumpLoBits;
+constexpr uint32_t kJumpHi = 4096;            // offsets below 2^24
+struct JumpTables { const ulonglong2* lo; const ulonglong2* hi; };
+YG_INLINE uint64_t pcg_jump(const JumpTables& J, uint64_t s, uint64_t inc, uint32_t n) {
+    const ulonglong2 b = J.hi[n >> kJumpLoBits];
+    const ulonglong2 a = J.lo[n & (kJumpLo - 1u)];
+    s = b.x * s + b.y * inc;
+    return a.x * s + a.y * inc;
+}
+
 // pcg32_srandom_r(rng, 0, stream): seeded state of a pixel stream.
 YG_INLINE uint64_t pcg_seed_state(uint64_t stream, uint64_t& inc) {
     inc = (stream << 1u) | 1u;

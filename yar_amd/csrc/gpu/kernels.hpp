@@ -83,10 +83,14 @@ struct Chunk {
     const int* pix_view;            // view of each pixel
     const long long* pix_first;     // prefix sums of spp over the chunk's pixels (n_pixels + 1)
     const DevView* views;
-    const float* table;             // sample tables, pixel p at pix_first[p] * dps floats
+    // Shuffle permutations of the sampler's 1D and 2D dimensions: pixel p's runs start at entry
+    // pix_first[p] * (d1 + d2), one run of spp entries per dimension (1D first), entries of
+    // perm_bytes (1 or 2) bytes.  The jittered values are recomputed from the pixel stream (jt).
+    const uint8_t* perm;
+    JumpTables jt;
     long long n_paths;              // paths of the current pass
     int n_pixels, max_spp;
-    int d1, d2, dps;                // sample-vector dims; floats per sample = 2 + d1 + 2*d2
+    int d1, d2, perm_bytes;         // sample-vector dims (reference_renderer.cpp:87-99)
     int pass, sequential;           // sequential: one sample per pixel per pass (RNG fallback)
     int max_bounces, rr_bounce_threshold;
     float rr_threshold, film_scale, max_component;
@@ -127,7 +131,7 @@ struct LaunchCfg {
     int guide_grid;
 };
 
-void launch_sample_table(const Chunk& host_chunk, const Chunk* dev_chunk, float* table, hipStream_t s);
+void launch_sample_perms(const Chunk& host_chunk, const Chunk* dev_chunk, hipStream_t s);
 void launch_raygen(const Chunk* c, long long n_paths_max, const Paths& p, Queues q, hipStream_t s);
 // Stages of one bounce-loop iteration, in launch order.
 enum IterStage {

@@ -96,126 +96,156 @@ YG_INLINE void scene_frame(const DevScene& s, V3& lo, V3& inv_ext) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Sample tables (Stratified_Pixel_Sampler::next_pixel, pixel_sampling.cpp:54-85;
-// generate_stratified_sequence_1d/2d, sampling.cpp:79-96; shuffle, math.h:138-144).
-// One work item per (pixel, dimension); the dimension's RNG draws start at a fixed offset of the
-// pixel's pcg32 stream, reached with the LCG jump-ahead.  The Fisher-Yates shuffle runs in place
-// on the item's own output run (a few hundred bytes that stay in L1/L2).
-// Table layout per pixel: image plane [2*spp], 1D dims [d1][spp], 2D dims [d2][spp][2].
+// Stratified pixel sampler (Stratified_Pixel_Sampler::next_pixel, pixel_sampling.cpp:54-85;
+// generate_stratified_sequence_1d/2d, sampling.cpp:79-96; shuffle, math.h:138-144;
+// get_next_1d/2d_sample, pixel_sampling.cpp:159-173).
+// next_pixel() draws, from the pixel's pcg32 stream in this order: the image plane (spp jittered 2D
+// values, row-major strata, x then y), then per 1D dimension spp jittered values and spp-1 shuffle
+// draws, then per 2D dimension 2 spp jittered values and spp-1 shuffle draws.  The value at shuffled
+// position k of a run is therefore stratum perm[k] jittered by the stream draw at a fixed offset, so
+// only the permutations are stored (k_sample_perms); consumers recompute values with the jump-ahead.
 // ---------------------------------------------------------------------------------------------
-// Offset of (pixel p, dimension dim) in the chunk's table and the length of its run, in floats.
-YG_INLINE long long table_run(const Chunk& c, int p, int dim, int spp, int& len) {
-    long long off = c.pix_first[p] * c.dps;
-    if (dim == 0) { len = 2 * spp; return off; }
-    if (dim <= c.d1) { len = spp; return off + 2 * spp + (long long)(dim - 1) * spp; }
-    len = 2 * spp;
-    return off + 2 * spp + (long long)c.d1 * spp + 2 * (long long)(dim - 1 - c.d1) * spp;
+YG_INLINE uint32_t dim1_base(uint32_t spp, uint32_t i) { return 2u * spp + i * (2u * spp - 1u); }
+YG_INLINE uint32_t dim2_base(uint32_t spp, uint32_t d1, uint32_t i) {
+    return 2u * spp + d1 * (2u * spp - 1u) + i * (3u * spp - 1u);
+}
+// Draws next_pixel() makes in total (the RNG fallback stream continues after them).
+YG_INLINE uint32_t sampler_draws(uint32_t spp, uint32_t d1, uint32_t d2) { return dim2_base(spp, d1, d2); }
+
+// A path's view of its pixel's sampler: seeded stream and first permutation entry.
+struct PixelSampler {
+    uint64_t s0, inc;
+    long long perm0;
+    int spp, nx, ny;
+};
+YG_INLINE PixelSampler pixel_sampler(const Chunk& c, int p) {
+    const DevView& v = c.views[c.pix_view[p]];
+    PixelSampler ps;
+    ps.s0 = pcg_seed_state(stream_id(c.pixels[p], v.seed_offset), ps.inc);
+    ps.perm0 = c.pix_first[p] * (long long)(c.d1 + c.d2);
+    ps.spp = v.spp; ps.nx = v.nx; ps.ny = v.ny;
+    return ps;
+}
+YG_INLINE uint32_t perm_entry(const Chunk& c, long long e) {
+    return c.perm_bytes == 1 ? (uint32_t)c.perm[e] : (uint32_t)reinterpret_cast<const uint16_t*>(c.perm)[e];
+}
+// generate_stratified_sequence_2d value of stratum j from the draws starting at stream offset n
+YG_INLINE void strat2(const PixelSampler& ps, const JumpTables& jt, uint32_t n, uint32_t j, float& vx, float& vy) {
+    uint64_t st = pcg_jump(jt, ps.s0, ps.inc, n);
+    const float dx = 1.f / float(ps.nx), dy = 1.f / float(ps.ny);
+    const uint32_t x = j % (uint32_t)ps.nx, y = j / (uint32_t)ps.nx;
+    vx = fminf((float(x) + pcg_float(st, ps.inc)) * dx, kOneMinusEps);
+    vy = fminf((float(y) + pcg_float(st, ps.inc)) * dy, kOneMinusEps);
+}
+// Image-plane sample k (not shuffled).
+YG_INLINE void image_sample(const PixelSampler& ps, const JumpTables& jt, int k, float& vx, float& vy) {
+    strat2(ps, jt, 2u * (uint32_t)k, (uint32_t)k, vx, vy);
+}
+// Sample k of 1D dimension i.
+YG_INLINE float sample_1d(const Chunk& c, const PixelSampler& ps, int i, int k) {
+    const uint32_t spp = (uint32_t)ps.spp;
+    const uint32_t j = perm_entry(c, ps.perm0 + (long long)i * spp + k);
+    uint64_t st = pcg_jump(c.jt, ps.s0, ps.inc, dim1_base(spp, (uint32_t)i) + j);
+    return fminf((float(j) + pcg_float(st, ps.inc)) * (1.f / float(spp)), kOneMinusEps);
+}
+// Sample k of 2D dimension i.
+YG_INLINE void sample_2d(const Chunk& c, const PixelSampler& ps, int i, int k, float& x, float& y) {
+    const uint32_t spp = (uint32_t)ps.spp;
+    const uint32_t j = perm_entry(c, ps.perm0 + (long long)(c.d1 + i) * spp + k);
+    strat2(ps, c.jt, dim2_base(spp, (uint32_t)c.d1, (uint32_t)i) + 2u * j, j, x, y);
 }
 
-// Stage index with one pad float per 32 so lanes writing runs 64 floats apart spread over banks.
-YG_INLINE int stage_index(int i) { return i + (i >> 5); }
-
-// STAGED: the block's items (consecutive (pixel, dimension) pairs, whose runs are adjacent in the
-// table) are generated and shuffled in LDS and then written out as one contiguous, coalesced range.
-template <bool STAGED>
-__global__ void k_sample_table(const Chunk* __restrict__ cp, float* __restrict__ table, int n_dims) {
-    extern __shared__ float stage[];
+// Shuffle permutations: one work item per (pixel, 1D or 2D dimension) applies the run's Fisher-Yates
+// swaps (math.h:138-144, the swap draws follow the run's value draws) to an index array and writes
+// the spp resulting stratum indices.  STAGED: the index arrays live in LDS ([entry][thread]) and the
+// block's runs, adjacent in the table, are written as one contiguous range; otherwise the swaps run in
+// place on the output (large spp).
+template <class T, bool STAGED>
+__global__ void k_sample_perms(const Chunk* __restrict__ cp, int n_dims) {
+    extern __shared__ uint32_t perm_lds[];
+    __shared__ long long run_off[256];
     const Chunk c = *cp;
+    T* out = reinterpret_cast<T*>(const_cast<uint8_t*>(c.perm));
     const long long total = (long long)c.n_pixels * n_dims;
     const long long g0 = (long long)blockIdx.x * blockDim.x;
     const long long g = g0 + threadIdx.x;
-    const bool valid = g < total;
-    long long gbase = 0;
-    if (STAGED) {
-        const int p0 = (int)(g0 / n_dims), d0 = (int)(g0 % n_dims);
-        int len0;
-        gbase = table_run(c, p0, d0, c.views[c.pix_view[p0]].spp, len0);
-    }
-    if (valid) {
-        const int p = (int)(g / n_dims), dim = (int)(g % n_dims);
+    const int nt = blockDim.x;
+    long long off = 0;
+    int spp = 0;
+    if (g < total) {
+        const int p = (int)(g / n_dims), i = (int)(g % n_dims);
         const DevView& v = c.views[c.pix_view[p]];
-        const int spp = v.spp, nx = v.nx, ny = v.ny;
-        int len;
-        const long long off = table_run(c, p, dim, spp, len);
+        spp = v.spp;
+        off = c.pix_first[p] * (long long)n_dims + (long long)i * spp;
         uint64_t inc;
-        uint64_t st = pcg_seed_state(stream_id(c.pixels[p], v.seed_offset), inc);
-        float* gout = table + off;
-        const int rel = (int)(off - gbase);
-        auto put = [&](int k, float x) {
-            if (STAGED) stage[stage_index(rel + k)] = x; else gout[k] = x;
-        };
-        auto get = [&](int k) -> float { return STAGED ? stage[stage_index(rel + k)] : gout[k]; };
-        if (dim == 0) {
-            const float dx = 1.f / float(nx), dy = 1.f / float(ny);
-            int k = 0;
-            for (int y = 0; y < ny; y++)
-                for (int x = 0; x < nx; x++, k++) {
-                    const float vx = fminf((float(x) + pcg_float(st, inc)) * dx, kOneMinusEps);
-                    const float vy = fminf((float(y) + pcg_float(st, inc)) * dy, kOneMinusEps);
-                    put(2 * k, vx);
-                    put(2 * k + 1, vy);
-                }
-        } else if (dim <= c.d1) {
-            const int i = dim - 1;
-            st = pcg_advance(st, inc, (uint64_t)(2 * spp) + (uint64_t)i * (uint64_t)(2 * spp - 1));
-            const float dx = 1.f / float(spp);
-            for (int x = 0; x < spp; x++) put(x, fminf((float(x) + pcg_float(st, inc)) * dx, kOneMinusEps));
+        const uint64_t s0 = pcg_seed_state(stream_id(c.pixels[p], v.seed_offset), inc);
+        const uint32_t first = i < c.d1 ? dim1_base((uint32_t)spp, (uint32_t)i) + (uint32_t)spp
+                                        : dim2_base((uint32_t)spp, (uint32_t)c.d1, (uint32_t)(i - c.d1)) + 2u * (uint32_t)spp;
+        uint64_t st = pcg_jump(c.jt, s0, inc, first);
+        if (STAGED) {
+            uint32_t* a = perm_lds + threadIdx.x;
+            for (int k = 0; k < spp; k++) a[k * nt] = (uint32_t)k;
             for (int n = spp; n > 1; n--) {
                 const int k = (int)(((uint64_t)pcg_next(st, inc) * (uint64_t)n) >> 32);
-                const float a = get(n - 1);
-                put(n - 1, get(k));
-                put(k, a);
+                const uint32_t x = a[(n - 1) * nt];
+                a[(n - 1) * nt] = a[k * nt];
+                a[k * nt] = x;
             }
         } else {
-            const int i = dim - 1 - c.d1;
-            st = pcg_advance(st, inc, (uint64_t)(2 * spp) + (uint64_t)c.d1 * (uint64_t)(2 * spp - 1) +
-                                          (uint64_t)i * (uint64_t)(3 * spp - 1));
-            const float dx = 1.f / float(nx), dy = 1.f / float(ny);
-            int k = 0;
-            for (int y = 0; y < ny; y++)
-                for (int x = 0; x < nx; x++, k++) {
-                    const float vx = fminf((float(x) + pcg_float(st, inc)) * dx, kOneMinusEps);
-                    const float vy = fminf((float(y) + pcg_float(st, inc)) * dy, kOneMinusEps);
-                    put(2 * k, vx);
-                    put(2 * k + 1, vy);
-                }
+            T* a = out + off;
+            for (int k = 0; k < spp; k++) a[k] = (T)k;
             for (int n = spp; n > 1; n--) {
-                const int kk = (int)(((uint64_t)pcg_next(st, inc) * (uint64_t)n) >> 32);
-                const float ax = get(2 * (n - 1)), ay = get(2 * (n - 1) + 1);
-                put(2 * (n - 1), get(2 * kk));
-                put(2 * (n - 1) + 1, get(2 * kk + 1));
-                put(2 * kk, ax);
-                put(2 * kk + 1, ay);
+                const int k = (int)(((uint64_t)pcg_next(st, inc) * (uint64_t)n) >> 32);
+                const T x = a[n - 1];
+                a[n - 1] = a[k];
+                a[k] = x;
             }
         }
     }
     if (!STAGED) return;
+    // coalesced write-out of the block's contiguous range [run_off[0], end)
+    run_off[threadIdx.x] = g < total ? off : -1;
     __syncthreads();
-    // the block's runs cover [gbase, end of its last item) without gaps
-    const long long glast = (total < g0 + blockDim.x ? total : g0 + blockDim.x) - 1;
-    const int pl = (int)(glast / n_dims), dl = (int)(glast % n_dims);
-    int lenl;
-    const long long gend = table_run(c, pl, dl, c.views[c.pix_view[pl]].spp, lenl) + lenl;
-    const int n_out = (int)(gend - gbase);
-    for (int j = threadIdx.x; j < n_out; j += blockDim.x) table[gbase + j] = stage[stage_index(j)];
+    const long long base = run_off[0];
+    const int last = (int)((total < g0 + nt ? total : g0 + nt) - g0) - 1;
+    __shared__ long long end_s;
+    if ((int)threadIdx.x == last) end_s = off + spp;
+    __syncthreads();
+    const int n_out = (int)(end_s - base);
+    const int uni = c.uniform_spp;
+    for (int j = threadIdx.x; j < n_out; j += nt) {
+        int t, k;
+        if (uni > 0) {
+            t = j / uni;
+            k = j - t * uni;
+        } else {                                    // the run holding entry j: last t with run_off[t] - base <= j
+            int lo = 0, hi = last;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (run_off[mid] - base <= j) lo = mid; else hi = mid - 1;
+            }
+            t = lo;
+            k = (int)(base + j - run_off[t]);
+        }
+        out[base + j] = (T)perm_lds[k * nt + t];
+    }
 }
 
-constexpr int kTableBlock = 64;
+constexpr int kPermBlock = 64;
 
-void launch_sample_table(const Chunk& hc, const Chunk* dc, float* table, hipStream_t s) {
-    const int n_dims = 1 + hc.d1 + hc.d2;
+void launch_sample_perms(const Chunk& hc, const Chunk* dc, hipStream_t s) {
+    const int n_dims = hc.d1 + hc.d2;
+    if (n_dims == 0) return;
     const long long items = (long long)hc.n_pixels * n_dims;
-    if (hc.max_spp <= 64) {
-        // at most kTableBlock runs of 2 * max_spp floats per block, plus the bank padding
-        const int floats = kTableBlock * 2 * hc.max_spp;
-        const size_t shm = (size_t)(floats + floats / 32 + 1) * sizeof(float);
-        hipLaunchKernelGGL(k_sample_table<true>, dim3((unsigned)((items + kTableBlock - 1) / kTableBlock)),
-                           dim3(kTableBlock), shm, s, dc, table, n_dims);
+    const unsigned blocks = (unsigned)((items + kPermBlock - 1) / kPermBlock);
+    if (hc.max_spp <= 128) {
+        const size_t shm = (size_t)kPermBlock * hc.max_spp * sizeof(uint32_t);
+        if (hc.perm_bytes == 1) hipLaunchKernelGGL((k_sample_perms<uint8_t, true>), dim3(blocks), dim3(kPermBlock), shm, s, dc, n_dims);
+        else hipLaunchKernelGGL((k_sample_perms<uint16_t, true>), dim3(blocks), dim3(kPermBlock), shm, s, dc, n_dims);
         return;
     }
-    const int T = 256;
-    const long long blocks = (items + T - 1) / T;
-    hipLaunchKernelGGL(k_sample_table<false>, dim3((unsigned)blocks), dim3(T), 0, s, dc, table, n_dims);
+    if (hc.perm_bytes == 1) hipLaunchKernelGGL((k_sample_perms<uint8_t, false>), dim3(blocks), dim3(kPermBlock), 0, s, dc, n_dims);
+    else hipLaunchKernelGGL((k_sample_perms<uint16_t, false>), dim3(blocks), dim3(kPermBlock), 0, s, dc, n_dims);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -255,10 +285,12 @@ __global__ void k_raygen(const Chunk* __restrict__ cp, Paths P, Queues q) {
     }
     if (active) {
         const DevView& v = c.views[c.pix_view[p]];
-        const float* tab = c.table + c.pix_first[p] * c.dps;
+        const PixelSampler ps = pixel_sampler(c, p);
         const int2 px = c.pixels[p];
-        const float fx = (float)px.x + tab[2 * k];
-        const float fy = (float)px.y + tab[2 * k + 1];
+        float jx, jy;
+        image_sample(ps, c.jt, k, jx, jy);
+        const float fx = (float)px.x + jx;
+        const float fy = (float)px.y + jy;
         const float u = 2.f * (fx / v.W) - 1.f;
         const float vv = 2.f * (fy / v.H) - 1.f;
         const float right = u * v.horz, up = -vv * v.vert;
@@ -279,11 +311,7 @@ __global__ void k_raygen(const Chunk* __restrict__ cp, Paths P, Queues q) {
         }
         if (c.sequential && c.pass == 0) {
             // fallback stream: the pixel's RNG after next_pixel() drew every table value
-            uint64_t inc;
-            uint64_t st = pcg_seed_state(stream_id(px, v.seed_offset), inc);
-            const uint64_t spp = (uint64_t)v.spp;
-            uint64_t draws = 2 * spp + (uint64_t)c.d1 * (2 * spp - 1) + (uint64_t)c.d2 * (3 * spp - 1);
-            P.rng[s] = pcg_advance(st, inc, draws);
+            P.rng[s] = pcg_jump(c.jt, ps.s0, ps.inc, sampler_draws((uint32_t)v.spp, (uint32_t)c.d1, (uint32_t)c.d2));
         }
     }
     const uint32_t idx = wave_append(&q.counts[0], active);
@@ -513,7 +541,7 @@ YG_INLINE void finalize(const Chunk& c, const Paths& P, uint2 pk, RGB L) {      
 // (smaller) register allocation.
 template <bool GUIDED>
 __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queues q, int parity) {
-    const Chunk c = *cp;
+    const Chunk& c = *cp;
     const uint32_t* in = parity ? q.active1 : q.active0;
     uint32_t* out = parity ? q.active0 : q.active1;
     uint32_t* n_out = &q.counts[parity ^ 1];
@@ -544,8 +572,7 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
             const uint2 pk = P.pk[slot];
             const int p = (int)pk.x, k = (int)pk.y;
             const DevView& v = c.views[c.pix_view[p]];
-            const int spp = v.spp;
-            const float* tab = c.table + c.pix_first[p] * c.dps;
+            const PixelSampler ps = pixel_sampler(c, p);
             const bool seq = c.sequential != 0;
             uint64_t rs = 0, rinc = 0;
             if (GUIDED && c.guided) {
@@ -557,13 +584,13 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
             }
             int nrec = (int)((st >> kRecShift) & 7u);
             auto next1 = [&]() -> float {                               // pixel_sampling.cpp:159
-                if (c1 < c.d1) { float x = tab[2 * spp + c1 * spp + k]; c1++; return x; }
+                if (c1 < c.d1) { const float x = sample_1d(c, ps, c1, k); c1++; return x; }
                 return pcg_float(rs, rinc);
             };
             auto next2 = [&](float& x, float& y) {                      // pixel_sampling.cpp:167
                 if (c2 < c.d2) {
-                    const float* q2 = tab + 2 * spp + c.d1 * spp + 2 * (c2 * spp + k);
-                    x = q2[0]; y = q2[1]; c2++;
+                    sample_2d(c, ps, c2, k, x, y);
+                    c2++;
                     return;
                 }
                 x = pcg_float(rs, rinc);
@@ -991,6 +1018,27 @@ void launch_iteration(const DevScene& sc, const Chunk* c, const Paths& p, Queues
 // ---------------------------------------------------------------------------------------------
 struct Footprint { int x0, x1, y0, y1; };   // [x0, x1) x [y0, y1)
 
+// The image-plane samples of a pixel from sample k0 on, in order (consecutive stream draws).
+struct ImageSamples {
+    uint64_t st, inc;
+    float dx, dy;
+    int nx, x, y;
+    YG_INLINE ImageSamples(const PixelSampler& ps, const JumpTables& jt, int k0) {
+        inc = ps.inc;
+        st = pcg_jump(jt, ps.s0, ps.inc, 2u * (uint32_t)k0);
+        dx = 1.f / float(ps.nx);
+        dy = 1.f / float(ps.ny);
+        nx = ps.nx;
+        x = k0 % ps.nx;
+        y = k0 / ps.nx;
+    }
+    YG_INLINE void next(float& vx, float& vy) {
+        vx = fminf((float(x) + pcg_float(st, inc)) * dx, kOneMinusEps);
+        vy = fminf((float(y) + pcg_float(st, inc)) * dy, kOneMinusEps);
+        if (++x == nx) { x = 0; y++; }
+    }
+};
+
 YG_INLINE Footprint footprint(int x, int y, float jx, float jy) {
     const float fx = (float)x + jx, fy = (float)y + jy;   // film_pos, reference_renderer.cpp:363
     Footprint f;
@@ -1023,12 +1071,15 @@ __global__ void k_film_spill(const Chunk* __restrict__ cp, Paths P, FilmParams f
     const int vi = c.pix_view[p];
     const DevView& v = c.views[vi];
     const int2 px = c.pixels[p];
-    const float* tab = c.table + c.pix_first[p] * c.dps;
+    const PixelSampler ps = pixel_sampler(c, p);
     const TileBox t = tile_of(v, f.tile_first, px.x, px.y);
     uint32_t bits = 0;
     const int ka = c.guided ? c.k0 : 0, kb = c.guided ? min(c.k1, v.spp) : v.spp;
+    ImageSamples is(ps, c.jt, ka);
     for (int k = ka; k < kb; k++) {
-        const Footprint fp = footprint(px.x, px.y, tab[2 * k], tab[2 * k + 1]);
+        float jx, jy;
+        is.next(jx, jy);
+        const Footprint fp = footprint(px.x, px.y, jx, jy);
         if (fp.x0 == px.x && fp.x1 == px.x + 1 && fp.y0 == px.y && fp.y1 == px.y + 1) continue;
         bool margin = false;
         for (int yy = fp.y0; yy < fp.y1; yy++)
@@ -1063,9 +1114,11 @@ YG_INLINE void gather(const Chunk& c, const Paths& P, const FilmParams& f, const
     if (nx < t.sx0 || nx >= t.sx1 || ny < t.sy0 || ny >= t.sy1) return;
     const long long li = t.first + (long long)(ny - t.sy0) * (t.sx1 - t.sx0) + (nx - t.sx0) - f.base;   // chunk-local
     if (!spill_bit(f.spill[li], x - nx, y - ny)) return;
-    const float* tab = c.table + c.pix_first[li] * c.dps;
+    ImageSamples is(pixel_sampler(c, (int)li), c.jt, ka);
     for (int k = ka; k < kb; k++) {
-        const Footprint fp = footprint(nx, ny, tab[2 * k], tab[2 * k + 1]);
+        float jx, jy;
+        is.next(jx, jy);
+        const Footprint fp = footprint(nx, ny, jx, jy);
         if (x >= fp.x0 && x < fp.x1 && y >= fp.y0 && y < fp.y1) {
             const float4 r = P.radiance[c.pix_first[li] + k];
             acc.x += 1.f * r.x; acc.y += 1.f * r.y; acc.z += 1.f * r.z; acc.w += 1.f;
