@@ -281,6 +281,49 @@ typedef struct yar_gpu_info {
 } yar_gpu_info;
 int  yar_gpu_get_info(const yar_gpu_ctx* ctx, yar_gpu_info* out_info);
 
+/* ---- multi-session checkpoints and image output -------------------------------------------
+ * The reference's checkpoint directory format (src/ref/reference_renderer.cpp:162-324): a text
+ * file "checkpoint" (input file name, total tile count, samples per pixel) and one binary file
+ * "tile_NNNN" per finished tile (render time f32, tile variance f64, pixel bounds 4 x i32, Film_Pixels),
+ * written as "temp_tile_NNNN" and renamed. */
+typedef struct yar_checkpoint yar_checkpoint;
+/* start_or_resume_checkpoint (reference_renderer.cpp:179): creates the directory and its metadata
+ * when absent or empty, else checks the metadata against the project and loads every finished tile. */
+int  yar_checkpoint_open(const char* dir, const char* input_filename, int32_t total_tile_count,
+                         int32_t samples_per_pixel, yar_checkpoint** out_checkpoint);
+int32_t yar_checkpoint_finished_count(const yar_checkpoint* checkpoint);
+int  yar_checkpoint_finished_tiles(const yar_checkpoint* checkpoint, int32_t* out_indices);   /* ascending */
+int  yar_checkpoint_tile(const yar_checkpoint* checkpoint, int32_t tile_index, int32_t out_pixel_bounds[4],
+                         yar_film_pixel* out_pixels, double* out_variance);
+float yar_checkpoint_previous_time(const yar_checkpoint* checkpoint);   /* time spent in earlier sessions */
+void yar_checkpoint_free(yar_checkpoint* checkpoint);
+/* write_tile_to_checkpoint_directory (reference_renderer.cpp:283). */
+int  yar_checkpoint_write_tile(const char* dir, int32_t tile_index, const int32_t pixel_bounds[4],
+                               const yar_film_pixel* pixels, double tile_variance, float render_time);
+
+/* render_scene with a checkpoint directory (reference_renderer.cpp:479-584): loads the finished
+ * tiles, renders the others on the GPU in batches of tiles_per_batch (0 = 64), writes each tile's
+ * file as its batch completes, then merges every tile in index order.  out_render_time = time of
+ * earlier sessions + this call. */
+int  yar_gpu_render_checkpointed(yar_gpu_ctx* ctx, int32_t view, const char* checkpoint_dir, const char* input_filename,
+                                 int32_t tiles_per_batch, yar_film_pixel* out_film, float* out_image,
+                                 double* out_variance, float* out_render_time, yar_render_stats* out_stats);
+
+typedef struct yar_exr_attributes {     /* EXR_Attributes + EXR_Write_Params, reference_renderer.h:29-46 */
+    const char* input_file;
+    int32_t spp;
+    float   variance;
+    float   load_time;                  /* written only with enable_varying_attributes */
+    float   render_time;
+    int32_t enable_varying_attributes;
+    int32_t dump_attributes;            /* also write "<name> <value>" lines to <image>.txt */
+} yar_exr_attributes;
+/* write_openexr_image (reference_renderer.cpp:673) / Image::write_exr (src/lib/image.cpp:163): HALF
+ * channels B, G, R, uncompressed or ZIP, with the yar_* attributes; rgb is width*height*3 floats,
+ * top row first. */
+int  yar_image_write_exr(const char* path, const float* rgb, int32_t width, int32_t height,
+                         const yar_exr_attributes* attributes, int32_t compress_zip);
+
 /* ---- learned directional proposal (guided form) ------------------------------------------
  * No counterpart in the reference: a small MLP (9 inputs -> 16-frequency encoding, 297 features ->
  * 4 x 64 ReLU -> 40 outputs = 8 von Mises-Fisher lobes) with bf16 weights and fp32 accumulation,

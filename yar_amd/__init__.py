@@ -13,7 +13,8 @@ from . import _abi
 from ._abi import (FilmPixel, GpuInfo, GpuOptions, GuideConfig, Hit, Ray, RenderStats, SceneDesc, View,
                    film_dtype, hit_dtype, ray_dtype)
 
-__all__ = ["lib", "YarError", "Scene", "Context", "Guide", "GuideConfig", "generate_scene", "build_kdtree", "build_box_kdtree",
+__all__ = ["lib", "YarError", "Scene", "Context", "Guide", "GuideConfig", "Checkpoint", "checkpoint_write_tile", "write_exr",
+           "generate_scene", "build_kdtree", "build_box_kdtree",
            "GpuOptions", "View", "RenderStats", "ray_dtype", "hit_dtype", "film_dtype"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libyar_gpu.so")
@@ -213,6 +214,20 @@ class Context:
         _check(lib().yar_gpu_render_tiles(self._h, view, idx, len(tiles), ptrs, var, C.byref(st)), self._h)
         return bufs, list(var), st
 
+    def render_checkpointed(self, checkpoint_dir, input_filename, view=0, tiles_per_batch=0, want_film=True):
+        """render_scene with a checkpoint directory: (image, film or None, variance, render time, stats)."""
+        v = self.view(view)
+        w, hgt = v.region_x1 - v.region_x0, v.region_y1 - v.region_y0
+        img = np.empty((hgt, w, 3), dtype=np.float32)
+        film = np.empty((hgt, w), dtype=film_dtype()) if want_film else None
+        var = C.c_double()
+        t = C.c_float()
+        st = RenderStats()
+        _check(lib().yar_gpu_render_checkpointed(self._h, view, os.fsencode(checkpoint_dir), os.fsencode(input_filename),
+                                                 tiles_per_batch, film.ctypes.data if want_film else None, img.ctypes.data,
+                                                 C.byref(var), C.byref(t), C.byref(st)), self._h)
+        return img, film, var.value, t.value, st
+
     def render_batch(self, views):
         imgs = []
         for vi in views:
@@ -254,6 +269,66 @@ class Context:
         st = RenderStats()
         _check(lib().yar_gpu_last_trace_counters(self._h, C.byref(st)), self._h)
         return st
+
+
+class Checkpoint:
+    """A checkpoint directory in the reference's format (start_or_resume_checkpoint,
+    src/ref/reference_renderer.cpp:179)."""
+
+    def __init__(self, directory, input_filename, total_tile_count, samples_per_pixel):
+        h = C.c_void_p()
+        _check(lib().yar_checkpoint_open(os.fsencode(directory), os.fsencode(input_filename), total_tile_count,
+                                         samples_per_pixel, C.byref(h)))
+        self._h = h
+
+    def finished_tiles(self):
+        n = lib().yar_checkpoint_finished_count(self._h)
+        out = (C.c_int32 * max(1, n))()
+        _check(lib().yar_checkpoint_finished_tiles(self._h, out))
+        return list(out[:n])
+
+    def tile(self, index):
+        """(pixel bounds x0, y0, x1, y1; Film_Pixels as a structured array; tile variance)."""
+        b = (C.c_int32 * 4)()
+        _check(lib().yar_checkpoint_tile(self._h, index, b, None, None))
+        px = np.zeros((b[3] - b[1], b[2] - b[0]), dtype=film_dtype())
+        var = C.c_double()
+        _check(lib().yar_checkpoint_tile(self._h, index, b, px.ctypes.data, C.byref(var)))
+        return tuple(b), px, var.value
+
+    @property
+    def previous_time(self):
+        return lib().yar_checkpoint_previous_time(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().yar_checkpoint_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def checkpoint_write_tile(directory, tile_index, pixel_bounds, pixels, variance, render_time):
+    """write_tile_to_checkpoint_directory (src/ref/reference_renderer.cpp:283)."""
+    b = (C.c_int32 * 4)(*pixel_bounds)
+    px = np.ascontiguousarray(pixels, dtype=film_dtype())
+    _check(lib().yar_checkpoint_write_tile(os.fsencode(directory), tile_index, b, px.ctypes.data, float(variance),
+                                           float(render_time)))
+
+
+def write_exr(path, image, input_file="", spp=0, variance=0.0, load_time=0.0, render_time=0.0,
+              varying_attributes=False, dump_attributes=False, compress=False):
+    """write_openexr_image (src/ref/reference_renderer.cpp:673): HALF B, G, R with the yar_* attributes."""
+    img = np.ascontiguousarray(image, dtype=np.float32)
+    a = _abi.ExrAttributes(os.fsencode(input_file), int(spp), float(variance), float(load_time), float(render_time),
+                           1 if varying_attributes else 0, 1 if dump_attributes else 0)
+    _check(lib().yar_image_write_exr(os.fsencode(path), img.ctypes.data, img.shape[1], img.shape[0], C.byref(a),
+                                     1 if compress else 0))
+    return path
 
 
 class Guide:

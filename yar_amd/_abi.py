@@ -132,6 +132,11 @@ def film_dtype():
     return np.dtype([("color_sum", "<f4", 3), ("weight_sum", "<f4")])
 
 
+class ExrAttributes(C.Structure):
+    _fields_ = [("input_file", C.c_char_p), ("spp", C.c_int32), ("variance", C.c_float), ("load_time", C.c_float),
+                ("render_time", C.c_float), ("enable_varying_attributes", C.c_int32), ("dump_attributes", C.c_int32)]
+
+
 _P = C.c_void_p
 
 # name -> (restype, argtypes)
@@ -166,6 +171,16 @@ SIGNATURES = {
     "yar_gpu_last_trace_counters": (C.c_int, [_P, C.POINTER(RenderStats)]),
     "yar_gpu_get_info": (C.c_int, [_P, C.POINTER(GpuInfo)]),
     "yar_gpu_render_guided": (C.c_int, [_P, _P, C.POINTER(C.c_int32), C.c_int32, _P, _P, _P]),
+    "yar_checkpoint_open": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, C.c_int32, C.POINTER(_P)]),
+    "yar_checkpoint_finished_count": (C.c_int32, [_P]),
+    "yar_checkpoint_finished_tiles": (C.c_int, [_P, C.POINTER(C.c_int32)]),
+    "yar_checkpoint_tile": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int32), _P, C.POINTER(C.c_double)]),
+    "yar_checkpoint_previous_time": (C.c_float, [_P]),
+    "yar_checkpoint_free": (None, [_P]),
+    "yar_checkpoint_write_tile": (C.c_int, [C.c_char_p, C.c_int32, C.POINTER(C.c_int32), _P, C.c_double, C.c_float]),
+    "yar_gpu_render_checkpointed": (C.c_int, [_P, C.c_int32, C.c_char_p, C.c_char_p, C.c_int32, _P, _P,
+                                              C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(RenderStats)]),
+    "yar_image_write_exr": (C.c_int, [C.c_char_p, _P, C.c_int32, C.c_int32, C.POINTER(ExrAttributes), C.c_int32]),
     "yar_guide_param_count": (C.c_int64, []),
     "yar_guide_create": (C.c_int, [C.c_int32, C.POINTER(GuideConfig), C.POINTER(_P)]),
     "yar_guide_destroy": (None, [_P]),

@@ -14,7 +14,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libyar_gpu.so")
 
-HOST_SOURCES = ["host/common.cpp", "host/pbrt_loader.cpp", "host/kdtree_builder.cpp", "host/scene_gen.cpp"]
+HOST_SOURCES = ["host/common.cpp", "host/pbrt_loader.cpp", "host/kdtree_builder.cpp", "host/scene_gen.cpp",
+                "host/checkpoint.cpp", "host/exr_writer.cpp"]
 HIP_SOURCES = ["gpu/render.hip", "gpu/context.hip", "gpu/guide.hip"]
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
@@ -52,7 +53,7 @@ def build(verbose=False):
                 print(o)
     objs = [os.path.join(BUILD, os.path.basename(s) + ".o") for s in HOST_SOURCES + HIP_SOURCES]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp] + objs + ["-pthread"])
+    _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", tmp] + objs + ["-pthread", "-lz"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -1260,6 +1260,68 @@ int yar_gpu_render(yar_gpu_ctx* ctx, int32_t view, yar_film_pixel* out_film, flo
     });
 }
 
+int yar_gpu_render_checkpointed(yar_gpu_ctx* ctx, int32_t view, const char* checkpoint_dir, const char* input_filename,
+                                int32_t tiles_per_batch, yar_film_pixel* out_film, float* out_image, double* out_variance,
+                                float* out_render_time, yar_render_stats* out_stats) {
+    int32_t n = 0;
+    if (!ctx || !checkpoint_dir || !input_filename || tiles_per_batch < 0 || yar_film_tile_count(ctx, view, &n) != YAR_OK)
+        return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_checkpointed: invalid argument");
+    return guarded(ctx, "yar_gpu_render_checkpointed", [&] {
+        const auto t0 = std::chrono::steady_clock::now();
+        const yar_view v = ctx->views[view];
+        yar_checkpoint* ck = nullptr;
+        const int rc = yar_checkpoint_open(checkpoint_dir, input_filename, n, v.x_pixel_sample_count * v.y_pixel_sample_count, &ck);
+        if (rc != YAR_OK) throw HipError{yar::get_last_error(), rc};
+        std::unique_ptr<yar_checkpoint, void (*)(yar_checkpoint*)> hold(ck, yar_checkpoint_free);
+        const float previous = yar_checkpoint_previous_time(ck);
+        PartResult all;
+        all.tiles.resize(n);
+        all.tile_var.assign(n, 0.0);
+        std::vector<int> todo;
+        for (int t = 0; t < n; t++) {
+            const TileGeom g = tile_geom(v, t);
+            int32_t b[4];
+            if (yar_checkpoint_tile(ck, t, b, nullptr, nullptr) != YAR_OK) { todo.push_back(t); continue; }
+            if (b[0] != g.px0 || b[1] != g.py0 || b[2] != g.px1 || b[3] != g.py1)
+                throw HipError{"checkpoint tile " + std::to_string(t) + " has other pixel bounds than this film", YAR_ERR_INVALID_ARGUMENT};
+            all.tiles[t].resize((size_t)(g.px1 - g.px0) * (g.py1 - g.py0));
+                if (yar_checkpoint_tile(ck, t, b, all.tiles[t].data(), &all.tile_var[t]) != YAR_OK)
+                throw HipError{yar::get_last_error(), YAR_ERR_INTERNAL};
+        }
+        HIPX(hipSetDevice(ctx->device));
+        const int batch = tiles_per_batch > 0 ? tiles_per_batch : 64;
+        yar_render_stats total{};
+        for (size_t i0 = 0; i0 < todo.size(); i0 += (size_t)batch) {
+            std::vector<JobPart> parts(1);
+            parts[0].view = view;
+            parts[0].tiles.assign(todo.begin() + i0, todo.begin() + std::min(todo.size(), i0 + (size_t)batch));
+            std::vector<PartResult> rr;
+            yar_render_stats st{};
+            render_job(ctx, parts, rr, st);
+            const float now = previous + (float)std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            for (size_t k = 0; k < parts[0].tiles.size(); k++) {
+                const int t = parts[0].tiles[k];
+                const TileGeom g = tile_geom(v, t);
+                const int32_t b[4] = {g.px0, g.py0, g.px1, g.py1};
+                all.tiles[t] = std::move(rr[0].tiles[k]);
+                all.tile_var[t] = rr[0].tile_var[k];
+                const int wrc = yar_checkpoint_write_tile(checkpoint_dir, t, b, all.tiles[t].data(), all.tile_var[t], now);
+                if (wrc != YAR_OK) throw HipError{yar::get_last_error(), wrc};
+            }
+            total.samples += st.samples;
+            total.closest_rays += st.closest_rays;
+            total.any_rays += st.any_rays;
+            total.gpu_seconds += st.gpu_seconds;
+        }
+        merge_view(v, all, out_film, out_image, out_variance);
+        const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        total.render_seconds = elapsed;
+        total.passes = 1;
+        if (out_render_time) *out_render_time = previous + (float)elapsed;
+        if (out_stats) *out_stats = total;
+    });
+}
+
 int yar_gpu_render_batch(yar_gpu_ctx* ctx, const int32_t* views, int32_t count, float* const* out_images,
                          double* out_variances, yar_render_stats* out_stats) {
     if (!ctx || !views || count <= 0) return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_render_batch: invalid argument");
