@@ -264,6 +264,14 @@ int  yar_gpu_trace_rays(yar_gpu_ctx* ctx, const yar_ray* rays, int64_t count, in
 /* Same, on device-resident buffers and the given HIP stream (NULL = the context's stream). */
 int  yar_gpu_trace_rays_device(yar_gpu_ctx* ctx, const yar_ray* d_rays, int64_t count, int32_t any_hit,
                                yar_hit* d_hits, void* hip_stream);
+/* Allocation-free form of yar_gpu_trace_rays_device.  No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  It calls no allocating routine; the per-ray hit staging and the traversal
+ * counters live in a static device array of 24 MB (each device the library runs on holds its own copy
+ * until the process ends), so count must not exceed 2^20 rays (YAR_ERR_CAPACITY otherwise).  Every
+ * call on one device shares that array: calls that overlap on one device race. */
+int  yar_gpu_trace_rays_device_noalloc(yar_gpu_ctx* ctx, const yar_ray* d_rays, int64_t count, int32_t any_hit,
+                                       yar_hit* d_hits, void* hip_stream);
 /* Per-ray traversal counters of the last trace_rays call (requires enable_counters). */
 int  yar_gpu_last_trace_counters(yar_gpu_ctx* ctx, yar_render_stats* out_stats);
 

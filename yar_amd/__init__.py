@@ -265,6 +265,12 @@ class Context:
                                                C.c_void_p(d_hits_ptr), C.c_void_p(stream) if stream else None),
                self._h)
 
+    def trace_rays_device_noalloc(self, d_rays_ptr, count, d_hits_ptr, any_hit=False, stream=None):
+        """Allocation-free device query (at most 2^20 rays; calls overlapping on one device race)."""
+        _check(lib().yar_gpu_trace_rays_device_noalloc(self._h, C.c_void_p(d_rays_ptr), count, 1 if any_hit else 0,
+                                                       C.c_void_p(d_hits_ptr), C.c_void_p(stream) if stream else None),
+               self._h)
+
     def last_trace_counters(self):
         st = RenderStats()
         _check(lib().yar_gpu_last_trace_counters(self._h, C.byref(st)), self._h)

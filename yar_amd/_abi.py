@@ -168,6 +168,7 @@ SIGNATURES = {
                                        C.POINTER(C.c_double), C.POINTER(RenderStats)]),
     "yar_gpu_trace_rays": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P]),
     "yar_gpu_trace_rays_device": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P]),
+    "yar_gpu_trace_rays_device_noalloc": (C.c_int, [_P, _P, C.c_int64, C.c_int32, _P, _P]),
     "yar_gpu_last_trace_counters": (C.c_int, [_P, C.POINTER(RenderStats)]),
     "yar_gpu_get_info": (C.c_int, [_P, C.POINTER(GpuInfo)]),
     "yar_gpu_render_guided": (C.c_int, [_P, _P, C.POINTER(C.c_int32), C.c_int32, _P, _P, _P]),

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <memory>
+#include <mutex>
 #include <cstdio>
 #include <string>
 #include <thread>
@@ -92,12 +93,45 @@ struct KernelTimer {
     ~KernelTimer() { for (auto e : pool) (void)hipEventDestroy(e); }
 };
 
+// Scratch of the allocation-free ray queries (yar_gpu_trace_rays_device_noalloc): per-ray hit staging
+// (t and barycentrics, object and triangle) for up to kNoAllocMaxRays rays, then the 16 traversal
+// counters.  Each device holds its own copy from module load until the process ends, and every call
+// on a device shares it.  In host code `scratch` names no device address: kernels receive the address
+// hipGetSymbolAddress returned for the current device (scratch_ptr below).
+constexpr int64_t kNoAllocMaxRays = (int64_t)1 << 20;
+constexpr size_t kNoAllocTbBytes = (size_t)kNoAllocMaxRays * sizeof(float4);
+constexpr size_t kNoAllocOtBytes = (size_t)kNoAllocMaxRays * sizeof(int2);
+constexpr size_t kNoAllocScratchBytes = kNoAllocTbBytes + kNoAllocOtBytes + 16 * sizeof(unsigned long long);
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be current), looked up once per device.
+unsigned char* scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) throw HipError{"device ordinal beyond the scratch table", YAR_ERR_CAPACITY};
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        HIPX(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw HipError{"hipGetSymbolAddress returned no address for the scratch array"};
+        g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_ptr[device];
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
 }  // namespace
 
 struct yar_gpu_ctx {
     std::string error;
     int device = 0;
     int cu_count = 1;
+    unsigned char* scratch = nullptr;     // this device's address of the allocation-free scratch array
     hipStream_t stream = nullptr;
     yar_gpu_options opt{};
     yar_render_config cfg{};
@@ -1155,6 +1189,7 @@ int yar_gpu_create(const yar_scene_desc* scene, const yar_gpu_options* options, 
         std::snprintf(ctx->info.device_name, sizeof(ctx->info.device_name), "%s", prop.gcnArchName);
         ctx->cu_count = std::max(1, prop.multiProcessorCount);
         HIPX(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ctx->scratch = scratch_ptr(ctx->device);
         upload_scene(ctx.get(), scene);
         upload_jump_tables(ctx.get());
         yar_view v{};
@@ -1461,3 +1496,28 @@ extern "C" int yar_gpu_get_info(const yar_gpu_ctx* ctx, yar_gpu_info* out_info) 
     return YAR_OK;
 }
 
+// Allocation-free form of yar_gpu_trace_rays_device: the same kernels, with the hit staging and the
+// counters in this device's `scratch` array instead of context buffers, cleared by a kernel (no
+// memset node), so the call can sit inside a HIP graph capture or a loop that must not allocate.
+extern "C" int yar_gpu_trace_rays_device_noalloc(yar_gpu_ctx* ctx, const yar_ray* d_rays, int64_t count, int32_t any_hit,
+                                                 yar_hit* d_hits, void* hip_stream) {
+    if (!ctx || count < 0 || (count > 0 && (!d_rays || !d_hits)))
+        return fail(ctx, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_trace_rays_device_noalloc: invalid argument");
+    if (count > kNoAllocMaxRays)
+        return fail(ctx, YAR_ERR_CAPACITY, "yar_gpu_trace_rays_device_noalloc: more than 2^20 rays in one call (scratch limit)");
+    if (count == 0) return YAR_OK;
+    return guarded(ctx, "yar_gpu_trace_rays_device_noalloc", [&] {
+        HIPX(hipSetDevice(ctx->device));
+        if (!ctx->scratch) throw HipError{"scratch array not set up for this device"};
+        float4* tb = reinterpret_cast<float4*>(ctx->scratch);
+        int2* ot = reinterpret_cast<int2*>(ctx->scratch + kNoAllocTbBytes);
+        unsigned long long* totals = reinterpret_cast<unsigned long long*>(ctx->scratch + kNoAllocTbBytes + kNoAllocOtBytes);
+        LaunchCfg lc = ctx->launch_cfg();
+        if (hip_stream) lc.stream = (hipStream_t)hip_stream;
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, lc.stream, totals, 16);
+        launch_trace_rays(ctx->scene, reinterpret_cast<const float4*>(d_rays), count, any_hit, tb, ot, totals, lc);
+        hipLaunchKernelGGL(k_pack_hits, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, lc.stream, tb, ot,
+                           (long long)count, d_hits);
+        HIPX(hipGetLastError());
+    });
+}
