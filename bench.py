@@ -36,6 +36,24 @@ def _scene(config, scale, seed):
     return yar_amd.Scene(path)
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _host_threads():
+    """Every host core the process may use (the CPU baseline runs the oracle on all of them)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def _cpu_baseline(scene, view, crop, threads):
     from tests import oracle_lib
     import yar_amd
@@ -54,8 +72,9 @@ def _cpu_baseline(scene, view, crop, threads):
     spp = v.x_pixel_sample_count * v.y_pixel_sample_count
     samples = crop * crop * spp
     return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
             "sample": f"{crop}x{crop} centre crop of the same render ({samples} samples, {spp} spp), "
-                      f"{dt:.2f} s on {threads} threads (oracle/cpu_ref.cpp, g++ -O2 -ffp-contract=off)"}
+                      f"{dt:.2f} s on {threads} threads = all host cores (oracle/cpu_ref.cpp, g++ -O2 -ffp-contract=off)"}
 
 
 def _lum(img):
@@ -118,37 +137,68 @@ def _batch_leg():
             "mrays_s": round((st.closest_rays + st.any_rays) / dt / 1e6, 1)}
 
 
-def _guided_leg(n_views=4, side=256, grid=8, ref_grid=32):
-    """Guided vs plain on config-3 views at equal sample count: MSE against a plain reference of
-    ref_grid^2 spp, wall times, and the time ratio guided/plain to reach the same MSE (MSE ~ 1/time)."""
+def _mse_legs(ctx, plain_views, ref_views, runs=2, guide_seed=1):
+    """Renders the plain views, their references and the guided views; MSE against the references and
+    the guided/plain time ratio to reach the same MSE (MSE ~ 1/time for both estimators)."""
     import numpy as np
+    import yar_amd
+    n = len(plain_views)
+    ctx.set_views(plain_views + ref_views)
+    ref, _, _ = ctx.render_batch(list(range(n, 2 * n)))
+    idx = list(range(n))
+    ctx.render_batch(idx)
+    tp = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        plain, _, _ = ctx.render_batch(idx)
+        tp.append(time.perf_counter() - t0)
+    tg = []
+    for _ in range(runs):
+        guide = yar_amd.Guide(seed=guide_seed)
+        t0 = time.perf_counter()
+        guided, _, gst = ctx.render_guided(guide, idx)
+        tg.append(time.perf_counter() - t0)
+    tp, tg = min(tp), min(tg)
+    mp = float(np.mean([((a - b) ** 2).mean() for a, b in zip(plain, ref)]))
+    mg = float(np.mean([((a - b) ** 2).mean() for a, b in zip(guided, ref)]))
+    return {"plain_ms": round(tp * 1e3, 1), "guided_ms": round(tg * 1e3, 1), "mse_plain": mp, "mse_guided": mg,
+            "equal_spp_mse_ratio": round(mg / mp, 4), "time_ratio_to_equal_mse": round((mg * tg) / (mp * tp), 3),
+            "train_nats": round(gst.train_nats, 3), "passes": gst.passes, "train_records": gst.train_records}
+
+
+def _guided_c3_leg(side=128, grid=8, ref_grid=64):
+    """Config 3 target: the 32 seeded poses of the batched workload (films reduced to side^2), guided
+    vs plain at grid^2 spp against a ref_grid^2-spp plain reference."""
     import yar_amd
     from yar_amd import views as yv
     scene = _scene(1, 1.0, 1)
     ctx = yar_amd.Context(scene)
-    base = yv.batch_views(ctx.view(0), n_views)
+    base = yv.batch_views(ctx.view(0), 32)
     vs = [yv.with_film(v, side, side, grid, seed_offset=v.rng_seed_offset) for v in base]
-    refs = [yv.with_film(v, side, side, ref_grid, seed_offset=1000 + i) for i, v in enumerate(base)]
-    ctx.set_views(vs + refs)
-    ref, _, _ = ctx.render_batch(list(range(n_views, 2 * n_views)))
-    idx = list(range(n_views))
-    ctx.render_batch(idx)
-    t0 = time.perf_counter()
-    plain, _, _ = ctx.render_batch(idx)
-    tp = time.perf_counter() - t0
-    guide = yar_amd.Guide(seed=1)
-    ctx.render_guided(guide, idx)
-    guide = yar_amd.Guide(seed=1)
-    t0 = time.perf_counter()
-    guided, _, gst = ctx.render_guided(guide, idx)
-    tg = time.perf_counter() - t0
+    refs = [yv.with_film(v, side, side, ref_grid, seed_offset=7000 + i) for i, v in enumerate(base)]
+    out = {"workload": f"config 3: the 32 batch poses at {side}x{side}, {grid * grid} spp, reference {ref_grid ** 2} spp plain"}
+    out.update(_mse_legs(ctx, vs, refs))
     ctx.close()
-    mp = float(np.mean([((a - b) ** 2).mean() for a, b in zip(plain, ref)]))
-    mg = float(np.mean([((a - b) ** 2).mean() for a, b in zip(guided, ref)]))
-    return {"workload": f"config-3 scene, {n_views} views at {side}x{side}, {grid * grid} spp, reference {ref_grid ** 2} spp plain",
-            "plain_ms": round(tp * 1e3, 1), "guided_ms": round(tg * 1e3, 1), "mse_plain": mp, "mse_guided": mg,
-            "equal_spp_mse_ratio": round(mg / mp, 4), "time_ratio_to_equal_mse": round((mg * tg) / (mp * tp), 3),
-            "train_nats": round(gst.train_nats, 3), "passes": gst.passes}
+    return out
+
+
+def _guided_c4_leg(side=256, grid=8, ref_grid=64):
+    """Config 4 (light through a slit): a side^2 centre crop, guided vs plain at grid^2 spp against a
+    ref_grid^2-spp plain reference."""
+    import yar_amd
+    from yar_amd import views as yv
+    scene = _scene(4, 1.0, 1)
+    ctx = yar_amd.Context(scene)
+    v0 = ctx.view(0)
+    cx, cy = (v0.region_x0 + v0.region_x1) // 2, (v0.region_y0 + v0.region_y1) // 2
+    region = (cx - side // 2, cy - side // 2, cx + side // 2, cy + side // 2)
+    v = yv.with_film(v0, v0.film_width, v0.film_height, grid, seed_offset=0, region=region)
+    r = yv.with_film(v0, v0.film_width, v0.film_height, ref_grid, seed_offset=9000, region=region)
+    out = {"workload": f"config 4: {side}x{side} centre crop of the 1024x1024 film, {grid * grid} spp, "
+                       f"reference {ref_grid ** 2} spp plain"}
+    out.update(_mse_legs(ctx, [v], [r]))
+    ctx.close()
+    return out
 
 
 def main():
@@ -164,7 +214,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--no-extra", action="store_true", help="skip the config-1, batch and guided legs")
-    ap.add_argument("--leg", choices=["config1", "batch", "guided"], help="run only this leg (profiling)")
+    ap.add_argument("--leg", choices=["config1", "batch", "guided_c3", "guided_c4"], help="run only this leg (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device with a torch copy kernel (HBM counter calibration)")
     args = ap.parse_args()
@@ -186,8 +236,9 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "launches": 3}))
         return
     if args.leg:
-        threads = min(16, os.cpu_count() or 1)
-        fn = {"config1": lambda: _config1_leg(threads, False), "batch": _batch_leg, "guided": _guided_leg}[args.leg]
+        threads = _host_threads()
+        fn = {"config1": lambda: _config1_leg(threads, False), "batch": _batch_leg, "guided_c3": _guided_c3_leg,
+              "guided_c4": _guided_c4_leg}[args.leg]
         print(json.dumps(fn()), flush=True)
         return
 
@@ -255,7 +306,7 @@ def main():
                                 "objects": round(cs.extend_object_visits / cs.extend_rays, 2)}}
 
     cpu = None
-    threads = min(16, os.cpu_count() or 1)
+    threads = _host_threads()
     if args.cpu_crop > 0:
         cpu = _cpu_baseline(scene, view, args.cpu_crop, threads)
     extra = {}
@@ -264,8 +315,10 @@ def main():
         print(f"# config 1: {extra['config1']}", file=sys.stderr, flush=True)
         extra["batch"] = _batch_leg()
         print(f"# batch: {extra['batch']}", file=sys.stderr, flush=True)
-        extra["guided"] = _guided_leg()
-        print(f"# guided: {extra['guided']}", file=sys.stderr, flush=True)
+        extra["guided_c3"] = _guided_c3_leg()
+        print(f"# guided config 3: {extra['guided_c3']}", file=sys.stderr, flush=True)
+        extra["guided_c4"] = _guided_c4_leg()
+        print(f"# guided config 4: {extra['guided_c4']}", file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1,

@@ -62,18 +62,22 @@ typedef struct yar_mesh_desc {          /* Triangle_Mesh, src/lib/triangle_mesh.
 #define YAR_MATERIAL_DIFFUSE 1           /* Diffuse_Material  -> Diffuse_BRDF   (bsdf.cpp:60) */
 #define YAR_MATERIAL_METAL   2           /* Metal_Material    -> Metal_BRDF     (bsdf.cpp:146) */
 #define YAR_MATERIAL_GLASS   3           /* Glass_Material, roughness 0 -> delta (delta_scattering.cpp:53) */
+#define YAR_MATERIAL_MIRROR  4           /* Perfect_Reflector_Material -> delta reflection (delta_scattering.cpp:27) */
+#define YAR_MATERIAL_PLASTIC 5           /* Plastic_Material -> Pbrt3_Plastic_BRDF (bsdf_pbrt.cpp:258) in pbrt3
+                                          * scenes, Plastic_BRDF (bsdf.cpp:204) otherwise */
 
 typedef struct yar_material_desc {
     int32_t type;
     int32_t roughness_is_alpha;         /* pbrt "remaproughness" false */
-    float   reflectance[3];             /* matte Kd / glass Kr */
+    float   reflectance[3];             /* matte/plastic Kd, glass/mirror Kr */
     float   transmittance[3];           /* glass Kt */
     float   eta[3];                     /* metal eta (RGB) */
     float   k[3];                       /* metal k (RGB) */
     float   eta_i;                      /* metal: IOR of adjacent dielectric (1 for pbrt) */
-    float   u_roughness;                /* metal uroughness / glass roughness */
+    float   u_roughness;                /* metal uroughness / glass, plastic roughness */
     float   v_roughness;                /* metal vroughness */
     float   index_of_refraction;        /* glass */
+    float   specular_r0;                /* plastic: specular scale r0 (luminance of pbrt Ks) */
 } yar_material_desc;
 
 typedef struct yar_object_desc {        /* Scene_Object, src/lib/scene_object.h:8 */
@@ -149,6 +153,11 @@ int  yar_scene_generate(int32_t config, uint64_t seed, double scale, const char*
 typedef struct yar_kdtree yar_kdtree;
 int  yar_kdtree_build_mesh(const float* vertices, const int32_t* indices, int32_t triangle_count,
                            yar_kdtree** out_tree);
+/* Binned fast path (no reference counterpart): nodes above 1024 triangles choose their plane among 31
+ * uniform candidates per axis with the same SAH cost, partition and clipping; the tree differs from the
+ * reference's but lists every triangle that overlaps a leaf, so traversal finds the same hits. */
+int  yar_kdtree_build_mesh_binned(const float* vertices, const int32_t* indices, int32_t triangle_count,
+                                  yar_kdtree** out_tree);
 /* boxes: 6 floats per primitive (min xyz, max xyz); builds the object-level (scene) tree. */
 int  yar_kdtree_build_boxes(const float* boxes, int32_t count, yar_kdtree** out_tree);
 uint32_t        yar_kdtree_node_count(const yar_kdtree* tree);
@@ -172,7 +181,9 @@ typedef struct yar_gpu_options {
     int32_t stack_mode;                 /* traversal stack storage: 0 = default, 1 = LDS, 2 = private */
     int32_t lane_refill;                /* 1 = traversal lanes take new rays individually, 0/2 = batches of 64 */
     int32_t disable_mis_cull;           /* 1 = trace every MIS BSDF ray (no analytic light-rectangle test) */
-    int32_t reserved[1];
+    int32_t bin_rays;                   /* 1 = group each bounce's rays by direction octant before traversal */
+    int32_t kdtree_builder;             /* 0 = exact (the reference's trees), 1 = binned SAH fast path */
+    int32_t reserved[3];
 } yar_gpu_options;
 
 typedef struct yar_view {               /* Scene_Overrides analog, reference_renderer.h:23 */

@@ -442,6 +442,8 @@ struct Mesh {
 struct Object { int mesh, material, light; M34 o2w, w2o, nrm; };
 struct Material {
     int type; RGB refl, trans, eta, k; float eta_i, alpha_x, alpha_y, ior, roughness;
+    float r0 = 0.f;           // plastic specular scale
+    bool pbrt3 = false;       // plastic: the pbrt3 variant (Fresnel of a 1.5 dielectric, pbrt3's flipped ior)
 };
 struct RectLight { M34 l2w; RGB L; V2 size; };
 struct PointLight { V3 p; RGB I; };
@@ -801,9 +803,66 @@ static float vndf_wi_pdf(V3 wo, V3 wh, float ax, float ay) {                    
     return wh_pdf / (4 * dot(wh, wo));
 }
 
+// Isotropic GGX against the shading normal: GGX_Distribution::D, lambda, G, G1 (scattering.cpp:153-238)
+// and the visible-normal reflection pdf (sampling.cpp:337-347, scattering.cpp:107-116).
+static float D_iso(V3 wh, V3 n, float alpha) {
+    float c = dot(wh, n);
+    if (c <= 0.f) return 0.f;
+    float c2 = c * c;
+    float a2 = alpha * alpha;
+    float k = c2 * (a2 - 1) + 1;
+    return a2 / (Pi * k * k);
+}
+static float lambda_iso(V3 v, V3 n, float alpha) {
+    float c = dot(v, n);
+    float c2 = c * c;
+    float t2 = std::max(0.f, (1.f - c2) / c2);
+    return 0.5f * (-1.f + std::sqrt(1.f + alpha * alpha * t2));
+}
+static float G_iso(V3 wi, V3 wo, V3 n, float a) { return 1.f / (1.f + lambda_iso(wi, n, a) + lambda_iso(wo, n, a)); }
+static float G1_iso(V3 v, V3 n, float a) { return 1.f / (1.f + lambda_iso(v, n, a)); }
+static float vndf_wi_pdf_iso(V3 wo, V3 wh, V3 n, float alpha) {
+    float wh_pdf = G1_iso(wo, n, alpha) * D_iso(wh, n, alpha) * std::max(0.f, dot(wo, wh)) / dot(wo, n);
+    return wh_pdf / (4 * dot(wh, wo));
+}
+static RGB schlick_fresnel(RGB r0, float cos_i) {                                    // scattering.cpp:10
+    float k = 1.f - std::fabs(cos_i);
+    float k5 = (k * k) * (k * k) * k;
+    return r0 + (splat(1.f) - r0) * k5;
+}
+
+// Plastic_BRDF::evaluate (bsdf.cpp:214-233) / Pbrt3_Plastic_BRDF::evaluate (bsdf_pbrt.cpp:264-291)
+static RGB plastic_eval(const Material& m, const Shading& sc, V3 wo, V3 wi) {
+    V3 wh = normalized(wo + wi);
+    float cos_i = dot(wi, wh);
+    RGB F;
+    if (m.pbrt3) {
+        float rel = 1.f / 1.5f;
+        V3 original = sc.flipped ? -sc.n : sc.n;
+        if (dot(original, wi) < 0.f) rel = 1.5f / 1.f;
+        F = splat(dielectric_fresnel(cos_i, rel));
+    } else {
+        F = schlick_fresnel(splat(0.04f), cos_i);
+    }
+    float G = G_iso(wi, wo, sc.n, m.alpha_x);
+    float D = D_iso(wh, sc.n, m.alpha_x);
+    float won = dot(wo, sc.n), win = dot(wi, sc.n);
+    RGB base = F * ((G * D) / (4.f * won * win));
+    RGB spec = m.r0 * base;
+    RGB diffuse = m.refl * Pi_Inv;
+    return diffuse + spec;
+}
+static float plastic_pdf(const Material& m, const Shading& sc, V3 wo, V3 wi) {       // bsdf.cpp:253-262
+    float diffuse_pdf = dot(sc.n, wi) / Pi;
+    V3 wh = normalized(wo + wi);
+    float spec_pdf = vndf_wi_pdf_iso(wo, wh, sc.n, m.alpha_x);
+    return 0.5f * (diffuse_pdf + spec_pdf);
+}
+
 static RGB bsdf_eval(const Scene& s, const Shading& sc, V3 wo, V3 wi) {
     const Material& m = s.materials[sc.material];
     if (m.type == YAR_MATERIAL_DIFFUSE) return Pi_Inv * m.refl;                      // bsdf.cpp:68
+    if (m.type == YAR_MATERIAL_PLASTIC) return plastic_eval(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);                               // bsdf.cpp:160
     V3 wh = normalized(wol + wil);
     float ci = dot(wil, wh);
@@ -815,16 +874,28 @@ static RGB bsdf_eval(const Scene& s, const Shading& sc, V3 wo, V3 wi) {
 static float bsdf_pdf(const Scene& s, const Shading& sc, V3 wo, V3 wi) {
     const Material& m = s.materials[sc.material];
     if (m.type == YAR_MATERIAL_DIFFUSE) return dot(sc.n, wi) / Pi;                   // bsdf.cpp:80
+    if (m.type == YAR_MATERIAL_PLASTIC) return plastic_pdf(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);                               // bsdf.cpp:190
     V3 wh = normalized(wol + wil);
     return vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
 }
-static RGB bsdf_sample(const Scene& s, const Shading& sc, V2 u, V3 wo, V3* wi, float* pdf) {
+static RGB bsdf_sample(const Scene& s, const Shading& sc, V2 u, float u_scat, V3 wo, V3* wi, float* pdf) {
     const Material& m = s.materials[sc.material];
     if (m.type == YAR_MATERIAL_DIFFUSE) {                                           // bsdf.cpp:71
         *wi = to_world(sc, sample_cosine(u));
         *pdf = bsdf_pdf(s, sc, wo, *wi);
         return bsdf_eval(s, sc, wo, *wi);
+    }
+    if (m.type == YAR_MATERIAL_PLASTIC) {                                           // bsdf.cpp:235-251
+        if (u_scat < 0.5f) {
+            *wi = to_world(sc, sample_cosine(u));
+        } else {
+            V3 wh = to_world(sc, vndf_sample(u, to_local(sc, wo), m.alpha_x, m.alpha_x));   // bsdf.cpp:41-48
+            *wi = reflect(wo, wh);
+        }
+        if (dot(sc.n, *wi) <= 0.f) return RGB{};
+        *pdf = plastic_pdf(m, sc, wo, *wi);
+        return plastic_eval(m, sc, wo, *wi);
     }
     V3 wol = to_local(sc, wo);                                                       // bsdf.cpp:176
     V3 wh = vndf_sample(u, wol, m.alpha_x, m.alpha_y);
@@ -888,7 +959,7 @@ static RGB light_rect(const Scene& s, const Shading& sc, int li, V2 ul, V2 ub, f
     }
     {
         V3 wi; float bpdf;
-        RGB f = bsdf_sample(s, sc, ub, sc.wo, &wi, &bpdf);
+        RGB f = bsdf_sample(s, sc, ub, us, sc.wo, &wi, &bpdf);
         if (!is_black(f)) {
             float lndw = dot(ln, -wi);
             if (lndw > 1e-4f) {
@@ -975,22 +1046,26 @@ static RGB trace_path(const Scene& s, Ray ray, PixelSampler& ps, PathCounters& p
         V2 u_bsdf = ps.next2d();
         V2 u_bsdf_next = ps.next2d();
         const Material& m = s.materials[sc.material];
-        bool delta = (m.type == YAR_MATERIAL_GLASS && m.roughness <= 0.f);
+        bool delta = (m.type == YAR_MATERIAL_GLASS && m.roughness <= 0.f) || m.type == YAR_MATERIAL_MIRROR;
         if (!delta) {
             RGB dl = direct_single(s, sc, u_sel, u_light, u_bsdf, u_scat, pc, tc);
             L = L + coeff * dl;
             bounce++;
             if (bounce == cfg.max_light_bounces) break;
             V3 wi; float pdf;
-            RGB f = bsdf_sample(s, sc, u_bsdf_next, sc.wo, &wi, &pdf);
-            (void)u_scat_next;
+            RGB f = bsdf_sample(s, sc, u_bsdf_next, u_scat_next, sc.wo, &wi, &pdf);
             if (is_black(f)) break;
             coeff = coeff * (f * (std::fabs(dot(sc.n, wi)) / (pdf * 1.f)));
             ray.o = sc.origin_dir(wi);
             ray.d = wi;
         } else {
             RGB att; V3 dir;
-            glass_scatter(sc, m, u_scat, &att, &dir);
+            if (m.type == YAR_MATERIAL_MIRROR) {                                     // delta_scattering.cpp:27-33
+                att = m.refl;
+                dir = reflect(sc.wo, sc.n);
+            } else {
+                glass_scatter(sc, m, u_scat, &att, &dir);
+            }
             Ray dray{sc.origin_dir(dir), dir};
             hit = Hit{Inf, -1, 0, {}};
             pc.closest++;
@@ -1188,6 +1263,8 @@ static bool load_scene(const yar_scene_desc* d, Scene& s, int threads) {
         m.eta_i = md.eta_i;
         m.ior = md.index_of_refraction;
         m.roughness = md.u_roughness;
+        m.r0 = md.specular_r0;
+        m.pbrt3 = d->config.pbrt3_scene != 0;
         auto remap = [&](float r) {                                                   // scattering.cpp:240-269
             if (md.roughness_is_alpha) return r;
             if (d->config.pbrt3_scene) {
@@ -1453,7 +1530,7 @@ int32_t oref_bsdf_sample_local(int32_t type, const float* refl, const float* eta
     Shading sc = probe_frame();
     V3 wi;
     float pdf = 0.f;
-    RGB f = bsdf_sample(s, sc, V2{u[0], u[1]}, V3(wo[0], wo[1], wo[2]), &wi, &pdf);
+    RGB f = bsdf_sample(s, sc, V2{u[0], u[1]}, 0.f, V3(wo[0], wo[1], wo[2]), &wi, &pdf);
     wi_out[0] = wi.x; wi_out[1] = wi.y; wi_out[2] = wi.z;
     f_out[0] = f.r; f_out[1] = f.g; f_out[2] = f.b;
     *pdf_out = pdf;
@@ -1461,6 +1538,38 @@ int32_t oref_bsdf_sample_local(int32_t type, const float* refl, const float* eta
 }
 
 float oref_dielectric_fresnel(float cos_i, float eta) { return dielectric_fresnel(cos_i, eta); }
+
+// Plastic BRDF in the local frame (n = z): evaluate + pdf at (wo, wi), or sample with (u0, u1, u_scat).
+// pbrt3 selects Pbrt3_Plastic_BRDF; flipped = the original shading normal faced away from wo.
+void oref_plastic_eval(int32_t pbrt3, const float* kd, float r0, float alpha, int32_t flipped, const float* wo,
+                       const float* wi, float* f_out, float* pdf_out) {
+    Material m{};
+    m.type = YAR_MATERIAL_PLASTIC; m.refl = RGB{kd[0], kd[1], kd[2]}; m.r0 = r0; m.alpha_x = m.alpha_y = alpha;
+    m.pbrt3 = pbrt3 != 0;
+    Shading sc = probe_frame();
+    sc.flipped = flipped != 0;
+    const V3 o(wo[0], wo[1], wo[2]), i(wi[0], wi[1], wi[2]);
+    const RGB f = plastic_eval(m, sc, o, i);
+    f_out[0] = f.r; f_out[1] = f.g; f_out[2] = f.b;
+    *pdf_out = plastic_pdf(m, sc, o, i);
+}
+int32_t oref_plastic_sample(int32_t pbrt3, const float* kd, float r0, float alpha, int32_t flipped, const float* wo,
+                            const float* u3, float* wi_out, float* f_out, float* pdf_out) {
+    Scene s;
+    Material m{};
+    m.type = YAR_MATERIAL_PLASTIC; m.refl = RGB{kd[0], kd[1], kd[2]}; m.r0 = r0; m.alpha_x = m.alpha_y = alpha;
+    m.pbrt3 = pbrt3 != 0;
+    s.materials.push_back(m);
+    Shading sc = probe_frame();
+    sc.flipped = flipped != 0;
+    V3 wi;
+    float pdf = 0.f;
+    const RGB f = bsdf_sample(s, sc, V2{u3[0], u3[1]}, u3[2], V3(wo[0], wo[1], wo[2]), &wi, &pdf);
+    wi_out[0] = wi.x; wi_out[1] = wi.y; wi_out[2] = wi.z;
+    f_out[0] = f.r; f_out[1] = f.g; f_out[2] = f.b;
+    *pdf_out = pdf;
+    return (f.r != 0.f || f.g != 0.f || f.b != 0.f) ? 1 : 0;
+}
 
 // Smooth-glass event at a surface with shading normal n (facing wo), original orientation flipped or not.
 int32_t oref_glass_scatter(const float* n, const float* wo, int32_t flipped, float ior, const float* kr, const float* kt,

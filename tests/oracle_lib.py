@@ -9,7 +9,7 @@ from yar_amd import _abi
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 PCG_REF_SO = os.path.join(ROOT, "oracle", "_ref", "libpcg_ref.so")
-THREADS = int(os.environ.get("ORACLE_THREADS", str(min(16, os.cpu_count() or 1))))
+THREADS = int(os.environ.get("ORACLE_THREADS", str(len(os.sched_getaffinity(0)))))
 
 _o = None
 

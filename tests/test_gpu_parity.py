@@ -17,11 +17,13 @@ def _hits_agree(g, o):
 
 @pytest.mark.parametrize("which,opts", [("c1", {}), ("c2_small", {}),
                                         ("c2_small", {"traversal_stack_depth": 2}),
-                                        ("c2_small", {"traversal_stack_depth": 32, "stack_mode": 2})],
-                         ids=["c1", "c2_small", "c2_small-restart", "c2_small-private"])
+                                        ("c2_small", {"traversal_stack_depth": 32, "stack_mode": 2}),
+                                        ("c2_small", {"kdtree_builder": 1})],
+                         ids=["c1", "c2_small", "c2_small-restart", "c2_small-private", "c2_small-binned-trees"])
 def test_trace_rays_closest_matches_oracle(which, opts, c1_path, c2_small_path, oracle):
     """Short stacks restart from the root (kd-restart) when they overflow; the hit records must not
-    depend on the stack storage or depth."""
+    depend on the stack storage or depth, nor on the tree (binned SAH fast path against the oracle's
+    reference-builder trees)."""
     path = c1_path if which == "c1" else c2_small_path
     scene = yar_amd.Scene(path)
     rays = raygen.random_rays(scene, 1 << 18, seed=7)
@@ -91,3 +93,21 @@ def test_render_c2_small_matches_oracle(c2_small_path, oracle):
     exact, close, mean_rel = _compare_images(img, oimg, film, ofilm, "c2-small crop 16spp")
     assert close >= 0.98
     assert mean_rel <= 5e-3
+
+
+def test_render_plastic_and_mirror_matches_oracle(c1_path, oracle, tmp_path):
+    # the config-1 room with a mirror wall, a plastic wall and plastic triangles (pbrt3 plastic,
+    # perfect reflector): Pbrt3_Plastic_BRDF (bsdf_pbrt.cpp:264), delta reflection (delta_scattering.cpp:27)
+    lines = open(c1_path).read().splitlines()
+    mats = [i for i, l in enumerate(lines) if l.startswith('Material "matte"')]
+    lines[mats[0]] = 'Material "plastic" "rgb Kd" [ 0.3 0.4 0.5 ] "rgb Ks" [ 0.5 0.5 0.5 ] "float roughness" [ 0.08 ]'
+    lines[mats[2]] = 'Material "mirror" "rgb Kr" [ 0.85 0.8 0.75 ]'
+    lines[mats[-1]] = 'Material "plastic" "rgb Kd" [ 0.6 0.3 0.2 ] "float roughness" [ 0.3 ]'
+    p = tmp_path / "c1_plastic.pbrt"
+    p.write_text("\n".join(lines) + "\n")
+    img, film, var, st, oimg, ofilm, ovar = _render_pair(str(p), oracle, nx=4)
+    exact, close, mean_rel = _compare_images(img, oimg, film, ofilm, "c1 plastic+mirror 16spp")
+    assert close >= 0.99
+    assert mean_rel <= 1e-3
+    assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
+    assert (film["weight_sum"] == ofilm["weight_sum"]).all()

@@ -182,3 +182,69 @@ def test_oracle_kd_traversal_equals_brute_force(which, c1_path):
     # identical hit ids except exact ties between coincident triangles
     assert same[hit].mean() > 0.9999
     assert hit.mean() > 0.05
+
+
+def _sah_cost(words, bounds):
+    """Expected cost of a ray query against the tree: surface-area weighted interior visits plus
+    leaf primitive tests, relative to the root box."""
+    def area(lo, hi):
+        d = np.maximum(hi - lo, 0)
+        return 2 * (d[0] * d[1] + d[0] * d[2] + d[1] * d[2])
+    b = bounds.astype(np.float64)
+    root = area(b[:3], b[3:])
+    cost = 0.0
+    stack = [(0, b[:3].copy(), b[3:].copy())]
+    while stack:
+        n, lo, hi = stack.pop()
+        w0, w1 = int(words[2 * n]), int(words[2 * n + 1])
+        a = area(lo, hi) / root
+        if (w0 & 3) == 3:
+            cost += a * (w0 >> 2)
+            continue
+        cost += a
+        axis = w0 & 3
+        split = float(np.array([w1], np.uint32).view(np.float32)[0])
+        bh, al = hi.copy(), lo.copy()
+        bh[axis] = split
+        al[axis] = split
+        stack.append((n + 1, lo, bh))
+        stack.append((w0 >> 2, al, hi))
+    return cost
+
+
+@pytest.mark.parametrize("which", ["c1", "c2_ground", "c5_sphere"])
+def test_binned_builder_invariants_and_quality(which, c1_path, c2_small_path, tmp_path):
+    # the binned fast path (nodes above 1024 triangles) keeps the kd-tree invariants that make
+    # traversal exact, and its SAH cost stays close to the reference builder's
+    if which == "c1":
+        v, idx = list(_meshes(yar_amd.Scene(c1_path)))[-1]
+    elif which == "c2_ground":
+        meshes = list(_meshes(yar_amd.Scene(c2_small_path)))
+        v, idx = max(meshes, key=lambda m: m[1].size)
+    else:
+        p = yar_amd.generate_scene(5, str(tmp_path / "c5.pbrt"), seed=1, scale=0.1)
+        v, idx = list(_meshes(yar_amd.Scene(p)))[0]
+    ntri = idx.size // 3
+    assert ntri > 1024
+    exact = yar_amd.build_kdtree(v, idx)
+    binned = yar_amd.build_kdtree(v, idx, binned=True)
+    assert not np.array_equal(exact.words, binned.words)
+    assert np.array_equal(exact.bounds, binned.bounds)
+    deepest, leaves = _leaf_lists(binned.words)
+    assert deepest <= min(int(np.round(8 + 1.3 * np.floor(np.log2(ntri)))), 40)
+    assert set(i for lst in leaves for i in lst) == set(range(ntri))
+    tri = v.reshape(-1, 3).astype(np.float64)[idx.reshape(-1, 3)]
+    tlo, thi = tri.min(1), tri.max(1)
+    b = binned.bounds.astype(np.float64)
+    eps = 1e-6 * np.abs(b).max()
+    for ids, lo, hi in _leaf_boxes(binned.words, b[:3], b[3:]):
+        ext = hi - lo
+        slo, shi = lo + 1e-4 * ext + eps, hi - 1e-4 * ext - eps
+        cand = np.nonzero((tlo < shi).all(1) & (thi > slo).all(1))[0]
+        if cand.size == 0 or np.any(shi <= slo):
+            continue
+        inside = cand[_tris_overlap_open_box(tri[cand], slo, shi)]
+        assert not (set(inside.tolist()) - set(ids))
+    ce, cb = _sah_cost(exact.words, exact.bounds), _sah_cost(binned.words, binned.bounds)
+    print(f"{which}: {ntri} tris, SAH cost exact {ce:.3f} binned {cb:.3f} (ratio {cb / ce:.3f})")
+    assert cb <= 1.25 * ce

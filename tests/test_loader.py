@@ -104,7 +104,7 @@ Shape "trianglemesh" "integer indices" [ 0 1 2 ] "point P" [ 0 0 1  1 0 1  0 1 1
 
 
 def test_unsupported_inputs_are_parse_errors(tmp_path):
-    for body in ['Material "plastic"\nShape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]\n',
+    for body in ['Material "substrate"\nShape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]\n',
                  'Material "glass" "float uroughness" [ 0.2 ]\nShape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]\n',
                  'Material "metal"\nShape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]\n']:
         with pytest.raises(yar_amd.YarError) as e:
@@ -180,3 +180,26 @@ AttributeEnd
     assert np.allclose(P2, [[10, 0, 0], [12, 0, 0], [10, 1, 0]])
     N2 = _arr(d.meshes[1].normals, 9).reshape(3, 3)
     assert np.allclose(N2, [[0.5, 1, 0]] * 3)
+
+
+def test_plastic_and_mirror_materials(tmp_path):
+    # pbrt-parser defaults (Kd = Ks = 0.25, roughness 0.1, mirror Kr = 0.9) and r0 = Y(sRGB -> XYZ of Ks)
+    # (pbrt_scene.cpp:443-447, 503-525)
+    body = LIGHT + '''Material "plastic" "rgb Kd" [ 0.1 0.2 0.3 ] "rgb Ks" [ 0.4 0.5 0.6 ] "float roughness" [ 0.05 ]
+Shape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]
+Material "plastic"
+Shape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 1 1 0 1 0 1 1]
+Material "mirror" "rgb Kr" [ 0.8 0.7 0.6 ]
+Shape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 2 1 0 2 0 1 2]
+Material "mirror"
+Shape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 3 1 0 3 0 1 3]
+'''
+    d = _load(tmp_path, body).desc
+    m = [d.materials[i] for i in range(d.material_count)]
+    assert [x.type for x in m] == [5, 5, 4, 4]
+    assert list(m[0].reflectance) == pytest.approx([0.1, 0.2, 0.3])
+    assert m[0].specular_r0 == pytest.approx(0.2126729 * 0.4 + 0.7151522 * 0.5 + 0.0721750 * 0.6)
+    assert m[0].u_roughness == pytest.approx(0.05) and m[0].roughness_is_alpha == 0
+    assert list(m[1].reflectance) == pytest.approx([0.25] * 3) and m[1].u_roughness == pytest.approx(0.1)
+    assert m[1].specular_r0 == pytest.approx(0.25, rel=1e-6)
+    assert list(m[2].reflectance) == pytest.approx([0.8, 0.7, 0.6]) and list(m[3].reflectance) == pytest.approx([0.9] * 3)

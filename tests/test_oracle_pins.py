@@ -392,3 +392,120 @@ def test_sah_root_split_matches_brute_force():
         assert np.array([w1], np.uint32).view(np.float32)[0] == np.float32(best[2])
         checked += 1
     assert checked > 150 and leaves >= 0
+
+
+def plastic(pbrt3, kd, r0, alpha, wo, wi=None, u=None, flipped=0):
+    L = O.lib()
+    P = C.c_void_p
+    L.oref_plastic_eval.argtypes = [C.c_int32, P, C.c_float, C.c_float, C.c_int32, P, P, P, P]
+    L.oref_plastic_sample.argtypes = [C.c_int32, P, C.c_float, C.c_float, C.c_int32, P, P, P, P, P]
+    L.oref_plastic_sample.restype = C.c_int32
+    kd, wo = _f32(kd), _f32(wo)
+    f = np.zeros(3, np.float32)
+    pdf = C.c_float()
+    if wi is not None:
+        wi = _f32(wi)
+        L.oref_plastic_eval(pbrt3, kd.ctypes.data, r0, alpha, flipped, wo.ctypes.data, wi.ctypes.data, f.ctypes.data,
+                            C.byref(pdf))
+        return f.astype(np.float64), pdf.value
+    u = _f32(u)
+    out = np.zeros(3, np.float32)
+    ok = L.oref_plastic_sample(pbrt3, kd.ctypes.data, r0, alpha, flipped, wo.ctypes.data, u.ctypes.data,
+                               out.ctypes.data, f.ctypes.data, C.byref(pdf))
+    return bool(ok), out.astype(np.float64), f.astype(np.float64), pdf.value
+
+
+def _hemi(theta, phi):
+    return np.array([np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi), np.cos(theta)])
+
+
+@pytest.mark.parametrize("pbrt3", [0, 1])
+def test_plastic_closed_forms(pbrt3):
+    # r0 = 0: the diffuse lobe alone, Kd / pi.  Normal incidence: both Fresnel forms give
+    # ((1.5 - 1) / (1.5 + 1))^2 = 0.04, G = 1 and D(n) = 1 / (pi alpha^2), so the specular term is
+    # r0 * 0.04 / (4 pi alpha^2).
+    kd, n = [0.3, 0.5, 0.7], [0, 0, 1]
+    f, _ = plastic(pbrt3, kd, 0.0, 0.2, _hemi(0.7, 0.3), _hemi(0.4, 2.0))
+    assert np.allclose(f, np.array(kd) / np.pi, rtol=1e-6)
+    for alpha in (0.1, 0.3):
+        f, pdf = plastic(pbrt3, kd, 0.8, alpha, n, n)
+        assert np.allclose(f, np.array(kd) / np.pi + 0.8 * 0.04 / (4 * np.pi * alpha ** 2), rtol=2e-5)
+        # pdf = (cos / pi + G1 D (wo.wh) / (wo.n) / (4 wo.wh)) / 2 with wh = n
+        assert pdf == pytest.approx(0.5 * (1 / np.pi + 1 / (np.pi * alpha ** 2) / 4), rel=2e-5)
+
+
+@pytest.mark.parametrize("pbrt3,alpha", [(1, 0.15), (0, 0.15), (1, 0.4)])
+def test_plastic_sampling_matches_quadrature_and_reciprocity(pbrt3, alpha):
+    rng = np.random.default_rng(13)
+    kd, r0 = [0.4, 0.3, 0.2], 0.6
+    for theta_o in (0.3, 1.0):
+        wo = _hemi(theta_o, 0.4)
+        # reciprocity f(wo, wi) = f(wi, wo)
+        for _ in range(20):
+            wi = _hemi(rng.uniform(0, 1.5), rng.uniform(0, 2 * np.pi))
+            f1, _ = plastic(pbrt3, kd, r0, alpha, wo, wi)
+            f2, _ = plastic(pbrt3, kd, r0, alpha, wi, wo)
+            assert np.allclose(f1, f2, rtol=1e-5)
+        # quadrature of the albedo int f cos and of the pdf over the hemisphere (midpoint rule in
+        # (cos theta, phi)) against the importance-sampled estimate and the fraction of valid samples
+        nt, nphi = 400, 200
+        ct = (np.arange(nt) + 0.5) / nt
+        ph = (np.arange(nphi) + 0.5) / nphi * 2 * np.pi
+        alb, mass = 0.0, 0.0
+        for c in ct:
+            st = np.sqrt(1 - c * c)
+            for p in ph[::4]:
+                wi = np.array([st * np.cos(p), st * np.sin(p), c])
+                f, pdf = plastic(pbrt3, kd, r0, alpha, wo, wi)
+                alb += f[1] * c
+                mass += pdf
+        dA = (1.0 / nt) * (2 * np.pi / (nphi / 4))
+        alb *= dA
+        mass *= dA
+        n = 6000
+        est, valid = 0.0, 0
+        for u in rng.uniform(size=(n, 3)):
+            ok, wi, f, pdf = plastic(pbrt3, kd, r0, alpha, wo, u=u)
+            if ok:
+                est += f[1] * wi[2] / pdf
+                valid += 1
+        est /= n
+        assert est == pytest.approx(alb, rel=0.03), (theta_o, est, alb)
+        assert mass == pytest.approx(valid / n, abs=0.02), (theta_o, mass, valid / n)
+
+
+def test_mirror_reflects_point_lit_plane_closed_form():
+    # Camera at (0, 1, 0) looking along +x at a 45-degree mirror centred at (1, 1, 0) (normal
+    # (-1, -1, 0)/sqrt 2): every camera ray turns straight down onto the diffuse plane y = 0 near
+    # (1, 0, 0), lit by a point light at (4, 2, 0) that the mirror does not shadow.  With maxdepth 2
+    # (the delta bounce counts, delta_scattering.cpp:27-33, path_tracing.cpp:60-84) the radiance is
+    # Kr * albedo/pi * I * cos / d^2 with cos = 2 / sqrt 13, d^2 = 13.
+    from tests.scenes import DescScene
+    from tests.test_oracle_fixtures import _view_from_desc
+    from yar_amd import _abi
+    s = 0.3 / np.sqrt(2)
+    c = np.array([1.0, 1.0, 0.0])
+    e1, e2 = np.array([0, 0, 0.3]), np.array([s, -s, 0])
+    quad = {"vertices": [c - e1 - e2, c + e1 - e2, c + e1 + e2, c - e1 + e2], "indices": [0, 1, 2, 0, 2, 3]}
+    plane = {"vertices": [[-50, 0, -50], [-50, 0, 50], [50, 0, 50], [50, 0, -50]], "indices": [0, 1, 2, 0, 2, 3]}
+    kr, albedo, intensity = [0.9, 0.6, 0.3], 0.5, 2.0
+    sc = DescScene(
+        meshes=[plane, quad],
+        materials=[{"type": _abi.YAR_MATERIAL_DIFFUSE, "reflectance": [albedo] * 3},
+                   {"type": _abi.YAR_MATERIAL_MIRROR, "reflectance": kr}],
+        objects=[{"mesh": 0, "material": 0}, {"mesh": 1, "material": 1}],
+        points=[{"position": [4.0, 2.0, 0.0], "intensity": [intensity] * 3}],
+        camera={"c2w": [0, 0, -1, 0, 0, 1, 0, 1, 1, 0, 0, 0], "fov": 1.0},
+        config={"film_width": 16, "film_height": 16, "region": (0, 0, 16, 16), "nx": 2, "ny": 2, "maxdepth": 2})
+    img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
+    want = np.array(kr) * albedo / np.pi * intensity * 2.0 / 13.0 ** 1.5
+    # +-0.5 degree of view moves the floor point by +-0.017: +-1.2% across the image, linear, so the
+    # image mean keeps only the quadratic residue
+    assert np.allclose(img, want, rtol=2e-2), (img.min(axis=(0, 1)), img.max(axis=(0, 1)), want)
+    assert np.allclose(img.mean(axis=(0, 1)), want, rtol=1e-3)
+    # the mirror tints by Kr, channel by channel
+    assert np.allclose(img[..., 0] / img[..., 1], kr[0] / kr[1], rtol=1e-5)
+    # maxdepth 1 ends the path at the delta bounce: nothing reaches the camera
+    sc.desc.config.max_light_bounces = 1
+    img1, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
+    assert (img1 == 0).all()

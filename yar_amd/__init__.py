@@ -119,11 +119,13 @@ class KdTree:
             pass
 
 
-def build_kdtree(vertices, indices):
+def build_kdtree(vertices, indices, binned=False):
+    """Mesh kd-tree: the reference builder's exact tree, or the binned SAH fast path."""
     v = np.ascontiguousarray(vertices, dtype=np.float32)
     i = np.ascontiguousarray(indices, dtype=np.int32)
     h = C.c_void_p()
-    _check(lib().yar_kdtree_build_mesh(v.ctypes.data, i.ctypes.data, i.size // 3, C.byref(h)))
+    f = lib().yar_kdtree_build_mesh_binned if binned else lib().yar_kdtree_build_mesh
+    _check(f(v.ctypes.data, i.ctypes.data, i.size // 3, C.byref(h)))
     return KdTree(h)
 
 

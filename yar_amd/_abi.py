@@ -3,6 +3,7 @@ import ctypes as C
 
 YAR_OK = 0
 YAR_MATERIAL_NONE, YAR_MATERIAL_DIFFUSE, YAR_MATERIAL_METAL, YAR_MATERIAL_GLASS = 0, 1, 2, 3
+YAR_MATERIAL_MIRROR, YAR_MATERIAL_PLASTIC = 4, 5
 
 
 class MeshDesc(C.Structure):
@@ -17,7 +18,7 @@ class MaterialDesc(C.Structure):
                 ("reflectance", C.c_float * 3), ("transmittance", C.c_float * 3),
                 ("eta", C.c_float * 3), ("k", C.c_float * 3), ("eta_i", C.c_float),
                 ("u_roughness", C.c_float), ("v_roughness", C.c_float),
-                ("index_of_refraction", C.c_float)]
+                ("index_of_refraction", C.c_float), ("specular_r0", C.c_float)]
 
 
 class ObjectDesc(C.Structure):
@@ -64,7 +65,8 @@ class GpuOptions(C.Structure):
     _fields_ = [("device", C.c_int32), ("block_size", C.c_int32), ("max_paths_in_flight", C.c_int64),
                 ("traversal_stack_depth", C.c_int32), ("enable_counters", C.c_int32),
                 ("use_graph", C.c_int32), ("enable_timing", C.c_int32), ("stack_mode", C.c_int32),
-                ("lane_refill", C.c_int32), ("disable_mis_cull", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("lane_refill", C.c_int32), ("disable_mis_cull", C.c_int32), ("bin_rays", C.c_int32),
+                ("kdtree_builder", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class View(C.Structure):
@@ -147,6 +149,7 @@ SIGNATURES = {
     "yar_scene_free": (None, [_P]),
     "yar_scene_generate": (C.c_int, [C.c_int32, C.c_uint64, C.c_double, C.c_char_p]),
     "yar_kdtree_build_mesh": (C.c_int, [_P, _P, C.c_int32, C.POINTER(_P)]),
+    "yar_kdtree_build_mesh_binned": (C.c_int, [_P, _P, C.c_int32, C.POINTER(_P)]),
     "yar_kdtree_build_boxes": (C.c_int, [_P, C.c_int32, C.POINTER(_P)]),
     "yar_kdtree_node_count": (C.c_uint32, [_P]),
     "yar_kdtree_nodes": (C.POINTER(C.c_uint32), [_P]),

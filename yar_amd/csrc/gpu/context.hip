@@ -163,7 +163,7 @@ struct yar_gpu_ctx {
     DevBuf<uint2> pk;
     DevBuf<uint32_t> state;
     DevBuf<uint64_t> rng;
-    DevBuf<uint32_t> q_a, q_b, sh_path, mi_path, dl_path, counts;
+    DevBuf<uint32_t> q_a, q_b, sh_path, mi_path, dl_path, counts, q_unsorted, bins;
     DevBuf<float4> sh_o, sh_d, sh_c, mi_o, mi_d, mi_f;
     DevBuf<float> mi_ndw;
     DevBuf<unsigned long long> totals;
@@ -214,7 +214,7 @@ struct yar_gpu_ctx {
         vtx_uv.release(); materials.release(); rects.release(); points.release();
         pixels.release(); pix_first.release(); dviews.release(); chunks.release(); chunk_slot.release();
         perm.release(); jump_lo.release(); jump_hi.release(); hit_ot.release(); tr_ot.release(); state.release(); rng.release();
-        DevBuf<uint32_t>* bu[] = {&q_a, &q_b, &sh_path, &mi_path, &dl_path, &counts};
+        DevBuf<uint32_t>* bu[] = {&q_a, &q_b, &sh_path, &mi_path, &dl_path, &counts, &q_unsorted, &bins};
         for (auto* b : bu) b->release();
         mi_ndw.release(); totals.release(); out_var.release(); spill.release();
         grng.release(); rec.release(); guide_in.release(); guide_out.release(); guide_path.release(); train.release();
@@ -287,8 +287,9 @@ static float pbrt3_remap(float r, const yar_render_config& cfg, bool is_alpha) {
 static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
     using namespace yar;
     auto t0 = std::chrono::steady_clock::now();
-    int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<KdBuildResult> trees = build_mesh_kdtrees(d, threads);
+    // every host core builds mesh trees, as the reference's per-mesh build does (scene_context.cpp:33-61)
+    int threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::vector<KdBuildResult> trees = build_mesh_kdtrees(d, threads, ctx->opt.kdtree_builder == 1);
     // scene tree over object world bounds (build_scene_kdtree, kdtree_builder.cpp:443-463)
     std::vector<float> boxes(6 * (size_t)d->object_count);
     for (int i = 0; i < d->object_count; i++) {
@@ -403,8 +404,10 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
         g.trans = make_float4(m.transmittance[0], m.transmittance[1], m.transmittance[2], m.index_of_refraction);
         g.eta = make_float4(m.eta[0], m.eta[1], m.eta[2], 0.f);
         g.k = make_float4(m.k[0], m.k[1], m.k[2], 0.f);
+        if (m.type == YAR_MATERIAL_PLASTIC)                      // trans.x = r0, trans.y = pbrt3 form
+            g.trans = make_float4(m.specular_r0, d->config.pbrt3_scene ? 1.f : 0.f, 0.f, 0.f);
         if (m.type == YAR_MATERIAL_GLASS && m.u_roughness > 0.f) throw HipError{"rough glass is not supported", YAR_ERR_UNSUPPORTED};
-        if (m.type < YAR_MATERIAL_DIFFUSE || m.type > YAR_MATERIAL_GLASS) throw HipError{"unsupported material type", YAR_ERR_UNSUPPORTED};
+        if (m.type < YAR_MATERIAL_DIFFUSE || m.type > YAR_MATERIAL_PLASTIC) throw HipError{"unsupported material type", YAR_ERR_UNSUPPORTED};
     }
     std::vector<DevRect> rl(std::max(1, d->rect_light_count));
     for (int i = 0; i < d->rect_light_count; i++) {
@@ -731,6 +734,11 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     for (auto* b : qb) b->reserve(np);
     ctx->mi_ndw.reserve(np);
     ctx->counts.reserve(16);
+    const bool bin_rays = ctx->opt.bin_rays != 0;
+    if (bin_rays) {
+        ctx->q_unsorted.reserve(np);
+        ctx->bins.reserve(16);
+    }
     ctx->totals.reserve(24);
     ctx->out_film.reserve((size_t)std::max<int64_t>(1, total_pixels));
     ctx->out_var.reserve((size_t)std::max<int64_t>(1, total_pixels));
@@ -823,6 +831,11 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     q.shadow_path = ctx->sh_path.p; q.shadow_o = ctx->sh_o.p; q.shadow_d = ctx->sh_d.p; q.shadow_c = ctx->sh_c.p;
     q.mis_path = ctx->mi_path.p; q.mis_o = ctx->mi_o.p; q.mis_d = ctx->mi_d.p; q.mis_f = ctx->mi_f.p;
     q.mis_ndw = ctx->mi_ndw.p; q.dl_path = ctx->dl_path.p; q.counts = ctx->counts.p; q.totals = ctx->totals.p;
+    if (bin_rays) {
+        q.unsorted = ctx->q_unsorted.p;
+        q.bins = ctx->bins.p;
+        q.bin_rays = 1;
+    }
     if (guided) {
         q.guide_path = ctx->guide_path.p;
         q.guide_in = ctx->guide_in.p;
@@ -919,7 +932,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
             }
             const int parity = it & 1;
             if (kt.on) {
-                static const int kind_of_stage[kStageCount] = {4, 0, 1, 5, 5, 2, 3, 4};
+                static const int kind_of_stage[kStageCount] = {4, 0, 1, 5, 5, 4, 2, 3, 4};
                 for (int stage = 0; stage < kStageCount; stage++) {
                     kt.begin(kind_of_stage[stage]);
                     launch_stage(stage, ctx->scene, slot, P, q, parity, lc);

@@ -201,11 +201,19 @@ YG_INLINE uint32_t leaf_id(const uint2* __restrict__ nodes, uint32_t nd, uint32_
     return (j & 1) ? w.x : w.y;
 }
 
+// Short-stack bookkeeping of one tree level packed in one word: push pointer in bits 0-7 (used
+// modulo the stack size, a power of two dividing 256), live entries in bits 8-15, and bit 16 set
+// when entries were dropped since the last restart.
+constexpr uint32_t kSsCount = 1u << 8, kSsDropped = 1u << 16;
+YG_INLINE uint32_t ss_count(uint32_t ss) { return (ss >> 8) & 0xffu; }
+YG_INLINE uint32_t ss_push(uint32_t ss) { return (ss & ~0xffu) | ((ss + 1u) & 0xffu); }
+YG_INLINE uint32_t ss_pop(uint32_t ss) { return ((ss & ~0xffu) | ((ss - 1u) & 0xffu)) - kSsCount; }
+
 // One step of a kd-tree descent shared by both levels: interior node -> child (pushing the far
 // child when the ray straddles the split), as KdTree::intersect's inner loop (kdtree.cpp:245-290).
 template <class Stack>
 YG_INLINE void kd_interior(uint2 w, uint32_t& node, V3 o, V3 d, V3 inv, float& tmin, float& tmax, Stack& st,
-                           int mask, int depth, int& sp, int& count, bool& dropped) {
+                           int mask, int depth, uint32_t& ss) {
     const int axis = (int)(w.x & 3u);
     const float oa = axis == 0 ? o.x : (axis == 1 ? o.y : o.z);
     const float da = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
@@ -221,9 +229,8 @@ YG_INLINE void kd_interior(uint2 w, uint32_t& node, V3 o, V3 d, V3 inv, float& t
         } else if (ts < tmin) {
             node = second;
         } else {
-            st.put(sp & mask, second, tmax);
-            sp++;
-            if (count == depth) dropped = true; else count++;
+            st.put((int)(ss & (uint32_t)mask), second, tmax);
+            ss = ss_count(ss) == (uint32_t)depth ? (ss_push(ss) | kSsDropped) : (ss_push(ss) + kSsCount);
             node = first;
             tmax = ts;
         }
@@ -236,19 +243,19 @@ YG_INLINE void kd_interior(uint2 w, uint32_t& node, V3 o, V3 d, V3 inv, float& t
 // entries (kd-restart; a restart that would not advance steps past the degenerate point), or finish.
 // Returns false when the level's traversal is complete.
 template <class Stack>
-YG_INLINE bool kd_next(uint32_t& node, uint32_t root, float& tmin, float& tmax, float t1, Stack& st, int mask, int& sp,
-                       int& count, bool& dropped, float& restart_t) {
-    if (count > 0) {
-        sp--; count--;
+YG_INLINE bool kd_next(uint32_t& node, uint32_t root, float& tmin, float& tmax, float t1, Stack& st, int mask, uint32_t& ss,
+                       float& restart_t) {
+    if (ss_count(ss) > 0) {
+        ss = ss_pop(ss);
         uint32_t nn; float tm;
-        st.get(sp & mask, nn, tm);
+        st.get((int)(ss & (uint32_t)mask), nn, tm);
         node = nn;
         tmin = tmax;
         tmax = tm;
         return true;
     }
-    if (dropped && tmax < t1) {
-        dropped = false;
+    if ((ss & kSsDropped) && tmax < t1) {
+        ss &= ~kSsDropped;
         node = root;
         tmin = tmax > restart_t ? tmax : nextafterf(restart_t, f_inf());
         restart_t = tmin;
@@ -275,18 +282,16 @@ struct Tracer {
     // scene level
     uint32_t snode;
     float stmin, stmax, srestart;
-    int ssp, scount;
-    bool sdropped;
+    uint32_t sss;                   // packed short-stack state
     // objects of the current scene leaf still to visit
     uint32_t lnode, lfirst, ln, lj;
     // mesh level
     bool in_mesh;
     uint32_t mroot, mnode, oi;
     float mtmin, mtmax, mt1, mrestart, mesh_tmax;
-    int msp, mcount;
-    bool mdropped;
+    uint32_t mss;                   // packed short-stack state
     V3 lo, ld, linv;
-    int kx, ky, kz;
+    uint32_t kxyz;                  // watertight axis permutation kx | ky << 2 | kz << 4
     float sx, sy, sz;
     int iter;
 
@@ -299,7 +304,7 @@ struct Tracer {
         if (!slab(s.scene_lo, s.scene_hi, o, d, t0, t1)) return false;
         inv = v3(1.f / d.x, 1.f / d.y, 1.f / d.z);
         st1 = t1;
-        snode = 0; stmin = t0; stmax = t1; srestart = -1.f; ssp = 0; scount = 0; sdropped = false;
+        snode = 0; stmin = t0; stmax = t1; srestart = -1.f; sss = 0u;
         lnode = lfirst = ln = lj = 0;
         in_mesh = false;
         iter = 0;
@@ -322,7 +327,7 @@ struct Tracer {
                 if (COUNT) cnt.nodes++;
                 uint2 w = s.mesh_nodes[mnode];
                 while ((w.x & 3u) != 3u) {
-                    kd_interior(w, mnode, lo, ld, linv, mtmin, mtmax, mst, mdepth - 1, mdepth, msp, mcount, mdropped);
+                    kd_interior(w, mnode, lo, ld, linv, mtmin, mtmax, mst, mdepth - 1, mdepth, mss);
                     if (COUNT) cnt.nodes++;
                     w = s.mesh_nodes[mnode];
                 }
@@ -334,7 +339,8 @@ struct Tracer {
                                  p2 = s.tri_verts[3 * (size_t)ti + 2];
                     if (COUNT) cnt.tri_tests++;
                     V3 b;
-                    const float t = watertight(lo, ld, kx, ky, kz, sx, sy, sz, p0, p1, p2, b);
+                    const float t = watertight(lo, ld, (int)(kxyz & 3u), (int)((kxyz >> 2) & 3u), (int)(kxyz >> 4), sx, sy,
+                                               sz, p0, p1, p2, b);
                     if (ANY) {
                         if (t >= any_t) continue;
                         if (degenerate(p0, p1, p2)) continue;
@@ -345,7 +351,7 @@ struct Tracer {
                         hit.t = t; hit.tri = ti; hit.b = b;
                     }
                 }
-                done = !kd_next(mnode, mroot, mtmin, mtmax, mt1, mst, mdepth - 1, msp, mcount, mdropped, mrestart);
+                done = !kd_next(mnode, mroot, mtmin, mtmax, mt1, mst, mdepth - 1, mss, mrestart);
             }
             if (done) {
                 in_mesh = false;
@@ -368,27 +374,28 @@ struct Tracer {
             mroot = fbits(ob.lo_node.w);
             // watertight-test setup hoisted per mesh ray (intersection.cpp:55-61)
             const float ax = fabsf(ld.x), ay = fabsf(ld.y), az = fabsf(ld.z);
-            kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
-            kx = (kz == 2 ? 0 : kz + 1);
-            ky = (kz == 0 ? 2 : kz - 1);
+            const int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+            const int kx = (kz == 2 ? 0 : kz + 1);
+            const int ky = (kz == 0 ? 2 : kz - 1);
+            kxyz = (uint32_t)kx | ((uint32_t)ky << 2) | ((uint32_t)kz << 4);
             const float dx = ld[kx], dy = ld[ky], dz = ld[kz];
             sx = -dx / dz; sy = -dy / dz; sz = 1.f / dz;
             mesh_tmax = hit.t;
             mnode = mroot; mtmin = mt0; mtmax = mt1; mrestart = -1.f;
-            msp = 0; mcount = 0; mdropped = false;
+            mss = 0u;
             in_mesh = true;
             return false;
         }
         if (ln > 0) {
             // the leaf's objects are done: continue the scene traversal behind it
             ln = 0;
-            return !kd_next(snode, 0u, stmin, stmax, st1, sst, sdepth - 1, ssp, scount, sdropped, srestart);
+            return !kd_next(snode, 0u, stmin, stmax, st1, sst, sdepth - 1, sss, srestart);
         }
         if (!(loop_t > stmin)) return true;
         if (COUNT) cnt.nodes++;
         uint2 w = s.scene_nodes[snode];
         while ((w.x & 3u) != 3u) {
-            kd_interior(w, snode, o, d, inv, stmin, stmax, sst, sdepth - 1, sdepth, ssp, scount, sdropped);
+            kd_interior(w, snode, o, d, inv, stmin, stmax, sst, sdepth - 1, sdepth, sss);
             if (COUNT) cnt.nodes++;
             w = s.scene_nodes[snode];
         }
@@ -398,7 +405,7 @@ struct Tracer {
             lnode = snode; lfirst = w.y; ln = n; lj = 0;
             return false;
         }
-        return !kd_next(snode, 0u, stmin, stmax, st1, sst, sdepth - 1, ssp, scount, sdropped, srestart);
+        return !kd_next(snode, 0u, stmin, stmax, st1, sst, sdepth - 1, sss, srestart);
     }
 };
 

@@ -74,6 +74,13 @@ struct Queues {
     // [0..15] traversal counters (4 per kernel kind), [16] extend rays, [17] shadow rays, [18] mis rays,
     // [19] extend launches
     unsigned long long* totals;
+    // Direction-octant binning of the next iteration's active list (bin_rays != 0): shade and guide
+    // sample append to `unsorted`; k_bin_count counts the new rays per octant in bins[0..7] and
+    // k_bin_scatter writes the active list grouped by octant (bins[8..15] are its cursors), so a
+    // traversal wave takes rays of one octant from neighbouring pixels.
+    uint32_t* unsorted;
+    uint32_t* bins;
+    int bin_rays;
 };
 
 // One chunk of pixels (whole film tiles, possibly of several views).  Lives in device memory so
@@ -135,7 +142,7 @@ void launch_sample_perms(const Chunk& host_chunk, const Chunk* dev_chunk, hipStr
 void launch_raygen(const Chunk* c, long long n_paths_max, const Paths& p, Queues q, hipStream_t s);
 // Stages of one bounce-loop iteration, in launch order.
 enum IterStage {
-    kStageReset, kStageExtend, kStageShade, kStageGuideForward, kStageGuideSample, kStageShadow, kStageMis,
+    kStageReset, kStageExtend, kStageShade, kStageGuideForward, kStageGuideSample, kStageBin, kStageShadow, kStageMis,
     kStageResolve, kStageCount
 };
 void launch_stage(int stage, const DevScene& sc, const Chunk* c, const Paths& p, Queues q, int parity, const LaunchCfg& cfg);

@@ -543,7 +543,7 @@ template <bool GUIDED>
 __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queues q, int parity) {
     const Chunk& c = *cp;
     const uint32_t* in = parity ? q.active1 : q.active0;
-    uint32_t* out = parity ? q.active0 : q.active1;
+    uint32_t* out = q.bin_rays ? q.unsorted : (parity ? q.active0 : q.active1);
     uint32_t* n_out = &q.counts[parity ^ 1];
     const uint32_t n = q.counts[parity];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -625,13 +625,21 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
                 init_shading(s, from4(d4), ot.x, (uint32_t)ot.y, v3(tb.y, tb.z, tb.w), sc);
                 const float u_scat = next1();
                 const float u_sel = next1();
-                next1();                                                // u_scattering_type_next_segment
+                const float u_scat_next = next1();                      // u_scattering_type_next_segment
                 float ulx, uly, ubx, uby, unx, uny;
                 next2(ulx, uly);
                 next2(ubx, uby);
                 next2(unx, uny);
                 const DevMaterial m = s.materials[sc.material];
-                if (m.type == 3) {
+                if (m.type == 4) {
+                    // perfect reflector (delta_scattering.cpp:27-33)
+                    const V3 dir = reflect(sc.wo, sc.n);
+                    P.ray_o[slot] = f4(origin_dir(sc, dir), 0.f);
+                    P.ray_d[slot] = f4(dir, 0.f);
+                    coeff = coeff * rgb4(m.refl);
+                    flags |= kDelta;
+                    want_active = true;
+                } else if (m.type == 3) {
                     // smooth glass, get_glass_info (delta_scattering.cpp:53-100); enter/exit from the
                     // original shading-normal orientation (no nested-dielectric tracking in pbrt scenes)
                     const float ior = m.trans.w;
@@ -707,7 +715,7 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
                         {
                             V3 wi = v3(0.f, 0.f, 0.f);
                             float bpdf = 0.f;
-                            const RGB f = bsdf_sample(m, sc, ubx, uby, sc.wo, wi, bpdf);
+                            const RGB f = bsdf_sample(m, sc, ubx, uby, u_scat, sc.wo, wi, bpdf);
                             if (!is_black(f)) {
                                 const float lndw = dot(ln, -wi);
                                 if (lndw > 1e-4f) {
@@ -747,11 +755,11 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
                             g_x0 = make_float4(x[0], x[1], x[2], x[3]);
                             g_x1 = make_float4(x[4], x[5], x[6], x[7]);
                             g_x2 = make_float4(x[8], (float)ridx, unx, uny);
-                            g_x3 = make_float4(u_term, 0.f, 0.f, 0.f);
+                            g_x3 = make_float4(u_term, u_scat_next, 0.f, 0.f);
                         } else {
                             V3 wi = v3(0.f, 0.f, 0.f);
                             float pdf = 0.f;
-                            const RGB f = bsdf_sample(m, sc, unx, uny, sc.wo, wi, pdf);
+                            const RGB f = bsdf_sample(m, sc, unx, uny, u_scat_next, sc.wo, wi, pdf);
                             if (is_black(f)) {
                                 finished = true;
                             } else {
@@ -776,7 +784,7 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
                     } else {
                         V3 wi = v3(0.f, 0.f, 0.f);
                         float pdf = 0.f;
-                        const RGB f = bsdf_sample(m, sc, unx, uny, sc.wo, wi, pdf);
+                        const RGB f = bsdf_sample(m, sc, unx, uny, u_scat_next, sc.wo, wi, pdf);
                         if (is_black(f)) {
                             finished = true;
                         } else {
@@ -845,7 +853,7 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
 // either way, so the estimator stays unbiased for any network output.
 __global__ void k_guide_sample(DevScene s, const Chunk* __restrict__ cp, Paths P, Queues q, int parity) {
     const Chunk c = *cp;
-    uint32_t* out = parity ? q.active0 : q.active1;
+    uint32_t* out = q.bin_rays ? q.unsorted : (parity ? q.active0 : q.active1);
     uint32_t* n_out = &q.counts[parity ^ 1];
     const uint32_t n = q.counts[9];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -858,7 +866,8 @@ __global__ void k_guide_sample(DevScene s, const Chunk* __restrict__ cp, Paths P
             const float4* gi = reinterpret_cast<const float4*>(q.guide_in + (size_t)i * 16);
             const float4 g2 = gi[2];
             const int ridx = (int)g2.y;
-            const float unx = g2.z, uny = g2.w, u_term = gi[3].x;
+            const float4 g3 = gi[3];
+            const float unx = g2.z, uny = g2.w, u_term = g3.x, u_scat_next = g3.y;
             const uint2 pk = P.pk[slot];
             const uint64_t rinc = guide_inc(c, pk);
             uint64_t rs = P.grng[slot];
@@ -882,7 +891,7 @@ __global__ void k_guide_sample(DevScene s, const Chunk* __restrict__ cp, Paths P
                     pb = bsdf_pdf(m, sc, sc.wo, wi);
                 }
             } else {
-                f = bsdf_sample(m, sc, unx, uny, sc.wo, wi, pb);
+                f = bsdf_sample(m, sc, unx, uny, u_scat_next, sc.wo, wi, pb);
                 if (!(dot(sc.n, wi) > 0.f)) f = rgb(0.f, 0.f, 0.f);
             }
             const float pm = is_black(f) ? 0.f : a * mixture_pdf(lobes, wi) + (1.f - a) * pb;
@@ -946,11 +955,59 @@ __global__ void k_resolve(DevScene s, const Chunk* __restrict__ cp, Paths P, Que
 }
 
 // Resets the output counters of an iteration (a kernel, not a memset, so it can sit in a graph).
-__global__ void k_iter_reset(uint32_t* counts, int parity) {
+__global__ void k_iter_reset(uint32_t* counts, uint32_t* bins, int parity) {
     const int t = threadIdx.x;
     if (t == 0) counts[parity ^ 1] = 0u;
     else if (t >= 2 && t <= 4) counts[t] = 0u;
     else if (t >= 6 && t <= 9) counts[t] = 0u;
+    if (bins && t < 16) bins[t] = 0u;
+}
+
+YG_INLINE uint32_t octant(float4 d) { return (d.x < 0.f ? 1u : 0u) | (d.y < 0.f ? 2u : 0u) | (d.z < 0.f ? 4u : 0u); }
+
+// Octant histogram of the unsorted next active list (one atomic per wave and octant present).
+__global__ void k_bin_count(Paths P, Queues q, int parity) {
+    const uint32_t n = q.counts[parity ^ 1];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t blk = blockIdx.x * blockDim.x; blk < n; blk += stride) {
+        const uint32_t i = blk + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t o = valid ? octant(P.ray_d[q.unsorted[i]]) : 8u;
+        for (uint32_t b = 0; b < 8; b++) {
+            const uint64_t m = __ballot(o == b);
+            if (m && __lane_id() == (uint32_t)(__ffsll((unsigned long long)m) - 1))
+                atomicAdd(&q.bins[b], (uint32_t)__popcll(m));
+        }
+    }
+}
+
+// Writes the next active list grouped by octant: entry i goes to its octant's range (prefix of the
+// histogram) at a slot reserved with one atomic per wave and octant.
+__global__ void k_bin_scatter(Paths P, Queues q, int parity) {
+    uint32_t* out = parity ? q.active0 : q.active1;
+    const uint32_t n = q.counts[parity ^ 1];
+    uint32_t prefix[8];
+    uint32_t acc = 0;
+    for (int b = 0; b < 8; b++) { prefix[b] = acc; acc += q.bins[b]; }
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t lane = __lane_id();
+    for (uint32_t blk = blockIdx.x * blockDim.x; blk < n; blk += stride) {
+        const uint32_t i = blk + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t slot = valid ? q.unsorted[i] : 0u;
+        const uint32_t o = valid ? octant(P.ray_d[slot]) : 8u;
+        uint32_t pos = 0;
+        for (uint32_t b = 0; b < 8; b++) {
+            const uint64_t m = __ballot(o == b);
+            if (!m) continue;
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(&q.bins[8 + b], (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            if (o == b) pos = prefix[b] + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        }
+        if (valid) out[pos] = slot;
+    }
 }
 
 static size_t stack_lds(const LaunchCfg& cfg) {
@@ -984,7 +1041,15 @@ int traversal_blocks_per_cu(const LaunchCfg& cfg) {
 
 void launch_stage(int stage, const DevScene& sc, const Chunk* c, const Paths& p, Queues q, int parity, const LaunchCfg& cfg) {
     switch (stage) {
-    case kStageReset: hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(64), 0, cfg.stream, q.counts, parity); break;
+    case kStageReset:
+        hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(64), 0, cfg.stream, q.counts, q.bin_rays ? q.bins : nullptr, parity);
+        break;
+    case kStageBin:
+        if (q.bin_rays) {
+            hipLaunchKernelGGL(k_bin_count, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, p, q, parity);
+            hipLaunchKernelGGL(k_bin_scatter, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, p, q, parity);
+        }
+        break;
     case kStageExtend: YG_LAUNCH_TRAV(k_extend, cfg.trav_grid, sc, p, q, parity); break;
     case kStageShade:
         if (cfg.guided) hipLaunchKernelGGL(k_shade<true>, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q, parity);

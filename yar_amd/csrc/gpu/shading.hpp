@@ -207,8 +207,64 @@ YG_INLINE float vndf_wi_pdf(V3 wo, V3 wh, float ax, float ay) {
     return wh_pdf / (4 * dot(wh, wo));
 }
 
+// Isotropic GGX against the shading normal (GGX_Distribution::D / lambda / G / G1, scattering.cpp:153-238)
+// and the visible-normal reflection pdf (sampling.cpp:337-347, scattering.cpp:107-116).
+YG_INLINE float D_iso(V3 wh, V3 n, float alpha) {
+    float c = dot(wh, n);
+    if (c <= 0.f) return 0.f;
+    float c2 = c * c;
+    float a2 = alpha * alpha;
+    float k = c2 * (a2 - 1.f) + 1.f;
+    return a2 / (kPi * k * k);
+}
+YG_INLINE float lambda_iso(V3 v, V3 n, float alpha) {
+    float c = dot(v, n);
+    float c2 = c * c;
+    float t2 = fmaxf(0.f, (1.f - c2) / c2);
+    return 0.5f * (-1.f + sqrtf(1.f + alpha * alpha * t2));
+}
+YG_INLINE float G_iso(V3 wi, V3 wo, V3 n, float a) { return 1.f / (1.f + lambda_iso(wi, n, a) + lambda_iso(wo, n, a)); }
+YG_INLINE float G1_iso(V3 v, V3 n, float a) { return 1.f / (1.f + lambda_iso(v, n, a)); }
+YG_INLINE float vndf_wi_pdf_iso(V3 wo, V3 wh, V3 n, float alpha) {
+    float wh_pdf = G1_iso(wo, n, alpha) * D_iso(wh, n, alpha) * fmaxf(0.f, dot(wo, wh)) / dot(wo, n);
+    return wh_pdf / (4.f * dot(wh, wo));
+}
+
+// Plastic_BRDF (bsdf.cpp:214-262) and its pbrt3 form Pbrt3_Plastic_BRDF (bsdf_pbrt.cpp:264-291, Fresnel
+// of a 1.5 dielectric with pbrt3's flipped relative ior).  Material: refl = Kd, alpha_x, trans.x = r0,
+// trans.y != 0 for the pbrt3 form.
+YG_INLINE RGB plastic_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
+    V3 wh = normalized(wo + wi);
+    float cos_i = dot(wi, wh);
+    RGB F;
+    if (m.trans.y != 0.f) {
+        float rel = 1.f / 1.5f;
+        V3 original = sc.flipped ? -sc.n : sc.n;
+        if (dot(original, wi) < 0.f) rel = 1.5f / 1.f;
+        F = splat(dielectric_fresnel(cos_i, rel));
+    } else {
+        float k = 1.f - fabsf(cos_i);                                                  // scattering.cpp:10
+        float k5 = (k * k) * (k * k) * k;
+        F = splat(0.04f) + (splat(1.f) - splat(0.04f)) * k5;
+    }
+    float G = G_iso(wi, wo, sc.n, m.alpha_x);
+    float D = D_iso(wh, sc.n, m.alpha_x);
+    float won = dot(wo, sc.n), win = dot(wi, sc.n);
+    RGB base = F * ((G * D) / (4.f * won * win));
+    RGB spec = m.trans.x * base;
+    RGB diffuse = rgb4(m.refl) * kPiInv;
+    return diffuse + spec;
+}
+YG_INLINE float plastic_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
+    float diffuse_pdf = dot(sc.n, wi) / kPi;
+    V3 wh = normalized(wo + wi);
+    float spec_pdf = vndf_wi_pdf_iso(wo, wh, sc.n, m.alpha_x);
+    return 0.5f * (diffuse_pdf + spec_pdf);
+}
+
 YG_INLINE RGB bsdf_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
     if (m.type == 1) return kPiInv * rgb4(m.refl);
+    if (m.type == 5) return plastic_eval(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
     V3 wh = normalized(wol + wil);
     float ci = dot(wil, wh);
@@ -219,19 +275,36 @@ YG_INLINE RGB bsdf_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
 }
 YG_INLINE float bsdf_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
     if (m.type == 1) return dot(sc.n, wi) / kPi;
+    if (m.type == 5) return plastic_pdf(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
     V3 wh = normalized(wol + wil);
     return vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
 }
-YG_INLINE RGB bsdf_sample(const DevMaterial& m, const Shading& sc, float u0, float u1, V3 wo, V3& wi, float& pdf) {
+YG_INLINE V3 sample_cosine(float u0, float u1) {                                    // sampling.cpp:43
+    float z = sqrtf(1.f - u0);
+    float r = sqrtf(u0);
+    float phi = 2.f * kPi * u1;
+    return v3(r * cosf(phi), r * sinf(phi), z);
+}
+
+// u_scat selects the lobe of layered BSDFs (plastic: diffuse below 0.5, specular above).
+YG_INLINE RGB bsdf_sample(const DevMaterial& m, const Shading& sc, float u0, float u1, float u_scat, V3 wo, V3& wi,
+                          float& pdf) {
     if (m.type == 1) {
-        float z = sqrtf(1.f - u0);                                                     // sampling.cpp:43
-        float r = sqrtf(u0);
-        float phi = 2.f * kPi * u1;
-        V3 l = v3(r * cosf(phi), r * sinf(phi), z);
-        wi = to_world(sc, l);
+        wi = to_world(sc, sample_cosine(u0, u1));
         pdf = bsdf_pdf(m, sc, wo, wi);
         return bsdf_eval(m, sc, wo, wi);
+    }
+    if (m.type == 5) {                                                                 // bsdf.cpp:235-251
+        if (u_scat < 0.5f) {
+            wi = to_world(sc, sample_cosine(u0, u1));
+        } else {
+            V3 wh = to_world(sc, vndf_sample(u0, u1, to_local(sc, wo), m.alpha_x, m.alpha_x));   // bsdf.cpp:41-48
+            wi = reflect(wo, wh);
+        }
+        if (dot(sc.n, wi) <= 0.f) return rgb(0.f, 0.f, 0.f);
+        pdf = plastic_pdf(m, sc, wo, wi);
+        return plastic_eval(m, sc, wo, wi);
     }
     V3 wol = to_local(sc, wo);
     V3 wh = vndf_sample(u0, u1, wol, m.alpha_x, m.alpha_y);

@@ -11,6 +11,12 @@
 // (orderable position | start flag | local index), so an unstable sort reproduces the stable
 // order, scratch buffers are reused per depth level instead of reallocated, and independent
 // meshes are built on worker threads.
+//
+// A binned fast path (not in the reference) replaces the sorted edge sweep of nodes with more than
+// kBinnedMin primitives by the same SAH cost evaluated at kBins uniform planes per axis, with the
+// same partition and clipping rules below the chosen plane; smaller nodes use the exact sweep.  Its
+// trees differ from the reference's but satisfy the same invariants (every triangle overlapping a
+// leaf's box is listed in the leaf), so traversal returns the same closest hits.
 #include "kdtree_builder.hpp"
 
 #include <memory>
@@ -33,7 +39,7 @@ class KdBuilder {
 public:
     struct Prim { uint32_t id; Box3 b; };
 
-    KdBuilder(const float* verts, const int32_t* idx) : verts_(verts), idx_(idx) {}
+    KdBuilder(const float* verts, const int32_t* idx, bool binned = false) : verts_(verts), idx_(idx), binned_(binned) {}
 
     KdBuildResult build(std::vector<Prim>& prims) {
         KdBuildResult r;
@@ -58,8 +64,12 @@ private:
         std::vector<Prim> below, above;
     };
 
+    static constexpr int kBins = 32;
+    static constexpr uint32_t kBinnedMin = 1024;
+
     const float* verts_;
     const int32_t* idx_;
+    bool binned_ = false;
     std::vector<uint32_t> words_;
     std::vector<Level> levels_;
     int deepest_ = 0;
@@ -159,10 +169,72 @@ private:
         std::sort(ids, ids + n);
     }
 
+    // SAH at kBins - 1 uniform planes per axis (the cost formula of split_cost): primitives reaching
+    // below a plane t are those with lo < t, above it those with hi > t, counted from per-bin
+    // histograms of lo and hi.  Returns the best (axis, plane) whose cost beats the leaf cost n.
+    bool binned_split(const Box3& nb, const std::vector<Prim>& prims, int* out_axis, float* out_split) const {
+        static const int other[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+        const uint32_t n = (uint32_t)prims.size();
+        F3 diag = nb.hi - nb.lo;
+        const float inv_total_area = 1.0f / (2.0f * (diag.x * diag.y + diag.x * diag.z + diag.y * diag.z));
+        float best_cost = (float)n;
+        bool found = false;
+        for (int axis = 0; axis < 3; axis++) {
+            const float lo = nb.lo[axis], hi = nb.hi[axis];
+            if (!(hi > lo)) continue;
+            const int o0 = other[axis][0], o1 = other[axis][1];
+            const float s0 = 2.0f * (diag[o0] * diag[o1]);
+            const float d0 = 2.0f * (diag[o0] + diag[o1]);
+            const float scale = (float)kBins / (hi - lo);
+            uint32_t starts[kBins] = {}, ends[kBins] = {};
+            for (const Prim& p : prims) {
+                const int b0 = std::min(kBins - 1, std::max(0, (int)((p.b.lo[axis] - lo) * scale)));
+                const int b1 = std::min(kBins - 1, std::max(0, (int)((p.b.hi[axis] - lo) * scale)));
+                starts[b0]++;
+                ends[b1]++;
+            }
+            uint32_t nbelow = 0, nabove = n;
+            for (int j = 1; j < kBins; j++) {
+                nbelow += starts[j - 1];
+                nabove -= ends[j - 1];
+                const float t = lo + (hi - lo) * ((float)j / (float)kBins);
+                if (!(t > lo && t < hi)) continue;
+                const float pb = (s0 + d0 * (t - lo)) * inv_total_area;
+                const float pa = (s0 + d0 * (hi - t)) * inv_total_area;
+                const float bonus = (nbelow == 0 || nabove == 0) ? 0.3f : 0.0f;
+                const float cost = (1.f - bonus) * (pb * nbelow + pa * nabove);
+                if (cost < best_cost) { best_cost = cost; *out_axis = axis; *out_split = t; found = true; }
+            }
+        }
+        return found;
+    }
+
     void build_node(const Box3& nb, const std::vector<Prim>& prims, int depth, int level) {
         const uint32_t n = (uint32_t)prims.size();
         if (n <= 2u || depth == 0) { make_leaf(prims, level); return; }
         Level& L = levels_[level];
+        if (binned_ && n > kBinnedMin) {
+            int axis = -1;
+            float split = 0.f;
+            if (!binned_split(nb, prims, &axis, &split)) { make_leaf(prims, level); return; }
+            L.below.clear();
+            L.above.clear();
+            for (const Prim& src : prims) {
+                const bool planar_on = src.b.lo[axis] == src.b.hi[axis] && src.b.hi[axis] == split;
+                if (src.b.lo[axis] < split || planar_on) {
+                    Prim p = src;
+                    if (p.b.hi[axis] > split && verts_) clip(p.id, split, axis, true, p.b);
+                    L.below.push_back(p);
+                }
+                if (src.b.hi[axis] > split) {
+                    Prim p = src;
+                    if (p.b.lo[axis] < split && verts_) clip(p.id, split, axis, false, p.b);
+                    L.above.push_back(p);
+                }
+            }
+            emit_interior(nb, axis, split, depth, level);
+            return;
+        }
         float best_cost = kInf;
         int best_axis = -1;
         uint32_t best_edge = 0;
@@ -202,6 +274,12 @@ private:
                 L.above.push_back(p);
             }
         }
+        emit_interior(nb, axis, split, depth, level);
+    }
+
+    // Interior node over this level's child lists, then both subtrees (below first).
+    void emit_interior(const Box3& nb, int axis, float split, int depth, int level) {
+        Level& L = levels_[level];
         // The recursion below reuses deeper levels only, so this level's child lists stay valid;
         // the above list is moved out because the below subtree does not touch it.
         std::vector<Prim> above;
@@ -237,9 +315,9 @@ static std::vector<KdBuilder::Prim> triangle_prims(const float* v, const int32_t
     return prims;
 }
 
-KdBuildResult build_mesh_kdtree(const float* v, const int32_t* idx, int32_t ntri) {
+KdBuildResult build_mesh_kdtree(const float* v, const int32_t* idx, int32_t ntri, bool binned) {
     std::vector<KdBuilder::Prim> prims = triangle_prims(v, idx, ntri);
-    KdBuilder b(v, idx);
+    KdBuilder b(v, idx, binned);
     return b.build(prims);
 }
 
@@ -255,7 +333,7 @@ KdBuildResult build_box_kdtree(const float* boxes, int32_t n) {
 }
 
 // Builds one tree per mesh on `threads` workers (load_geometry_kdtrees, scene_context.cpp:33-61).
-std::vector<KdBuildResult> build_mesh_kdtrees(const yar_scene_desc* d, int threads) {
+std::vector<KdBuildResult> build_mesh_kdtrees(const yar_scene_desc* d, int threads, bool binned) {
     std::vector<KdBuildResult> out(d->mesh_count);
     std::atomic<int> next{0};
     // Largest meshes first so one big mesh does not finish last on a single worker.
@@ -268,7 +346,7 @@ std::vector<KdBuildResult> build_mesh_kdtrees(const yar_scene_desc* d, int threa
         for (int j = next++; j < d->mesh_count; j = next++) {
             int i = order[j];
             const yar_mesh_desc& m = d->meshes[i];
-            out[i] = build_mesh_kdtree(m.vertices, m.indices, m.triangle_count);
+            out[i] = build_mesh_kdtree(m.vertices, m.indices, m.triangle_count, binned);
         }
     };
     int nth = std::max(1, std::min(threads, d->mesh_count));
@@ -287,7 +365,8 @@ struct yar_kdtree {
 
 extern "C" {
 
-int yar_kdtree_build_mesh(const float* vertices, const int32_t* indices, int32_t triangle_count, yar_kdtree** out_tree) {
+static int build_mesh_entry(const char* name, const float* vertices, const int32_t* indices, int32_t triangle_count,
+                            yar_kdtree** out_tree, bool binned) {
     if (!vertices || !indices || triangle_count <= 0 || !out_tree) {
         yar::set_last_error("yar_kdtree_build_mesh: invalid argument");
         return YAR_ERR_INVALID_ARGUMENT;
@@ -296,11 +375,20 @@ int yar_kdtree_build_mesh(const float* vertices, const int32_t* indices, int32_t
         yar::set_last_error("yar_kdtree_build_mesh: more than 2^30-1 primitives");
         return YAR_ERR_CAPACITY;
     }
-    return yar::abi_guard("yar_kdtree_build_mesh", [&] {
+    return yar::abi_guard(name, [&] {
         std::unique_ptr<yar_kdtree> t(new yar_kdtree);
-        t->r = yar::build_mesh_kdtree(vertices, indices, triangle_count);
+        t->r = yar::build_mesh_kdtree(vertices, indices, triangle_count, binned);
         *out_tree = t.release();
     });
+}
+
+int yar_kdtree_build_mesh(const float* vertices, const int32_t* indices, int32_t triangle_count, yar_kdtree** out_tree) {
+    return build_mesh_entry("yar_kdtree_build_mesh", vertices, indices, triangle_count, out_tree, false);
+}
+
+int yar_kdtree_build_mesh_binned(const float* vertices, const int32_t* indices, int32_t triangle_count,
+                                 yar_kdtree** out_tree) {
+    return build_mesh_entry("yar_kdtree_build_mesh_binned", vertices, indices, triangle_count, out_tree, true);
 }
 
 int yar_kdtree_build_boxes(const float* boxes, int32_t count, yar_kdtree** out_tree) {

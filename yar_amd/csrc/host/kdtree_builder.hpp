@@ -15,8 +15,8 @@ struct KdBuildResult {
     int depth = 0;                 // level of the deepest leaf, root = 0
 };
 
-KdBuildResult build_mesh_kdtree(const float* v, const int32_t* idx, int32_t ntri);
+KdBuildResult build_mesh_kdtree(const float* v, const int32_t* idx, int32_t ntri, bool binned = false);
 KdBuildResult build_box_kdtree(const float* boxes, int32_t n);
-std::vector<KdBuildResult> build_mesh_kdtrees(const yar_scene_desc* d, int threads);
+std::vector<KdBuildResult> build_mesh_kdtrees(const yar_scene_desc* d, int threads, bool binned = false);
 
 }  // namespace yar

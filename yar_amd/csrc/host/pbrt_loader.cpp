@@ -303,6 +303,24 @@ struct Loader {
             m.v_roughness = get_f(ms->params, "vroughness", 0.f);
             m.roughness_is_alpha = get_b(ms->params, "remaproughness", true) ? 0 : 1;
             if (m.u_roughness > 0.f) throw ParseError{"rough glass (Rough_Glass_BSDF) is not supported"};
+        } else if (ms->type == "mirror") {                                 // pbrt_scene.cpp:443-447
+            m.type = YAR_MATERIAL_MIRROR;
+            m.reflectance[0] = m.reflectance[1] = m.reflectance[2] = 0.9f;    // pbrt-parser MirrorMaterial kr
+            if (find(ms->params, "Kr") && find(ms->params, "Kr")->type == "texture")
+                throw ParseError{"textured mirror Kr is not supported"};
+            get_rgb(ms->params, "Kr", m.reflectance);
+        } else if (ms->type == "plastic") {                                // pbrt_scene.cpp:503-525
+            m.type = YAR_MATERIAL_PLASTIC;
+            float kd[3] = {0.25f, 0.25f, 0.25f}, ks[3] = {0.25f, 0.25f, 0.25f};   // pbrt-parser PlasticMaterial
+            for (const char* name : {"Kd", "Ks", "roughness"})
+                if (find(ms->params, name) && find(ms->params, name)->type == "texture")
+                    throw ParseError{std::string("textured plastic ") + name + " is not supported"};
+            get_rgb(ms->params, "Kd", kd);
+            get_rgb(ms->params, "Ks", ks);
+            for (int i = 0; i < 3; i++) m.reflectance[i] = kd[i];
+            m.specular_r0 = 0.2126729f * ks[0] + 0.7151522f * ks[1] + 0.0721750f * ks[2];   // Y of sRGB_to_XYZ(Ks)
+            m.u_roughness = m.v_roughness = get_f(ms->params, "roughness", 0.1f);
+            m.roughness_is_alpha = get_b(ms->params, "remaproughness", true) ? 0 : 1;
         } else {
             throw ParseError{"unsupported material type '" + ms->type + "'"};
         }
