@@ -213,6 +213,8 @@ def main():
     ap.add_argument("--no-counters", action="store_true", help="skip the counted render (roofline bytes)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
+    ap.add_argument("--opt", action="append", default=[], metavar="FIELD=INT",
+                    help="set a yar_gpu_options field for the config-2 render (e.g. lane_refill=1)")
     ap.add_argument("--no-extra", action="store_true", help="skip the config-1, batch and guided legs")
     ap.add_argument("--leg", choices=["config1", "batch", "guided_c3", "guided_c4"], help="run only this leg (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -252,6 +254,9 @@ def main():
         opts["block_size"] = args.block
     if args.stack:
         opts["traversal_stack_depth"] = args.stack
+    for kv in args.opt:
+        k, v = kv.split("=", 1)
+        opts[k] = int(v)
     ctx = yar_amd.Context(scene, **opts)
     info = ctx.info()
     view = ctx.view(0)
@@ -284,7 +289,7 @@ def main():
     ext_launches = stats[0].extend_launches
     roofline = None
     if not args.no_counters:
-        cctx = yar_amd.Context(scene, enable_counters=1)
+        cctx = yar_amd.Context(scene, **dict(opts, enable_timing=0, enable_counters=1))
         _, _, _, cs = cctx.render(0, want_film=False)
         cctx.close()
         ext_bytes = (BYTES_NODE * cs.extend_node_visits + BYTES_LEAF_REF * cs.extend_leaf_refs +

@@ -181,7 +181,7 @@ typedef struct yar_gpu_options {
     int32_t stack_mode;                 /* traversal stack storage: 0 = default, 1 = LDS, 2 = private */
     int32_t lane_refill;                /* 1 = traversal lanes take new rays individually, 0/2 = batches of 64 */
     int32_t disable_mis_cull;           /* 1 = trace every MIS BSDF ray (no analytic light-rectangle test) */
-    int32_t bin_rays;                   /* 1 = group each bounce's rays by direction octant before traversal */
+    int32_t bin_rays;                   /* 1, 2, 3 = sort each bounce's rays by direction cell (2x2, 4x4, 8x8 per cube face) within windows of 4096 before traversal */
     int32_t kdtree_builder;             /* 0 = exact (the reference's trees), 1 = binned SAH fast path */
     int32_t reserved[3];
 } yar_gpu_options;

@@ -37,8 +37,9 @@ def _alone(scene, views, **opts):
 
 
 @pytest.mark.parametrize("opts", [{}, {"max_paths_in_flight": 9000}, {"use_graph": 1}, {"bin_rays": 1},
-                                  {"bin_rays": 1, "use_graph": 1, "max_paths_in_flight": 20000}],
-                         ids=["default", "small_chunks", "graph", "binned", "binned_graph"])
+                                  {"bin_rays": 3, "use_graph": 1, "max_paths_in_flight": 20000},
+                                  {"lane_refill": 1}],
+                         ids=["default", "small_chunks", "graph", "binned", "binned_graph", "refill"])
 def test_batch_equals_single_view_renders(c1_path, opts):
     scene = yar_amd.Scene(c1_path)
     ctx = yar_amd.Context(scene, **opts)

@@ -18,12 +18,14 @@ def _hits_agree(g, o):
 @pytest.mark.parametrize("which,opts", [("c1", {}), ("c2_small", {}),
                                         ("c2_small", {"traversal_stack_depth": 2}),
                                         ("c2_small", {"traversal_stack_depth": 32, "stack_mode": 2}),
-                                        ("c2_small", {"kdtree_builder": 1})],
-                         ids=["c1", "c2_small", "c2_small-restart", "c2_small-private", "c2_small-binned-trees"])
+                                        ("c2_small", {"kdtree_builder": 1}),
+                                        ("c2_small", {"traversal_stack_depth": 2, "lane_refill": 1})],
+                         ids=["c1", "c2_small", "c2_small-restart", "c2_small-private", "c2_small-binned-trees",
+                              "c2_small-restart-refill"])
 def test_trace_rays_closest_matches_oracle(which, opts, c1_path, c2_small_path, oracle):
     """Short stacks restart from the root (kd-restart) when they overflow; the hit records must not
-    depend on the stack storage or depth, nor on the tree (binned SAH fast path against the oracle's
-    reference-builder trees)."""
+    depend on the stack storage or depth, on per-lane refill, nor on
+    the tree (binned SAH fast path against the oracle's reference-builder trees)."""
     path = c1_path if which == "c1" else c2_small_path
     scene = yar_amd.Scene(path)
     rays = raygen.random_rays(scene, 1 << 18, seed=7)

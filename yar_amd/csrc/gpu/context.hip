@@ -163,7 +163,7 @@ struct yar_gpu_ctx {
     DevBuf<uint2> pk;
     DevBuf<uint32_t> state;
     DevBuf<uint64_t> rng;
-    DevBuf<uint32_t> q_a, q_b, sh_path, mi_path, dl_path, counts, q_unsorted, bins;
+    DevBuf<uint32_t> q_a, q_b, sh_path, mi_path, dl_path, counts, q_unsorted;
     DevBuf<float4> sh_o, sh_d, sh_c, mi_o, mi_d, mi_f;
     DevBuf<float> mi_ndw;
     DevBuf<unsigned long long> totals;
@@ -214,7 +214,7 @@ struct yar_gpu_ctx {
         vtx_uv.release(); materials.release(); rects.release(); points.release();
         pixels.release(); pix_first.release(); dviews.release(); chunks.release(); chunk_slot.release();
         perm.release(); jump_lo.release(); jump_hi.release(); hit_ot.release(); tr_ot.release(); state.release(); rng.release();
-        DevBuf<uint32_t>* bu[] = {&q_a, &q_b, &sh_path, &mi_path, &dl_path, &counts, &q_unsorted, &bins};
+        DevBuf<uint32_t>* bu[] = {&q_a, &q_b, &sh_path, &mi_path, &dl_path, &counts, &q_unsorted};
         for (auto* b : bu) b->release();
         mi_ndw.release(); totals.release(); out_var.release(); spill.release();
         grng.release(); rec.release(); guide_in.release(); guide_out.release(); guide_path.release(); train.release();
@@ -734,10 +734,10 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     for (auto* b : qb) b->reserve(np);
     ctx->mi_ndw.reserve(np);
     ctx->counts.reserve(16);
-    const bool bin_rays = ctx->opt.bin_rays != 0;
+    const int bin_mode = std::min(std::max(ctx->opt.bin_rays, 0), 3);
+    const bool bin_rays = bin_mode > 0;
     if (bin_rays) {
         ctx->q_unsorted.reserve(np);
-        ctx->bins.reserve(16);
     }
     ctx->totals.reserve(24);
     ctx->out_film.reserve((size_t)std::max<int64_t>(1, total_pixels));
@@ -833,8 +833,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     q.mis_ndw = ctx->mi_ndw.p; q.dl_path = ctx->dl_path.p; q.counts = ctx->counts.p; q.totals = ctx->totals.p;
     if (bin_rays) {
         q.unsorted = ctx->q_unsorted.p;
-        q.bins = ctx->bins.p;
-        q.bin_rays = 1;
+        q.bin_rays = bin_mode;
     }
     if (guided) {
         q.guide_path = ctx->guide_path.p;

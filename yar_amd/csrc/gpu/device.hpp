@@ -202,9 +202,12 @@ YG_INLINE uint32_t leaf_id(const uint2* __restrict__ nodes, uint32_t nd, uint32_
 }
 
 // Short-stack bookkeeping of one tree level packed in one word: push pointer in bits 0-7 (used
-// modulo the stack size, a power of two dividing 256), live entries in bits 8-15, and bit 16 set
-// when entries were dropped since the last restart.
-constexpr uint32_t kSsCount = 1u << 8, kSsDropped = 1u << 16;
+// modulo the stack size, a power of two dividing 256), live entries in bits 8-15, bit 16 set
+// when entries were dropped since the last restart, bit 17 set while a descent is suspended.
+constexpr uint32_t kSsCount = 1u << 8, kSsDropped = 1u << 16, kSsResume = 1u << 17;
+
+// Interior nodes a lane descends per traversal step (a constant, so the descent is unrolled).
+constexpr int kDescentBudget = 6;
 YG_INLINE uint32_t ss_count(uint32_t ss) { return (ss >> 8) & 0xffu; }
 YG_INLINE uint32_t ss_push(uint32_t ss) { return (ss & ~0xffu) | ((ss + 1u) & 0xffu); }
 YG_INLINE uint32_t ss_pop(uint32_t ss) { return ((ss & ~0xffu) | ((ss - 1u) & 0xffu)) - kSsCount; }
@@ -214,29 +217,30 @@ YG_INLINE uint32_t ss_pop(uint32_t ss) { return ((ss & ~0xffu) | ((ss - 1u) & 0x
 template <class Stack>
 YG_INLINE void kd_interior(uint2 w, uint32_t& node, V3 o, V3 d, V3 inv, float& tmin, float& tmax, Stack& st,
                            int mask, int depth, uint32_t& ss) {
-    const int axis = (int)(w.x & 3u);
-    const float oa = axis == 0 ? o.x : (axis == 1 ? o.y : o.z);
-    const float da = axis == 0 ? d.x : (axis == 1 ? d.y : d.z);
-    const float ia = axis == 0 ? inv.x : (axis == 1 ? inv.y : inv.z);
+    // Written as selects so a wave runs one straight-line sequence whatever mix of cases its lanes
+    // are in; only the stack write is a branch.
+    const uint32_t axis = w.x & 3u;
+    const float oa = axis == 0u ? o.x : (axis == 1u ? o.y : o.z);
+    const float da = axis == 0u ? d.x : (axis == 1u ? d.y : d.z);
+    const float ia = axis == 0u ? inv.x : (axis == 1u ? inv.y : inv.z);
     const float dist = bitsf(w.y) - oa;
     const uint32_t below = node + 1, above = w.x >> 2;
-    if (dist != 0.f) {
-        const uint32_t first = dist > 0.f ? below : above;
-        const uint32_t second = dist > 0.f ? above : below;
-        const float ts = dist * ia;
-        if (ts > tmax || ts < 0.f) {
-            node = first;
-        } else if (ts < tmin) {
-            node = second;
-        } else {
-            st.put((int)(ss & (uint32_t)mask), second, tmax);
-            ss = ss_count(ss) == (uint32_t)depth ? (ss_push(ss) | kSsDropped) : (ss_push(ss) + kSsCount);
-            node = first;
-            tmax = ts;
-        }
-    } else {
-        node = (da > 0.f) ? above : below;
-    }
+    const bool pos = dist > 0.f;
+    const uint32_t first = pos ? below : above;
+    const uint32_t second = pos ? above : below;
+    const float ts = dist * ia;
+    const bool zero = dist == 0.f;
+    const bool only_first = ts > tmax || ts < 0.f;
+    const bool only_second = !only_first && ts < tmin;
+    const bool push = !zero && !only_first && !only_second;
+    const uint32_t next = only_second ? second : first;
+    const uint32_t on_plane = da > 0.f ? above : below;
+    if (push) st.put((int)(ss & (uint32_t)mask), second, tmax);
+    const uint32_t pushed = ss_push(ss);
+    const uint32_t ss_new = ss_count(ss) == (uint32_t)depth ? (pushed | kSsDropped) : (pushed + kSsCount);
+    ss = push ? ss_new : ss;
+    tmax = push ? ts : tmax;
+    node = zero ? on_plane : next;
 }
 
 // After a leaf: pop the next subtree, or restart behind the last leaf when the short stack dropped
@@ -314,7 +318,9 @@ struct Tracer {
     // closest hit: hit.t < tmax_in; any hit: found_any
     YG_INLINE bool result() const { return ANY ? found_any : hit.t < tmax_in; }
 
-    // One traversal step; returns true when the query is complete.
+    // One traversal step; returns true when the query is complete.  A descent stops after
+    // kDescentBudget interior nodes and resumes from the same node in the next step, so a lane far
+    // from its next leaf does not hold the wave while the other lanes wait at theirs.
     template <class SStack, class MStack>
     YG_INLINE bool step(const DevScene& s, SStack& sst, int sdepth, MStack& mst, int mdepth, TravCounters& cnt) {
         // The iteration cap only guarantees that every wave terminates; real trees need far fewer.
@@ -324,9 +330,14 @@ struct Tracer {
             bool done = !(loop_t > mtmin);
             if (!done) {
                 // descend to the next leaf: neither the loop bound nor tmin changes on the way down
-                if (COUNT) cnt.nodes++;
+                if (COUNT && !(mss & kSsResume)) cnt.nodes++;
+                mss &= ~kSsResume;
                 uint2 w = s.mesh_nodes[mnode];
-                while ((w.x & 3u) != 3u) {
+                for (int b = kDescentBudget; (w.x & 3u) != 3u;) {
+                    if (b-- == 0) {
+                        mss |= kSsResume;
+                        return false;
+                    }
                     kd_interior(w, mnode, lo, ld, linv, mtmin, mtmax, mst, mdepth - 1, mdepth, mss);
                     if (COUNT) cnt.nodes++;
                     w = s.mesh_nodes[mnode];
@@ -392,9 +403,14 @@ struct Tracer {
             return !kd_next(snode, 0u, stmin, stmax, st1, sst, sdepth - 1, sss, srestart);
         }
         if (!(loop_t > stmin)) return true;
-        if (COUNT) cnt.nodes++;
+        if (COUNT && !(sss & kSsResume)) cnt.nodes++;
+        sss &= ~kSsResume;
         uint2 w = s.scene_nodes[snode];
-        while ((w.x & 3u) != 3u) {
+        for (int b = kDescentBudget; (w.x & 3u) != 3u;) {
+            if (b-- == 0) {
+                sss |= kSsResume;
+                return false;
+            }
             kd_interior(w, snode, o, d, inv, stmin, stmax, sst, sdepth - 1, sdepth, sss);
             if (COUNT) cnt.nodes++;
             w = s.scene_nodes[snode];

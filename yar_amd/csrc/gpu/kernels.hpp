@@ -74,12 +74,11 @@ struct Queues {
     // [0..15] traversal counters (4 per kernel kind), [16] extend rays, [17] shadow rays, [18] mis rays,
     // [19] extend launches
     unsigned long long* totals;
-    // Direction-octant binning of the next iteration's active list (bin_rays != 0): shade and guide
-    // sample append to `unsorted`; k_bin_count counts the new rays per octant in bins[0..7] and
-    // k_bin_scatter writes the active list grouped by octant (bins[8..15] are its cursors), so a
-    // traversal wave takes rays of one octant from neighbouring pixels.
+    // Direction binning of the next iteration's active list (bin_rays = 1, 2, 3: cube-map cells of
+    // 2x2, 4x4, 8x8 per face): shade and guide sample append to `unsorted`; k_bin_local sorts each
+    // window of 4,096 consecutive entries by direction cell in LDS and writes it to the active list
+    // in place, so a traversal wave takes rays of similar direction from nearby samples.
     uint32_t* unsorted;
-    uint32_t* bins;
     int bin_rays;
 };
 

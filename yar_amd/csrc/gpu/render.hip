@@ -955,58 +955,84 @@ __global__ void k_resolve(DevScene s, const Chunk* __restrict__ cp, Paths P, Que
 }
 
 // Resets the output counters of an iteration (a kernel, not a memset, so it can sit in a graph).
-__global__ void k_iter_reset(uint32_t* counts, uint32_t* bins, int parity) {
+__global__ void k_iter_reset(uint32_t* counts, int parity) {
     const int t = threadIdx.x;
     if (t == 0) counts[parity ^ 1] = 0u;
     else if (t >= 2 && t <= 4) counts[t] = 0u;
     else if (t >= 6 && t <= 9) counts[t] = 0u;
-    if (bins && t < 16) bins[t] = 0u;
 }
 
-YG_INLINE uint32_t octant(float4 d) { return (d.x < 0.f ? 1u : 0u) | (d.y < 0.f ? 2u : 0u) | (d.z < 0.f ? 4u : 0u); }
-
-// Octant histogram of the unsorted next active list (one atomic per wave and octant present).
-__global__ void k_bin_count(Paths P, Queues q, int parity) {
-    const uint32_t n = q.counts[parity ^ 1];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t blk = blockIdx.x * blockDim.x; blk < n; blk += stride) {
-        const uint32_t i = blk + threadIdx.x;
-        const bool valid = i < n;
-        const uint32_t o = valid ? octant(P.ray_d[q.unsorted[i]]) : 8u;
-        for (uint32_t b = 0; b < 8; b++) {
-            const uint64_t m = __ballot(o == b);
-            if (m && __lane_id() == (uint32_t)(__ffsll((unsigned long long)m) - 1))
-                atomicAdd(&q.bins[b], (uint32_t)__popcll(m));
-        }
-    }
+// Cube-map cell of a direction: dominant axis and sign pick one of 6 faces, the other two
+// components over the dominant one pick one of C x C cells.
+template <int C>
+YG_INLINE uint32_t dir_cell(float4 d) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    uint32_t f;
+    float u, v, m;
+    if (ax >= ay && ax >= az) { f = d.x < 0.f ? 1u : 0u; u = d.y; v = d.z; m = ax; }
+    else if (ay >= az) { f = d.y < 0.f ? 3u : 2u; u = d.x; v = d.z; m = ay; }
+    else { f = d.z < 0.f ? 5u : 4u; u = d.x; v = d.y; m = az; }
+    const float sc = 0.5f * (float)C / m;
+    const uint32_t iu = (uint32_t)fminf(fmaxf((u + m) * sc, 0.f), (float)(C - 1));
+    const uint32_t iv = (uint32_t)fminf(fmaxf((v + m) * sc, 0.f), (float)(C - 1));
+    return (f * C + iu) * C + iv;
 }
 
-// Writes the next active list grouped by octant: entry i goes to its octant's range (prefix of the
-// histogram) at a slot reserved with one atomic per wave and octant.
-__global__ void k_bin_scatter(Paths P, Queues q, int parity) {
+// Sorts the unsorted next active list window by window (4,096 entries per window) by direction
+// cell: a counting sort in LDS, written back to the same window of the active list.  The order
+// inside a cell is arbitrary; every path's result depends only on its own slot, so the images do
+// not depend on it.
+constexpr int kBinWindow = 4096;
+template <int C>
+__global__ __launch_bounds__(256) void k_bin_local(Paths P, Queues q, int parity) {
+    constexpr int NB = 6 * C * C;
+    __shared__ uint32_t s_slot[kBinWindow];
+    __shared__ uint16_t s_key[kBinWindow];
+    __shared__ uint32_t s_cur[NB];
     uint32_t* out = parity ? q.active0 : q.active1;
     const uint32_t n = q.counts[parity ^ 1];
-    uint32_t prefix[8];
-    uint32_t acc = 0;
-    for (int b = 0; b < 8; b++) { prefix[b] = acc; acc += q.bins[b]; }
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t lane = __lane_id();
-    for (uint32_t blk = blockIdx.x * blockDim.x; blk < n; blk += stride) {
-        const uint32_t i = blk + threadIdx.x;
-        const bool valid = i < n;
-        const uint32_t slot = valid ? q.unsorted[i] : 0u;
-        const uint32_t o = valid ? octant(P.ray_d[slot]) : 8u;
-        uint32_t pos = 0;
-        for (uint32_t b = 0; b < 8; b++) {
-            const uint64_t m = __ballot(o == b);
-            if (!m) continue;
-            const int leader = __ffsll((unsigned long long)m) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(&q.bins[8 + b], (uint32_t)__popcll(m));
-            base = __shfl(base, leader);
-            if (o == b) pos = prefix[b] + base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    const int t = threadIdx.x;
+    for (uint32_t base = blockIdx.x * (uint32_t)kBinWindow; base < n; base += gridDim.x * (uint32_t)kBinWindow) {
+        const int m = (int)min((uint32_t)kBinWindow, n - base);
+        for (int j = t; j < NB; j += 256) s_cur[j] = 0u;
+        __syncthreads();
+        for (int j = t; j < m; j += 256) {
+            const uint32_t slot = q.unsorted[base + j];
+            const uint32_t key = dir_cell<C>(P.ray_d[slot]);
+            s_slot[j] = slot;
+            s_key[j] = (uint16_t)key;
+            atomicAdd(&s_cur[key], 1u);
         }
-        if (valid) out[pos] = slot;
+        __syncthreads();
+        if (t < 64) {
+            // exclusive prefix over the NB cells: each lane sums a run of cells, then a wave scan
+            constexpr int RUN = (NB + 63) / 64;
+            uint32_t v[RUN], sum = 0;
+#pragma unroll
+            for (int r = 0; r < RUN; r++) {
+                const int c = t * RUN + r;
+                v[r] = c < NB ? s_cur[c] : 0u;
+                sum += v[r];
+            }
+            uint32_t incl = sum;
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t y = __shfl_up(incl, off);
+                if (t >= off) incl += y;
+            }
+            uint32_t acc = incl - sum;
+#pragma unroll
+            for (int r = 0; r < RUN; r++) {
+                const int c = t * RUN + r;
+                if (c < NB) s_cur[c] = acc;
+                acc += v[r];
+            }
+        }
+        __syncthreads();
+        for (int j = t; j < m; j += 256) {
+            const uint32_t pos = atomicAdd(&s_cur[s_key[j]], 1u);
+            out[base + pos] = s_slot[j];
+        }
+        __syncthreads();
     }
 }
 
@@ -1042,13 +1068,12 @@ int traversal_blocks_per_cu(const LaunchCfg& cfg) {
 void launch_stage(int stage, const DevScene& sc, const Chunk* c, const Paths& p, Queues q, int parity, const LaunchCfg& cfg) {
     switch (stage) {
     case kStageReset:
-        hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(64), 0, cfg.stream, q.counts, q.bin_rays ? q.bins : nullptr, parity);
+        hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(64), 0, cfg.stream, q.counts, parity);
         break;
     case kStageBin:
-        if (q.bin_rays) {
-            hipLaunchKernelGGL(k_bin_count, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, p, q, parity);
-            hipLaunchKernelGGL(k_bin_scatter, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, p, q, parity);
-        }
+        if (q.bin_rays == 1) hipLaunchKernelGGL(k_bin_local<2>, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, p, q, parity);
+        else if (q.bin_rays == 2) hipLaunchKernelGGL(k_bin_local<4>, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, p, q, parity);
+        else if (q.bin_rays >= 3) hipLaunchKernelGGL(k_bin_local<8>, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, p, q, parity);
         break;
     case kStageExtend: YG_LAUNCH_TRAV(k_extend, cfg.trav_grid, sc, p, q, parity); break;
     case kStageShade:
