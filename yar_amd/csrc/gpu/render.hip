@@ -18,24 +18,48 @@ namespace yg {
 // ---------------------------------------------------------------------------------------------
 // wave-level helpers
 // ---------------------------------------------------------------------------------------------
-// Queue append with one atomic per wave; lane order is preserved inside the wave.  Every lane of
-// the wave must call it (pred may be false).
-YG_INLINE uint32_t wave_append(uint32_t* counter, bool pred) {
-    uint64_t mask = __ballot(pred);
-    if (mask == 0) return 0;
-    uint32_t lane = __lane_id();
-    uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    return base + prefix;
+// Queue appends aggregated over the block: one atomic per block and queue.  Atomics on one address
+// serialize in L2 (about 10 ns each), so with one atomic per wave the appends of a 4M-entry launch
+// alone would take most of a millisecond.  Every thread of the block must make the call, the same
+// number of times (pred may be false).  Entries keep wave order inside the block and lane order
+// inside a wave.  Blocks of at most 1024 threads.
+template <int NQ>
+YG_INLINE void block_append(uint32_t* const (&ctr)[NQ], const bool (&pred)[NQ], uint32_t (&idx)[NQ]) {
+    __shared__ uint32_t s_cnt[NQ][16];
+    __shared__ uint32_t s_base[NQ];
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6, nw = (blockDim.x + 63u) >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t pre[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+        const uint64_t m = __ballot(pred[k]);
+        pre[k] = (uint32_t)__popcll(m & below);
+        if (lane == 0) s_cnt[k][wave] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+        if (threadIdx.x == (uint32_t)k) {
+            uint32_t tot = 0;
+            for (uint32_t w = 0; w < nw; w++) {
+                const uint32_t c = s_cnt[k][w];
+                s_cnt[k][w] = tot;
+                tot += c;
+            }
+            s_base[k] = tot ? atomicAdd(ctr[k], tot) : 0u;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NQ; k++) idx[k] = s_base[k] + s_cnt[k][wave] + pre[k];
+    __syncthreads();
 }
 
-// Next batch of 64 work items for this wave from a device cursor (persistent traversal kernels).
-YG_INLINE uint32_t wave_fetch(uint32_t* cursor) {
+// Next `size` work items (a multiple of 64) for this wave from a device cursor (persistent
+// traversal kernels).
+YG_INLINE uint32_t wave_fetch(uint32_t* cursor, uint32_t size) {
     uint32_t base = 0;
-    if (__lane_id() == 0) base = atomicAdd(cursor, 64u);
+    if (__lane_id() == 0) base = atomicAdd(cursor, size);
     return __shfl(base, 0);
 }
 
@@ -314,8 +338,17 @@ __global__ void k_raygen(const Chunk* __restrict__ cp, Paths P, Queues q) {
             P.rng[s] = pcg_jump(c.jt, ps.s0, ps.inc, sampler_draws((uint32_t)v.spp, (uint32_t)c.d1, (uint32_t)c.d2));
         }
     }
-    const uint32_t idx = wave_append(&q.counts[0], active);
-    if (active) q.active0[idx] = (uint32_t)s;
+    if (c.uniform_spp > 0 && !c.guided && !c.sequential) {
+        // every slot below n_paths is a path: the queue is the identity, no appends
+        if (active) q.active0[s] = (uint32_t)s;
+        if (s == 0) q.counts[0] = (uint32_t)c.n_paths;
+        return;
+    }
+    uint32_t* const ctr[1] = {&q.counts[0]};
+    const bool pred[1] = {active};
+    uint32_t idx[1];
+    block_append<1>(ctr, pred, idx);
+    if (active) q.active0[idx[0]] = (uint32_t)s;
 }
 
 // Clears the queue counters and cursors before a pass; the film side-list counter [5] and the
@@ -371,17 +404,25 @@ YG_INLINE void trace_queue(const DevScene& sc, uint32_t n, uint32_t* cursor, SSt
                            TravCounters& cnt, bool refill, Setup setup, Finish finish) {
     Tracer<ANY, COUNT> tr;
     if (!refill) {
-        // batches of 64: every lane of the wave runs its query to completion before the next fetch
-        for (uint32_t base = wave_fetch(cursor); base < n; base = wave_fetch(cursor)) {
-            const uint32_t i = base + __lane_id();
-            if (i < n) {
-                V3 o, d;
-                float tmax;
-                const uint32_t payload = setup(i, o, d, tmax);
-                if (tr.start(sc, o, d, tmax))
-                    while (!tr.step(sc, sst, sd, mst, md, cnt)) {}
-                finish(payload, tr);
+        // batches of 64: every lane of the wave runs its query to completion before the next batch.
+        // A wave takes 4 batches per cursor update while more than 8 fetches per wave remain (fewer
+        // same-address atomics), then single batches (balanced tail).
+        const uint32_t waves = gridDim.x * ((blockDim.x + 63u) >> 6);
+        uint32_t size = n > 8u * 256u * waves ? 256u : 64u;
+        for (uint32_t base = wave_fetch(cursor, size); base < n; base = wave_fetch(cursor, size)) {
+            const uint32_t end = min(base + size, n);
+            for (uint32_t b = base; b < end; b += 64u) {
+                const uint32_t i = b + __lane_id();
+                if (i < end) {
+                    V3 o, d;
+                    float tmax;
+                    const uint32_t payload = setup(i, o, d, tmax);
+                    if (tr.start(sc, o, d, tmax))
+                        while (!tr.step(sc, sst, sd, mst, md, cnt)) {}
+                    finish(payload, tr);
+                }
             }
+            if (n - end < 8u * 256u * waves) size = 64u;
         }
         return;
     }
@@ -817,10 +858,11 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
             if (GUIDED && c.guided) P.grng[slot] = rs;
             else if (seq) P.rng[slot] = rs;
         }
-        const uint32_t ia = wave_append(n_out, want_active);
-        const uint32_t is = wave_append(&q.counts[2], want_shadow);
-        const uint32_t im = wave_append(&q.counts[3], want_mis);
-        const uint32_t id = wave_append(&q.counts[4], want_dl);
+        uint32_t* const ctr[4] = {n_out, &q.counts[2], &q.counts[3], &q.counts[4]};
+        const bool pred[4] = {want_active, want_shadow, want_mis, want_dl};
+        uint32_t qi[4];
+        block_append<4>(ctr, pred, qi);
+        const uint32_t ia = qi[0], is = qi[1], im = qi[2], id = qi[3];
         if (want_active) out[ia] = slot;
         if (want_shadow) {
             q.shadow_path[is] = slot;
@@ -837,7 +879,11 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
         }
         if (want_dl) q.dl_path[id] = slot;
         if (GUIDED && c.guided) {
-            const uint32_t ig = wave_append(&q.counts[9], want_guide);
+            uint32_t* const gctr[1] = {&q.counts[9]};
+            const bool gpred[1] = {want_guide};
+            uint32_t gidx[1];
+            block_append<1>(gctr, gpred, gidx);
+            const uint32_t ig = gidx[0];
             if (want_guide) {
                 q.guide_path[ig] = slot;
                 float4* gi = reinterpret_cast<float4*>(q.guide_in + (size_t)ig * 16);
@@ -920,8 +966,11 @@ __global__ void k_guide_sample(DevScene s, const Chunk* __restrict__ cp, Paths P
             P.coeff[slot] = f4(coeff, 0.f);
             P.grng[slot] = rs;
         }
-        const uint32_t ia = wave_append(n_out, alive);
-        if (alive) out[ia] = slot;
+        uint32_t* const ctr[1] = {n_out};
+        const bool pred[1] = {alive};
+        uint32_t idx[1];
+        block_append<1>(ctr, pred, idx);
+        if (alive) out[idx[0]] = slot;
     }
 }
 
