@@ -318,59 +318,41 @@ struct Tracer {
     // closest hit: hit.t < tmax_in; any hit: found_any
     YG_INLINE bool result() const { return ANY ? found_any : hit.t < tmax_in; }
 
-    // One traversal step; returns true when the query is complete.  A descent stops after
+    // One traversal step; returns true when the query is complete.  A lane outside a mesh tree
+    // takes the scene-level actions in traversal order, each at most once per step: finish the
+    // current scene leaf, descend to the next one, set up its next object; a lane inside a mesh
+    // tree (possibly just entered) then processes one mesh leaf.  A descent stops after
     // kDescentBudget interior nodes and resumes from the same node in the next step, so a lane far
     // from its next leaf does not hold the wave while the other lanes wait at theirs.
     template <class SStack, class MStack>
     YG_INLINE bool step(const DevScene& s, SStack& sst, int sdepth, MStack& mst, int mdepth, TravCounters& cnt) {
         // The iteration cap only guarantees that every wave terminates; real trees need far fewer.
         if (++iter > (1 << 24)) return true;
-        const float loop_t = ANY ? any_t : hit.t;
-        if (in_mesh) {
-            bool done = !(loop_t > mtmin);
-            if (!done) {
-                // descend to the next leaf: neither the loop bound nor tmin changes on the way down
-                if (COUNT && !(mss & kSsResume)) cnt.nodes++;
-                mss &= ~kSsResume;
-                uint2 w = s.mesh_nodes[mnode];
+        if (!in_mesh) {
+            if (ln > 0 && lj >= ln) {
+                // the leaf's objects are done: continue the scene traversal behind it
+                ln = 0;
+                if (!kd_next(snode, 0u, stmin, stmax, st1, sst, sdepth - 1, sss, srestart)) return true;
+            }
+            if (ln == 0) {
+                if (!((ANY ? any_t : hit.t) > stmin)) return true;
+                if (COUNT && !(sss & kSsResume)) cnt.nodes++;
+                sss &= ~kSsResume;
+                uint2 w = s.scene_nodes[snode];
                 for (int b = kDescentBudget; (w.x & 3u) != 3u;) {
                     if (b-- == 0) {
-                        mss |= kSsResume;
+                        sss |= kSsResume;
                         return false;
                     }
-                    kd_interior(w, mnode, lo, ld, linv, mtmin, mtmax, mst, mdepth - 1, mdepth, mss);
+                    kd_interior(w, snode, o, d, inv, stmin, stmax, sst, sdepth - 1, sdepth, sss);
                     if (COUNT) cnt.nodes++;
-                    w = s.mesh_nodes[mnode];
+                    w = s.scene_nodes[snode];
                 }
                 const uint32_t n = w.x >> 2;
                 if (COUNT) cnt.leaf_refs += n;
-                for (uint32_t k = 0; k < n; k++) {
-                    const uint32_t ti = leaf_id(s.mesh_nodes, mnode, w.y, k);
-                    const float4 p0 = s.tri_verts[3 * (size_t)ti], p1 = s.tri_verts[3 * (size_t)ti + 1],
-                                 p2 = s.tri_verts[3 * (size_t)ti + 2];
-                    if (COUNT) cnt.tri_tests++;
-                    V3 b;
-                    const float t = watertight(lo, ld, (int)(kxyz & 3u), (int)((kxyz >> 2) & 3u), (int)(kxyz >> 4), sx, sy,
-                                               sz, p0, p1, p2, b);
-                    if (ANY) {
-                        if (t >= any_t) continue;
-                        if (degenerate(p0, p1, p2)) continue;
-                        found_any = true;
-                        return true;
-                    } else if (t < hit.t) {
-                        if (degenerate(p0, p1, p2)) continue;
-                        hit.t = t; hit.tri = ti; hit.b = b;
-                    }
-                }
-                done = !kd_next(mnode, mroot, mtmin, mtmax, mt1, mst, mdepth - 1, mss, mrestart);
+                if (n == 0) return !kd_next(snode, 0u, stmin, stmax, st1, sst, sdepth - 1, sss, srestart);
+                lnode = snode; lfirst = w.y; ln = n; lj = 0;
             }
-            if (done) {
-                in_mesh = false;
-                if (!ANY && hit.t < mesh_tmax) hit.obj = (int)oi;
-            }
-            return false;
-        }
-        if (lj < ln) {
             // next object of the scene leaf: ray to object space, mesh bounds, watertight setup
             oi = leaf_id(s.scene_nodes, lnode, lfirst, lj);
             lj++;
@@ -395,33 +377,50 @@ struct Tracer {
             mnode = mroot; mtmin = mt0; mtmax = mt1; mrestart = -1.f;
             mss = 0u;
             in_mesh = true;
-            return false;
         }
-        if (ln > 0) {
-            // the leaf's objects are done: continue the scene traversal behind it
-            ln = 0;
-            return !kd_next(snode, 0u, stmin, stmax, st1, sst, sdepth - 1, sss, srestart);
-        }
-        if (!(loop_t > stmin)) return true;
-        if (COUNT && !(sss & kSsResume)) cnt.nodes++;
-        sss &= ~kSsResume;
-        uint2 w = s.scene_nodes[snode];
-        for (int b = kDescentBudget; (w.x & 3u) != 3u;) {
-            if (b-- == 0) {
-                sss |= kSsResume;
-                return false;
+        // one leaf of the current mesh tree
+        bool done = !((ANY ? any_t : hit.t) > mtmin);
+        if (!done) {
+            // descend to the next leaf: neither the loop bound nor tmin changes on the way down
+            if (COUNT && !(mss & kSsResume)) cnt.nodes++;
+            mss &= ~kSsResume;
+            uint2 w = s.mesh_nodes[mnode];
+            for (int b = kDescentBudget; (w.x & 3u) != 3u;) {
+                if (b-- == 0) {
+                    mss |= kSsResume;
+                    return false;
+                }
+                kd_interior(w, mnode, lo, ld, linv, mtmin, mtmax, mst, mdepth - 1, mdepth, mss);
+                if (COUNT) cnt.nodes++;
+                w = s.mesh_nodes[mnode];
             }
-            kd_interior(w, snode, o, d, inv, stmin, stmax, sst, sdepth - 1, sdepth, sss);
-            if (COUNT) cnt.nodes++;
-            w = s.scene_nodes[snode];
+            const uint32_t n = w.x >> 2;
+            if (COUNT) cnt.leaf_refs += n;
+            for (uint32_t k = 0; k < n; k++) {
+                const uint32_t ti = leaf_id(s.mesh_nodes, mnode, w.y, k);
+                const float4 p0 = s.tri_verts[3 * (size_t)ti], p1 = s.tri_verts[3 * (size_t)ti + 1],
+                             p2 = s.tri_verts[3 * (size_t)ti + 2];
+                if (COUNT) cnt.tri_tests++;
+                V3 b;
+                const float t = watertight(lo, ld, (int)(kxyz & 3u), (int)((kxyz >> 2) & 3u), (int)(kxyz >> 4), sx, sy,
+                                           sz, p0, p1, p2, b);
+                if (ANY) {
+                    if (t >= any_t) continue;
+                    if (degenerate(p0, p1, p2)) continue;
+                    found_any = true;
+                    return true;
+                } else if (t < hit.t) {
+                    if (degenerate(p0, p1, p2)) continue;
+                    hit.t = t; hit.tri = ti; hit.b = b;
+                }
+            }
+            done = !kd_next(mnode, mroot, mtmin, mtmax, mt1, mst, mdepth - 1, mss, mrestart);
         }
-        const uint32_t n = w.x >> 2;
-        if (COUNT) cnt.leaf_refs += n;
-        if (n > 0) {
-            lnode = snode; lfirst = w.y; ln = n; lj = 0;
-            return false;
+        if (done) {
+            in_mesh = false;
+            if (!ANY && hit.t < mesh_tmax) hit.obj = (int)oi;
         }
-        return !kd_next(snode, 0u, stmin, stmax, st1, sst, sdepth - 1, sss, srestart);
+        return false;
     }
 };
 
