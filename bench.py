@@ -288,10 +288,12 @@ def main():
     ext_sec = sum(s.extend_seconds for s in stats) / len(stats)
     ext_launches = stats[0].extend_launches
     roofline = None
+    culled = None
     if not args.no_counters:
         cctx = yar_amd.Context(scene, **dict(opts, enable_timing=0, enable_counters=1))
         _, _, _, cs = cctx.render(0, want_film=False)
         cctx.close()
+        culled = cs.mis_culled_rays
         ext_bytes = (BYTES_NODE * cs.extend_node_visits + BYTES_LEAF_REF * cs.extend_leaf_refs +
                      BYTES_TRI * cs.extend_triangle_tests + BYTES_OBJ * cs.extend_object_visits)
         achieved = ext_bytes / ext_sec / 1e9
@@ -299,9 +301,16 @@ def main():
         tpath = os.path.join(ROOT, "profiles", "extend_traffic.json")
         if os.path.exists(tpath):
             traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        counters = None
+        cpath = os.path.join(ROOT, "profiles", "extend_counters.json")
+        if os.path.exists(cpath):
+            counters = json.load(open(cpath))
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": "k_extend (closest-hit kd traversal of path rays)",
+                    "limiter": "instruction issue: dependent kd-node loads under SIMD divergence (HBM traffic "
+                               "and bandwidth are far below peak; see counters)",
+                    "counters": counters,
                     "algorithmic_bytes_per_launch": int(ext_bytes / max(1, ext_launches)),
                     "avg_launch_ms": round(ext_sec / max(1, ext_launches) * 1e3, 4),
                     "launches_per_step": ext_launches,
@@ -334,6 +343,10 @@ def main():
                                f"{spp} spp, max depth {ctx.scene.desc.config.max_light_bounces}",
                    "samples_per_step": samples, "triangles": ntri, "scale": args.scale, "seed": args.seed},
         "rays_per_s": round(rays_per_s, 1), "mrays_per_s": round(rays_per_s / 1e6, 2),
+        # the reference counts every MIS BSDF ray toward a light as traced; this build answers some of
+        # them with an analytic light-rectangle test (count from the counted render of the same step)
+        "mrays_per_s_reference_definition": (round((rays + culled) / (elapsed / args.steps) / 1e6, 2)
+                                             if culled is not None else None),
         "rays_per_sample": round(rays / samples, 3),
         "gpu_ms_per_step": round(sum(s.gpu_seconds for s in stats) / len(stats) * 1e3, 2),
         "kernel_ms_per_step": {"extend": round(ext_sec * 1e3, 2),

@@ -238,6 +238,8 @@ typedef struct yar_render_stats {
     int64_t train_records;              /* guided: training records collected over all passes */
     int64_t passes;                     /* guided: sample passes (1 for plain renders) */
     double  train_nats;                 /* guided: last batch's log-likelihood gain over uniform */
+    int64_t mis_culled_rays;            /* only with enable_counters: MIS BSDF rays toward a light that the
+                                           analytic light-rectangle test answered without a traversal */
 } yar_render_stats;
 
 int  yar_gpu_create(const yar_scene_desc* scene, const yar_gpu_options* options, yar_gpu_ctx** out_ctx);

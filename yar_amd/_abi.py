@@ -102,7 +102,7 @@ class RenderStats(C.Structure):
                 ("extend_seconds", C.c_double), ("shade_seconds", C.c_double),
                 ("shadow_seconds", C.c_double), ("mis_seconds", C.c_double), ("other_seconds", C.c_double),
                 ("guide_seconds", C.c_double), ("train_seconds", C.c_double), ("train_records", C.c_int64),
-                ("passes", C.c_int64), ("train_nats", C.c_double)]
+                ("passes", C.c_int64), ("train_nats", C.c_double), ("mis_culled_rays", C.c_int64)]
 
 
 class GuideConfig(C.Structure):

@@ -831,6 +831,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     q.shadow_path = ctx->sh_path.p; q.shadow_o = ctx->sh_o.p; q.shadow_d = ctx->sh_d.p; q.shadow_c = ctx->sh_c.p;
     q.mis_path = ctx->mi_path.p; q.mis_o = ctx->mi_o.p; q.mis_d = ctx->mi_d.p; q.mis_f = ctx->mi_f.p;
     q.mis_ndw = ctx->mi_ndw.p; q.dl_path = ctx->dl_path.p; q.counts = ctx->counts.p; q.totals = ctx->totals.p;
+    q.count_culls = ctx->opt.enable_counters != 0;
     if (bin_rays) {
         q.unsorted = ctx->q_unsorted.p;
         q.bin_rays = bin_mode;
@@ -1097,6 +1098,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     st.extend_object_visits = (int64_t)tot[3];
     st.extend_rays = (int64_t)tot[16];
     st.extend_launches = (int64_t)tot[19];
+    st.mis_culled_rays = (int64_t)tot[20];
     st.extend_seconds = ksec[0];
     st.shade_seconds = ksec[1];
     st.shadow_seconds = ksec[2];

@@ -372,7 +372,7 @@ struct Tracer {
             const int ky = (kz == 0 ? 2 : kz - 1);
             kxyz = (uint32_t)kx | ((uint32_t)ky << 2) | ((uint32_t)kz << 4);
             const float dx = ld[kx], dy = ld[ky], dz = ld[kz];
-            sx = -dx / dz; sy = -dy / dz; sz = 1.f / dz;
+            sx = -dx / dz; sy = -dy / dz; sz = linv[kz];           // linv[kz] is 1.f / dz
             mesh_tmax = hit.t;
             mnode = mroot; mtmin = mt0; mtmax = mt1; mrestart = -1.f;
             mss = 0u;

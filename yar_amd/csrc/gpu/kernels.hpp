@@ -72,8 +72,9 @@ struct Queues {
     uint32_t train_cap;
     float keep_prob;
     // [0..15] traversal counters (4 per kernel kind), [16] extend rays, [17] shadow rays, [18] mis rays,
-    // [19] extend launches
+    // [19] extend launches, [20] MIS rays answered by the light-rectangle test (count_culls only)
     unsigned long long* totals;
+    int count_culls;
     // Direction binning of the next iteration's active list (bin_rays = 1, 2, 3: cube-map cells of
     // 2x2, 4x4, 8x8 per face): shade and guide sample append to `unsorted`; k_bin_local sorts each
     // window of 4,096 consecutive entries by direction cell in LDS and writes it to the active list

@@ -592,6 +592,7 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
         const uint32_t i = blk + threadIdx.x;
         const bool valid = i < n;
         bool want_active = false, want_shadow = false, want_mis = false, want_dl = false, want_guide = false;
+        bool mis_culled = false;
         float4 g_x0, g_x1, g_x2, g_x3;
         uint32_t slot = 0;
         V3 sh_o = v3(0, 0, 0), sh_d = v3(0, 0, 0), mi_o = v3(0, 0, 0), mi_d = v3(0, 0, 0);
@@ -764,6 +765,7 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
                                     mi_ndw = dot(sc.n, wi); mi_lndw = lndw; mi_light = ri;
                                     // only a ray that can reach the light's rectangle can contribute
                                     want_mis = ray_may_hit_rect(R, mi_o, wi);
+                                    mis_culled = !want_mis;
                                 }
                             }
                         }
@@ -857,6 +859,10 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
             P.L[slot] = f4(L, 0.f);
             if (GUIDED && c.guided) P.grng[slot] = rs;
             else if (seq) P.rng[slot] = rs;
+        }
+        if (q.count_culls) {
+            const uint64_t m = __ballot(mis_culled);
+            if (__lane_id() == 0 && m) atomicAdd(&q.totals[20], (unsigned long long)__popcll(m));
         }
         uint32_t* const ctr[4] = {n_out, &q.counts[2], &q.counts[3], &q.counts[4]};
         const bool pred[4] = {want_active, want_shadow, want_mis, want_dl};
