@@ -113,3 +113,14 @@ def test_render_plastic_and_mirror_matches_oracle(c1_path, oracle, tmp_path):
     assert mean_rel <= 1e-3
     assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
     assert (film["weight_sum"] == ofilm["weight_sum"]).all()
+
+
+@pytest.mark.parametrize("nx", [12, 20], ids=["144spp-u8", "400spp-u16"])
+def test_render_high_spp_matches_oracle(c1_path, oracle, nx):
+    # sample counts above 128 take the one-wave-per-run shuffle kernel (u8 entries up to 256 spp,
+    # u16 beyond): pixel_sampling.cpp:54-85 with every sampler dimension used by depth-5 paths
+    img, film, var, st, oimg, ofilm, ovar = _render_pair(c1_path, oracle, nx=nx, region=(96, 96, 112, 112))
+    exact, close, mean_rel = _compare_images(img, oimg, film, ofilm, f"c1 16x16 crop {nx * nx} spp")
+    assert close >= 0.99
+    assert mean_rel <= 1e-3
+    assert (film["weight_sum"] == ofilm["weight_sum"]).all()

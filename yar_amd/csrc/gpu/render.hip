@@ -179,6 +179,50 @@ YG_INLINE void sample_2d(const Chunk& c, const PixelSampler& ps, int i, int k, f
     strat2(ps, c.jt, dim2_base(spp, (uint32_t)c.d1, (uint32_t)i) + 2u * j, j, x, y);
 }
 
+// Shuffle permutations of large runs (spp > 128): one wave per (pixel, dimension) run.  The lanes
+// draw the run's swap indices in parallel (pcg32 jump-ahead to each lane's first draw, then steps
+// of 64 draws) into LDS and initialise the index array there; one lane then applies the swaps in
+// order (math.h:138-144) and the wave writes the permutation out.  Every array access is in LDS.
+template <class T>
+__global__ __launch_bounds__(64) void k_sample_perms_large(const Chunk* __restrict__ cp, int n_dims, ulonglong2 step64) {
+    extern __shared__ uint16_t big_lds[];
+    const Chunk c = *cp;
+    const long long total = (long long)c.n_pixels * n_dims;
+    const long long g = blockIdx.x;
+    if (g >= total) return;
+    const int p = (int)(g / n_dims), i = (int)(g % n_dims);
+    const DevView& v = c.views[c.pix_view[p]];
+    const int spp = v.spp;
+    uint16_t* a = big_lds;                       // index array, spp entries
+    uint16_t* jv = big_lds + c.max_spp;          // swap index of draw m (array size n = spp - m)
+    const int lane = (int)threadIdx.x;
+    uint64_t inc;
+    const uint64_t s0 = pcg_seed_state(stream_id(c.pixels[p], v.seed_offset), inc);
+    const uint32_t first = i < c.d1 ? dim1_base((uint32_t)spp, (uint32_t)i) + (uint32_t)spp
+                                    : dim2_base((uint32_t)spp, (uint32_t)c.d1, (uint32_t)(i - c.d1)) + 2u * (uint32_t)spp;
+    uint64_t st = pcg_jump(c.jt, s0, inc, first + (uint32_t)lane);
+    const uint64_t g64 = step64.y * inc;
+    for (int m = lane; m < spp - 1; m += 64) {
+        const uint32_t n = (uint32_t)(spp - m);
+        jv[m] = (uint16_t)((pcg_output(st) * (uint64_t)n) >> 32);
+        st = step64.x * st + g64;
+    }
+    for (int k = lane; k < spp; k += 64) a[k] = (uint16_t)k;
+    __syncthreads();
+    if (lane == 0) {
+        for (int m = 0; m < spp - 1; m++) {
+            const int n = spp - m;
+            const uint32_t k = jv[m];
+            const uint16_t x = a[n - 1];
+            a[n - 1] = a[k];
+            a[k] = x;
+        }
+    }
+    __syncthreads();
+    T* out = reinterpret_cast<T*>(const_cast<uint8_t*>(c.perm)) + c.pix_first[p] * (long long)n_dims + (long long)i * spp;
+    for (int k = lane; k < spp; k += 64) out[k] = (T)a[k];
+}
+
 // Shuffle permutations: one work item per (pixel, 1D or 2D dimension) applies the run's Fisher-Yates
 // swaps (math.h:138-144, the swap draws follow the run's value draws) to an index array and writes
 // the spp resulting stratum indices.  STAGED: the index arrays live in LDS ([entry][thread]) and the
@@ -256,6 +300,7 @@ __global__ void k_sample_perms(const Chunk* __restrict__ cp, int n_dims) {
 }
 
 constexpr int kPermBlock = 64;
+constexpr int kPermLargeMaxSpp = 8192;     // 2 x 2 B x spp of LDS per run (32 KB)
 
 void launch_sample_perms(const Chunk& hc, const Chunk* dc, hipStream_t s) {
     const int n_dims = hc.d1 + hc.d2;
@@ -268,8 +313,19 @@ void launch_sample_perms(const Chunk& hc, const Chunk* dc, hipStream_t s) {
         else hipLaunchKernelGGL((k_sample_perms<uint16_t, true>), dim3(blocks), dim3(kPermBlock), shm, s, dc, n_dims);
         return;
     }
-    if (hc.perm_bytes == 1) hipLaunchKernelGGL((k_sample_perms<uint8_t, false>), dim3(blocks), dim3(kPermBlock), 0, s, dc, n_dims);
-    else hipLaunchKernelGGL((k_sample_perms<uint16_t, false>), dim3(blocks), dim3(kPermBlock), 0, s, dc, n_dims);
+    if (hc.max_spp > kPermLargeMaxSpp) {          // runs beyond the LDS of one workgroup: in place in HBM
+        if (hc.perm_bytes == 1) hipLaunchKernelGGL((k_sample_perms<uint8_t, false>), dim3(blocks), dim3(kPermBlock), 0, s, dc, n_dims);
+        else hipLaunchKernelGGL((k_sample_perms<uint16_t, false>), dim3(blocks), dim3(kPermBlock), 0, s, dc, n_dims);
+        return;
+    }
+    // one wave per run; pcg32 multiplier and increment factor of 64 steps (A_64, G_64)
+    uint64_t a64 = 1, g64 = 0;
+    for (int k = 0; k < 64; k++) { g64 = g64 * kPcgMult + 1; a64 *= kPcgMult; }
+    const ulonglong2 step64 = make_ulonglong2(a64, g64);
+    const size_t shm = (size_t)2 * hc.max_spp * sizeof(uint16_t);
+    const unsigned runs = (unsigned)items;
+    if (hc.perm_bytes == 1) hipLaunchKernelGGL((k_sample_perms_large<uint8_t>), dim3(runs), dim3(64), shm, s, dc, n_dims, step64);
+    else hipLaunchKernelGGL((k_sample_perms_large<uint16_t>), dim3(runs), dim3(64), shm, s, dc, n_dims, step64);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -405,11 +461,16 @@ YG_INLINE void trace_queue(const DevScene& sc, uint32_t n, uint32_t* cursor, SSt
     Tracer<ANY, COUNT> tr;
     if (!refill) {
         // batches of 64: every lane of the wave runs its query to completion before the next batch.
-        // A wave takes 4 batches per cursor update while more than 8 fetches per wave remain (fewer
-        // same-address atomics), then single batches (balanced tail).
-        const uint32_t waves = gridDim.x * ((blockDim.x + 63u) >> 6);
+        // The first `size` entries of every wave are assigned statically (no atomic, so a launch
+        // over a short queue costs almost nothing); further work comes from the cursor, 4 batches
+        // per update while more than 8 fetches per wave remain (fewer same-address atomics), then
+        // single batches (balanced tail).
+        const uint32_t wpb = (blockDim.x + 63u) >> 6;
+        const uint32_t waves = gridDim.x * wpb;
         uint32_t size = n > 8u * 256u * waves ? 256u : 64u;
-        for (uint32_t base = wave_fetch(cursor, size); base < n; base = wave_fetch(cursor, size)) {
+        const uint32_t static_end = waves * size;
+        for (uint32_t base = (blockIdx.x * wpb + (threadIdx.x >> 6)) * size; base < n;
+             base = static_end + wave_fetch(cursor, size)) {
             const uint32_t end = min(base + size, n);
             for (uint32_t b = base; b < end; b += 64u) {
                 const uint32_t i = b + __lane_id();
