@@ -361,9 +361,11 @@ typedef struct yar_guide_config {
     float    beta1, beta2, epsilon;     /* 0 = 0.9, 0.99, 1e-8 */
     int32_t  steps_per_pass;            /* Adam steps after each guided render pass (0 = 32) */
     float    guide_fraction;            /* one-sample MIS probability of the learned mixture (0 = 0.5) */
-    int32_t  first_pass_spp;            /* samples per pixel of the first (untrained) pass (0 = 4) */
+    int32_t  first_pass_spp;            /* samples per pixel of the first (untrained) pass (0 = 8) */
     int32_t  record_capacity;           /* training records kept per pass (0 = 2^22) */
-    int32_t  reserved[6];
+    int32_t  pass_count;                /* sample passes per render (0 = 2): the first pass, then the rest
+                                           split into passes of doubling size, training after each */
+    int32_t  reserved[5];
 } yar_guide_config;
 
 int64_t yar_guide_param_count(void);
@@ -386,10 +388,11 @@ int  yar_guide_pdf(yar_guide* guide, const float* inputs, const float* dirs, int
 int  yar_guide_sample(yar_guide* guide, const float* inputs, const float* uniforms, int64_t n, float* out_dirs,
                       float* out_pdf);
 
-/* Guided form of yar_gpu_render_batch: renders the views in sample passes (sizes first_pass_spp,
- * first_pass_spp, 2x, 4x, ...), traces continuations with one-sample MIS between the guide's
- * mixture and the BSDF (the first pass uses the BSDF alone), and trains the guide on the pass's
- * radiance samples between passes.  The image converges to the plain render's mean. */
+/* Guided form of yar_gpu_render_batch: renders the views in pass_count sample passes (first_pass_spp
+ * samples per pixel with BSDF sampling alone, then the rest in passes of doubling size), trains the
+ * guide on the radiance samples of the passes so far between passes, and in the later passes traces
+ * the continuation at diffuse vertices with one-sample MIS between the guide's mixture and the
+ * BSDF.  The image converges to the plain render's mean. */
 int  yar_gpu_render_guided(yar_gpu_ctx* ctx, yar_guide* guide, const int32_t* views, int32_t count,
                            float* const* out_images, double* out_variances, yar_render_stats* out_stats);
 

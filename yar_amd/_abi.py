@@ -109,7 +109,8 @@ class GuideConfig(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("batch_size", C.c_int32), ("learning_rate", C.c_float),
                 ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
                 ("steps_per_pass", C.c_int32), ("guide_fraction", C.c_float), ("first_pass_spp", C.c_int32),
-                ("record_capacity", C.c_int32), ("reserved", C.c_int32 * 6)]
+                ("record_capacity", C.c_int32), ("pass_count", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
 
 
 class GpuInfo(C.Structure):

@@ -597,14 +597,24 @@ struct GuidedRun {
     std::vector<std::pair<int, int>> passes;      // sample ranges [k0, k1)
 };
 
-static std::vector<std::pair<int, int>> guided_passes(int max_spp, int first) {
+// `first` samples per pixel with BSDF sampling only, then the rest in count - 1 passes of doubling
+// size, each guided by the proposal trained on the records of the passes before it.  Every pass
+// runs the whole bounce loop, and a small pass leaves the GPU idle in the long tail of each
+// iteration, so the default is two passes.
+static std::vector<std::pair<int, int>> guided_passes(int max_spp, int first, int count) {
     std::vector<std::pair<int, int>> p;
-    int k0 = 0, size = std::max(1, first);
-    while (k0 < max_spp) {
-        const int k1 = std::min(max_spp, k0 + size);
-        p.push_back({k0, k1});
-        if (k0 > 0) size *= 2;                    // sizes first, first, 2 first, 4 first, ...
-        k0 = k1;
+    const int k1 = std::min(max_spp, std::max(1, first));
+    p.push_back({0, k1});
+    const int rest = max_spp - k1, n = std::max(1, count - 1);
+    // sizes proportional to 1, 2, 4, ... 2^(n-1) over the remaining samples
+    const double unit = (double)rest / (double)((1ll << std::min(n, 30)) - 1);
+    int k0 = k1;
+    for (int i = 0; i < n && k0 < max_spp; i++) {
+        int k = i + 1 == n ? max_spp : k1 + (int)std::llround(unit * (double)((1ll << (i + 1)) - 1));
+        k = std::max(k, k0 + 1);
+        k = std::min(k, max_spp);
+        p.push_back({k0, k});
+        k0 = k;
     }
     return p;
 }
@@ -654,7 +664,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         }
     }
     const int64_t total_pixels = (int64_t)plist.size();
-    if (guided) gr->passes = guided_passes(job_max_spp, guide_config(gr->guide).first_pass_spp);
+    if (guided) gr->passes = guided_passes(job_max_spp, guide_config(gr->guide).first_pass_spp, guide_config(gr->guide).pass_count);
     int pass_max = 1;                                                          // widest guided pass
     if (guided)
         for (auto& pr : gr->passes) pass_max = std::max(pass_max, pr.second - pr.first);

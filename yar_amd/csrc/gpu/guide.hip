@@ -607,8 +607,9 @@ void fill_defaults(yar_guide_config& c) {
     if (c.epsilon <= 0.f) c.epsilon = 1e-8f;
     if (c.steps_per_pass <= 0) c.steps_per_pass = 32;
     if (c.guide_fraction <= 0.f || c.guide_fraction > 1.f) c.guide_fraction = 0.5f;
-    if (c.first_pass_spp <= 0) c.first_pass_spp = 4;
+    if (c.first_pass_spp <= 0) c.first_pass_spp = 8;
     if (c.record_capacity <= 0) c.record_capacity = 1 << 22;
+    if (c.pass_count <= 0) c.pass_count = 2;
 }
 
 std::vector<GuideRecord> pack_records(const float* x, const float* dirs, const float* wt, int64_t n) {

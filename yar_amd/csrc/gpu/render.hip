@@ -837,9 +837,12 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
                     bounce++;
                     if (bounce == c.max_bounces) {
                         finished = true;
-                    } else if (GUIDED && c.guided) {
-                        // guided continuation: record the vertex, then either sample the BSDF here
-                        // (untrained pass) or queue a network query for the guide-sample stage
+                    } else if (GUIDED && c.guided && m.type == 1) {
+                        // guided continuation at a diffuse vertex: record the vertex, then either
+                        // sample the BSDF here (untrained pass) or queue a network query for the
+                        // guide-sample stage.  Glossy vertices keep BSDF sampling: their product
+                        // f.Li.cos is a narrow lobe that the mixture cannot match, and a broad
+                        // proposal mixed into a glossy lobe only raises the variance.
                         float x[kGIn];
                         V3 lo, inv_ext;
                         scene_frame(s, lo, inv_ext);
