@@ -65,19 +65,22 @@ typedef struct yar_mesh_desc {          /* Triangle_Mesh, src/lib/triangle_mesh.
 #define YAR_MATERIAL_MIRROR  4           /* Perfect_Reflector_Material -> delta reflection (delta_scattering.cpp:27) */
 #define YAR_MATERIAL_PLASTIC 5           /* Plastic_Material -> Pbrt3_Plastic_BRDF (bsdf_pbrt.cpp:258) in pbrt3
                                           * scenes, Plastic_BRDF (bsdf.cpp:204) otherwise */
+#define YAR_MATERIAL_COATED_DIFFUSE 6    /* Coated_Diffuse_Material (pbrt "substrate") ->
+                                          * Ashikhmin_Shirley_Phong_BRDF (bsdf.cpp:427) */
 
 typedef struct yar_material_desc {
     int32_t type;
     int32_t roughness_is_alpha;         /* pbrt "remaproughness" false */
-    float   reflectance[3];             /* matte/plastic Kd, glass/mirror Kr */
+    float   reflectance[3];             /* matte/plastic/substrate Kd, glass/mirror Kr */
     float   transmittance[3];           /* glass Kt */
     float   eta[3];                     /* metal eta (RGB) */
     float   k[3];                       /* metal k (RGB) */
     float   eta_i;                      /* metal: IOR of adjacent dielectric (1 for pbrt) */
-    float   u_roughness;                /* metal uroughness / glass, plastic roughness */
-    float   v_roughness;                /* metal vroughness */
+    float   u_roughness;                /* metal/substrate uroughness, glass/plastic roughness */
+    float   v_roughness;                /* metal/substrate vroughness */
     float   index_of_refraction;        /* glass */
     float   specular_r0;                /* plastic: specular scale r0 (luminance of pbrt Ks) */
+    float   specular[3];                /* coated diffuse: Fresnel r0 per channel (pbrt substrate Ks) */
 } yar_material_desc;
 
 typedef struct yar_object_desc {        /* Scene_Object, src/lib/scene_object.h:8 */

@@ -859,10 +859,33 @@ static float plastic_pdf(const Material& m, const Shading& sc, V3 wo, V3 wi) {  
     return 0.5f * (diffuse_pdf + spec_pdf);
 }
 
+// Ashikhmin_Shirley_Phong_BRDF::evaluate / pdf (bsdf.cpp:440-495); Material: refl = diffuse
+// reflectance, k = Fresnel r0 per channel.
+static RGB coated_eval(const Material& m, const Shading& sc, V3 wo, V3 wi) {
+    V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
+    V3 wh = normalized(wol + wil);
+    float cos_i = dot(wil, wh);
+    RGB F = schlick_fresnel(m.k, cos_i);
+    float D = D_aniso(wh, m.alpha_x, m.alpha_y);
+    RGB spec = F * (D / (4.f * cos_i * std::max(wol.z, wil.z)));
+    auto pow5 = [](float v) { return (v * v) * (v * v) * v; };
+    RGB diffuse = (m.refl * (splat(1.f) - m.k)) *
+                  (28.f / (23.f * Pi) * (1.f - pow5(1.f - 0.5f * wil.z)) * (1.f - pow5(1.f - 0.5f * wol.z)));
+    return diffuse + spec;
+}
+static float coated_pdf(const Material& m, const Shading& sc, V3 wo, V3 wi) {
+    float diffuse_pdf = dot(sc.n, wi) / Pi;
+    V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
+    V3 wh = normalized(wol + wil);
+    float spec_pdf = vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
+    return 0.5f * (diffuse_pdf + spec_pdf);
+}
+
 static RGB bsdf_eval(const Scene& s, const Shading& sc, V3 wo, V3 wi) {
     const Material& m = s.materials[sc.material];
     if (m.type == YAR_MATERIAL_DIFFUSE) return Pi_Inv * m.refl;                      // bsdf.cpp:68
     if (m.type == YAR_MATERIAL_PLASTIC) return plastic_eval(m, sc, wo, wi);
+    if (m.type == YAR_MATERIAL_COATED_DIFFUSE) return coated_eval(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);                               // bsdf.cpp:160
     V3 wh = normalized(wol + wil);
     float ci = dot(wil, wh);
@@ -875,6 +898,7 @@ static float bsdf_pdf(const Scene& s, const Shading& sc, V3 wo, V3 wi) {
     const Material& m = s.materials[sc.material];
     if (m.type == YAR_MATERIAL_DIFFUSE) return dot(sc.n, wi) / Pi;                   // bsdf.cpp:80
     if (m.type == YAR_MATERIAL_PLASTIC) return plastic_pdf(m, sc, wo, wi);
+    if (m.type == YAR_MATERIAL_COATED_DIFFUSE) return coated_pdf(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);                               // bsdf.cpp:190
     V3 wh = normalized(wol + wil);
     return vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
@@ -896,6 +920,18 @@ static RGB bsdf_sample(const Scene& s, const Shading& sc, V2 u, float u_scat, V3
         if (dot(sc.n, *wi) <= 0.f) return RGB{};
         *pdf = plastic_pdf(m, sc, wo, *wi);
         return plastic_eval(m, sc, wo, *wi);
+    }
+    if (m.type == YAR_MATERIAL_COATED_DIFFUSE) {                                    // bsdf.cpp:462-479
+        if (u_scat < 0.5f) {
+            *wi = to_world(sc, sample_cosine(u));
+        } else {
+            V3 wol = to_local(sc, wo);
+            V3 wh = vndf_sample(u, wol, m.alpha_x, m.alpha_y);
+            *wi = to_world(sc, reflect(wol, wh));
+        }
+        if (dot(sc.n, *wi) <= 0.f) return RGB{};
+        *pdf = coated_pdf(m, sc, wo, *wi);
+        return coated_eval(m, sc, wo, *wi);
     }
     V3 wol = to_local(sc, wo);                                                       // bsdf.cpp:176
     V3 wh = vndf_sample(u, wol, m.alpha_x, m.alpha_y);
@@ -1260,6 +1296,7 @@ static bool load_scene(const yar_scene_desc* d, Scene& s, int threads) {
         m.trans = {md.transmittance[0], md.transmittance[1], md.transmittance[2]};
         m.eta = {md.eta[0], md.eta[1], md.eta[2]};
         m.k = {md.k[0], md.k[1], md.k[2]};
+        if (md.type == YAR_MATERIAL_COATED_DIFFUSE) m.k = {md.specular[0], md.specular[1], md.specular[2]};   // r0
         m.eta_i = md.eta_i;
         m.ior = md.index_of_refraction;
         m.roughness = md.u_roughness;
@@ -1562,6 +1599,35 @@ int32_t oref_plastic_sample(int32_t pbrt3, const float* kd, float r0, float alph
     s.materials.push_back(m);
     Shading sc = probe_frame();
     sc.flipped = flipped != 0;
+    V3 wi;
+    float pdf = 0.f;
+    const RGB f = bsdf_sample(s, sc, V2{u3[0], u3[1]}, u3[2], V3(wo[0], wo[1], wo[2]), &wi, &pdf);
+    wi_out[0] = wi.x; wi_out[1] = wi.y; wi_out[2] = wi.z;
+    f_out[0] = f.r; f_out[1] = f.g; f_out[2] = f.b;
+    *pdf_out = pdf;
+    return (f.r != 0.f || f.g != 0.f || f.b != 0.f) ? 1 : 0;
+}
+
+// Coated diffuse BRDF in the local frame (n = z, T = x, B = y): evaluate + pdf, or sample.
+void oref_coated_eval(const float* kd, const float* r0, float ax, float ay, const float* wo, const float* wi,
+                      float* f_out, float* pdf_out) {
+    Material m{};
+    m.type = YAR_MATERIAL_COATED_DIFFUSE; m.refl = RGB{kd[0], kd[1], kd[2]}; m.k = RGB{r0[0], r0[1], r0[2]};
+    m.alpha_x = ax; m.alpha_y = ay;
+    Shading sc = probe_frame();
+    const V3 o(wo[0], wo[1], wo[2]), i(wi[0], wi[1], wi[2]);
+    const RGB f = coated_eval(m, sc, o, i);
+    f_out[0] = f.r; f_out[1] = f.g; f_out[2] = f.b;
+    *pdf_out = coated_pdf(m, sc, o, i);
+}
+int32_t oref_coated_sample(const float* kd, const float* r0, float ax, float ay, const float* wo, const float* u3,
+                           float* wi_out, float* f_out, float* pdf_out) {
+    Scene s;
+    Material m{};
+    m.type = YAR_MATERIAL_COATED_DIFFUSE; m.refl = RGB{kd[0], kd[1], kd[2]}; m.k = RGB{r0[0], r0[1], r0[2]};
+    m.alpha_x = ax; m.alpha_y = ay;
+    s.materials.push_back(m);
+    Shading sc = probe_frame();
     V3 wi;
     float pdf = 0.f;
     const RGB f = bsdf_sample(s, sc, V2{u3[0], u3[1]}, u3[2], V3(wo[0], wo[1], wo[2]), &wi, &pdf);

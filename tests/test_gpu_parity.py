@@ -124,3 +124,21 @@ def test_render_high_spp_matches_oracle(c1_path, oracle, nx):
     assert close >= 0.99
     assert mean_rel <= 1e-3
     assert (film["weight_sum"] == ofilm["weight_sum"]).all()
+
+
+def test_render_coated_diffuse_matches_oracle(c1_path, oracle, tmp_path):
+    # the config-1 room with an anisotropic substrate wall and substrate triangles:
+    # Ashikhmin_Shirley_Phong_BRDF (bsdf.cpp:427-495)
+    lines = open(c1_path).read().splitlines()
+    mats = [i for i, l in enumerate(lines) if l.startswith('Material "matte"')]
+    lines[mats[1]] = ('Material "substrate" "rgb Kd" [ 0.5 0.3 0.2 ] "rgb Ks" [ 0.08 0.08 0.08 ] '
+                      '"float uroughness" [ 0.05 ] "float vroughness" [ 0.25 ]')
+    lines[mats[-1]] = 'Material "substrate" "rgb Kd" [ 0.2 0.4 0.6 ]'
+    p = tmp_path / "c1_substrate.pbrt"
+    p.write_text("\n".join(lines) + "\n")
+    img, film, var, st, oimg, ofilm, ovar = _render_pair(str(p), oracle, nx=4)
+    exact, close, mean_rel = _compare_images(img, oimg, film, ofilm, "c1 substrate 16spp")
+    assert close >= 0.99
+    assert mean_rel <= 1e-3
+    assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
+    assert (film["weight_sum"] == ofilm["weight_sum"]).all()

@@ -203,3 +203,27 @@ Shape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 3 1 0 3 0 1 3]
     assert list(m[1].reflectance) == pytest.approx([0.25] * 3) and m[1].u_roughness == pytest.approx(0.1)
     assert m[1].specular_r0 == pytest.approx(0.25, rel=1e-6)
     assert list(m[2].reflectance) == pytest.approx([0.8, 0.7, 0.6]) and list(m[3].reflectance) == pytest.approx([0.9] * 3)
+
+
+def test_substrate_material(tmp_path):
+    # pbrt "substrate" -> Coated_Diffuse_Material (pbrt_scene.cpp:531-561); pbrt-parser defaults
+    # Kd = Ks = 0.5, uroughness = vroughness = 0.1, remaproughness true
+    body = LIGHT + '''Material "substrate" "rgb Kd" [ 0.1 0.2 0.3 ] "rgb Ks" [ 0.04 0.05 0.06 ] "float uroughness" [ 0.02 ] "float vroughness" [ 0.3 ] "bool remaproughness" "false"
+Shape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]
+Material "substrate"
+Shape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 1 1 0 1 0 1 1]
+'''
+    d = _load(tmp_path, body).desc
+    m = [d.materials[i] for i in range(d.material_count)]
+    assert [x.type for x in m] == [6, 6]
+    assert list(m[0].reflectance) == pytest.approx([0.1, 0.2, 0.3])
+    assert list(m[0].specular) == pytest.approx([0.04, 0.05, 0.06])
+    assert (m[0].u_roughness, m[0].v_roughness, m[0].roughness_is_alpha) == (pytest.approx(0.02), pytest.approx(0.3), 1)
+    assert list(m[1].reflectance) == pytest.approx([0.5] * 3) and list(m[1].specular) == pytest.approx([0.5] * 3)
+    assert (m[1].u_roughness, m[1].v_roughness, m[1].roughness_is_alpha) == (pytest.approx(0.1), pytest.approx(0.1), 0)
+    bad = LIGHT + '''Texture "t" "spectrum" "imagemap" "string filename" "x.png"
+Material "substrate" "texture Kd" "t"
+Shape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]
+'''
+    with pytest.raises(yar_amd.YarError):
+        _load(tmp_path, bad)

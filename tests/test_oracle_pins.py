@@ -509,3 +509,85 @@ def test_mirror_reflects_point_lit_plane_closed_form():
     sc.desc.config.max_light_bounces = 1
     img1, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
     assert (img1 == 0).all()
+
+
+def coated(kd, r0, ax, ay, wo, wi=None, u=None):
+    L = O.lib()
+    P = C.c_void_p
+    L.oref_coated_eval.argtypes = [P, P, C.c_float, C.c_float, P, P, P, P]
+    L.oref_coated_sample.argtypes = [P, P, C.c_float, C.c_float, P, P, P, P, P]
+    L.oref_coated_sample.restype = C.c_int32
+    kd, r0, wo = _f32(kd), _f32(r0), _f32(wo)
+    f = np.zeros(3, np.float32)
+    pdf = C.c_float()
+    if wi is not None:
+        wi = _f32(wi)
+        L.oref_coated_eval(kd.ctypes.data, r0.ctypes.data, ax, ay, wo.ctypes.data, wi.ctypes.data, f.ctypes.data,
+                           C.byref(pdf))
+        return f.astype(np.float64), pdf.value
+    u = _f32(u)
+    out = np.zeros(3, np.float32)
+    ok = L.oref_coated_sample(kd.ctypes.data, r0.ctypes.data, ax, ay, wo.ctypes.data, u.ctypes.data, out.ctypes.data,
+                              f.ctypes.data, C.byref(pdf))
+    return bool(ok), out.astype(np.float64), f.astype(np.float64), pdf.value
+
+
+def test_coated_diffuse_closed_forms():
+    # Ashikhmin-Shirley coated diffuse (bsdf.cpp:440-495).  Normal incidence: F = r0, anisotropic
+    # GGX D(n) = 1/(pi ax ay), so f = Rd (1 - r0) 28/(23 pi) (31/32)^2 + r0 / (4 pi ax ay).
+    kd, r0 = np.array([0.3, 0.5, 0.7]), np.array([0.05, 0.1, 0.2])
+    for ax, ay in ((0.1, 0.1), (0.08, 0.3)):
+        f, pdf = coated(kd, r0, ax, ay, [0, 0, 1], [0, 0, 1])
+        want = kd * (1 - r0) * 28 / (23 * np.pi) * (31 / 32) ** 2 + r0 / (4 * np.pi * ax * ay)
+        assert np.allclose(f, want, rtol=2e-5), (f, want)
+        # pdf = (cos/pi + VNDF reflection pdf) / 2, at normal incidence G1 = 1: D(n) / 4
+        assert pdf == pytest.approx(0.5 * (1 / np.pi + 1 / (4 * np.pi * ax * ay)), rel=2e-5)
+    # r0 = 1: no diffuse term, F = 1, and the specular term is the textbook anisotropic GGX
+    # D(h) = 1 / (pi ax ay (hx^2/ax^2 + hy^2/ay^2 + hz^2)^2) over 4 (wi.h) max(wo.z, wi.z)
+    ax, ay = 0.15, 0.4
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        wo = _hemi(rng.uniform(0, 1.4), rng.uniform(0, 2 * np.pi))
+        wi = _hemi(rng.uniform(0, 1.4), rng.uniform(0, 2 * np.pi))
+        h = (wo + wi) / np.linalg.norm(wo + wi)
+        D = 1 / (np.pi * ax * ay * (h[0] ** 2 / ax ** 2 + h[1] ** 2 / ay ** 2 + h[2] ** 2) ** 2)
+        want = D / (4 * wi.dot(h) * max(wo[2], wi[2]))
+        f, _ = coated(kd, [1, 1, 1], ax, ay, wo, wi)
+        assert np.allclose(f, want, rtol=1e-4), (f, want)
+
+
+@pytest.mark.parametrize("ax,ay", [(0.15, 0.15), (0.1, 0.35)])
+def test_coated_diffuse_sampling_matches_quadrature_and_reciprocity(ax, ay):
+    rng = np.random.default_rng(17)
+    kd, r0 = [0.4, 0.3, 0.2], [0.3, 0.3, 0.3]
+    for theta_o in (0.3, 1.0):
+        wo = _hemi(theta_o, 0.7)
+        for _ in range(20):
+            wi = _hemi(rng.uniform(0, 1.5), rng.uniform(0, 2 * np.pi))
+            f1, _ = coated(kd, r0, ax, ay, wo, wi)
+            f2, _ = coated(kd, r0, ax, ay, wi, wo)
+            assert np.allclose(f1, f2, rtol=1e-5)
+        nt, nphi = 400, 200
+        ct = (np.arange(nt) + 0.5) / nt
+        ph = (np.arange(nphi) + 0.5) / nphi * 2 * np.pi
+        alb, mass = 0.0, 0.0
+        for c in ct:
+            st = np.sqrt(1 - c * c)
+            for p in ph[::2]:
+                wi = np.array([st * np.cos(p), st * np.sin(p), c])
+                f, pdf = coated(kd, r0, ax, ay, wo, wi)
+                alb += f[1] * c
+                mass += pdf
+        dA = (1.0 / nt) * (2 * np.pi / (nphi / 2))
+        alb *= dA
+        mass *= dA
+        n = 6000
+        est, valid = 0.0, 0
+        for u in rng.uniform(size=(n, 3)):
+            ok, wi, f, pdf = coated(kd, r0, ax, ay, wo, u=u)
+            if ok:
+                est += f[1] * wi[2] / pdf
+                valid += 1
+        est /= n
+        assert est == pytest.approx(alb, rel=0.03), (theta_o, est, alb)
+        assert mass == pytest.approx(valid / n, abs=0.02), (theta_o, mass, valid / n)

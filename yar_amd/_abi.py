@@ -3,7 +3,7 @@ import ctypes as C
 
 YAR_OK = 0
 YAR_MATERIAL_NONE, YAR_MATERIAL_DIFFUSE, YAR_MATERIAL_METAL, YAR_MATERIAL_GLASS = 0, 1, 2, 3
-YAR_MATERIAL_MIRROR, YAR_MATERIAL_PLASTIC = 4, 5
+YAR_MATERIAL_MIRROR, YAR_MATERIAL_PLASTIC, YAR_MATERIAL_COATED_DIFFUSE = 4, 5, 6
 
 
 class MeshDesc(C.Structure):
@@ -18,7 +18,8 @@ class MaterialDesc(C.Structure):
                 ("reflectance", C.c_float * 3), ("transmittance", C.c_float * 3),
                 ("eta", C.c_float * 3), ("k", C.c_float * 3), ("eta_i", C.c_float),
                 ("u_roughness", C.c_float), ("v_roughness", C.c_float),
-                ("index_of_refraction", C.c_float), ("specular_r0", C.c_float)]
+                ("index_of_refraction", C.c_float), ("specular_r0", C.c_float),
+                ("specular", C.c_float * 3)]
 
 
 class ObjectDesc(C.Structure):

@@ -79,7 +79,7 @@ struct DevMaterial {      // 64 bytes
     float4 refl;          // diffuse reflectance / glass Kr
     float4 trans;         // glass Kt (w = index of refraction)
     float4 eta;           // metal eta
-    float4 k;             // metal k
+    float4 k;             // metal k / coated diffuse Fresnel r0
 };
 // light_to_world rows, radiance, size (xy); size.z != 0 when the analytic MIS-ray test may not be used
 // for this light (its emitter geometry is not known to lie inside the rectangle)

@@ -262,9 +262,35 @@ YG_INLINE float plastic_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 w
     return 0.5f * (diffuse_pdf + spec_pdf);
 }
 
+// Ashikhmin_Shirley_Phong_BRDF (bsdf.cpp:427-495): Schlick-Fresnel anisotropic GGX specular over a
+// diffuse base scaled by (1 - r0).  Material: refl = diffuse reflectance, k = r0, alpha_x/y.
+YG_INLINE RGB coated_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
+    const V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
+    const V3 wh = normalized(wol + wil);
+    const float cos_i = dot(wil, wh);
+    const RGB r0 = rgb4(m.k);
+    const float k = 1.f - fabsf(cos_i);                                                 // scattering.cpp:10
+    const float k5 = (k * k) * (k * k) * k;
+    const RGB F = r0 + (splat(1.f) - r0) * k5;
+    const float D = D_aniso(wh, m.alpha_x, m.alpha_y);
+    const RGB spec = F * (D / (4.f * cos_i * fmaxf(wol.z, wil.z)));
+    const float ai = 1.f - 0.5f * wil.z, ao = 1.f - 0.5f * wol.z;
+    const float pi5 = (ai * ai) * (ai * ai) * ai, po5 = (ao * ao) * (ao * ao) * ao;
+    const RGB diffuse = (rgb4(m.refl) * (splat(1.f) - r0)) * (28.f / (23.f * kPi) * (1.f - pi5) * (1.f - po5));
+    return diffuse + spec;
+}
+YG_INLINE float coated_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
+    const float diffuse_pdf = dot(sc.n, wi) / kPi;
+    const V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
+    const V3 wh = normalized(wol + wil);
+    const float spec_pdf = vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
+    return 0.5f * (diffuse_pdf + spec_pdf);
+}
+
 YG_INLINE RGB bsdf_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
     if (m.type == 1) return kPiInv * rgb4(m.refl);
     if (m.type == 5) return plastic_eval(m, sc, wo, wi);
+    if (m.type == 6) return coated_eval(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
     V3 wh = normalized(wol + wil);
     float ci = dot(wil, wh);
@@ -276,6 +302,7 @@ YG_INLINE RGB bsdf_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
 YG_INLINE float bsdf_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
     if (m.type == 1) return dot(sc.n, wi) / kPi;
     if (m.type == 5) return plastic_pdf(m, sc, wo, wi);
+    if (m.type == 6) return coated_pdf(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
     V3 wh = normalized(wol + wil);
     return vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
@@ -287,7 +314,7 @@ YG_INLINE V3 sample_cosine(float u0, float u1) {                                
     return v3(r * cosf(phi), r * sinf(phi), z);
 }
 
-// u_scat selects the lobe of layered BSDFs (plastic: diffuse below 0.5, specular above).
+// u_scat selects the lobe of layered BSDFs (plastic, coated diffuse: diffuse below 0.5, specular above).
 YG_INLINE RGB bsdf_sample(const DevMaterial& m, const Shading& sc, float u0, float u1, float u_scat, V3 wo, V3& wi,
                           float& pdf) {
     if (m.type == 1) {
@@ -305,6 +332,18 @@ YG_INLINE RGB bsdf_sample(const DevMaterial& m, const Shading& sc, float u0, flo
         if (dot(sc.n, wi) <= 0.f) return rgb(0.f, 0.f, 0.f);
         pdf = plastic_pdf(m, sc, wo, wi);
         return plastic_eval(m, sc, wo, wi);
+    }
+    if (m.type == 6) {                                                                 // bsdf.cpp:462-479
+        if (u_scat < 0.5f) {
+            wi = to_world(sc, sample_cosine(u0, u1));
+        } else {
+            const V3 wol = to_local(sc, wo);
+            const V3 wh = vndf_sample(u0, u1, wol, m.alpha_x, m.alpha_y);
+            wi = to_world(sc, reflect(wol, wh));
+        }
+        if (dot(sc.n, wi) <= 0.f) return rgb(0.f, 0.f, 0.f);
+        pdf = coated_pdf(m, sc, wo, wi);
+        return coated_eval(m, sc, wo, wi);
     }
     V3 wol = to_local(sc, wo);
     V3 wh = vndf_sample(u0, u1, wol, m.alpha_x, m.alpha_y);

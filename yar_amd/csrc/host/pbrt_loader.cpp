@@ -321,6 +321,18 @@ struct Loader {
             m.specular_r0 = 0.2126729f * ks[0] + 0.7151522f * ks[1] + 0.0721750f * ks[2];   // Y of sRGB_to_XYZ(Ks)
             m.u_roughness = m.v_roughness = get_f(ms->params, "roughness", 0.1f);
             m.roughness_is_alpha = get_b(ms->params, "remaproughness", true) ? 0 : 1;
+        } else if (ms->type == "substrate") {                              // pbrt_scene.cpp:531-561
+            m.type = YAR_MATERIAL_COATED_DIFFUSE;
+            float kd[3] = {0.5f, 0.5f, 0.5f}, ks[3] = {0.5f, 0.5f, 0.5f};     // pbrt-parser SubstrateMaterial
+            for (const char* name : {"Kd", "Ks", "uroughness", "vroughness", "bumpmap"})
+                if (find(ms->params, name) && find(ms->params, name)->type == "texture")
+                    throw ParseError{std::string("textured substrate ") + name + " is not supported"};
+            get_rgb(ms->params, "Kd", kd);
+            get_rgb(ms->params, "Ks", ks);
+            for (int i = 0; i < 3; i++) { m.reflectance[i] = kd[i]; m.specular[i] = ks[i]; }
+            m.u_roughness = get_f(ms->params, "uroughness", 0.1f);
+            m.v_roughness = get_f(ms->params, "vroughness", 0.1f);
+            m.roughness_is_alpha = get_b(ms->params, "remaproughness", true) ? 0 : 1;
         } else {
             throw ParseError{"unsupported material type '" + ms->type + "'"};
         }
