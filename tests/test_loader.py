@@ -104,7 +104,7 @@ Shape "trianglemesh" "integer indices" [ 0 1 2 ] "point P" [ 0 0 1  1 0 1  0 1 1
 
 
 def test_unsupported_inputs_are_parse_errors(tmp_path):
-    for body in ['Material "substrate"\nShape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]\n',
+    for body in ['Material "translucent"\nShape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]\n',
                  'Material "glass" "float uroughness" [ 0.2 ]\nShape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]\n',
                  'Material "metal"\nShape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]\n']:
         with pytest.raises(yar_amd.YarError) as e:
