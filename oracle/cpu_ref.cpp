@@ -881,8 +881,79 @@ static float coated_pdf(const Material& m, const Shading& sc, V3 wo, V3 wi) {
     return 0.5f * (diffuse_pdf + spec_pdf);
 }
 
+// Rough_Glass_BSDF (bsdf.cpp:268-420) with scattering.cpp:65-150 and math.h:115-127.  Material:
+// refl = Kr, trans = Kt, ior, alpha_x = GGX alpha; enter/exit from the original normal orientation.
+static void rough_glass_etas(const Material& m, const Shading& sc, float* eta_o, float* eta_i) {
+    bool enter = !sc.flipped;
+    *eta_o = enter ? 1.f : m.ior;
+    *eta_i = enter ? m.ior : 1.f;
+}
+static float max_component(RGB c) { return std::max(c.r, std::max(c.g, c.b)); }
+static V3 refraction_half_direction(float eta_o, V3 wo, float eta_i, V3 wi, V3 n) {
+    V3 wh = -normalized(eta_o * wo + eta_i * wi);
+    if (dot(wh, n) < 0.f) wh = -wh;
+    return wh;
+}
+static bool refract(V3 w, V3 n, float eta, V3* out) {
+    float ci = dot(w, n);
+    float st2 = eta * eta * std::max(0.f, 1.f - ci * ci);
+    if (st2 >= 1.f) return false;
+    float ct = std::sqrt(1.f - st2);
+    *out = (-eta) * w + (eta * ci - ct) * n;
+    return true;
+}
+static float glass_reflection_probability(const Material& m, float fresnel) {
+    float max_r = max_component(m.refl), max_t = max_component(m.trans);
+    float ratio = max_r / (max_r + max_t);
+    float r = fresnel * ratio;
+    float t = (1.f - fresnel) * (1 - ratio);
+    return ((r + t) == 0) ? 0.f : r / (r + t);
+}
+static RGB rough_glass_eval(const Material& m, const Shading& sc, V3 wo, V3 wi) {
+    float eta_o, eta_i;
+    rough_glass_etas(m, sc, &eta_o, &eta_i);
+    V3 n = sc.n;
+    float a = m.alpha_x;
+    if (dot(wo, n) * dot(wi, n) > 0.f) {
+        V3 wh = normalized(wo + wi);
+        float F = dielectric_fresnel(dot(wi, wh), eta_i / eta_o);
+        float G = G_iso(wi, wo, n, a), D = D_iso(wh, n, a);
+        float base = (F * G * D) / (4.f * dot(wo, n) * dot(wi, n));
+        return m.refl * base;
+    }
+    V3 wh = refraction_half_direction(eta_o, wo, eta_i, wi, n);
+    float wow = dot(wo, wh), wiw = dot(wi, wh);
+    if (wow * wiw > 0.f) return RGB{};
+    float F = dielectric_fresnel(wiw, eta_o / eta_i);
+    if (F == 1.f) return RGB{};
+    float G = G_iso(wi, wo, n, a), D = D_iso(wh, n, a);
+    float k = std::fabs((wow * wiw) / (dot(wo, n) * dot(wi, n)));
+    float k2 = eta_o * wow + eta_i * wiw;
+    float base = (eta_o * eta_o * k * G * D * (1.f - F)) / (k2 * k2);
+    return m.trans * base;
+}
+static float rough_glass_pdf(const Material& m, const Shading& sc, V3 wo, V3 wi) {
+    float eta_o, eta_i;
+    rough_glass_etas(m, sc, &eta_o, &eta_i);
+    V3 n = sc.n;
+    float a = m.alpha_x;
+    if (dot(wo, n) * dot(wi, n) > 0.f) {
+        V3 wh = normalized(wo + wi);
+        float rpdf = vndf_wi_pdf_iso(wo, wh, n, a);
+        return rpdf * glass_reflection_probability(m, dielectric_fresnel(dot(wi, wh), eta_i / eta_o));
+    }
+    V3 wh = refraction_half_direction(eta_o, wo, eta_i, wi, n);
+    if (dot(wo, wh) * dot(wi, wh) > 0.f) return 0.f;
+    float rp = glass_reflection_probability(m, dielectric_fresnel(dot(wo, wh), eta_i / eta_o));
+    float wh_pdf = G1_iso(wo, n, a) * D_iso(wh, n, a) * std::max(0.f, dot(wo, wh)) / dot(wo, n);
+    float denom = eta_o * dot(wo, wh) + eta_i * dot(wi, wh);
+    float dwh_dwi = eta_i * eta_i * std::fabs(dot(wi, wh)) / (denom * denom);
+    return wh_pdf * dwh_dwi * (1.f - rp);
+}
+
 static RGB bsdf_eval(const Scene& s, const Shading& sc, V3 wo, V3 wi) {
     const Material& m = s.materials[sc.material];
+    if (m.type == YAR_MATERIAL_GLASS) return rough_glass_eval(m, sc, wo, wi);
     if (m.type == YAR_MATERIAL_DIFFUSE) return Pi_Inv * m.refl;                      // bsdf.cpp:68
     if (m.type == YAR_MATERIAL_PLASTIC) return plastic_eval(m, sc, wo, wi);
     if (m.type == YAR_MATERIAL_COATED_DIFFUSE) return coated_eval(m, sc, wo, wi);
@@ -899,6 +970,7 @@ static float bsdf_pdf(const Scene& s, const Shading& sc, V3 wo, V3 wi) {
     if (m.type == YAR_MATERIAL_DIFFUSE) return dot(sc.n, wi) / Pi;                   // bsdf.cpp:80
     if (m.type == YAR_MATERIAL_PLASTIC) return plastic_pdf(m, sc, wo, wi);
     if (m.type == YAR_MATERIAL_COATED_DIFFUSE) return coated_pdf(m, sc, wo, wi);
+    if (m.type == YAR_MATERIAL_GLASS) return rough_glass_pdf(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);                               // bsdf.cpp:190
     V3 wh = normalized(wol + wil);
     return vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
@@ -920,6 +992,23 @@ static RGB bsdf_sample(const Scene& s, const Shading& sc, V2 u, float u_scat, V3
         if (dot(sc.n, *wi) <= 0.f) return RGB{};
         *pdf = plastic_pdf(m, sc, wo, *wi);
         return plastic_eval(m, sc, wo, *wi);
+    }
+    if (m.type == YAR_MATERIAL_GLASS) {                                             // bsdf.cpp:339-376
+        float eta_o, eta_i;
+        rough_glass_etas(m, sc, &eta_o, &eta_i);
+        V3 wh = to_world(sc, vndf_sample(u, to_local(sc, wo), m.alpha_x, m.alpha_x));
+        V3 rwi = reflect(wo, wh);
+        if (dot(rwi, sc.n) <= 0.f) return RGB{};
+        float rp = glass_reflection_probability(m, dielectric_fresnel(dot(wo, wh), eta_i / eta_o));
+        if (u_scat < rp) {
+            *wi = rwi;
+        } else {
+            if (!refract(wo, wh, eta_o / eta_i, wi)) return RGB{};
+            if (dot(*wi, sc.n) >= 0.f) return RGB{};
+        }
+        *pdf = rough_glass_pdf(m, sc, wo, *wi);
+        if (*pdf == 0.f) return RGB{};
+        return rough_glass_eval(m, sc, wo, *wi);
     }
     if (m.type == YAR_MATERIAL_COATED_DIFFUSE) {                                    // bsdf.cpp:462-479
         if (u_scat < 0.5f) {
@@ -979,7 +1068,8 @@ static RGB light_rect(const Scene& s, const Shading& sc, int li, V2 ul, V2 ub, f
         float lndw = dot(ln, -wi);
         if (lndw > 1e-4f) {
             float ndw = dot(sc.n, wi);
-            if (ndw > 0.f) {   // reflection-only BSDFs (Diffuse_BRDF, Metal_BRDF)
+            const bool rough_glass = s.materials[sc.material].type == YAR_MATERIAL_GLASS;
+            if (ndw > 0.f || (ndw < 0.f && rough_glass)) {   // scattering_possible (direct_lighting.cpp:133-135)
                 RGB f = bsdf_eval(s, sc, sc.wo, wi);
                 if (!is_black(f)) {
                     pc.any++;
@@ -1628,6 +1718,38 @@ int32_t oref_coated_sample(const float* kd, const float* r0, float ax, float ay,
     m.alpha_x = ax; m.alpha_y = ay;
     s.materials.push_back(m);
     Shading sc = probe_frame();
+    V3 wi;
+    float pdf = 0.f;
+    const RGB f = bsdf_sample(s, sc, V2{u3[0], u3[1]}, u3[2], V3(wo[0], wo[1], wo[2]), &wi, &pdf);
+    wi_out[0] = wi.x; wi_out[1] = wi.y; wi_out[2] = wi.z;
+    f_out[0] = f.r; f_out[1] = f.g; f_out[2] = f.b;
+    *pdf_out = pdf;
+    return (f.r != 0.f || f.g != 0.f || f.b != 0.f) ? 1 : 0;
+}
+
+// Rough glass BSDF in the local frame (n = z): evaluate + pdf, or sample; flipped = leaving the dielectric.
+static Material rough_glass_probe(float ior, float alpha, const float* kr, const float* kt) {
+    Material m{};
+    m.type = YAR_MATERIAL_GLASS; m.ior = ior; m.alpha_x = m.alpha_y = alpha; m.roughness = 1.f;
+    m.refl = RGB{kr[0], kr[1], kr[2]}; m.trans = RGB{kt[0], kt[1], kt[2]};
+    return m;
+}
+void oref_rough_glass_eval(float ior, float alpha, const float* kr, const float* kt, int32_t flipped, const float* wo,
+                           const float* wi, float* f_out, float* pdf_out) {
+    Material m = rough_glass_probe(ior, alpha, kr, kt);
+    Shading sc = probe_frame();
+    sc.flipped = flipped != 0;
+    const V3 o(wo[0], wo[1], wo[2]), i(wi[0], wi[1], wi[2]);
+    const RGB f = rough_glass_eval(m, sc, o, i);
+    f_out[0] = f.r; f_out[1] = f.g; f_out[2] = f.b;
+    *pdf_out = rough_glass_pdf(m, sc, o, i);
+}
+int32_t oref_rough_glass_sample(float ior, float alpha, const float* kr, const float* kt, int32_t flipped,
+                                const float* wo, const float* u3, float* wi_out, float* f_out, float* pdf_out) {
+    Scene s;
+    s.materials.push_back(rough_glass_probe(ior, alpha, kr, kt));
+    Shading sc = probe_frame();
+    sc.flipped = flipped != 0;
     V3 wi;
     float pdf = 0.f;
     const RGB f = bsdf_sample(s, sc, V2{u3[0], u3[1]}, u3[2], V3(wo[0], wo[1], wo[2]), &wi, &pdf);

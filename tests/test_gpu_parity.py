@@ -142,3 +142,21 @@ def test_render_coated_diffuse_matches_oracle(c1_path, oracle, tmp_path):
     assert mean_rel <= 1e-3
     assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
     assert (film["weight_sum"] == ofilm["weight_sum"]).all()
+
+
+def test_render_rough_glass_matches_oracle(c1_path, oracle, tmp_path):
+    # the config-1 room with rough glass triangles and a rough glass wall (Rough_Glass_BSDF,
+    # bsdf.cpp:268-420: GGX reflection and transmission, light sampled through the surface)
+    lines = open(c1_path).read().splitlines()
+    mats = [i for i, l in enumerate(lines) if l.startswith('Material "matte"')]
+    lines[mats[3]] = 'Material "glass" "float uroughness" [ 0.3 ] "float vroughness" [ 0.3 ]'
+    lines[mats[-1]] = ('Material "glass" "rgb Kr" [ 0.9 0.9 0.9 ] "rgb Kt" [ 0.8 0.9 1.0 ] "float index" [ 1.4 ] '
+                       '"float uroughness" [ 0.15 ] "float vroughness" [ 0.15 ]')
+    p = tmp_path / "c1_rough_glass.pbrt"
+    p.write_text("\n".join(lines) + "\n")
+    img, film, var, st, oimg, ofilm, ovar = _render_pair(str(p), oracle, nx=4)
+    exact, close, mean_rel = _compare_images(img, oimg, film, ofilm, "c1 rough glass 16spp")
+    assert close >= 0.99
+    assert mean_rel <= 1e-3
+    assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
+    assert (film["weight_sum"] == ofilm["weight_sum"]).all()

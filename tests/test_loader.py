@@ -227,3 +227,19 @@ Shape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]
 '''
     with pytest.raises(yar_amd.YarError):
         _load(tmp_path, bad)
+
+
+def test_rough_glass_material(tmp_path):
+    # glass with u = v roughness > 0 -> Rough_Glass_BSDF (pbrt_scene.cpp:449-462); u != v is rejected
+    body = LIGHT + '''Material "glass" "float uroughness" [ 0.2 ] "float vroughness" [ 0.2 ] "float index" [ 1.33 ]
+Shape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]
+'''
+    d = _load(tmp_path, body).desc
+    m = d.materials[0]
+    assert m.type == 3 and m.u_roughness == pytest.approx(0.2) and m.v_roughness == pytest.approx(0.2)
+    assert m.index_of_refraction == pytest.approx(1.33) and m.roughness_is_alpha == 0
+    bad = LIGHT + '''Material "glass" "float uroughness" [ 0.2 ] "float vroughness" [ 0.1 ]
+Shape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]
+'''
+    with pytest.raises(yar_amd.YarError):
+        _load(tmp_path, bad)

@@ -591,3 +591,98 @@ def test_coated_diffuse_sampling_matches_quadrature_and_reciprocity(ax, ay):
         est /= n
         assert est == pytest.approx(alb, rel=0.03), (theta_o, est, alb)
         assert mass == pytest.approx(valid / n, abs=0.02), (theta_o, mass, valid / n)
+
+
+def rough_glass(ior, alpha, wo, wi=None, u=None, flipped=0, kr=(1, 1, 1), kt=(1, 1, 1)):
+    L = O.lib()
+    P = C.c_void_p
+    L.oref_rough_glass_eval.argtypes = [C.c_float, C.c_float, P, P, C.c_int32, P, P, P, P]
+    L.oref_rough_glass_sample.argtypes = [C.c_float, C.c_float, P, P, C.c_int32, P, P, P, P, P]
+    L.oref_rough_glass_sample.restype = C.c_int32
+    kr, kt, wo = _f32(kr), _f32(kt), _f32(wo)
+    f = np.zeros(3, np.float32)
+    pdf = C.c_float()
+    if wi is not None:
+        wi = _f32(wi)
+        L.oref_rough_glass_eval(ior, alpha, kr.ctypes.data, kt.ctypes.data, flipped, wo.ctypes.data, wi.ctypes.data,
+                                f.ctypes.data, C.byref(pdf))
+        return f.astype(np.float64), pdf.value
+    u = _f32(u)
+    out = np.zeros(3, np.float32)
+    ok = L.oref_rough_glass_sample(ior, alpha, kr.ctypes.data, kt.ctypes.data, flipped, wo.ctypes.data, u.ctypes.data,
+                                   out.ctypes.data, f.ctypes.data, C.byref(pdf))
+    return bool(ok), out.astype(np.float64), f.astype(np.float64), pdf.value
+
+
+@pytest.mark.parametrize("flipped", [0, 1], ids=["entering", "leaving"])
+def test_rough_glass_smooth_limit_is_fresnel_with_radiance_scaling(flipped):
+    # As alpha -> 0 the sampled albedo of Rough_Glass_BSDF (Kr = Kt = 1) tends to the smooth
+    # interface's F + (1 - F) (eta_o / eta_i)^2 (radiance scaling of the transmitted part), with F
+    # the unpolarised Fresnel reflectance, and to 1 past the critical angle.
+    ior = 1.5
+    eta_o, eta_i = (1.0, ior) if not flipped else (ior, 1.0)
+    rng = np.random.default_rng(23)
+    for theta in (0.2, 0.6, 1.0, 1.3):
+        wo = _hemi(theta, 0.3)
+        F = _textbook_fresnel(np.cos(theta), eta_o, eta_i)
+        want = F + (1 - F) * (eta_o / eta_i) ** 2
+        n = 4000
+        est = 0.0
+        for u in rng.uniform(size=(n, 3)):
+            ok, wi, f, pdf = rough_glass(ior, 0.003, wo, u=u, flipped=flipped)
+            if ok:
+                est += f[1] * abs(wi[2]) / pdf
+        est /= n
+        assert est == pytest.approx(want, rel=0.03), (theta, est, want, F)
+
+
+@pytest.mark.parametrize("flipped,alpha", [(0, 0.2), (1, 0.2), (0, 0.45)])
+def test_rough_glass_generalised_reciprocity_and_sampling_bounds(flipped, alpha):
+    ior = 1.5
+    eta_o, eta_i = (1.0, ior) if not flipped else (ior, 1.0)
+    rng = np.random.default_rng(29)
+    prev_gap = None
+    for theta_o in (0.35, 0.9):
+        wo = _hemi(theta_o, 1.1)
+        # transmission: f(wo -> wi) seen from the other side is scaled by (eta_i / eta_o)^2
+        # (Walter et al. 2007, eq. 21 in the radiance convention); reflection is reciprocal
+        for _ in range(20):
+            wt = _hemi(rng.uniform(0.1, 1.3), rng.uniform(0, 2 * np.pi)) * np.array([1, 1, -1])
+            f1, _ = rough_glass(ior, alpha, wo, wt, flipped=flipped)
+            f2, _ = rough_glass(ior, alpha, wt * np.array([1, 1, -1]), wo * np.array([1, 1, -1]), flipped=1 - flipped)
+            if f1[1] > 0:
+                assert f2[1] / f1[1] == pytest.approx((eta_i / eta_o) ** 2, rel=1e-3)   # float32 Fresnel and half-vector
+            wr = _hemi(rng.uniform(0.1, 1.3), rng.uniform(0, 2 * np.pi))
+            g1, _ = rough_glass(ior, alpha, wo, wr, flipped=flipped)
+            g2, _ = rough_glass(ior, alpha, wr, wo, flipped=flipped)
+            assert np.allclose(g1, g2, rtol=1e-5)
+        # Rough_Glass_BSDF::sample rejects a microfacet normal whose mirror direction falls below the
+        # surface before it chooses reflection or refraction (bsdf.cpp:341-344), so its samples cover
+        # less than the support of pdf(): the sampled albedo and the valid-sample fraction are bounded
+        # by the quadratures of f|cos| and of pdf over the sphere (equality in the smooth limit, see
+        # the test above)
+        nt, nphi = 300, 100
+        ct = (np.arange(2 * nt) + 0.5) / nt - 1
+        ph = (np.arange(nphi) + 0.5) / nphi * 2 * np.pi
+        alb, mass = 0.0, 0.0
+        for c in ct:
+            st = np.sqrt(1 - c * c)
+            for p in ph:
+                wi = np.array([st * np.cos(p), st * np.sin(p), c])
+                f, pdf = rough_glass(ior, alpha, wo, wi, flipped=flipped)
+                alb += f[1] * abs(c)
+                mass += pdf
+        dA = (1.0 / nt) * (2 * np.pi / nphi)
+        alb *= dA
+        mass *= dA
+        n = 6000
+        est, valid = 0.0, 0
+        for u in rng.uniform(size=(n, 3)):
+            ok, wi, f, pdf = rough_glass(ior, alpha, wo, u=u, flipped=flipped)
+            if ok:
+                est += f[1] * abs(wi[2]) / pdf
+                valid += 1
+        est /= n
+        assert mass <= 1.02
+        assert est <= alb * 1.03 and valid / n <= mass + 0.02, (theta_o, est, alb, valid / n, mass)
+        assert est > 0.5 * alb

@@ -408,7 +408,7 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
             g.trans = make_float4(m.specular_r0, d->config.pbrt3_scene ? 1.f : 0.f, 0.f, 0.f);
         if (m.type == YAR_MATERIAL_COATED_DIFFUSE)               // k = Fresnel r0 per channel
             g.k = make_float4(m.specular[0], m.specular[1], m.specular[2], 0.f);
-        if (m.type == YAR_MATERIAL_GLASS && m.u_roughness > 0.f) throw HipError{"rough glass is not supported", YAR_ERR_UNSUPPORTED};
+        if (m.type == YAR_MATERIAL_GLASS && m.u_roughness > 0.f) g.type = kRoughGlassType;   // Rough_Glass_BSDF
         if (m.type < YAR_MATERIAL_DIFFUSE || m.type > YAR_MATERIAL_COATED_DIFFUSE) throw HipError{"unsupported material type", YAR_ERR_UNSUPPORTED};
     }
     std::vector<DevRect> rl(std::max(1, d->rect_light_count));

@@ -802,7 +802,7 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
                             const float lndw = dot(ln, -wi);
                             if (lndw > 1e-4f) {
                                 const float ndw = dot(sc.n, wi);
-                                if (ndw > 0.f) {
+                                if (scattering_possible(m.type, ndw)) {
                                     const RGB f = bsdf_eval(m, sc, sc.wo, wi);
                                     if (!is_black(f)) {
                                         const float lpdf = (dd * dd) / (R.size.x * R.size.y * lndw);

@@ -10,6 +10,12 @@
 
 namespace yg {
 
+// Whether a BSDF can scatter light arriving from direction wi: reflection above the shading
+// normal, transmission below it (direct_lighting.cpp:133-135).
+YG_INLINE bool scattering_possible(int type, float n_dot_wi) {
+    return n_dot_wi > 0.f || (n_dot_wi < 0.f && type == kRoughGlassType);
+}
+
 YG_INLINE float power_heuristic(float a, float b) {             // direct_lighting.cpp:18
     if (a == f_inf()) return 1.f;
     return a * a / (a * a + b * b);
@@ -287,8 +293,80 @@ YG_INLINE float coated_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi
     return 0.5f * (diffuse_pdf + spec_pdf);
 }
 
+// Rough_Glass_BSDF (bsdf.cpp:268-420; scattering.cpp:65-150): GGX reflection and transmission
+// through a dielectric boundary.  Material: refl = Kr, trans = Kt with w = index of refraction,
+// alpha_x = GGX alpha.  The side the ray enters from follows the original shading-normal
+// orientation (pbrt scenes do not track nested dielectrics).
+YG_INLINE void rough_glass_etas(const DevMaterial& m, const Shading& sc, float& eta_o, float& eta_i) {
+    const bool enter = !sc.flipped;
+    eta_o = enter ? 1.f : m.trans.w;
+    eta_i = enter ? m.trans.w : 1.f;
+}
+YG_INLINE float max_component(RGB c) { return fmaxf(c.r, fmaxf(c.g, c.b)); }
+YG_INLINE V3 refraction_half(float eta_o, V3 wo, float eta_i, V3 wi, V3 n) {
+    V3 wh = -normalized(eta_o * wo + eta_i * wi);
+    if (dot(wh, n) < 0.f) wh = -wh;
+    return wh;
+}
+YG_INLINE bool refract_dir(V3 w, V3 n, float eta, V3& out) {                          // math.h:115
+    const float ci = dot(w, n);
+    const float st2 = eta * eta * fmaxf(0.f, 1.f - ci * ci);
+    if (st2 >= 1.f) return false;
+    const float ct = sqrtf(1.f - st2);
+    out = (-eta) * w + (eta * ci - ct) * n;
+    return true;
+}
+YG_INLINE float reflection_probability(const DevMaterial& m, float fresnel) {
+    const float max_r = max_component(rgb4(m.refl)), max_t = max_component(rgb4(m.trans));
+    const float ratio = max_r / (max_r + max_t);
+    const float r = fresnel * ratio, t = (1.f - fresnel) * (1 - ratio);
+    return ((r + t) == 0) ? 0.f : r / (r + t);
+}
+YG_INLINE RGB rough_glass_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
+    float eta_o, eta_i;
+    rough_glass_etas(m, sc, eta_o, eta_i);
+    const V3 n = sc.n;
+    const float a = m.alpha_x;
+    if (dot(wo, n) * dot(wi, n) > 0.f) {
+        const V3 wh = normalized(wo + wi);
+        const float F = dielectric_fresnel(dot(wi, wh), eta_i / eta_o);
+        const float G = G_iso(wi, wo, n, a), D = D_iso(wh, n, a);
+        const float base = (F * G * D) / (4.f * dot(wo, n) * dot(wi, n));
+        return rgb4(m.refl) * base;
+    }
+    const V3 wh = refraction_half(eta_o, wo, eta_i, wi, n);
+    const float wow = dot(wo, wh), wiw = dot(wi, wh);
+    if (wow * wiw > 0.f) return rgb(0.f, 0.f, 0.f);
+    const float F = dielectric_fresnel(wiw, eta_o / eta_i);
+    if (F == 1.f) return rgb(0.f, 0.f, 0.f);
+    const float G = G_iso(wi, wo, n, a), D = D_iso(wh, n, a);
+    const float k = fabsf((wow * wiw) / (dot(wo, n) * dot(wi, n)));
+    const float k2 = eta_o * wow + eta_i * wiw;
+    const float base = (eta_o * eta_o * k * G * D * (1.f - F)) / (k2 * k2);
+    return rgb4(m.trans) * base;
+}
+YG_INLINE float rough_glass_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
+    float eta_o, eta_i;
+    rough_glass_etas(m, sc, eta_o, eta_i);
+    const V3 n = sc.n;
+    const float a = m.alpha_x;
+    if (dot(wo, n) * dot(wi, n) > 0.f) {
+        const V3 wh = normalized(wo + wi);
+        const float rpdf = vndf_wi_pdf_iso(wo, wh, n, a);
+        return rpdf * reflection_probability(m, dielectric_fresnel(dot(wi, wh), eta_i / eta_o));
+    }
+    const V3 wh = refraction_half(eta_o, wo, eta_i, wi, n);
+    if (dot(wo, wh) * dot(wi, wh) > 0.f) return 0.f;
+    const float rp = reflection_probability(m, dielectric_fresnel(dot(wo, wh), eta_i / eta_o));
+    const float wh_pdf = G1_iso(wo, n, a) * D_iso(wh, n, a) * fmaxf(0.f, dot(wo, wh)) / dot(wo, n);
+    const float denom = eta_o * dot(wo, wh) + eta_i * dot(wi, wh);
+    const float dwh_dwi = eta_i * eta_i * fabsf(dot(wi, wh)) / (denom * denom);
+    return wh_pdf * dwh_dwi * (1.f - rp);
+}
+
 YG_INLINE RGB bsdf_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
     if (m.type == 1) return kPiInv * rgb4(m.refl);
+    if (m.type == kRoughGlassType) return rough_glass_eval(m, sc, wo, wi);
     if (m.type == 5) return plastic_eval(m, sc, wo, wi);
     if (m.type == 6) return coated_eval(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
@@ -303,6 +381,7 @@ YG_INLINE float bsdf_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) 
     if (m.type == 1) return dot(sc.n, wi) / kPi;
     if (m.type == 5) return plastic_pdf(m, sc, wo, wi);
     if (m.type == 6) return coated_pdf(m, sc, wo, wi);
+    if (m.type == kRoughGlassType) return rough_glass_pdf(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
     V3 wh = normalized(wol + wil);
     return vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
@@ -332,6 +411,23 @@ YG_INLINE RGB bsdf_sample(const DevMaterial& m, const Shading& sc, float u0, flo
         if (dot(sc.n, wi) <= 0.f) return rgb(0.f, 0.f, 0.f);
         pdf = plastic_pdf(m, sc, wo, wi);
         return plastic_eval(m, sc, wo, wi);
+    }
+    if (m.type == kRoughGlassType) {                                                   // bsdf.cpp:339-376
+        float eta_o, eta_i;
+        rough_glass_etas(m, sc, eta_o, eta_i);
+        const V3 wh = to_world(sc, vndf_sample(u0, u1, to_local(sc, wo), m.alpha_x, m.alpha_x));
+        const V3 rwi = reflect(wo, wh);
+        if (dot(rwi, sc.n) <= 0.f) return rgb(0.f, 0.f, 0.f);
+        const float rp = reflection_probability(m, dielectric_fresnel(dot(wo, wh), eta_i / eta_o));
+        if (u_scat < rp) {
+            wi = rwi;
+        } else {
+            if (!refract_dir(wo, wh, eta_o / eta_i, wi)) return rgb(0.f, 0.f, 0.f);
+            if (dot(wi, sc.n) >= 0.f) return rgb(0.f, 0.f, 0.f);
+        }
+        pdf = rough_glass_pdf(m, sc, wo, wi);
+        if (pdf == 0.f) return rgb(0.f, 0.f, 0.f);
+        return rough_glass_eval(m, sc, wo, wi);
     }
     if (m.type == 6) {                                                                 // bsdf.cpp:462-479
         if (u_scat < 0.5f) {

@@ -302,7 +302,8 @@ struct Loader {
             m.u_roughness = get_f(ms->params, "uroughness", 0.f);
             m.v_roughness = get_f(ms->params, "vroughness", 0.f);
             m.roughness_is_alpha = get_b(ms->params, "remaproughness", true) ? 0 : 1;
-            if (m.u_roughness > 0.f) throw ParseError{"rough glass (Rough_Glass_BSDF) is not supported"};
+            if (m.u_roughness != m.v_roughness)                         // pbrt_scene.cpp:450
+                throw ParseError{"glass with different u and v roughness is not supported"};
         } else if (ms->type == "mirror") {                                 // pbrt_scene.cpp:443-447
             m.type = YAR_MATERIAL_MIRROR;
             m.reflectance[0] = m.reflectance[1] = m.reflectance[2] = 0.9f;    // pbrt-parser MirrorMaterial kr
