@@ -86,7 +86,8 @@ typedef struct yar_material_desc {
 typedef struct yar_object_desc {        /* Scene_Object, src/lib/scene_object.h:8 */
     int32_t mesh;                       /* index into meshes */
     int32_t material;                   /* index into materials, -1 for area lights */
-    int32_t area_light;                 /* index into rect_lights, -1 if none */
+    int32_t area_light;                 /* index into rect_lights, rect_light_count + i for
+                                           mesh_lights[i], -1 if none */
     int32_t reserved;
     float   object_to_world[12];
     float   world_to_object[12];
@@ -103,6 +104,27 @@ typedef struct yar_point_light_desc {   /* Point_Light, src/lib/light.h:33 */
     float position[3];
     float intensity[3];
 } yar_point_light_desc;
+
+typedef struct yar_spot_light_desc {    /* Spot_Light, src/lib/light.h:40 */
+    float position[3];
+    float direction[3];                 /* unit axis, from the light towards what it lights */
+    float cone_angle;                   /* radians */
+    float penumbra_angle;               /* radians, measured inwards from the cone edge */
+    float intensity[3];
+} yar_spot_light_desc;
+
+typedef struct yar_directional_light_desc {   /* Directional_Light, src/lib/light.h:48 */
+    float direction[3];                 /* unit vector towards the light */
+    float irradiance[3];                /* on a surface perpendicular to direction */
+} yar_directional_light_desc;
+
+/* Diffuse_Triangle_Mesh_Light (src/lib/light.h:85): a one-sided emitter mesh.  Light samples are
+ * taken in the mesh's own coordinates, as Diffuse_Triangle_Mesh_Light_Sampler does
+ * (light_sampling.cpp:126-184), so the emitter object's transform must be the identity. */
+typedef struct yar_mesh_light_desc {
+    int32_t object;                     /* the emitter object (area_light = rect_light_count + index) */
+    float   emitted_radiance[3];
+} yar_mesh_light_desc;
 
 typedef struct yar_render_config {      /* Raytracer_Config, src/lib/raytracer_config.h:5 */
     int32_t film_width, film_height;
@@ -132,14 +154,18 @@ typedef struct yar_scene_desc {
     const yar_point_light_desc* point_lights; int32_t point_light_count;
     yar_camera_desc   camera;
     yar_render_config config;
+    const yar_spot_light_desc*        spot_lights;        int32_t spot_light_count;
+    const yar_directional_light_desc* directional_lights; int32_t directional_light_count;
+    const yar_mesh_light_desc*        mesh_lights;        int32_t mesh_light_count;
 } yar_scene_desc;
 
 /* ---- scene loading / generation -------------------------------------------------------- */
 typedef struct yar_scene yar_scene;
 
-/* Loads the pbrt3 subset the RAY path tracer consumes (trianglemesh, matte/metal/glass,
- * diffuse area lights, point lights, perspective camera, Film, Sampler, Integrator "path",
- * PixelFilter "box") with the reference's import semantics. */
+/* Loads the pbrt3 subset the RAY path tracer consumes (trianglemesh; matte, metal, glass, mirror,
+ * plastic, substrate; diffuse area lights (rectangles and triangle meshes), point, spot and distant
+ * lights; perspective camera, Film, Sampler, Integrator "path", PixelFilter "box") with the
+ * reference's import semantics. */
 int  yar_scene_load_pbrt(const char* path, yar_scene** out_scene);
 const yar_scene_desc* yar_scene_get_desc(const yar_scene* scene);
 void yar_scene_free(yar_scene* scene);

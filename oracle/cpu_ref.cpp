@@ -447,6 +447,11 @@ struct Material {
 };
 struct RectLight { M34 l2w; RGB L; V2 size; };
 struct PointLight { V3 p; RGB I; };
+struct SpotLight { V3 p, dir; float cone_angle, penumbra_angle; RGB I; };     // light.h:40
+struct DirectionalLight { V3 dir; RGB E; };                                   // light.h:48
+// Diffuse_Triangle_Mesh_Light + its sampler (light.h:85, light_sampling.h:48): the emitter mesh,
+// its area and the area CDF of Distribution_1D (sampling.cpp:149-166)
+struct MeshLight { int object, mesh; RGB L; float area; std::vector<float> cdf; };
 
 struct Counters { int64_t nodes = 0, leaf_refs = 0, tri_tests = 0, objects = 0; };
 
@@ -456,6 +461,9 @@ struct Scene {
     std::vector<Material> materials;
     std::vector<RectLight> rects;
     std::vector<PointLight> points;
+    std::vector<SpotLight> spots;
+    std::vector<DirectionalLight> dirs;
+    std::vector<MeshLight> mesh_lights;
     Tree scene_tree;
     yar_camera_desc camera;
     yar_render_config config;
@@ -1041,6 +1049,14 @@ static inline float power_heuristic(float a, float b) {
 
 struct PathCounters { int64_t closest = 0, any = 0; Counters trav; };
 
+// get_emitted_radiance (direct_lighting.cpp:388-410): rectangles emit on both sides, triangle-mesh
+// lights only where the original shading normal faces the viewer
+static RGB area_light_emission(const Scene& s, const Shading& sc) {
+    if (sc.light < (int)s.rects.size()) return s.rects[sc.light].L;
+    if (sc.flipped) return RGB{};
+    return s.mesh_lights[sc.light - (int)s.rects.size()].L;
+}
+
 static RGB light_point(const Scene& s, const Shading& sc, const PointLight& L, PathCounters& pc, Counters* tc) {
     V3 position = sc.origin_point(L.p);
     V3 lv = L.p - position;
@@ -1110,11 +1126,145 @@ static RGB light_rect(const Scene& s, const Shading& sc, int li, V2 ul, V2 ub, f
     return out;
 }
 
+static bool scattering_possible(const Scene& s, const Shading& sc, float n_dot_wi) {   // direct_lighting.cpp:133-135
+    const bool transmission = s.materials[sc.material].type == YAR_MATERIAL_GLASS;       // Rough_Glass_BSDF
+    return n_dot_wi > 0.f || (n_dot_wi < 0.f && transmission);
+}
+
+static RGB light_spot(const Scene& s, const Shading& sc, const SpotLight& L, PathCounters& pc, Counters* tc) {
+    V3 position = sc.origin_point(L.p);                                      // direct_lighting.cpp:49-89
+    V3 lv = L.p - position;
+    float dist = length(lv);
+    V3 wi = vdiv(lv, dist);
+    float cone_cos = std::cos(L.cone_angle);
+    float wi_cos = dot(-wi, L.dir);
+    if (wi_cos < cone_cos) return RGB{};
+    float penumbra = 1.f;
+    float penumbra_cos = std::cos(std::max(0.f, L.cone_angle - L.penumbra_angle));
+    if (wi_cos < penumbra_cos) {
+        float k = (wi_cos - cone_cos) / (penumbra_cos - cone_cos);
+        penumbra = (k * k) * (k * k);
+    }
+    float ndw = dot(sc.n, wi);
+    if (!scattering_possible(s, sc, ndw)) return RGB{};
+    RGB f = bsdf_eval(s, sc, sc.wo, wi);
+    if (is_black(f)) return RGB{};
+    pc.any++;
+    if (intersect_any(s, Ray{position, wi}, dist * (1.f - 1e-5f), tc)) return RGB{};
+    return (L.I * f) * (penumbra * std::fabs(ndw) / (dist * dist));
+}
+
+static RGB light_directional(const Scene& s, const Shading& sc, const DirectionalLight& L, PathCounters& pc, Counters* tc) {
+    float ndl = dot(sc.n, L.dir);                                            // direct_lighting.cpp:91-107
+    if (ndl <= 0.f) return RGB{};
+    V3 position = sc.origin_dir(L.dir);
+    pc.any++;
+    if (intersect_any(s, Ray{position, L.dir}, Inf, tc)) return RGB{};
+    RGB f = bsdf_eval(s, sc, sc.wo, L.dir);
+    return (L.E * f) * ndl;
+}
+
+// Distribution_1D::sample (sampling.cpp:98-147, 168-172): the sample in [0,1) and the remapped u
+static float sample_cdf(const std::vector<float>& cdf, float u, float* remapped) {
+    const int n = (int)cdf.size();
+    const float interval_length = 1.f / n;
+    const int k = (int)(std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin());
+    const float cdf_a = (k == 0) ? 0.f : cdf[k - 1];
+    const float cdf_b = cdf[k];
+    const float fraction = (u - cdf_a) / (cdf_b - cdf_a);
+    const float x = (float(k) + fraction) * interval_length;
+    *remapped = std::min(fraction, One_Minus_Epsilon);
+    return std::min(x, One_Minus_Epsilon);
+}
+
+// Emitter normal of a mesh light at (triangle, barycentrics): the interpolated vertex normal when the
+// mesh has normals (not normalised), else the geometric normal (light_sampling.cpp:137-142, 168-173)
+static V3 mesh_light_normal(const Mesh& m, uint32_t tri, V3 b) {
+    if (!m.n.empty()) {
+        V3 nv[3];
+        for (int k = 0; k < 3; k++) { const float* q = &m.n[3 * (size_t)m.idx[3 * (size_t)tri + k]]; nv[k] = {q[0], q[1], q[2]}; }
+        return b.x * nv[0] + b.y * nv[1] + b.z * nv[2];
+    }
+    V3 p[3]; m.pos(tri, p);
+    V3 ng = normalized(cross(p[1] - p[0], p[2] - p[0]));
+    return m.reverse ? -ng : ng;
+}
+
+static RGB light_mesh(const Scene& s, const Shading& sc, int li, V2 ul, V2 ub, float us, PathCounters& pc, Counters* tc) {
+    const MeshLight& ML = s.mesh_lights[li];
+    const Mesh& m = s.meshes[ML.mesh];
+    const int ntri = (int)(m.idx.size() / 3);
+    RGB out;
+    {   // Diffuse_Triangle_Mesh_Light_Sampler::sample (light_sampling.cpp:126-161)
+        float u0;
+        float smp = sample_cdf(ML.cdf, ul.x, &u0);
+        uint32_t tri = std::min(uint32_t(smp * ntri), (uint32_t)ntri - 1);
+        float a = std::sqrt(u0);                                             // sampling.cpp:69-77
+        V3 b;
+        b.x = 1 - a;
+        b.y = ul.y * a;
+        b.z = 1.f - b.x - b.y;
+        V3 p[3]; m.pos(tri, p);
+        V3 lp = b.x * p[0] + b.y * p[1] + b.z * p[2];
+        V3 ln = mesh_light_normal(m, tri, b);
+        V3 lv0 = lp - sc.pos;
+        float dsq = dot(lv0, lv0);
+        V3 w0 = vdiv(lv0, std::sqrt(dsq));
+        float lndw0 = dot(ln, -w0);
+        float lpdf = lndw0 <= 0.f ? 0.f : dsq / (lndw0 * ML.area);
+        if (lpdf != 0.f) {                                                   // direct_lighting.cpp:272-297
+            V3 position = sc.origin_point(lp);
+            V3 lv = lp - position;
+            float d = length(lv);
+            V3 wi = vdiv(lv, d);
+            float ndw = dot(sc.n, wi);
+            if (scattering_possible(s, sc, ndw)) {
+                RGB f = bsdf_eval(s, sc, sc.wo, wi);
+                if (!is_black(f)) {
+                    pc.any++;
+                    if (!intersect_any(s, Ray{position, wi}, d * (1.f - 1e-5f), tc)) {
+                        float bpdf = bsdf_pdf(s, sc, sc.wo, wi);
+                        float w = power_heuristic(lpdf, bpdf);
+                        out = out + (ML.L * f) * (w * std::fabs(ndw) / lpdf);
+                    }
+                }
+            }
+        }
+    }
+    {   // BSDF sampling half (direct_lighting.cpp:299-321, light_sampling.cpp:163-184)
+        V3 wi; float bpdf;
+        RGB f = bsdf_sample(s, sc, ub, us, sc.wo, &wi, &bpdf);
+        if (!is_black(f)) {
+            V3 position = sc.origin_dir(wi);
+            Hit h{Inf, -1, 0, {}};
+            pc.closest++;
+            bool found = intersect(s, Ray{position, wi}, h, tc);
+            if (found && s.objects[h.obj].light == (int)s.rects.size() + li) {
+                V3 ln = mesh_light_normal(m, h.tri, h.b);
+                float lndw = dot(ln, -wi);
+                float lpdf = lndw <= 0.f ? 0.f : (h.t * h.t) / (lndw * ML.area);
+                float w = power_heuristic(bpdf, lpdf);
+                out = out + (ML.L * f) * (w * std::fabs(dot(sc.n, wi)) / bpdf);
+            }
+        }
+    }
+    return out;
+}
+
+// estimate_direct_lighting_from_single_sample (direct_lighting.cpp:522-578): lights in the order
+// point, spot, directional, rectangular, triangle mesh
 static RGB direct_single(const Scene& s, const Shading& sc, float u_sel, V2 ul, V2 ub, float us, PathCounters& pc, Counters* tc) {
     int li = int(u_sel * s.total_lights);
-    if (li < (int)s.points.size()) return (float)s.total_lights * light_point(s, sc, s.points[li], pc, tc);
+    const float n = (float)s.total_lights;
+    if (li < (int)s.points.size()) return n * light_point(s, sc, s.points[li], pc, tc);
     li -= (int)s.points.size();
-    return (float)s.total_lights * light_rect(s, sc, li, ul, ub, us, pc, tc);
+    if (li < (int)s.spots.size()) return n * light_spot(s, sc, s.spots[li], pc, tc);
+    li -= (int)s.spots.size();
+    if (li < (int)s.dirs.size()) return n * light_directional(s, sc, s.dirs[li], pc, tc);
+    li -= (int)s.dirs.size();
+    if (li < (int)s.rects.size()) return n * light_rect(s, sc, li, ul, ub, us, pc, tc);
+    li -= (int)s.rects.size();
+    return n * light_mesh(s, sc, li, ul, ub, us, pc, tc);
 }
 
 // Smooth glass: get_glass_info (delta_scattering.cpp:53-100), enter/exit judged by the original
@@ -1159,7 +1309,7 @@ static RGB trace_path(const Scene& s, Ray ray, PixelSampler& ps, PathCounters& p
             hit_found = intersect(s, ray, hit, tc);
             if (hit_found) init_shading(s, ray, hit, sc);
             if (bounce == 0) {
-                if (hit_found && sc.light >= 0) L = L + coeff * s.rects[sc.light].L;
+                if (hit_found && sc.light >= 0) L = L + coeff * area_light_emission(s, sc);
                 if (cfg.max_light_bounces == 0) break;
             }
             if (!hit_found || sc.light >= 0) break;
@@ -1198,7 +1348,7 @@ static RGB trace_path(const Scene& s, Ray ray, PixelSampler& ps, PathCounters& p
             bool dh = intersect(s, dray, hit, tc);
             if (dh) init_shading(s, dray, hit, sc);
             RGB emitted;
-            if (dh && sc.light >= 0) emitted = s.rects[sc.light].L;
+            if (dh && sc.light >= 0) emitted = area_light_emission(s, sc);
             coeff = coeff * att;
             L = L + coeff * emitted;
             bounce++;
@@ -1417,7 +1567,40 @@ static bool load_scene(const yar_scene_desc* d, Scene& s, int threads) {
         s.points.push_back(PointLight{{ld.position[0], ld.position[1], ld.position[2]},
                                       {ld.intensity[0], ld.intensity[1], ld.intensity[2]}});
     }
-    s.total_lights = (int)s.points.size() + (int)s.rects.size();                     // light.cpp:35
+    for (int i = 0; i < d->spot_light_count; i++) {
+        const yar_spot_light_desc& ld = d->spot_lights[i];
+        s.spots.push_back(SpotLight{{ld.position[0], ld.position[1], ld.position[2]},
+                                    {ld.direction[0], ld.direction[1], ld.direction[2]}, ld.cone_angle, ld.penumbra_angle,
+                                    {ld.intensity[0], ld.intensity[1], ld.intensity[2]}});
+    }
+    for (int i = 0; i < d->directional_light_count; i++) {
+        const yar_directional_light_desc& ld = d->directional_lights[i];
+        s.dirs.push_back(DirectionalLight{{ld.direction[0], ld.direction[1], ld.direction[2]},
+                                          {ld.irradiance[0], ld.irradiance[1], ld.irradiance[2]}});
+    }
+    for (int i = 0; i < d->mesh_light_count; i++) {                          // reference_renderer.cpp:137-152
+        const yar_mesh_light_desc& ld = d->mesh_lights[i];
+        MeshLight ml;
+        ml.object = ld.object;
+        ml.mesh = s.objects[ld.object].mesh;
+        ml.L = {ld.emitted_radiance[0], ld.emitted_radiance[1], ld.emitted_radiance[2]};
+        const Mesh& m = s.meshes[ml.mesh];
+        const int n = (int)(m.idx.size() / 3);
+        std::vector<float> areas(n);
+        for (int t = 0; t < n; t++) {                                         // triangle_mesh.h:103-118
+            V3 p[3]; m.pos(t, p);
+            areas[t] = length(cross(p[1] - p[0], p[2] - p[0])) * 0.5f;
+        }
+        ml.area = 0.f;
+        for (int t = 0; t < n; t++) ml.area += areas[t];
+        float total = 0.f;
+        for (int t = 0; t < n; t++) total += areas[t];
+        ml.cdf.resize(n);
+        float sum = 0.f;
+        for (int t = 0; t < n; t++) { sum += areas[t]; ml.cdf[t] = sum / total; }
+        s.mesh_lights.push_back(std::move(ml));
+    }
+    s.total_lights = (int)(s.points.size() + s.spots.size() + s.dirs.size() + s.rects.size() + s.mesh_lights.size());
     s.camera = d->camera;
     s.config = d->config;
     return true;

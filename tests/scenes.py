@@ -13,7 +13,8 @@ def identity12():
 class DescScene:
     """Owns numpy buffers and the ctypes SceneDesc pointing at them."""
 
-    def __init__(self, meshes, materials, objects, rects=(), points=(), camera=None, config=None):
+    def __init__(self, meshes, materials, objects, rects=(), points=(), camera=None, config=None, spots=(), dirs=(),
+                 mesh_lights=()):
         self._keep = []
         md = (_abi.MeshDesc * len(meshes))()
         for i, m in enumerate(meshes):
@@ -62,13 +63,31 @@ class DescScene:
         for i, p in enumerate(points):
             pl[i].position[:] = p["position"]
             pl[i].intensity[:] = p["intensity"]
-        self._keep += [md, mt, ob, rl, pl]
+        sp = (_abi.SpotLightDesc * max(1, len(spots)))()
+        for i, l in enumerate(spots):
+            sp[i].position[:] = l["position"]
+            sp[i].direction[:] = l["direction"]
+            sp[i].cone_angle = l["cone_angle"]
+            sp[i].penumbra_angle = l["penumbra_angle"]
+            sp[i].intensity[:] = l["intensity"]
+        dl = (_abi.DirectionalLightDesc * max(1, len(dirs)))()
+        for i, l in enumerate(dirs):
+            dl[i].direction[:] = l["direction"]
+            dl[i].irradiance[:] = l["irradiance"]
+        ml = (_abi.MeshLightDesc * max(1, len(mesh_lights)))()
+        for i, l in enumerate(mesh_lights):
+            ml[i].object = l["object"]
+            ml[i].emitted_radiance[:] = l["L"]
+        self._keep += [md, mt, ob, rl, pl, sp, dl, ml]
         d = _abi.SceneDesc()
         d.meshes, d.mesh_count = md, len(meshes)
         d.materials, d.material_count = mt, len(materials)
         d.objects, d.object_count = ob, len(objects)
         d.rect_lights, d.rect_light_count = rl, len(rects)
         d.point_lights, d.point_light_count = pl, len(points)
+        d.spot_lights, d.spot_light_count = sp, len(spots)
+        d.directional_lights, d.directional_light_count = dl, len(dirs)
+        d.mesh_lights, d.mesh_light_count = ml, len(mesh_lights)
         cam = camera or {}
         d.camera.camera_to_world[:] = cam.get("c2w", [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 5])
         d.camera.fov_y_degrees = cam.get("fov", 45.0)
@@ -200,3 +219,37 @@ def point_light_scene(albedo=0.5, intensity=2.0, h=1.0, spp_side=4, res=16, fov=
         camera={"c2w": c2w, "fov": fov},
         config={"film_width": res, "film_height": res, "region": (0, 0, res, res), "nx": spp_side, "ny": spp_side,
                 "maxdepth": maxdepth})
+
+
+def lit_plane_scene(albedo=0.5, spots=(), dirs=(), emitters=(), spp_side=4, res=16, fov=2.0, h=1.0, maxdepth=1):
+    """Diffuse plane y=0 seen straight down from (0, h/2, 0) with a narrow field of view, lit by spot and
+    directional lights and by emitter meshes given as (vertices, indices, L) (triangle-mesh lights:
+    object index = 1 + their position in the list, area light = rect count (0) + that position)."""
+    plane = {"vertices": [[-50, 0, -50], [-50, 0, 50], [50, 0, 50], [50, 0, -50]], "indices": [0, 1, 2, 0, 2, 3]}
+    meshes = [plane] + [{"vertices": v, "indices": ix} for v, ix, _ in emitters]
+    objects = [{"mesh": 0, "material": 0}] + [{"mesh": 1 + i, "light": i} for i in range(len(emitters))]
+    c2w = [1, 0, 0, 0,
+           0, 0, 1, h / 2,
+           0, -1, 0, 0]
+    return DescScene(
+        meshes=meshes,
+        materials=[{"type": _abi.YAR_MATERIAL_DIFFUSE, "reflectance": [albedo] * 3}],
+        objects=objects,
+        spots=list(spots), dirs=list(dirs),
+        mesh_lights=[{"object": 1 + i, "L": L} for i, (_, _, L) in enumerate(emitters)],
+        camera={"c2w": c2w, "fov": fov},
+        config={"film_width": res, "film_height": res, "region": (0, 0, res, res), "nx": spp_side, "ny": spp_side,
+                "maxdepth": maxdepth})
+
+
+def square_fan(half, y, facing_down=True):
+    """A 2*half square at height y as four triangles around its centre (not a rectangle light: a
+    triangle-mesh emitter); faces down by default."""
+    c = [0.0, y, 0.0]
+    corners = [[-half, y, -half], [half, y, -half], [half, y, half], [-half, y, half]]
+    v = [c] + corners
+    idx = []
+    for i in range(4):
+        a, b = 1 + i, 1 + (i + 1) % 4
+        idx += [0, a, b] if facing_down else [0, b, a]
+    return v, idx

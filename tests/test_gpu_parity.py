@@ -160,3 +160,29 @@ def test_render_rough_glass_matches_oracle(c1_path, oracle, tmp_path):
     assert mean_rel <= 1e-3
     assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
     assert (film["weight_sum"] == ofilm["weight_sum"]).all()
+
+
+def test_render_spot_directional_mesh_lights_match_oracle(c1_path, oracle, tmp_path):
+    # the config-1 room lit by every light type at once: its rectangle light, a spot light aimed at
+    # the floor, a distant light through the open front and a triangle-fan emitter under the ceiling
+    # (import_pbrt_non_area_light, pbrt_scene.cpp:903-956; Triangle_Mesh_Light, light_sampling.cpp)
+    from tests import scenes
+    v, idx = scenes.square_fan(0.12, 0.0, facing_down=True)
+    pts = " ".join(f"{x + 0.25:.6g} {y + 0.92:.6g} {z + 0.6:.6g}" for x, y, z in v)
+    extra = ('LightSource "spot" "rgb I" [ 2 1.5 1 ] "point from" [ 0.7 0.9 0.5 ] "point to" [ 0.6 0 0.4 ] '
+             '"float coneangle" [ 25 ] "float conedeltaangle" [ 6 ]\n'
+             'LightSource "distant" "rgb L" [ 0.6 0.7 0.8 ] "point from" [ 0.3 0.8 4 ] "point to" [ 0.5 0.4 0 ]\n'
+             'AttributeBegin\nAreaLightSource "diffuse" "rgb L" [ 6 5 4 ]\n'
+             f'Shape "trianglemesh" "integer indices" [ {" ".join(map(str, idx))} ] "point P" [ {pts} ]\n'
+             'AttributeEnd\n')
+    text = open(c1_path).read().replace("WorldBegin\n", "WorldBegin\n" + extra, 1)
+    p = tmp_path / "c1_lights.pbrt"
+    p.write_text(text)
+    d = yar_amd.Scene(str(p)).desc
+    assert (d.spot_light_count, d.directional_light_count, d.mesh_light_count) == (1, 1, 1)
+    img, film, var, st, oimg, ofilm, ovar = _render_pair(str(p), oracle, nx=4)
+    exact, close, mean_rel = _compare_images(img, oimg, film, ofilm, "c1 spot+distant+mesh lights 16spp")
+    assert close >= 0.99
+    assert mean_rel <= 1e-3
+    assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
+    assert (film["weight_sum"] == ofilm["weight_sum"]).all()

@@ -243,3 +243,56 @@ Shape "trianglemesh" "integer indices" [0 1 2] "point P" [0 0 0 1 0 0 0 1 0]
 '''
     with pytest.raises(yar_amd.YarError):
         _load(tmp_path, bad)
+
+
+def test_spot_and_distant_lights(tmp_path):
+    """import_pbrt_non_area_light (pbrt_scene.cpp:903-956): spot from/to go through the CTM, cone
+    angles are degrees; a distant light shines along from - to with L * scale as irradiance."""
+    body = LIGHT + '''AttributeBegin
+Translate 0 1 0
+LightSource "spot" "rgb I" [ 3 2 1 ] "point from" [ 0 2 0 ] "point to" [ 0 2 -4 ] "float coneangle" [ 20 ] "float conedeltaangle" [ 4 ]
+AttributeEnd
+LightSource "spot" "rgb I" [ 1 1 1 ]
+LightSource "distant" "rgb L" [ 2 2 2 ] "rgb scale" [ 0.5 1 2 ] "point from" [ 0 0 0 ] "point to" [ 0 -3 0 ]
+LightSource "distant"
+'''
+    d = _load(tmp_path, body).desc
+    assert d.spot_light_count == 2 and d.directional_light_count == 2 and d.point_light_count == 0
+    s0, s1 = d.spot_lights[0], d.spot_lights[1]
+    assert list(s0.position) == [0, 3, 0]
+    assert np.allclose(list(s0.direction), [0, 0, -1])
+    assert s0.cone_angle == pytest.approx(np.radians(20), rel=1e-6)
+    assert s0.penumbra_angle == pytest.approx(np.radians(4), rel=1e-6)
+    assert list(s0.intensity) == [3, 2, 1]
+    assert s1.cone_angle == pytest.approx(np.radians(30), rel=1e-6)          # pbrt3 defaults
+    assert s1.penumbra_angle == pytest.approx(np.radians(5), rel=1e-6)
+    l0, l1 = d.directional_lights[0], d.directional_lights[1]
+    assert np.allclose(list(l0.direction), [0, 1, 0])                          # from - to, normalised
+    assert list(l0.irradiance) == [1, 2, 4]
+    assert np.allclose(list(l1.direction), [0, 0, -1])                         # default to = (0, 0, 1)
+    assert list(l1.irradiance) == [1, 1, 1]
+
+
+def test_point_light_from_goes_through_ctm(tmp_path):
+    body = LIGHT + 'AttributeBegin\nTranslate 1 0 0\nScale 2 2 2\nLightSource "point" "point from" [ 1 1 1 ]\nAttributeEnd\n'
+    d = _load(tmp_path, body).desc
+    assert list(d.point_lights[0].position) == [3, 2, 2]
+    assert list(d.point_lights[0].intensity) == [1, 1, 1]
+
+
+def test_triangle_mesh_area_light(tmp_path):
+    """import_pbrt_shape (pbrt_scene.cpp:786-794): an emitter that is not a rectangle becomes a
+    triangle-mesh light; its object's light index counts after every rectangle light."""
+    fan = '''AttributeBegin
+AreaLightSource "diffuse" "rgb L" [ 7 8 9 ]
+Shape "trianglemesh" "integer indices" [ 0 1 2 0 2 3 0 3 4 ] "point P" [ 0 3 0  1 3 0  1 3 1  0 3 2  -1 3 1 ]
+AttributeEnd
+'''
+    d = _load(tmp_path, fan + LIGHT + 'Material "matte"\nShape "trianglemesh" "integer indices" [ 0 1 2 ] '
+              '"point P" [ 0 0 0  1 0 0  0 1 0 ]\n').desc
+    assert d.rect_light_count == 1 and d.mesh_light_count == 1
+    assert d.mesh_lights[0].object == 0
+    assert list(d.mesh_lights[0].emitted_radiance) == [7, 8, 9]
+    objs = [d.objects[i] for i in range(d.object_count)]
+    assert [o.area_light for o in objs] == [1, 0, -1]                          # mesh light after 1 rect
+    assert objs[0].material == -1 and objs[2].material >= 0

@@ -686,3 +686,57 @@ def test_rough_glass_generalised_reciprocity_and_sampling_bounds(flipped, alpha)
         assert mass <= 1.02
         assert est <= alb * 1.03 and valid / n <= mass + 0.02, (theta_o, est, alb, valid / n, mass)
         assert est > 0.5 * alb
+
+
+def _render_lit_plane(**kw):
+    from tests import scenes
+    from tests.test_oracle_fixtures import _view_from_desc
+    sc = scenes.lit_plane_scene(**kw)
+    img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
+    return img
+
+
+def test_spot_light_cone_and_penumbra_closed_forms():
+    # direct_lighting.cpp:49-89: a spot at (0, 1, 0) lights the plane point below it with
+    # albedo/pi * I * cos / d^2 inside the cone, k^4 of that in the penumbra with
+    # k = (cos w - cos cone) / (cos(cone - delta) - cos cone), nothing outside.
+    rad = np.pi / 180
+    def spot(tilt_deg, cone=30.0, delta=10.0):
+        t = tilt_deg * rad
+        return {"position": [0, 1, 0], "direction": [np.sin(t), -np.cos(t), 0], "cone_angle": cone * rad,
+                "penumbra_angle": delta * rad, "intensity": [2.0, 2.0, 2.0]}
+    full = 0.5 / np.pi * 2.0
+    img = _render_lit_plane(spots=[spot(0.0)], fov=0.5)
+    assert np.allclose(img, full, rtol=1e-3), (img.min(), img.max(), full)
+    img = _render_lit_plane(spots=[spot(45.0)], fov=0.5)
+    assert (img == 0).all()
+    img = _render_lit_plane(spots=[spot(25.0)], fov=0.2)
+    k = (np.cos(25 * rad) - np.cos(30 * rad)) / (np.cos(20 * rad) - np.cos(30 * rad))
+    # the view covers +-0.1 degree around the point below the light; k^4 varies by ~3% per 0.1 degree
+    assert img.mean() == pytest.approx(full * k ** 4, rel=1e-2), (img.mean(), full * k ** 4)
+
+
+def test_directional_light_closed_form():
+    # direct_lighting.cpp:91-107: L = albedo/pi * E * cos(theta) for a light from direction theta
+    for theta in (0.0, 0.6, 1.2):
+        d = [np.sin(theta), np.cos(theta), 0.0]
+        img = _render_lit_plane(dirs=[{"direction": d, "irradiance": [3.0, 2.0, 1.0]}], fov=1.0)
+        want = 0.5 / np.pi * np.array([3.0, 2.0, 1.0]) * np.cos(theta)
+        assert np.allclose(img, want, rtol=1e-5), (theta, img.mean(axis=(0, 1)), want)
+
+
+def test_triangle_mesh_light_form_factor_and_one_sidedness():
+    # a 2x2 square emitter at height 1 built from four triangles around its centre is a triangle-mesh
+    # light (not a rectangle); with MIS of light and BSDF sampling (direct_lighting.cpp:262-322) the
+    # plane below it receives albedo * Le * F, F the point-to-parallel-square form factor
+    from tests import scenes
+    v, idx = scenes.square_fan(1.0, 1.0, facing_down=True)
+    img = _render_lit_plane(emitters=[(v, idx, [1.0, 1.0, 1.0])], spp_side=16)
+    want = 0.5 * scenes.parallel_rect_form_factor(1.0, 1.0)
+    assert img.mean() == pytest.approx(want, rel=0.01), (img.mean(), want)
+    # facing up, the plane sees the emitter's back: light samples there have pdf 0 and contribute
+    # nothing, while BSDF samples that hit the back keep MIS weight 1 (direct_lighting.cpp:313-320,
+    # light_sampling.cpp:175-177), so the BSDF half alone carries the same albedo * Le * F
+    v, idx = scenes.square_fan(1.0, 1.0, facing_down=False)
+    img = _render_lit_plane(emitters=[(v, idx, [1.0, 1.0, 1.0])], spp_side=16)
+    assert img.mean() == pytest.approx(want, rel=0.02), (img.mean(), want)

@@ -37,6 +37,19 @@ class PointLightDesc(C.Structure):
     _fields_ = [("position", C.c_float * 3), ("intensity", C.c_float * 3)]
 
 
+class SpotLightDesc(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("direction", C.c_float * 3), ("cone_angle", C.c_float),
+                ("penumbra_angle", C.c_float), ("intensity", C.c_float * 3)]
+
+
+class DirectionalLightDesc(C.Structure):
+    _fields_ = [("direction", C.c_float * 3), ("irradiance", C.c_float * 3)]
+
+
+class MeshLightDesc(C.Structure):
+    _fields_ = [("object", C.c_int32), ("emitted_radiance", C.c_float * 3)]
+
+
 class RenderConfig(C.Structure):
     _fields_ = [("film_width", C.c_int32), ("film_height", C.c_int32),
                 ("region_x0", C.c_int32), ("region_y0", C.c_int32),
@@ -59,7 +72,10 @@ class SceneDesc(C.Structure):
                 ("objects", C.POINTER(ObjectDesc)), ("object_count", C.c_int32),
                 ("rect_lights", C.POINTER(RectLightDesc)), ("rect_light_count", C.c_int32),
                 ("point_lights", C.POINTER(PointLightDesc)), ("point_light_count", C.c_int32),
-                ("camera", CameraDesc), ("config", RenderConfig)]
+                ("camera", CameraDesc), ("config", RenderConfig),
+                ("spot_lights", C.POINTER(SpotLightDesc)), ("spot_light_count", C.c_int32),
+                ("directional_lights", C.POINTER(DirectionalLightDesc)), ("directional_light_count", C.c_int32),
+                ("mesh_lights", C.POINTER(MeshLightDesc)), ("mesh_light_count", C.c_int32)]
 
 
 class GpuOptions(C.Structure):

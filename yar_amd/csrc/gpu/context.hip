@@ -147,6 +147,10 @@ struct yar_gpu_ctx {
     DevBuf<DevMaterial> materials;
     DevBuf<DevRect> rects;
     DevBuf<DevPoint> points;
+    DevBuf<DevSpot> spots;
+    DevBuf<DevDirLight> dirs;
+    DevBuf<DevMeshLight> mesh_lights;
+    DevBuf<float> mesh_light_cdf;
     DevScene scene{};
     std::vector<int32_t> tri_base;        // per object (for hit conversion on the host)
 
@@ -212,6 +216,7 @@ struct yar_gpu_ctx {
         DevBuf<int>* bi[] = {&pix_view, &tile_first, &side_view};
         for (auto* b : bi) b->release();
         vtx_uv.release(); materials.release(); rects.release(); points.release();
+        spots.release(); dirs.release(); mesh_lights.release(); mesh_light_cdf.release();
         pixels.release(); pix_first.release(); dviews.release(); chunks.release(); chunk_slot.release();
         perm.release(); jump_lo.release(); jump_hi.release(); hit_ot.release(); tr_ot.release(); state.release(); rng.release();
         DevBuf<uint32_t>* bu[] = {&q_a, &q_b, &sh_path, &mi_path, &dl_path, &counts, &q_unsorted};
@@ -327,6 +332,7 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
     int mesh_depth = 0;
     for (int i = 0; i < d->mesh_count; i++) {
         const yar_mesh_desc& m = d->meshes[i];
+        const int mflags = (m.normals ? 1 : 0) | (m.uvs ? 2 : 0) | (m.reverse_geometric_normal_orientation ? 4 : 0);
         for (int t = 0; t < m.triangle_count; t++) {
             int64_t g = mtri[i] + t;
             int vi[3];
@@ -335,13 +341,13 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
                 const float* p = m.vertices + 3 * (size_t)vi[k];
                 tv[3 * (size_t)g + k] = make_float4(p[0], p[1], p[2], 0.f);
             }
-            tvi[g] = make_int4((int)(mvtx[i] + vi[0]), (int)(mvtx[i] + vi[1]), (int)(mvtx[i] + vi[2]), 0);
+            tvi[g] = make_int4((int)(mvtx[i] + vi[0]), (int)(mvtx[i] + vi[1]), (int)(mvtx[i] + vi[2]), mflags);
         }
         for (int v = 0; v < m.vertex_count; v++) {
             if (m.normals) vn[mvtx[i] + v] = make_float4(m.normals[3 * v], m.normals[3 * v + 1], m.normals[3 * v + 2], 0.f);
             if (m.uvs) vuv[mvtx[i] + v] = make_float2(m.uvs[2 * v], m.uvs[2 * v + 1]);
         }
-        minfo[i] = make_int4((m.normals ? 1 : 0) | (m.uvs ? 2 : 0) | (m.reverse_geometric_normal_orientation ? 4 : 0), 0, 0, 0);
+        minfo[i] = make_int4(mflags, 0, 0, 0);
         // rebase the mesh tree: above_child and leaf triangle ids become global indices
         const std::vector<uint32_t>& w = trees[i].words;
         size_t nn = w.size() / 2;
@@ -422,7 +428,7 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
     // whose world-space emitter vertices fall outside it (by more than 1e-4 of the size) trace every ray.
     for (int i = 0; i < d->object_count; i++) {
         const yar_object_desc& o = d->objects[i];
-        if (o.area_light < 0) continue;
+        if (o.area_light < 0 || o.area_light >= d->rect_light_count) continue;
         DevRect& R = rl[o.area_light];
         yar::M34 m;
         std::memcpy(m.a, o.object_to_world, sizeof(m.a));
@@ -444,6 +450,52 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
         pl[i].I = make_float4(l.intensity[0], l.intensity[1], l.intensity[2], 0.f);
     }
 
+    std::vector<DevSpot> spl(std::max(1, d->spot_light_count));
+    for (int i = 0; i < d->spot_light_count; i++) {
+        const yar_spot_light_desc& l = d->spot_lights[i];
+        const float cone_cos = std::cos(l.cone_angle);
+        const float penumbra_cos = std::cos(std::max(0.f, l.cone_angle - l.penumbra_angle));
+        spl[i].p = make_float4(l.position[0], l.position[1], l.position[2], 0.f);
+        spl[i].dir = make_float4(l.direction[0], l.direction[1], l.direction[2], cone_cos);
+        spl[i].I = make_float4(l.intensity[0], l.intensity[1], l.intensity[2], penumbra_cos);
+    }
+    std::vector<DevDirLight> dl(std::max(1, d->directional_light_count));
+    for (int i = 0; i < d->directional_light_count; i++) {
+        const yar_directional_light_desc& l = d->directional_lights[i];
+        dl[i].dir = make_float4(l.direction[0], l.direction[1], l.direction[2], 0.f);
+        dl[i].E = make_float4(l.irradiance[0], l.irradiance[1], l.irradiance[2], 0.f);
+    }
+    // triangle-mesh lights: triangle areas and their CDF in the reference's float order
+    // (Triangle_Mesh::get_triangle_area / get_area, Distribution_1D::initialize; reference_renderer.cpp:137-152)
+    std::vector<DevMeshLight> ml(std::max(1, d->mesh_light_count));
+    std::vector<float> mcdf;
+    for (int i = 0; i < d->mesh_light_count; i++) {
+        const yar_mesh_light_desc& l = d->mesh_lights[i];
+        const yar_object_desc& o = d->objects[l.object];
+        const yar_mesh_desc& mesh = d->meshes[o.mesh];
+        const int n = mesh.triangle_count;
+        std::vector<float> areas((size_t)n);
+        for (int t = 0; t < n; t++) {
+            F3 p[3];
+            for (int k = 0; k < 3; k++) {
+                const float* v = mesh.vertices + 3 * (size_t)mesh.indices[3 * (size_t)t + k];
+                p[k] = F3(v[0], v[1], v[2]);
+            }
+            areas[(size_t)t] = yar::length(yar::cross(p[1] - p[0], p[2] - p[0])) * 0.5f;
+        }
+        float area = 0.f, total = 0.f, sum = 0.f;
+        for (int t = 0; t < n; t++) area += areas[(size_t)t];
+        for (int t = 0; t < n; t++) total += areas[(size_t)t];
+        ml[i].obj = l.object;
+        ml[i].tri_base = (int)mtri[o.mesh];
+        ml[i].tri_count = n;
+        ml[i].cdf = (int)mcdf.size();
+        ml[i].area = area;
+        ml[i].L = make_float4(l.emitted_radiance[0], l.emitted_radiance[1], l.emitted_radiance[2], 0.f);
+        for (int t = 0; t < n; t++) { sum += areas[(size_t)t]; mcdf.push_back(sum / total); }
+    }
+    if (mcdf.empty()) mcdf.push_back(1.f);
+
     hipStream_t s = ctx->stream;
     ctx->scene_nodes.upload(sn, s);
     ctx->mesh_nodes.upload(mn, s);
@@ -459,6 +511,10 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
     ctx->materials.upload(mats, s);
     ctx->rects.upload(rl, s);
     ctx->points.upload(pl, s);
+    ctx->spots.upload(spl, s);
+    ctx->dirs.upload(dl, s);
+    ctx->mesh_lights.upload(ml, s);
+    ctx->mesh_light_cdf.upload(mcdf, s);
     HIPX(hipStreamSynchronize(s));
     auto t2 = std::chrono::steady_clock::now();
 
@@ -479,9 +535,17 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
     S.materials = ctx->materials.p;
     S.rects = ctx->rects.p;
     S.points = ctx->points.p;
+    S.spots = ctx->spots.p;
+    S.dirs = ctx->dirs.p;
+    S.mesh_lights = ctx->mesh_lights.p;
+    S.mesh_light_cdf = ctx->mesh_light_cdf.p;
     S.n_rects = d->rect_light_count;
     S.n_points = d->point_light_count;
-    S.n_lights = d->rect_light_count + d->point_light_count;     // update_total_light_count, light.cpp:35
+    S.n_spots = d->spot_light_count;
+    S.n_dirs = d->directional_light_count;
+    S.n_mesh_lights = d->mesh_light_count;
+    S.n_lights = d->rect_light_count + d->point_light_count + d->spot_light_count + d->directional_light_count +
+                 d->mesh_light_count;                            // update_total_light_count, light.cpp:35
 
     yar_gpu_info& inf = ctx->info;
     inf.scene_bytes = (int64_t)(sn.size() * 8 + mn.size() * 8 + ot.size() * sizeof(ObjTrav) + tv.size() * 16 +
@@ -1185,12 +1249,28 @@ int yar_gpu_create(const yar_scene_desc* scene, const yar_gpu_options* options, 
     *out_ctx = nullptr;
     if (scene->object_count <= 0 || scene->mesh_count <= 0)
         return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_create: scene has no geometry");
-    if (scene->rect_light_count + scene->point_light_count <= 0)
-        return fail(nullptr, YAR_ERR_UNSUPPORTED, "yar_gpu_create: scene has no point or rectangular lights");
+    if (scene->rect_light_count < 0 || scene->point_light_count < 0 || scene->spot_light_count < 0 ||
+        scene->directional_light_count < 0 || scene->mesh_light_count < 0 ||
+        (scene->spot_light_count > 0 && !scene->spot_lights) ||
+        (scene->directional_light_count > 0 && !scene->directional_lights) ||
+        (scene->mesh_light_count > 0 && !scene->mesh_lights))
+        return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_create: invalid light arrays");
+    if (scene->rect_light_count + scene->point_light_count + scene->spot_light_count + scene->directional_light_count +
+            scene->mesh_light_count <= 0)
+        return fail(nullptr, YAR_ERR_UNSUPPORTED, "yar_gpu_create: scene has no lights");
+    for (int i = 0; i < scene->mesh_light_count; i++) {
+        const int ob = scene->mesh_lights[i].object;
+        if (ob < 0 || ob >= scene->object_count || scene->objects[ob].area_light != scene->rect_light_count + i)
+            return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "mesh light object does not reference the light");
+        static const float identity[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+        if (std::memcmp(scene->objects[ob].object_to_world, identity, sizeof(identity)) != 0)
+            return fail(nullptr, YAR_ERR_UNSUPPORTED, "mesh light objects must have identity transforms");
+    }
     for (int i = 0; i < scene->object_count; i++) {
         const yar_object_desc& o = scene->objects[i];
         if (o.mesh < 0 || o.mesh >= scene->mesh_count) return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "object mesh index out of range");
-        if (o.area_light >= scene->rect_light_count) return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "object light index out of range");
+        if (o.area_light >= scene->rect_light_count + scene->mesh_light_count)
+            return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "object light index out of range");
         if (o.area_light < 0 && (o.material < 0 || o.material >= scene->material_count))
             return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "object material index out of range");
     }

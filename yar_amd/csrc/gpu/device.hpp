@@ -87,6 +87,12 @@ struct DevMaterial {      // 64 bytes
 // for this light (its emitter geometry is not known to lie inside the rectangle)
 struct DevRect { float4 l0, l1, l2; float4 L; float4 size; };
 struct DevPoint { float4 p; float4 I; };
+// Spot_Light with cos(cone_angle) in dir.w and cos(max(0, cone_angle - penumbra_angle)) in I.w, both
+// computed on the host as the reference computes them (direct_lighting.cpp:58-63)
+struct DevSpot { float4 p; float4 dir; float4 I; };
+struct DevDirLight { float4 dir; float4 E; };          // direction towards the light, irradiance
+// Diffuse_Triangle_Mesh_Light: emitter object, its triangles' global range, area CDF offset, area
+struct DevMeshLight { int obj, tri_base, tri_count, cdf; float area, pad0, pad1, pad2; float4 L; };
 
 struct DevScene {
     const uint2* scene_nodes;
@@ -99,13 +105,17 @@ struct DevScene {
     const float4* obj_nrm;         // 3 rows per object (transpose of world_to_object 3x3)
     const int4* obj_info;          // material, area light, mesh, triangle base
     const int4* mesh_info;         // flags (1 normals, 2 uvs, 4 reverse), -, -, -
-    const int4* tri_vidx;          // global vertex indices per triangle
+    const int4* tri_vidx;          // global vertex indices per triangle, w = its mesh's flags
     const float4* vtx_n;           // per vertex normal
     const float2* vtx_uv;          // per vertex uv
     const DevMaterial* materials;
     const DevRect* rects;
     const DevPoint* points;
-    int n_rects, n_points, n_lights;
+    const DevSpot* spots;
+    const DevDirLight* dirs;
+    const DevMeshLight* mesh_lights;
+    const float* mesh_light_cdf;
+    int n_rects, n_points, n_spots, n_dirs, n_mesh_lights, n_lights;
 };
 
 // ---- ray/box slab test (Bounding_Box::intersect_by_ray_without_NaNs, bounding_box.h:63) ----

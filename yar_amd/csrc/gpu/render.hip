@@ -103,6 +103,53 @@ YG_INLINE bool ray_may_hit_rect(const DevRect& R, V3 o, V3 d) {
     return fabsf(dot(q, ex)) <= mu && fabsf(dot(q, ey)) <= mv;
 }
 
+// Distribution_1D::sample (sampling.cpp:98-147, 168-172) over a CDF of n entries ending in 1:
+// returns the sample in [0,1) and the remapped u of the chosen interval.
+YG_INLINE float sample_cdf(const float* cdf, int n, float u, float& remapped) {
+    int lo = 0, hi = n;                                       // lower_bound: first cdf[k] >= u
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] < u) lo = mid + 1; else hi = mid;
+    }
+    const int k = min(lo, n - 1);
+    const float interval_length = 1.f / (float)n;
+    const float a = k == 0 ? 0.f : cdf[k - 1], b = cdf[k];
+    const float fraction = (u - a) / (b - a);
+    const float x = ((float)k + fraction) * interval_length;
+    remapped = fminf(fraction, kOneMinusEps);
+    return fminf(x, kOneMinusEps);
+}
+
+// Point and emitter normal of a triangle-mesh light at (global triangle, barycentrics), in the mesh's
+// coordinates: the interpolated vertex normal (not normalised) when the mesh has normals, else the
+// geometric normal (Triangle_Mesh::get_position / get_normal / get_geometric_normal,
+// light_sampling.cpp:133-142, 168-173).
+YG_INLINE void mesh_light_point(const DevScene& s, uint32_t gt, V3 b, V3& p, V3& n) {
+    const V3 p0 = from4(s.tri_verts[3 * (size_t)gt]), p1 = from4(s.tri_verts[3 * (size_t)gt + 1]),
+             p2 = from4(s.tri_verts[3 * (size_t)gt + 2]);
+    p = b.x * p0 + b.y * p1 + b.z * p2;
+    const int4 vi = s.tri_vidx[gt];
+    const int mflags = vi.w;
+    if (mflags & 1) {
+        n = b.x * from4(s.vtx_n[vi.x]) + b.y * from4(s.vtx_n[vi.y]) + b.z * from4(s.vtx_n[vi.z]);
+    } else {
+        n = normalized(cross(p1 - p0, p2 - p0));
+        if (mflags & 4) n = -n;
+    }
+}
+
+// get_emitted_radiance (direct_lighting.cpp:388-410): rectangles emit on both sides, triangle-mesh
+// lights only where the original shading normal faces the viewer (not flipped by init_shading).
+YG_INLINE RGB mesh_light_emission(const DevScene& s, int j, int obj, uint32_t tri, V3 b, V3 ray_d) {
+    Shading sc;
+    init_shading(s, ray_d, obj, tri, b, sc);
+    return sc.flipped ? rgb(0.f, 0.f, 0.f) : rgb4(s.mesh_lights[j].L);
+}
+YG_INLINE RGB area_emission(const DevScene& s, int light, int obj, uint32_t tri, V3 b, V3 ray_d) {
+    if (light < s.n_rects) return rgb4(s.rects[light].L);
+    return mesh_light_emission(s, light - s.n_rects, obj, tri, b, ray_d);
+}
+
 // Guided form: a pcg32 stream per (pixel, sample), disjoint from the pixel streams (whose upper
 // 32 sequence bits are zero), for strategy choices, mixture sampling and fallback dimensions.
 YG_INLINE uint64_t guide_stream(const Chunk& c, uint2 pk) {
@@ -587,18 +634,32 @@ __global__ void k_mis(DevScene sc, Paths P, Queues q, StackCfg cfg) {
                 const float4 o4 = q.mis_o[i], d4 = q.mis_d[i];
                 const int light = (int)fbits(o4.w);
                 if (tr.result() && sc.obj_info[h.obj].y == light) {
-                    const DevRect L = sc.rects[light];
-                    const float4 v0 = sc.tri_verts[3 * (size_t)h.tri], v1 = sc.tri_verts[3 * (size_t)h.tri + 1],
-                                 v2 = sc.tri_verts[3 * (size_t)h.tri + 2];
-                    // mesh-space hit point, as Triangle_Mesh::get_position (direct_lighting.cpp:181)
-                    const V3 p = h.b.x * from4(v0) + h.b.y * from4(v1) + h.b.z * from4(v2);
-                    const float dd = length(p - from4(o4));
-                    const float lndw = d4.w;
-                    const float lpdf = (dd * dd) / (L.size.x * L.size.y * lndw);
+                    float lpdf;
+                    RGB Le;
+                    if (light < sc.n_rects) {
+                        const DevRect L = sc.rects[light];
+                        const float4 v0 = sc.tri_verts[3 * (size_t)h.tri], v1 = sc.tri_verts[3 * (size_t)h.tri + 1],
+                                     v2 = sc.tri_verts[3 * (size_t)h.tri + 2];
+                        // mesh-space hit point, as Triangle_Mesh::get_position (direct_lighting.cpp:181)
+                        const V3 p = h.b.x * from4(v0) + h.b.y * from4(v1) + h.b.z * from4(v2);
+                        const float dd = length(p - from4(o4));
+                        const float lndw = d4.w;
+                        lpdf = (dd * dd) / (L.size.x * L.size.y * lndw);
+                        Le = rgb4(L.L);
+                    } else {
+                        // Diffuse_Triangle_Mesh_Light_Sampler::pdf (light_sampling.cpp:163-184); a hit on
+                        // the emitter's back has pdf 0 and keeps the full BSDF-sample weight, as there
+                        const DevMeshLight ML = sc.mesh_lights[light - sc.n_rects];
+                        V3 lp, ln;
+                        mesh_light_point(sc, h.tri, h.b, lp, ln);
+                        const float lndw = dot(ln, -from4(d4));
+                        lpdf = lndw <= 0.f ? 0.f : (h.t * h.t) / (lndw * ML.area);
+                        Le = rgb4(ML.L);
+                    }
                     const float4 f4v = q.mis_f[i];
                     const float bpdf = f4v.w;
                     const float w = power_heuristic(bpdf, lpdf);
-                    const RGB contrib = (rgb4(L.L) * rgb4(f4v)) * (w * fabsf(q.mis_ndw[i]) / bpdf);
+                    const RGB contrib = (Le * rgb4(f4v)) * (w * fabsf(q.mis_ndw[i]) / bpdf);
                     P.dl_bsdf[q.mis_path[i]] = f4(contrib, 0.f);
                 }
             });
@@ -642,7 +703,7 @@ YG_INLINE void finalize(const Chunk& c, const Paths& P, uint2 pk, RGB L) {      
 // GUIDED selects the guided form's variant at compile time, so the plain kernel keeps its own
 // (smaller) register allocation.
 template <bool GUIDED>
-__global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queues q, int parity) {
+__global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queues q, int parity) {
     const Chunk& c = *cp;
     const uint32_t* in = parity ? q.active1 : q.active0;
     uint32_t* out = q.bin_rays ? q.unsorted : (parity ? q.active0 : q.active1);
@@ -702,13 +763,13 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
             bool finished = false, pending_dl = false, continue_body = true;
             if (!delta_traced) {
                 if (bounce == 0) {
-                    if (hit && light >= 0) L = L + coeff * rgb4(s.rects[light].L);
+                    if (hit && light >= 0) L = L + coeff * area_emission(s, light, ot.x, (uint32_t)ot.y, v3(tb.y, tb.z, tb.w), from4(d4));
                     if (c.max_bounces == 0) { finished = true; continue_body = false; }
                 }
                 if (continue_body && (!hit || light >= 0)) { finished = true; continue_body = false; }
             } else {
                 RGB emitted = rgb(0.f, 0.f, 0.f);
-                if (hit && light >= 0) emitted = rgb4(s.rects[light].L);
+                if (hit && light >= 0) emitted = area_emission(s, light, ot.x, (uint32_t)ot.y, v3(tb.y, tb.z, tb.w), from4(d4));
                 L = L + coeff * emitted;
                 bounce++;
                 if (bounce == c.max_bounces || !hit || light >= 0) { finished = true; continue_body = false; }
@@ -773,8 +834,9 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
                     // estimate_direct_lighting_from_single_sample (direct_lighting.cpp:522-578)
                     const int li = int(u_sel * s.n_lights);
                     pending_dl = true;
-                    if (li < s.n_points) {
-                        const DevPoint pl = s.points[li];               // direct_lighting.cpp:26-47
+                    int lj = li;
+                    if (lj < s.n_points) {
+                        const DevPoint pl = s.points[lj];               // direct_lighting.cpp:26-47
                         const V3 lp = from4(pl.p);
                         const V3 position = origin_dir(sc, lp - sc.pos);
                         const V3 lv = lp - position;
@@ -787,8 +849,44 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
                             sh_o = position; sh_d = ldir; sh_tmax = dist * (1.f - 1e-5f);
                             want_shadow = true;
                         }
-                    } else {
-                        const int ri = li - s.n_points;                 // direct_lighting.cpp:109-194
+                    } else if ((lj -= s.n_points) < s.n_spots) {
+                        const DevSpot sl = s.spots[lj];                 // direct_lighting.cpp:49-89
+                        const V3 lp = from4(sl.p);
+                        const V3 position = origin_dir(sc, lp - sc.pos);
+                        const V3 lv = lp - position;
+                        const float dist = length(lv);
+                        const V3 wi = vdiv(lv, dist);
+                        const float cone_cos = sl.dir.w;
+                        const float wi_cos = dot(-wi, from4(sl.dir));
+                        if (!(wi_cos < cone_cos)) {
+                            float penumbra = 1.f;
+                            const float penumbra_cos = sl.I.w;
+                            if (wi_cos < penumbra_cos) {
+                                const float kp = (wi_cos - cone_cos) / (penumbra_cos - cone_cos);
+                                penumbra = (kp * kp) * (kp * kp);
+                            }
+                            const float ndw = dot(sc.n, wi);
+                            if (scattering_possible(m.type, ndw)) {
+                                const RGB f = bsdf_eval(m, sc, sc.wo, wi);
+                                if (!is_black(f)) {
+                                    sh_c = (rgb4(sl.I) * f) * (penumbra * fabsf(ndw) / (dist * dist));
+                                    sh_o = position; sh_d = wi; sh_tmax = dist * (1.f - 1e-5f);
+                                    want_shadow = true;
+                                }
+                            }
+                        }
+                    } else if ((lj -= s.n_spots) < s.n_dirs) {
+                        const DevDirLight dl = s.dirs[lj];              // direct_lighting.cpp:91-107
+                        const V3 ldir = from4(dl.dir);
+                        const float ndl = dot(sc.n, ldir);
+                        if (ndl > 0.f) {
+                            const RGB f = bsdf_eval(m, sc, sc.wo, ldir);
+                            sh_c = (rgb4(dl.E) * f) * ndl;
+                            sh_o = origin_dir(sc, ldir); sh_d = ldir; sh_tmax = f_inf();
+                            want_shadow = true;
+                        }
+                    } else if ((lj -= s.n_dirs) < s.n_rects) {
+                        const int ri = lj;                              // direct_lighting.cpp:109-194
                         const DevRect R = s.rects[ri];
                         const M34 l2w = {R.l0, R.l1, R.l2};
                         const V3 ln = v3(R.l0.z, R.l1.z, R.l2.z);
@@ -828,6 +926,54 @@ __global__ void k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queue
                                     want_mis = ray_may_hit_rect(R, mi_o, wi);
                                     mis_culled = !want_mis;
                                 }
+                            }
+                        }
+                    } else {
+                        lj -= s.n_rects;                                // direct_lighting.cpp:262-322
+                        const DevMeshLight ML = s.mesh_lights[lj];
+                        {
+                            // Diffuse_Triangle_Mesh_Light_Sampler::sample (light_sampling.cpp:126-161)
+                            float u0;
+                            const float smp = sample_cdf(s.mesh_light_cdf + ML.cdf, ML.tri_count, ulx, u0);
+                            const uint32_t lt = min((uint32_t)(smp * (float)ML.tri_count), (uint32_t)ML.tri_count - 1u);
+                            const float ra = sqrtf(u0);                  // sampling.cpp:69-77
+                            V3 bl;
+                            bl.x = 1 - ra;
+                            bl.y = uly * ra;
+                            bl.z = 1.f - bl.x - bl.y;
+                            V3 lp, ln;
+                            mesh_light_point(s, (uint32_t)ML.tri_base + lt, bl, lp, ln);
+                            const V3 lv0 = lp - sc.pos;
+                            const float dsq = dot(lv0, lv0);
+                            const V3 w0 = vdiv(lv0, sqrtf(dsq));
+                            const float lndw0 = dot(ln, -w0);
+                            const float lpdf = lndw0 <= 0.f ? 0.f : dsq / (lndw0 * ML.area);
+                            if (lpdf != 0.f) {
+                                const V3 position = origin_dir(sc, lp - sc.pos);
+                                const V3 lv = lp - position;
+                                const float dd = length(lv);
+                                const V3 wi = vdiv(lv, dd);
+                                const float ndw = dot(sc.n, wi);
+                                if (scattering_possible(m.type, ndw)) {
+                                    const RGB f = bsdf_eval(m, sc, sc.wo, wi);
+                                    if (!is_black(f)) {
+                                        const float bpdf = bsdf_pdf(m, sc, sc.wo, wi);
+                                        const float w = power_heuristic(lpdf, bpdf);
+                                        sh_c = (rgb4(ML.L) * f) * (w * fabsf(ndw) / lpdf);
+                                        sh_o = position; sh_d = wi; sh_tmax = dd * (1.f - 1e-5f);
+                                        want_shadow = true;
+                                    }
+                                }
+                            }
+                        }
+                        {
+                            V3 wi = v3(0.f, 0.f, 0.f);
+                            float bpdf = 0.f;
+                            const RGB f = bsdf_sample(m, sc, ubx, uby, u_scat, sc.wo, wi, bpdf);
+                            if (!is_black(f)) {
+                                mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
+                                mi_ndw = dot(sc.n, wi); mi_lndw = 0.f; mi_light = s.n_rects + lj;
+                                want_mis = true;
                             }
                         }
                     }

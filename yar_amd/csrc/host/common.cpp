@@ -79,6 +79,12 @@ void SceneData::finalize_desc() {
     desc.rect_light_count = (int32_t)rect_lights.size();
     desc.point_lights = point_lights.data();
     desc.point_light_count = (int32_t)point_lights.size();
+    desc.spot_lights = spot_lights.data();
+    desc.spot_light_count = (int32_t)spot_lights.size();
+    desc.directional_lights = directional_lights.data();
+    desc.directional_light_count = (int32_t)directional_lights.size();
+    desc.mesh_lights = mesh_lights.data();
+    desc.mesh_light_count = (int32_t)mesh_lights.size();
 }
 
 }  // namespace yar

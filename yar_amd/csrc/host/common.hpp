@@ -105,6 +105,9 @@ struct SceneData {
     std::vector<yar_object_desc> objects;
     std::vector<yar_rect_light_desc> rect_lights;
     std::vector<yar_point_light_desc> point_lights;
+    std::vector<yar_spot_light_desc> spot_lights;
+    std::vector<yar_directional_light_desc> directional_lights;
+    std::vector<yar_mesh_light_desc> mesh_lights;
     yar_scene_desc desc{};
     void finalize_desc();
 };

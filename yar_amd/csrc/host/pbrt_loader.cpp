@@ -212,6 +212,12 @@ static M34 rotation(F3 axis, float r) {
 }
 
 // check_if_mesh_is_rectangle, src/lib/pbrt_scene.cpp:45-121
+static bool mesh_is_rectangle(const MeshData& mesh, float size[2], M34& xf);
+static bool mesh_is_rectangle_light(const MeshData& mesh) {
+    float size[2];
+    M34 xf{};
+    return mesh_is_rectangle(mesh, size, xf);
+}
 static bool mesh_is_rectangle(const MeshData& mesh, float size[2], M34& xf) {
     size_t nv = mesh.vertices.size() / 3;
     if (!((nv == 4 || nv == 6) && mesh.indices.size() == 6)) return false;
@@ -343,6 +349,57 @@ struct Loader {
         return id;
     }
 
+    // pbrt-parser's createLightSource_point/spot/distant (impl/semantic/Lights.cpp:50-168): "from" and
+    // "to" are transformed by the CTM when given; import_pbrt_non_area_light (pbrt_scene.cpp:903-956).
+    F3 light_point(const ParamList& ps, const char* name, F3 dflt) {
+        const Param* p = find(ps, name);
+        if (!p) return dflt;
+        if (p->fnums.size() != 3) throw ParseError{std::string("light '") + name + "' needs 3 values"};
+        return transform_point(gs.ctm, F3(p->fnums[0], p->fnums[1], p->fnums[2]));
+    }
+    void light_source(const std::string& type, const ParamList& ps) {
+        for (const char* spectral : {"I", "L"})
+            if (find(ps, spectral) && find(ps, spectral)->type != "rgb" && find(ps, spectral)->type != "color")
+                throw ParseError{std::string("light ") + spectral + " must be an rgb value"};
+        float sc[3] = {1, 1, 1};
+        get_rgb(ps, "scale", sc);
+        if (type == "point") {
+            yar_point_light_desc L{};
+            float I[3] = {1, 1, 1};
+            get_rgb(ps, "I", I);
+            const F3 from = light_point(ps, "from", F3(0, 0, 0));
+            L.position[0] = from.x; L.position[1] = from.y; L.position[2] = from.z;
+            for (int i = 0; i < 3; i++) L.intensity[i] = I[i] * sc[i];
+            sd.point_lights.push_back(L);
+        } else if (type == "spot") {
+            yar_spot_light_desc L{};
+            float I[3] = {1, 1, 1};
+            get_rgb(ps, "I", I);
+            const F3 from = light_point(ps, "from", F3(0, 0, 0)), to = light_point(ps, "to", F3(0, 0, 0));
+            const F3 dir = normalized(to - from);
+            L.position[0] = from.x; L.position[1] = from.y; L.position[2] = from.z;
+            L.direction[0] = dir.x; L.direction[1] = dir.y; L.direction[2] = dir.z;
+            constexpr float deg_2_rad = kPi / 180.f;                        // math.h:22 radians()
+            L.cone_angle = get_f(ps, "coneangle", 30.f) * deg_2_rad;
+            L.penumbra_angle = get_f(ps, "conedeltaangle", 5.f) * deg_2_rad;
+            for (int i = 0; i < 3; i++) L.intensity[i] = I[i] * sc[i];
+            sd.spot_lights.push_back(L);
+        } else if (type == "distant") {
+            if (find(ps, "L") && find(ps, "L")->fnums.size() == 2)
+                throw ParseError{"blackbody distant lights are not supported"};
+            yar_directional_light_desc L{};
+            float E[3] = {1, 1, 1};
+            get_rgb(ps, "L", E);
+            const F3 from = light_point(ps, "from", F3(0, 0, 0)), to = light_point(ps, "to", F3(0, 0, 1));
+            const F3 dir = normalized(from - to);                          // instance transform: identity
+            L.direction[0] = dir.x; L.direction[1] = dir.y; L.direction[2] = dir.z;
+            for (int i = 0; i < 3; i++) L.irradiance[i] = E[i] * sc[i];
+            sd.directional_lights.push_back(L);
+        } else {
+            throw ParseError{"unsupported light source '" + type + "'"};
+        }
+    }
+
     void shape(const std::string& type, const ParamList& ps) {
         if (type != "trianglemesh") throw ParseError{"unsupported shape '" + type + "'"};
         const Param* P = find(ps, "P");
@@ -403,11 +460,19 @@ struct Loader {
         M34 w2o = inverse_transform(o2w);
         std::memcpy(obj.object_to_world, o2w.a, sizeof(o2w.a));
         std::memcpy(obj.world_to_object, w2o.a, sizeof(w2o.a));
-        if (gs.area_light) {
+        if (gs.area_light && !mesh_is_rectangle_light(mesh)) {
+            // import_pbrt_shape (pbrt_scene.cpp:786-794): any other emitter is a triangle-mesh light
+            yar_mesh_light_desc L{};
+            L.object = (int32_t)sd.objects.size();
+            float Lrgb[3] = {1, 1, 1};
+            get_rgb(*gs.area_light, "L", Lrgb);
+            for (int i = 0; i < 3; i++) L.emitted_radiance[i] = Lrgb[i];
+            sd.mesh_lights.push_back(L);
+            obj.area_light = -2 - (int32_t)(sd.mesh_lights.size() - 1);   // resolved in finish()
+        } else if (gs.area_light) {
             float size[2];
             M34 rect{};
-            if (!mesh_is_rectangle(mesh, size, rect))
-                throw ParseError{"non-rectangular area lights (triangle-mesh lights) are not supported"};
+            mesh_is_rectangle(mesh, size, rect);
             if (gs.reverse_orientation) {
                 rect.set_column(0, -rect.column(0));
                 rect.set_column(2, -rect.column(2));
@@ -551,14 +616,7 @@ struct Loader {
             } else if (d == "LightSource") {
                 std::string type = lx.string();
                 ParamList ps = parse_params(lx);
-                if (type != "point") throw ParseError{"unsupported light source '" + type + "'"};
-                yar_point_light_desc L{};
-                float I[3] = {1, 1, 1}, sc[3] = {1, 1, 1}, from[3] = {0, 0, 0};
-                get_rgb(ps, "I", I);
-                get_rgb(ps, "scale", sc);
-                if (const Param* f = find(ps, "from")) for (int i = 0; i < 3 && i < (int)f->fnums.size(); i++) from[i] = f->fnums[i];
-                for (int i = 0; i < 3; i++) { L.position[i] = from[i]; L.intensity[i] = I[i] * sc[i]; }
-                sd.point_lights.push_back(L);
+                light_source(type, ps);
             } else if (d == "ReverseOrientation") {
                 gs.reverse_orientation = !gs.reverse_orientation;
             } else if (d == "Shape") {
@@ -615,8 +673,12 @@ struct Loader {
         } else {
             sd.desc.camera.fov_y_degrees = fov;
         }
-        if (sd.point_lights.empty() && sd.rect_lights.empty())
+        if (sd.point_lights.empty() && sd.rect_lights.empty() && sd.spot_lights.empty() &&
+            sd.directional_lights.empty() && sd.mesh_lights.empty())
             throw ParseError{"scene has no supported lights (the default directional light is not supported)"};
+        // mesh-light objects index the concatenation rect lights, then mesh lights
+        for (yar_object_desc& o : sd.objects)
+            if (o.area_light <= -2) o.area_light = (int32_t)sd.rect_lights.size() + (-2 - o.area_light);
         sd.finalize_desc();
     }
 };
