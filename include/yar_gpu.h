@@ -86,21 +86,23 @@ typedef struct yar_material_desc {
 typedef struct yar_object_desc {        /* Scene_Object, src/lib/scene_object.h:8 */
     int32_t mesh;                       /* index into meshes */
     int32_t material;                   /* index into materials, -1 for area lights */
-    int32_t area_light;                 /* index into rect_lights, rect_light_count + i for
-                                           mesh_lights[i], -1 if none */
+    int32_t area_light;                 /* index into the area lights in the order rect_lights,
+                                           sphere_lights, mesh_lights (rect_light_count + i for
+                                           sphere_lights[i], rect_light_count + sphere_light_count
+                                           + i for mesh_lights[i]), -1 if none */
     int32_t reserved;
     float   object_to_world[12];
     float   world_to_object[12];
 } yar_object_desc;
 
-typedef struct yar_rect_light_desc {    /* Diffuse_Rectangular_Light, src/lib/light.h:60 */
+typedef struct yar_rect_light_desc {    /* Diffuse_Rectangular_Light, src/lib/light.h:69 */
     float light_to_world[12];
     float emitted_radiance[3];
     float size[2];
     int32_t sample_count;
 } yar_rect_light_desc;
 
-typedef struct yar_point_light_desc {   /* Point_Light, src/lib/light.h:33 */
+typedef struct yar_point_light_desc {   /* Point_Light, src/lib/light.h:35 */
     float position[3];
     float intensity[3];
 } yar_point_light_desc;
@@ -122,9 +124,20 @@ typedef struct yar_directional_light_desc {   /* Directional_Light, src/lib/ligh
  * taken in the mesh's own coordinates, as Diffuse_Triangle_Mesh_Light_Sampler does
  * (light_sampling.cpp:126-184), so the emitter object's transform must be the identity. */
 typedef struct yar_mesh_light_desc {
-    int32_t object;                     /* the emitter object (area_light = rect_light_count + index) */
+    int32_t object;                     /* the emitter object */
     float   emitted_radiance[3];
 } yar_mesh_light_desc;
+
+/* Diffuse_Sphere_Light (src/lib/light.h:78): emits on its whole surface.  Light samples are taken
+ * on the analytic sphere (position, radius) by uniform cone sampling (light_sampling.cpp:76-121);
+ * BSDF samples count where they hit the emitter object, the tessellated sphere. */
+typedef struct yar_sphere_light_desc {
+    int32_t object;                     /* the emitter object */
+    float   position[3];                /* world-space centre */
+    float   radius;
+    float   emitted_radiance[3];
+    int32_t sample_count;               /* pbrt "nsamples" (the single-sample estimator ignores it) */
+} yar_sphere_light_desc;
 
 typedef struct yar_render_config {      /* Raytracer_Config, src/lib/raytracer_config.h:5 */
     int32_t film_width, film_height;
@@ -157,14 +170,15 @@ typedef struct yar_scene_desc {
     const yar_spot_light_desc*        spot_lights;        int32_t spot_light_count;
     const yar_directional_light_desc* directional_lights; int32_t directional_light_count;
     const yar_mesh_light_desc*        mesh_lights;        int32_t mesh_light_count;
+    const yar_sphere_light_desc*      sphere_lights;      int32_t sphere_light_count;
 } yar_scene_desc;
 
 /* ---- scene loading / generation -------------------------------------------------------- */
 typedef struct yar_scene yar_scene;
 
 /* Loads the pbrt3 subset the RAY path tracer consumes (trianglemesh; matte, metal, glass, mirror,
- * plastic, substrate; diffuse area lights (rectangles and triangle meshes), point, spot and distant
- * lights; perspective camera, Film, Sampler, Integrator "path", PixelFilter "box") with the
+ * plastic, substrate; sphere shapes, tessellated as the reference does; diffuse area lights
+ * (rectangles, spheres and triangle meshes), point, spot and distant lights; perspective camera, Film, Sampler, Integrator "path", PixelFilter "box") with the
  * reference's import semantics. */
 int  yar_scene_load_pbrt(const char* path, yar_scene** out_scene);
 const yar_scene_desc* yar_scene_get_desc(const yar_scene* scene);

@@ -452,6 +452,7 @@ struct DirectionalLight { V3 dir; RGB E; };                                   //
 // Diffuse_Triangle_Mesh_Light + its sampler (light.h:85, light_sampling.h:48): the emitter mesh,
 // its area and the area CDF of Distribution_1D (sampling.cpp:149-166)
 struct MeshLight { int object, mesh; RGB L; float area; std::vector<float> cdf; };
+struct SphereLight { V3 p; float radius; RGB L; };                           // light.h:78
 
 struct Counters { int64_t nodes = 0, leaf_refs = 0, tri_tests = 0, objects = 0; };
 
@@ -464,6 +465,7 @@ struct Scene {
     std::vector<SpotLight> spots;
     std::vector<DirectionalLight> dirs;
     std::vector<MeshLight> mesh_lights;
+    std::vector<SphereLight> spheres;
     Tree scene_tree;
     yar_camera_desc camera;
     yar_render_config config;
@@ -1049,12 +1051,15 @@ static inline float power_heuristic(float a, float b) {
 
 struct PathCounters { int64_t closest = 0, any = 0; Counters trav; };
 
-// get_emitted_radiance (direct_lighting.cpp:388-410): rectangles emit on both sides, triangle-mesh
-// lights only where the original shading normal faces the viewer
+// get_emitted_radiance (direct_lighting.cpp:388-410): rectangles and spheres emit on both sides,
+// triangle-mesh lights only where the original shading normal faces the viewer.  Area lights are
+// numbered rectangles, then spheres, then meshes.
 static RGB area_light_emission(const Scene& s, const Shading& sc) {
-    if (sc.light < (int)s.rects.size()) return s.rects[sc.light].L;
+    const int n_rects = (int)s.rects.size(), n_spheres = (int)s.spheres.size();
+    if (sc.light < n_rects) return s.rects[sc.light].L;
+    if (sc.light < n_rects + n_spheres) return s.spheres[sc.light - n_rects].L;
     if (sc.flipped) return RGB{};
-    return s.mesh_lights[sc.light - (int)s.rects.size()].L;
+    return s.mesh_lights[sc.light - n_rects - n_spheres].L;
 }
 
 static RGB light_point(const Scene& s, const Shading& sc, const PointLight& L, PathCounters& pc, Counters* tc) {
@@ -1239,7 +1244,7 @@ static RGB light_mesh(const Scene& s, const Shading& sc, int li, V2 ul, V2 ub, f
             Hit h{Inf, -1, 0, {}};
             pc.closest++;
             bool found = intersect(s, Ray{position, wi}, h, tc);
-            if (found && s.objects[h.obj].light == (int)s.rects.size() + li) {
+            if (found && s.objects[h.obj].light == (int)(s.rects.size() + s.spheres.size()) + li) {
                 V3 ln = mesh_light_normal(m, h.tri, h.b);
                 float lndw = dot(ln, -wi);
                 float lpdf = lndw <= 0.f ? 0.f : (h.t * h.t) / (lndw * ML.area);
@@ -1251,8 +1256,78 @@ static RGB light_mesh(const Scene& s, const Shading& sc, int li, V2 ul, V2 ub, f
     return out;
 }
 
+// Diffuse_Sphere_Light_Sampler (light_sampling.cpp:76-121): uniform directions in the cone the
+// sphere subtends from the shading point, mapped to the near intersection with the sphere.
+struct SphereSampler {
+    V3 ax[3];
+    float d_center, cos_theta_max, cone_pdf;
+    SphereSampler(const SphereLight& L, V3 shading_pos) {
+        ax[2] = shading_pos - L.p;
+        d_center = length(ax[2]);
+        ax[2] = vdiv(ax[2], d_center);
+        coordinate_system(ax[2], &ax[0], &ax[1]);
+        const float sin_theta_max = L.radius / d_center;
+        cos_theta_max = std::sqrt(std::max(0.f, 1.f - sin_theta_max * sin_theta_max));
+        cone_pdf = 1.f / (2.f * Pi * (1.f - cos_theta_max));
+    }
+    V3 sample(const SphereLight& L, V2 u) const {
+        const float r2 = L.radius * L.radius, d2 = d_center * d_center;
+        const float cos_theta = (1.f - u.x) + u.x * cos_theta_max;
+        const float phi = 2.f * Pi * u.y;
+        const float sin_theta2 = std::max(0.f, 1.f - cos_theta * cos_theta);
+        const float d_sample = d_center * cos_theta - std::sqrt(std::max(0.f, r2 - d2 * sin_theta2));
+        const float cos_alpha = (d2 + r2 - d_sample * d_sample) / (2.f * d_center * L.radius);   // law of cosines
+        const float sin_alpha = std::sqrt(std::max(0.f, 1.f - cos_alpha * cos_alpha));
+        const V3 dir = ((sin_alpha * std::cos(phi)) * ax[0] + (sin_alpha * std::sin(phi)) * ax[1]) + cos_alpha * ax[2];
+        return L.radius * dir + L.p;
+    }
+    bool in_cone(V3 wi) const { return dot(-ax[2], wi) >= cos_theta_max; }
+};
+
+static RGB light_sphere(const Scene& s, const Shading& sc, int li, V2 ul, V2 ub, float us, PathCounters& pc, Counters* tc) {
+    const SphereLight& L = s.spheres[li];                                    // direct_lighting.cpp:196-260
+    const SphereSampler smp(L, sc.pos);
+    RGB out;
+    {
+        const V3 lp = smp.sample(L, ul);
+        const V3 position = sc.origin_point(lp);
+        const V3 lv = lp - position;
+        const float d = length(lv);
+        const V3 wi = vdiv(lv, d);
+        const float ndw = dot(sc.n, wi);
+        if (scattering_possible(s, sc, ndw)) {
+            RGB f = bsdf_eval(s, sc, sc.wo, wi);
+            if (!is_black(f)) {
+                pc.any++;
+                if (!intersect_any(s, Ray{position, wi}, d * (1.f - 1e-5f), tc)) {
+                    const float lpdf = smp.cone_pdf;
+                    const float bpdf = bsdf_pdf(s, sc, sc.wo, wi);
+                    const float w = power_heuristic(lpdf, bpdf);
+                    out = out + (L.L * f) * (w * std::fabs(ndw) / lpdf);
+                }
+            }
+        }
+    }
+    {
+        V3 wi; float bpdf;
+        RGB f = bsdf_sample(s, sc, ub, us, sc.wo, &wi, &bpdf);
+        if (!is_black(f) && smp.in_cone(wi)) {
+            const V3 position = sc.origin_dir(wi);
+            Hit h{Inf, -1, 0, {}};
+            pc.closest++;
+            const bool found = intersect(s, Ray{position, wi}, h, tc);
+            if (found && s.objects[h.obj].light == (int)s.rects.size() + li) {
+                const float lpdf = smp.cone_pdf;
+                const float w = power_heuristic(bpdf, lpdf);
+                out = out + (L.L * f) * (w * std::fabs(dot(sc.n, wi)) / bpdf);
+            }
+        }
+    }
+    return out;
+}
+
 // estimate_direct_lighting_from_single_sample (direct_lighting.cpp:522-578): lights in the order
-// point, spot, directional, rectangular, triangle mesh
+// point, spot, directional, rectangular, sphere, triangle mesh
 static RGB direct_single(const Scene& s, const Shading& sc, float u_sel, V2 ul, V2 ub, float us, PathCounters& pc, Counters* tc) {
     int li = int(u_sel * s.total_lights);
     const float n = (float)s.total_lights;
@@ -1264,6 +1339,8 @@ static RGB direct_single(const Scene& s, const Shading& sc, float u_sel, V2 ul, 
     li -= (int)s.dirs.size();
     if (li < (int)s.rects.size()) return n * light_rect(s, sc, li, ul, ub, us, pc, tc);
     li -= (int)s.rects.size();
+    if (li < (int)s.spheres.size()) return n * light_sphere(s, sc, li, ul, ub, us, pc, tc);
+    li -= (int)s.spheres.size();
     return n * light_mesh(s, sc, li, ul, ub, us, pc, tc);
 }
 
@@ -1600,7 +1677,13 @@ static bool load_scene(const yar_scene_desc* d, Scene& s, int threads) {
         for (int t = 0; t < n; t++) { sum += areas[t]; ml.cdf[t] = sum / total; }
         s.mesh_lights.push_back(std::move(ml));
     }
-    s.total_lights = (int)(s.points.size() + s.spots.size() + s.dirs.size() + s.rects.size() + s.mesh_lights.size());
+    for (int i = 0; i < d->sphere_light_count; i++) {
+        const yar_sphere_light_desc& ld = d->sphere_lights[i];
+        s.spheres.push_back(SphereLight{{ld.position[0], ld.position[1], ld.position[2]}, ld.radius,
+                                        {ld.emitted_radiance[0], ld.emitted_radiance[1], ld.emitted_radiance[2]}});
+    }
+    s.total_lights = (int)(s.points.size() + s.spots.size() + s.dirs.size() + s.rects.size() + s.spheres.size() +
+                           s.mesh_lights.size());
     s.camera = d->camera;
     s.config = d->config;
     return true;

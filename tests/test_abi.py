@@ -38,9 +38,9 @@ def test_errors_are_status_codes_not_exits(tmp_path):
     assert L.yar_scene_load_pbrt(str(tmp_path / "missing.pbrt").encode(), C.byref(h)) == 2   # YAR_ERR_IO
     assert b"missing.pbrt" in L.yar_last_error()
     bad = tmp_path / "bad.pbrt"
-    bad.write_text('Camera "perspective"\nWorldBegin\nShape "sphere" "float radius" [1]\nWorldEnd\n')
+    bad.write_text('Camera "perspective"\nWorldBegin\nShape "cylinder" "float radius" [1]\nWorldEnd\n')
     assert L.yar_scene_load_pbrt(str(bad).encode(), C.byref(h)) == 3                         # YAR_ERR_PARSE
-    assert b"sphere" in L.yar_last_error()
+    assert b"cylinder" in L.yar_last_error()
     assert L.yar_gpu_create(None, None, C.byref(h)) == 1                                     # invalid argument
     assert L.yar_scene_generate(9, 1, 1.0, str(tmp_path / "x.pbrt").encode()) == 1
     assert L.yar_kdtree_build_mesh(None, None, 0, C.byref(h)) == 1

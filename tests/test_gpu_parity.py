@@ -186,3 +186,58 @@ def test_render_spot_directional_mesh_lights_match_oracle(c1_path, oracle, tmp_p
     assert mean_rel <= 1e-3
     assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
     assert (film["weight_sum"] == ofilm["weight_sum"]).all()
+
+
+SPHERES = '''AttributeBegin
+Translate 0.3 0.75 0.4
+AreaLightSource "diffuse" "rgb L" [ 8 7 6 ]
+Shape "sphere" "float radius" [ 0.06 ]
+AttributeEnd
+AttributeBegin
+Translate 0.7 0.22 0.5
+Rotate 35 0.3 1 0.2
+Scale 1 1.4 0.8
+Material "substrate" "rgb Kd" [ 0.5 0.3 0.2 ] "rgb Ks" [ 0.1 0.1 0.1 ] "float uroughness" [ 0.05 ] "float vroughness" [ 0.3 ]
+Shape "sphere" "float radius" [ 0.15 ]
+AttributeEnd
+AttributeBegin
+Translate 0.35 0.15 0.7
+Material "metal" "rgb eta" [ 0.2 0.9 1.1 ] "rgb k" [ 3.9 2.4 2.1 ] "float roughness" [ 0.1 ]
+Shape "sphere" "float radius" [ 0.12 ]
+AttributeEnd
+'''
+
+
+def test_render_spheres_and_sphere_light_match_oracle(c1_path, oracle, tmp_path):
+    # the config-1 room with a sphere light and two tessellated spheres under object transforms
+    # (rotation, non-uniform scale) with anisotropic coated-diffuse and GGX metal materials:
+    # import_pbrt_sphere (pbrt_scene.cpp:745-758), direct_lighting_from_sphere_light
+    # (direct_lighting.cpp:196-260), object-space traversal through world_to_object (kdtree.cpp:219-327)
+    text = open(c1_path).read().replace("WorldBegin\n", "WorldBegin\n" + SPHERES, 1)
+    p = tmp_path / "c1_spheres.pbrt"
+    p.write_text(text)
+    scene = yar_amd.Scene(str(p))
+    assert scene.desc.sphere_light_count == 1
+    rays = raygen.random_rays(scene, 1 << 18, seed=5)
+    # half the batch aimed from the room's interior at points around the three spheres
+    rng = np.random.default_rng(11)
+    h = rays.size // 2
+    centres = np.array([[0.3, 0.75, 0.4], [0.7, 0.22, 0.5], [0.35, 0.15, 0.7]], np.float32)
+    org = rng.uniform(0.02, 0.98, size=(h, 3)).astype(np.float32)
+    tgt = centres[rng.integers(0, 3, h)] + rng.normal(scale=0.12, size=(h, 3)).astype(np.float32)
+    dd = tgt - org
+    rays["origin"][:h] = org
+    rays["direction"][:h] = dd / np.linalg.norm(dd, axis=1, keepdims=True)
+    rays["tmax"][:h] = np.inf
+    g = yar_amd.Context(scene).trace_rays(rays)
+    o = oracle.OracleScene(scene).trace(rays)
+    same, max_rel, exact = _hits_agree(g, o)
+    print(f"spheres trace: hit-id agreement {same:.6f}, max rel t err {max_rel:.3g}, bit-exact t {exact:.6f}")
+    assert same >= 0.9999
+    assert max_rel <= 1e-5
+    img, film, var, st, oimg, ofilm, ovar = _render_pair(str(p), oracle, nx=4)
+    exact, close, mean_rel = _compare_images(img, oimg, film, ofilm, "c1 spheres + sphere light 16spp")
+    assert close >= 0.99
+    assert mean_rel <= 1e-3
+    assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
+    assert (film["weight_sum"] == ofilm["weight_sum"]).all()

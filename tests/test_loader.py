@@ -296,3 +296,71 @@ AttributeEnd
     objs = [d.objects[i] for i in range(d.object_count)]
     assert [o.area_light for o in objs] == [1, 0, -1]                          # mesh light after 1 rect
     assert objs[0].material == -1 and objs[2].material >= 0
+
+
+SPHERES = '''AttributeBegin
+Translate 1 2 3
+Scale 2 2 2
+AreaLightSource "diffuse" "rgb L" [ 3 4 5 ] "integer nsamples" [ 4 ]
+Shape "sphere" "float radius" [ 0.25 ]
+AttributeEnd
+AttributeBegin
+Translate -1 0 0
+Rotate 30 0 1 0
+Material "matte"
+Shape "sphere" "float radius" [ 0.25 ]
+AttributeEnd
+Material "matte"
+Shape "sphere"
+'''
+
+
+def test_sphere_shapes_are_tessellated_geodesic_spheres(tmp_path):
+    """import_pbrt_sphere (pbrt_scene.cpp:745-758): one level-6 geodesic tessellation per radius,
+    shared by every sphere of that radius, in object space under the shape's transform."""
+    d = _load(tmp_path, LIGHT + SPHERES).desc
+    assert d.mesh_count == 3 and d.object_count == 4                         # rect, two r=0.25, one r=1
+    objs = [d.objects[i] for i in range(d.object_count)]
+    assert objs[1].mesh == objs[2].mesh != objs[3].mesh
+    m = d.meshes[objs[1].mesh]
+    assert m.triangle_count == 20 * 4 ** 6 and m.vertex_count == 10 * 4 ** 6 + 2   # every shared vertex merged
+    v = _arr(m.vertices, 3 * m.vertex_count).reshape(-1, 3)
+    n = _arr(m.normals, 3 * m.vertex_count).reshape(-1, 3)
+    uv = _arr(m.uvs, 2 * m.vertex_count).reshape(-1, 2)
+    idx = _arr(m.indices, 3 * m.triangle_count).reshape(-1, 3)
+    assert np.allclose(np.linalg.norm(v, axis=1), 0.25, rtol=1e-6)
+    assert np.allclose(np.linalg.norm(n, axis=1), 1.0, rtol=1e-6)
+    assert (np.sum(n * v, axis=1) > 0.2499).all()                            # outward, near radial
+    # faces wind outwards: the geometric normal points away from the centre
+    a, b, c = v[idx[:, 0]], v[idx[:, 1]], v[idx[:, 2]]
+    assert (np.sum(np.cross(b - a, c - a) * (a + b + c), axis=1) > 0).all()
+    # the first level keeps the icosahedron's vertex 0 = (g, 1, 0)/sqrt(1 + g^2) first
+    g = (1 + np.sqrt(5)) / 2
+    assert np.allclose(v[0], np.array([g, 1, 0]) / np.sqrt(1 + g * g) * 0.25, atol=1e-7)
+    # uv: v = acos(-z)/pi is 0 at the -z pole, u = atan2(y, x)/2pi in [0, 1)
+    assert np.allclose(uv[:, 1], np.arccos(np.clip(-v[:, 2] / 0.25, -1, 1)) / np.pi, atol=2e-6)
+    assert (uv[:, 0] >= 0).all() and (uv[:, 0] < 1).all()
+    o2w = np.array(list(objs[1].object_to_world)).reshape(3, 4)
+    assert np.allclose(o2w, [[2, 0, 0, 1], [0, 2, 0, 2], [0, 0, 2, 3]])
+    w2o = np.array(list(objs[1].world_to_object)).reshape(3, 4)
+    assert np.allclose(w2o, [[0.5, 0, 0, -0.5], [0, 0.5, 0, -1], [0, 0, 0.5, -1.5]])
+    assert d.meshes[objs[3].mesh].vertex_count == m.vertex_count
+    assert np.allclose(np.linalg.norm(_arr(d.meshes[objs[3].mesh].vertices, 30).reshape(-1, 3), axis=1), 1.0)
+
+
+def test_sphere_light(tmp_path):
+    """Diffuse_Sphere_Light (pbrt_scene.cpp:857-866): centre = the transform's translation, radius
+    unscaled; area-light indices run rectangles, spheres, meshes."""
+    fan = '''AttributeBegin
+AreaLightSource "diffuse" "rgb L" [ 1 1 1 ]
+Shape "trianglemesh" "integer indices" [ 0 1 2 ] "point P" [ 0 3 0  1 3 0  1 3 1 ]
+AttributeEnd
+'''
+    d = _load(tmp_path, fan + LIGHT + SPHERES).desc
+    assert (d.rect_light_count, d.sphere_light_count, d.mesh_light_count) == (1, 1, 1)
+    sl = d.sphere_lights[0]
+    assert sl.object == 2 and list(sl.position) == [1, 2, 3] and sl.radius == 0.25
+    assert list(sl.emitted_radiance) == [3, 4, 5] and sl.sample_count == 4
+    objs = [d.objects[i] for i in range(d.object_count)]
+    assert [o.area_light for o in objs] == [2, 0, 1, -1, -1]
+    assert objs[2].material == -1 and objs[3].material >= 0

@@ -716,6 +716,16 @@ def test_spot_light_cone_and_penumbra_closed_forms():
     assert img.mean() == pytest.approx(full * k ** 4, rel=1e-2), (img.mean(), full * k ** 4)
 
 
+def test_triangle_mesh_light_seen_directly_is_one_sided():
+    # get_emitted_radiance (direct_lighting.cpp:399-408): a camera ray that hits a triangle-mesh
+    # light sees Le on its front and black on its back (the shading normal had to be flipped)
+    from tests import scenes
+    for facing_down, want in ((False, 2.5), (True, 0.0)):
+        v, idx = scenes.square_fan(1.0, 0.25, facing_down=facing_down)
+        img = _render_lit_plane(emitters=[(v, idx, [2.5, 2.5, 2.5])])
+        assert (img == np.float32(want)).all(), (facing_down, img.min(), img.max())
+
+
 def test_directional_light_closed_form():
     # direct_lighting.cpp:91-107: L = albedo/pi * E * cos(theta) for a light from direction theta
     for theta in (0.0, 0.6, 1.2):
@@ -740,3 +750,43 @@ def test_triangle_mesh_light_form_factor_and_one_sidedness():
     v, idx = scenes.square_fan(1.0, 1.0, facing_down=False)
     img = _render_lit_plane(emitters=[(v, idx, [1.0, 1.0, 1.0])], spp_side=16)
     assert img.mean() == pytest.approx(want, rel=0.02), (img.mean(), want)
+
+
+def _sphere_light_scene(tmp_path, centre, radius, spp_side=16, look_up=False, name="sl.pbrt"):
+    """A diffuse plane y = 0 (albedo 0.5) seen straight down from (0, 0.5, 0) through a 0.5-degree
+    view (or a view straight up at the light), lit only by a pbrt sphere light of radiance 1."""
+    look = "0 2 0  1 0 0" if look_up else "0 0 0  0 0 1"
+    body = (f'LookAt 0 0.5 0  {look}\nCamera "perspective" "float fov" [ 0.5 ]\n'
+            'Film "image" "integer xresolution" [ 16 ] "integer yresolution" [ 16 ]\n'
+            f'Sampler "stratified" "integer xsamples" [ {spp_side} ] "integer ysamples" [ {spp_side} ]\n'
+            'Integrator "path" "integer maxdepth" [ 1 ]\nWorldBegin\n'
+            'AttributeBegin\nMaterial "matte" "rgb Kd" [ 0.5 0.5 0.5 ]\n'
+            'Shape "trianglemesh" "integer indices" [ 0 1 2 0 2 3 ] "point P" [ -50 0 -50  -50 0 50  50 0 50  50 0 -50 ]\n'
+            'AttributeEnd\n'
+            f'AttributeBegin\nTranslate {centre[0]} {centre[1]} {centre[2]}\nAreaLightSource "diffuse" "rgb L" [ 1 1 1 ]\n'
+            f'Shape "sphere" "float radius" [ {radius} ]\nAttributeEnd\nWorldEnd\n')
+    p = tmp_path / name
+    p.write_text(body)
+    return yar_amd.Scene(str(p))
+
+
+def test_sphere_light_irradiance_closed_form(tmp_path):
+    # direct_lighting.cpp:196-260, light_sampling.cpp:76-121: a uniformly emitting sphere wholly above
+    # a surface gives irradiance pi L (R/d)^2 cos(theta) (theta between the normal and the centre), so
+    # the diffuse plane shows albedo * L * R^2 cos(theta) / d^2.  Light samples land on the analytic
+    # sphere, BSDF samples count where they hit its level-6 tessellation (inscribed, ~1e-4 smaller
+    # in solid angle), and the two MIS halves must sum to that value.
+    from tests.test_oracle_fixtures import _view_from_desc
+    for centre, cos_t, d2 in (((0, 2, 0), 1.0, 4.0), ((1, 2, 0), 2 / np.sqrt(5), 5.0), ((0.5, 1, 0.3), 1 / np.sqrt(1.34), 1.34)):
+        sc = _sphere_light_scene(tmp_path, centre, 0.5)
+        img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
+        want = 0.5 * 1.0 * 0.25 * cos_t / d2
+        assert img.mean() == pytest.approx(want, rel=1e-3), (centre, img.mean(), want)
+
+
+def test_sphere_light_seen_directly(tmp_path):
+    # get_emitted_radiance (direct_lighting.cpp:396-397): the sphere emits its L towards every viewer
+    from tests.test_oracle_fixtures import _view_from_desc
+    sc = _sphere_light_scene(tmp_path, (0, 2, 0), 0.5, spp_side=2, look_up=True)
+    img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
+    assert (img == 1.0).all(), (img.min(), img.max())

@@ -50,6 +50,11 @@ class MeshLightDesc(C.Structure):
     _fields_ = [("object", C.c_int32), ("emitted_radiance", C.c_float * 3)]
 
 
+class SphereLightDesc(C.Structure):
+    _fields_ = [("object", C.c_int32), ("position", C.c_float * 3), ("radius", C.c_float),
+                ("emitted_radiance", C.c_float * 3), ("sample_count", C.c_int32)]
+
+
 class RenderConfig(C.Structure):
     _fields_ = [("film_width", C.c_int32), ("film_height", C.c_int32),
                 ("region_x0", C.c_int32), ("region_y0", C.c_int32),
@@ -75,7 +80,8 @@ class SceneDesc(C.Structure):
                 ("camera", CameraDesc), ("config", RenderConfig),
                 ("spot_lights", C.POINTER(SpotLightDesc)), ("spot_light_count", C.c_int32),
                 ("directional_lights", C.POINTER(DirectionalLightDesc)), ("directional_light_count", C.c_int32),
-                ("mesh_lights", C.POINTER(MeshLightDesc)), ("mesh_light_count", C.c_int32)]
+                ("mesh_lights", C.POINTER(MeshLightDesc)), ("mesh_light_count", C.c_int32),
+                ("sphere_lights", C.POINTER(SphereLightDesc)), ("sphere_light_count", C.c_int32)]
 
 
 class GpuOptions(C.Structure):

@@ -151,6 +151,7 @@ struct yar_gpu_ctx {
     DevBuf<DevDirLight> dirs;
     DevBuf<DevMeshLight> mesh_lights;
     DevBuf<float> mesh_light_cdf;
+    DevBuf<DevSphere> spheres;
     DevScene scene{};
     std::vector<int32_t> tri_base;        // per object (for hit conversion on the host)
 
@@ -216,7 +217,7 @@ struct yar_gpu_ctx {
         DevBuf<int>* bi[] = {&pix_view, &tile_first, &side_view};
         for (auto* b : bi) b->release();
         vtx_uv.release(); materials.release(); rects.release(); points.release();
-        spots.release(); dirs.release(); mesh_lights.release(); mesh_light_cdf.release();
+        spots.release(); dirs.release(); mesh_lights.release(); mesh_light_cdf.release(); spheres.release();
         pixels.release(); pix_first.release(); dviews.release(); chunks.release(); chunk_slot.release();
         perm.release(); jump_lo.release(); jump_hi.release(); hit_ot.release(); tr_ot.release(); state.release(); rng.release();
         DevBuf<uint32_t>* bu[] = {&q_a, &q_b, &sh_path, &mi_path, &dl_path, &counts, &q_unsorted};
@@ -495,6 +496,12 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
         for (int t = 0; t < n; t++) { sum += areas[(size_t)t]; mcdf.push_back(sum / total); }
     }
     if (mcdf.empty()) mcdf.push_back(1.f);
+    std::vector<DevSphere> sph(std::max(1, d->sphere_light_count));
+    for (int i = 0; i < d->sphere_light_count; i++) {
+        const yar_sphere_light_desc& l = d->sphere_lights[i];
+        sph[i].p = make_float4(l.position[0], l.position[1], l.position[2], l.radius);
+        sph[i].L = make_float4(l.emitted_radiance[0], l.emitted_radiance[1], l.emitted_radiance[2], 0.f);
+    }
 
     hipStream_t s = ctx->stream;
     ctx->scene_nodes.upload(sn, s);
@@ -515,6 +522,7 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
     ctx->dirs.upload(dl, s);
     ctx->mesh_lights.upload(ml, s);
     ctx->mesh_light_cdf.upload(mcdf, s);
+    ctx->spheres.upload(sph, s);
     HIPX(hipStreamSynchronize(s));
     auto t2 = std::chrono::steady_clock::now();
 
@@ -539,13 +547,15 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
     S.dirs = ctx->dirs.p;
     S.mesh_lights = ctx->mesh_lights.p;
     S.mesh_light_cdf = ctx->mesh_light_cdf.p;
+    S.spheres = ctx->spheres.p;
     S.n_rects = d->rect_light_count;
     S.n_points = d->point_light_count;
     S.n_spots = d->spot_light_count;
     S.n_dirs = d->directional_light_count;
     S.n_mesh_lights = d->mesh_light_count;
+    S.n_spheres = d->sphere_light_count;
     S.n_lights = d->rect_light_count + d->point_light_count + d->spot_light_count + d->directional_light_count +
-                 d->mesh_light_count;                            // update_total_light_count, light.cpp:35
+                 d->sphere_light_count + d->mesh_light_count;                            // update_total_light_count, light.cpp:35
 
     yar_gpu_info& inf = ctx->info;
     inf.scene_bytes = (int64_t)(sn.size() * 8 + mn.size() * 8 + ot.size() * sizeof(ObjTrav) + tv.size() * 16 +
@@ -1250,17 +1260,28 @@ int yar_gpu_create(const yar_scene_desc* scene, const yar_gpu_options* options, 
     if (scene->object_count <= 0 || scene->mesh_count <= 0)
         return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_create: scene has no geometry");
     if (scene->rect_light_count < 0 || scene->point_light_count < 0 || scene->spot_light_count < 0 ||
-        scene->directional_light_count < 0 || scene->mesh_light_count < 0 ||
+        scene->directional_light_count < 0 || scene->mesh_light_count < 0 || scene->sphere_light_count < 0 ||
         (scene->spot_light_count > 0 && !scene->spot_lights) ||
+        (scene->sphere_light_count > 0 && !scene->sphere_lights) ||
         (scene->directional_light_count > 0 && !scene->directional_lights) ||
         (scene->mesh_light_count > 0 && !scene->mesh_lights))
         return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_create: invalid light arrays");
     if (scene->rect_light_count + scene->point_light_count + scene->spot_light_count + scene->directional_light_count +
-            scene->mesh_light_count <= 0)
+            scene->sphere_light_count + scene->mesh_light_count <= 0)
         return fail(nullptr, YAR_ERR_UNSUPPORTED, "yar_gpu_create: scene has no lights");
+    const int n_area = scene->rect_light_count + scene->sphere_light_count + scene->mesh_light_count;
+    for (int i = 0; i < scene->sphere_light_count; i++) {
+        const yar_sphere_light_desc& l = scene->sphere_lights[i];
+        if (l.object < 0 || l.object >= scene->object_count ||
+            scene->objects[l.object].area_light != scene->rect_light_count + i)
+            return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "sphere light object does not reference the light");
+        if (!(l.radius > 0.f) || !std::isfinite(l.radius))
+            return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "sphere light radius must be positive");
+    }
     for (int i = 0; i < scene->mesh_light_count; i++) {
         const int ob = scene->mesh_lights[i].object;
-        if (ob < 0 || ob >= scene->object_count || scene->objects[ob].area_light != scene->rect_light_count + i)
+        if (ob < 0 || ob >= scene->object_count ||
+            scene->objects[ob].area_light != scene->rect_light_count + scene->sphere_light_count + i)
             return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "mesh light object does not reference the light");
         static const float identity[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
         if (std::memcmp(scene->objects[ob].object_to_world, identity, sizeof(identity)) != 0)
@@ -1269,7 +1290,7 @@ int yar_gpu_create(const yar_scene_desc* scene, const yar_gpu_options* options, 
     for (int i = 0; i < scene->object_count; i++) {
         const yar_object_desc& o = scene->objects[i];
         if (o.mesh < 0 || o.mesh >= scene->mesh_count) return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "object mesh index out of range");
-        if (o.area_light >= scene->rect_light_count + scene->mesh_light_count)
+        if (o.area_light >= n_area)
             return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "object light index out of range");
         if (o.area_light < 0 && (o.material < 0 || o.material >= scene->material_count))
             return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "object material index out of range");

@@ -93,6 +93,8 @@ struct DevSpot { float4 p; float4 dir; float4 I; };
 struct DevDirLight { float4 dir; float4 E; };          // direction towards the light, irradiance
 // Diffuse_Triangle_Mesh_Light: emitter object, its triangles' global range, area CDF offset, area
 struct DevMeshLight { int obj, tri_base, tri_count, cdf; float area, pad0, pad1, pad2; float4 L; };
+// Diffuse_Sphere_Light: world-space centre with the radius in p.w, radiance
+struct DevSphere { float4 p; float4 L; };
 
 struct DevScene {
     const uint2* scene_nodes;
@@ -115,7 +117,9 @@ struct DevScene {
     const DevDirLight* dirs;
     const DevMeshLight* mesh_lights;
     const float* mesh_light_cdf;
-    int n_rects, n_points, n_spots, n_dirs, n_mesh_lights, n_lights;
+    const DevSphere* spheres;
+    // area lights are numbered rectangles, spheres, meshes (an object's area-light index)
+    int n_rects, n_points, n_spots, n_dirs, n_mesh_lights, n_spheres, n_lights;
 };
 
 // ---- ray/box slab test (Bounding_Box::intersect_by_ray_without_NaNs, bounding_box.h:63) ----

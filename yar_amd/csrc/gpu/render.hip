@@ -138,8 +138,37 @@ YG_INLINE void mesh_light_point(const DevScene& s, uint32_t gt, V3 b, V3& p, V3&
     }
 }
 
-// get_emitted_radiance (direct_lighting.cpp:388-410): rectangles emit on both sides, triangle-mesh
-// lights only where the original shading normal faces the viewer (not flipped by init_shading).
+// Diffuse_Sphere_Light_Sampler (light_sampling.cpp:76-121): the cone of directions the sphere
+// subtends from the shading point, its frame and the constant pdf of uniform sampling over it.
+struct SphereCone { V3 a0, a1, a2; float d_center, cos_max, pdf; };
+YG_INLINE SphereCone sphere_cone(const DevSphere& L, V3 pos) {
+    SphereCone c;
+    const V3 v = pos - from4(L.p);
+    c.d_center = length(v);
+    c.a2 = vdiv(v, c.d_center);
+    coordinate_system(c.a2, c.a0);
+    c.a1 = cross(c.a2, c.a0);
+    const float sin_max = L.p.w / c.d_center;
+    c.cos_max = sqrtf(fmaxf(0.f, 1.f - sin_max * sin_max));
+    c.pdf = 1.f / (2.f * kPi * (1.f - c.cos_max));
+    return c;
+}
+// A uniform direction of the cone taken to the near side of the sphere, as a point on the sphere.
+YG_INLINE V3 sphere_cone_point(const DevSphere& L, const SphereCone& c, float u0, float u1) {
+    const float r = L.p.w, r2 = r * r, d2 = c.d_center * c.d_center;
+    const float cos_theta = (1.f - u0) + u0 * c.cos_max;
+    const float phi = 2.f * kPi * u1;
+    const float sin_theta2 = fmaxf(0.f, 1.f - cos_theta * cos_theta);
+    const float d_sample = c.d_center * cos_theta - sqrtf(fmaxf(0.f, r2 - d2 * sin_theta2));
+    const float cos_alpha = (d2 + r2 - d_sample * d_sample) / (2.f * c.d_center * r);
+    const float sin_alpha = sqrtf(fmaxf(0.f, 1.f - cos_alpha * cos_alpha));
+    const V3 dir = ((sin_alpha * cosf(phi)) * c.a0 + (sin_alpha * sinf(phi)) * c.a1) + cos_alpha * c.a2;
+    return r * dir + from4(L.p);
+}
+
+// get_emitted_radiance (direct_lighting.cpp:388-410): rectangles and spheres emit on both sides,
+// triangle-mesh lights only where the original shading normal faces the viewer (not flipped by
+// init_shading).
 YG_INLINE RGB mesh_light_emission(const DevScene& s, int j, int obj, uint32_t tri, V3 b, V3 ray_d) {
     Shading sc;
     init_shading(s, ray_d, obj, tri, b, sc);
@@ -147,7 +176,8 @@ YG_INLINE RGB mesh_light_emission(const DevScene& s, int j, int obj, uint32_t tr
 }
 YG_INLINE RGB area_emission(const DevScene& s, int light, int obj, uint32_t tri, V3 b, V3 ray_d) {
     if (light < s.n_rects) return rgb4(s.rects[light].L);
-    return mesh_light_emission(s, light - s.n_rects, obj, tri, b, ray_d);
+    if (light < s.n_rects + s.n_spheres) return rgb4(s.spheres[light - s.n_rects].L);
+    return mesh_light_emission(s, light - s.n_rects - s.n_spheres, obj, tri, b, ray_d);
 }
 
 // Guided form: a pcg32 stream per (pixel, sample), disjoint from the pixel streams (whose upper
@@ -646,10 +676,13 @@ __global__ void k_mis(DevScene sc, Paths P, Queues q, StackCfg cfg) {
                         const float lndw = d4.w;
                         lpdf = (dd * dd) / (L.size.x * L.size.y * lndw);
                         Le = rgb4(L.L);
+                    } else if (light < sc.n_rects + sc.n_spheres) {
+                        lpdf = d4.w;                                // the cone pdf (direct_lighting.cpp:247)
+                        Le = rgb4(sc.spheres[light - sc.n_rects].L);
                     } else {
                         // Diffuse_Triangle_Mesh_Light_Sampler::pdf (light_sampling.cpp:163-184); a hit on
                         // the emitter's back has pdf 0 and keeps the full BSDF-sample weight, as there
-                        const DevMeshLight ML = sc.mesh_lights[light - sc.n_rects];
+                        const DevMeshLight ML = sc.mesh_lights[light - sc.n_rects - sc.n_spheres];
                         V3 lp, ln;
                         mesh_light_point(sc, h.tri, h.b, lp, ln);
                         const float lndw = dot(ln, -from4(d4));
@@ -928,8 +961,39 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                                 }
                             }
                         }
+                    } else if ((lj -= s.n_rects) < s.n_spheres) {
+                        const DevSphere SL = s.spheres[lj];             // direct_lighting.cpp:196-260
+                        const SphereCone cone = sphere_cone(SL, sc.pos);
+                        {
+                            const V3 lp = sphere_cone_point(SL, cone, ulx, uly);
+                            const V3 position = origin_dir(sc, lp - sc.pos);
+                            const V3 lv = lp - position;
+                            const float dd = length(lv);
+                            const V3 wi = vdiv(lv, dd);
+                            const float ndw = dot(sc.n, wi);
+                            if (scattering_possible(m.type, ndw)) {
+                                const RGB f = bsdf_eval(m, sc, sc.wo, wi);
+                                if (!is_black(f)) {
+                                    const float bpdf = bsdf_pdf(m, sc, sc.wo, wi);
+                                    const float w = power_heuristic(cone.pdf, bpdf);
+                                    sh_c = (rgb4(SL.L) * f) * (w * fabsf(ndw) / cone.pdf);
+                                    sh_o = position; sh_d = wi; sh_tmax = dd * (1.f - 1e-5f);
+                                    want_shadow = true;
+                                }
+                            }
+                        }
+                        {
+                            V3 wi = v3(0.f, 0.f, 0.f);
+                            float bpdf = 0.f;
+                            const RGB f = bsdf_sample(m, sc, ubx, uby, u_scat, sc.wo, wi, bpdf);
+                            if (!is_black(f) && dot(-cone.a2, wi) >= cone.cos_max) {
+                                mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
+                                mi_ndw = dot(sc.n, wi); mi_lndw = cone.pdf; mi_light = s.n_rects + lj;
+                                want_mis = true;
+                            }
+                        }
                     } else {
-                        lj -= s.n_rects;                                // direct_lighting.cpp:262-322
+                        lj -= s.n_spheres;                              // direct_lighting.cpp:262-322
                         const DevMeshLight ML = s.mesh_lights[lj];
                         {
                             // Diffuse_Triangle_Mesh_Light_Sampler::sample (light_sampling.cpp:126-161)
@@ -972,7 +1036,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                             const RGB f = bsdf_sample(m, sc, ubx, uby, u_scat, sc.wo, wi, bpdf);
                             if (!is_black(f)) {
                                 mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
-                                mi_ndw = dot(sc.n, wi); mi_lndw = 0.f; mi_light = s.n_rects + lj;
+                                mi_ndw = dot(sc.n, wi); mi_lndw = 0.f; mi_light = s.n_rects + s.n_spheres + lj;
                                 want_mis = true;
                             }
                         }

@@ -85,6 +85,8 @@ void SceneData::finalize_desc() {
     desc.directional_light_count = (int32_t)directional_lights.size();
     desc.mesh_lights = mesh_lights.data();
     desc.mesh_light_count = (int32_t)mesh_lights.size();
+    desc.sphere_lights = sphere_lights.data();
+    desc.sphere_light_count = (int32_t)sphere_lights.size();
 }
 
 }  // namespace yar

@@ -18,6 +18,8 @@ namespace yar {
 
 constexpr float kInf = std::numeric_limits<float>::infinity();
 constexpr float kPi = 3.14159265f;   // src/lib/math.h:9
+constexpr float kPi2 = 6.2831853f;   // src/lib/math.h:10
+constexpr float kOneMinusEpsilon = 0x1.fffffep-1;   // src/lib/math.h:14
 
 // Thread-local last-error text for calls that have no context (scene, kd-tree, generator).
 void set_last_error(const std::string& msg);
@@ -108,6 +110,7 @@ struct SceneData {
     std::vector<yar_spot_light_desc> spot_lights;
     std::vector<yar_directional_light_desc> directional_lights;
     std::vector<yar_mesh_light_desc> mesh_lights;
+    std::vector<yar_sphere_light_desc> sphere_lights;
     yar_scene_desc desc{};
     void finalize_desc();
 };

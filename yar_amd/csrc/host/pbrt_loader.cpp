@@ -6,7 +6,9 @@
 //                               remove_degenerate_triangles src/lib/triangle_mesh.cpp:5)
 //   check_if_mesh_is_rectangle  src/lib/pbrt_scene.cpp:45-121 (rectangular area lights)
 //   import_pbrt_material        src/lib/pbrt_scene.cpp:418-501 (matte, metal, glass)
-//   import_pbrt_non_area_light  src/lib/pbrt_scene.cpp:906-915 (point light)
+//   import_pbrt_non_area_light  src/lib/pbrt_scene.cpp:903-956 (point, spot, distant lights)
+//   import_pbrt_sphere          src/lib/pbrt_scene.cpp:745-758, 853-887 (tessellated spheres, sphere lights;
+//                               create_sphere_mesh src/lib/tessellation.cpp:43-171)
 //   import_pbrt_camera          src/lib/pbrt_scene.cpp:985-1045
 //   finalize_scene              src/lib/scene_loader.cpp:56-72
 // and the parameter defaults of the vendored pbrt-parser fork (third-party/pbrt-parser:
@@ -253,11 +255,94 @@ static bool mesh_is_rectangle(const MeshData& mesh, float size[2], M34& xf) {
     return true;
 }
 
+// The sphere of create_sphere_mesh (src/lib/tessellation.cpp:43-171).  A geodesic grid: the
+// unit-circumradius icosahedron (vertex and face tables of geometrictools.com PlatonicSolids.pdf,
+// in the reference's order), each face split `levels` times into four with the edge midpoints
+// pushed out to the unit sphere.  Every level renumbers vertices by first use while it walks the
+// faces in order, visiting each face's corners and then the midpoints of edges 01, 12 and 02, and
+// identical positions share one vertex.  Texture coordinates come from the unit position (v = 0 at
+// the -z pole), then positions are scaled by the radius and get area-weighted vertex normals
+// (calculate_normals with default parameters, triangle_mesh.cpp:285-345).
+static MeshData geodesic_sphere(float radius, int levels) {
+    const float g = (1.f + std::sqrt(5.f)) / 2.f;
+    const float k = 1.f / std::sqrt(1.f + g * g);
+    std::vector<F3> pos = {{g, 1, 0}, {-g, 1, 0}, {g, -1, 0}, {-g, -1, 0}, {1, 0, g}, {1, 0, -g},
+                           {-1, 0, g}, {-1, 0, -g}, {0, g, 1}, {0, -g, 1}, {0, g, -1}, {0, -g, -1}};
+    for (F3& p : pos) p = p * k;
+    std::vector<int32_t> idx = {0, 8, 4, 1, 10, 7, 2, 9, 11, 7, 3, 1, 0, 5, 10, 3, 9, 6, 3, 11, 9, 8, 6, 4,
+                                2, 4, 9, 3, 7, 11, 4, 2, 0, 9, 4, 6, 2, 11, 5, 0, 10, 8, 5, 0, 2, 10, 5, 7,
+                                1, 6, 8, 1, 8, 10, 6, 1, 3, 11, 7, 5};
+    struct Key {
+        uint32_t x, y, z;
+        bool operator==(const Key& o) const { return x == o.x && y == o.y && z == o.z; }
+    };
+    struct KeyHash {
+        size_t operator()(const Key& q) const { return ((size_t)q.x * 0x9E3779B97F4A7C15ull) ^ ((size_t)q.y << 21) ^ q.z; }
+    };
+    auto bits = [](float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; };
+    for (int level = 0; level < levels; level++) {
+        std::vector<F3> next_pos;
+        std::vector<int32_t> next_idx;
+        next_idx.reserve(idx.size() * 4);
+        // positions are never -0 or NaN here, so equal bit patterns are equal positions
+        std::unordered_map<Key, int32_t, KeyHash> first_use;
+        auto vertex = [&](F3 p) {
+            auto [it, fresh] = first_use.try_emplace(Key{bits(p.x), bits(p.y), bits(p.z)}, (int32_t)next_pos.size());
+            if (fresh) next_pos.push_back(p);
+            return it->second;
+        };
+        for (size_t f = 0; f < idx.size(); f += 3) {
+            const F3 a = pos[idx[f]], b = pos[idx[f + 1]], c = pos[idx[f + 2]];
+            const F3 ab = normalized(a + b), bc = normalized(b + c), ac = normalized(a + c);
+            const int32_t ia = vertex(a);
+            const int32_t ib = vertex(b);
+            const int32_t ic = vertex(c);
+            const int32_t iab = vertex(ab);
+            const int32_t ibc = vertex(bc);
+            const int32_t iac = vertex(ac);
+            next_idx.insert(next_idx.end(), {ia, iab, iac, ib, ibc, iab, ic, iac, ibc, iab, ibc, iac});
+        }
+        pos.swap(next_pos);
+        idx.swap(next_idx);
+    }
+    MeshData m;
+    m.uvs.reserve(2 * pos.size());
+    for (const F3& p : pos) {
+        const float cos_theta = std::clamp(-p.z, -1.f, 1.f);
+        const float v = std::clamp(std::acos(cos_theta) / kPi, 0.f, kOneMinusEpsilon);
+        float phi = std::atan2(p.y, p.x);
+        if (phi < 0) phi += kPi2;
+        const float u = std::clamp(phi / kPi2, 0.f, kOneMinusEpsilon);
+        m.uvs.push_back(u);
+        m.uvs.push_back(v);
+    }
+    for (F3& p : pos) p = F3(p.x * radius, p.y * radius, p.z * radius);
+    std::vector<F3> nrm(pos.size());
+    for (size_t f = 0; f < idx.size(); f += 3) {
+        const F3 a = pos[idx[f]], b = pos[idx[f + 1]], c = pos[idx[f + 2]];
+        const F3 n = cross(b - a, c - a);
+        for (int j = 0; j < 3; j++) nrm[idx[f + j]] = nrm[idx[f + j]] + n;
+    }
+    m.vertices.reserve(3 * pos.size());
+    m.normals.reserve(3 * pos.size());
+    for (size_t i = 0; i < pos.size(); i++) {
+        F3 n = nrm[i];
+        if (n.x == 0.f && n.y == 0.f && n.z == 0.f) n = F3(0, 0, 1);
+        const float l = length(n);
+        m.vertices.insert(m.vertices.end(), {pos[i].x, pos[i].y, pos[i].z});
+        m.normals.insert(m.normals.end(), {n.x / l, n.y / l, n.z / l});
+    }
+    m.indices = std::move(idx);
+    return m;
+}
+
 struct Loader {
     SceneData& sd;
     std::vector<GraphicsState> stack;
     GraphicsState gs;
     std::map<std::string, std::shared_ptr<MaterialState>> named;
+    std::map<float, int32_t> sphere_meshes;       // tessellation per radius (pbrt_scene.cpp:748-756)
+    std::vector<std::pair<int32_t, int32_t>> sphere_objects;   // (object, sphere light)
     // camera/film/sampler/integrator state
     bool have_camera = false;
     M34 camera_ctm = M34::identity();
@@ -400,7 +485,46 @@ struct Loader {
         }
     }
 
+    // import_pbrt_sphere and the sphere branch of import_pbrt_shape (pbrt_scene.cpp:745-758, 853-893):
+    // the unit-radius-scaled tessellation in object space under the current transform, which
+    // pbrt-parser keeps as the shape's transform (Geometry.cpp:323-332); an emitter becomes a sphere
+    // light centred at the transform's translation with the unscaled radius.
+    void sphere(const ParamList& ps) {
+        const float radius = get_f(ps, "radius", 1.f);
+        auto it = sphere_meshes.find(radius);
+        if (it == sphere_meshes.end()) {
+            sd.meshes.push_back(geodesic_sphere(radius, 6));
+            it = sphere_meshes.emplace(radius, (int32_t)sd.meshes.size() - 1).first;
+        }
+        yar_object_desc obj{};
+        obj.mesh = it->second;
+        obj.material = -1;
+        obj.area_light = -1;
+        const M34 w2o = inverse_transform(gs.ctm);
+        std::memcpy(obj.object_to_world, gs.ctm.a, sizeof(gs.ctm.a));
+        std::memcpy(obj.world_to_object, w2o.a, sizeof(w2o.a));
+        if (gs.area_light) {
+            if (find(*gs.area_light, "L") && find(*gs.area_light, "L")->fnums.size() == 2)
+                throw ParseError{"blackbody area lights are not supported"};
+            yar_sphere_light_desc L{};
+            L.object = (int32_t)sd.objects.size();
+            const F3 c = gs.ctm.column(3);
+            L.position[0] = c.x; L.position[1] = c.y; L.position[2] = c.z;
+            L.radius = radius;
+            float Lrgb[3] = {1, 1, 1};
+            get_rgb(*gs.area_light, "L", Lrgb);
+            for (int i = 0; i < 3; i++) L.emitted_radiance[i] = Lrgb[i];
+            L.sample_count = get_i(*gs.area_light, "nsamples", 1);
+            sphere_objects.emplace_back(L.object, (int32_t)sd.sphere_lights.size());
+            sd.sphere_lights.push_back(L);
+        } else {
+            obj.material = import_material(gs.material);
+        }
+        sd.objects.push_back(obj);
+    }
+
     void shape(const std::string& type, const ParamList& ps) {
+        if (type == "sphere") { sphere(ps); return; }
         if (type != "trianglemesh") throw ParseError{"unsupported shape '" + type + "'"};
         const Param* P = find(ps, "P");
         const Param* I = find(ps, "indices");
@@ -674,11 +798,13 @@ struct Loader {
             sd.desc.camera.fov_y_degrees = fov;
         }
         if (sd.point_lights.empty() && sd.rect_lights.empty() && sd.spot_lights.empty() &&
-            sd.directional_lights.empty() && sd.mesh_lights.empty())
+            sd.directional_lights.empty() && sd.mesh_lights.empty() && sd.sphere_lights.empty())
             throw ParseError{"scene has no supported lights (the default directional light is not supported)"};
-        // mesh-light objects index the concatenation rect lights, then mesh lights
+        // emitter objects index the concatenation rect lights, sphere lights, mesh lights
+        const int32_t n_rects = (int32_t)sd.rect_lights.size(), n_spheres = (int32_t)sd.sphere_lights.size();
         for (yar_object_desc& o : sd.objects)
-            if (o.area_light <= -2) o.area_light = (int32_t)sd.rect_lights.size() + (-2 - o.area_light);
+            if (o.area_light <= -2) o.area_light = n_rects + n_spheres + (-2 - o.area_light);
+        for (const auto& [object, light] : sphere_objects) sd.objects[object].area_light = n_rects + light;
         sd.finalize_desc();
     }
 };
