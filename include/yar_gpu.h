@@ -67,20 +67,26 @@ typedef struct yar_mesh_desc {          /* Triangle_Mesh, src/lib/triangle_mesh.
                                           * scenes, Plastic_BRDF (bsdf.cpp:204) otherwise */
 #define YAR_MATERIAL_COATED_DIFFUSE 6    /* Coated_Diffuse_Material (pbrt "substrate") ->
                                           * Ashikhmin_Shirley_Phong_BRDF (bsdf.cpp:427) */
+#define YAR_MATERIAL_PBRT3_UBER 7        /* Pbrt3_Uber_Material -> Pbrt3_Uber_BRDF (bsdf_pbrt.cpp:14-91) for
+                                            Kd/Ks, delta reflection Kr and opacity pass-through chosen per
+                                            component (delta_scattering.cpp:102-152); Kt must be black */
 
 typedef struct yar_material_desc {
     int32_t type;
     int32_t roughness_is_alpha;         /* pbrt "remaproughness" false */
-    float   reflectance[3];             /* matte/plastic/substrate Kd, glass/mirror Kr */
+    float   reflectance[3];             /* matte/plastic/substrate/uber Kd, glass/mirror Kr */
     float   transmittance[3];           /* glass Kt */
     float   eta[3];                     /* metal eta (RGB) */
     float   k[3];                       /* metal k (RGB) */
     float   eta_i;                      /* metal: IOR of adjacent dielectric (1 for pbrt) */
     float   u_roughness;                /* metal/substrate uroughness, glass/plastic roughness */
     float   v_roughness;                /* metal/substrate vroughness */
-    float   index_of_refraction;        /* glass */
+    float   index_of_refraction;        /* glass, uber */
     float   specular_r0;                /* plastic: specular scale r0 (luminance of pbrt Ks) */
-    float   specular[3];                /* coated diffuse: Fresnel r0 per channel (pbrt substrate Ks) */
+    float   specular[3];                /* coated diffuse: Fresnel r0 per channel (pbrt substrate Ks);
+                                           uber Ks */
+    float   delta_reflectance[3];       /* uber Kr */
+    float   opacity[3];                 /* uber opacity (1 = opaque) */
 } yar_material_desc;
 
 typedef struct yar_object_desc {        /* Scene_Object, src/lib/scene_object.h:8 */

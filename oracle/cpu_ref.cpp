@@ -444,6 +444,10 @@ struct Material {
     int type; RGB refl, trans, eta, k; float eta_i, alpha_x, alpha_y, ior, roughness;
     float r0 = 0.f;           // plastic specular scale
     bool pbrt3 = false;       // plastic: the pbrt3 variant (Fresnel of a 1.5 dielectric, pbrt3's flipped ior)
+    // pbrt3 uber (material_pbrt.h:51-84): Ks, Kr, opacity and the component list in the order
+    // diffuse 0, specular 1, delta reflection 2, (delta transmission 3), opacity 4
+    RGB ks, kr, opacity;
+    int ncomp = 0, comps[5] = {};
 };
 struct RectLight { M34 l2w; RGB L; V2 size; };
 struct PointLight { V3 p; RGB I; };
@@ -891,6 +895,29 @@ static float coated_pdf(const Material& m, const Shading& sc, V3 wo, V3 wi) {
     return 0.5f * (diffuse_pdf + spec_pdf);
 }
 
+// Pbrt3_Uber_BRDF (bsdf_pbrt.cpp:14-91): Kd/pi and an anisotropic GGX lobe with the Fresnel of a
+// dielectric (ior inverted when leaving), both scaled by the opacity; half of the samples cosine, half VNDF.
+static float uber_ior(const Material& m, const Shading& sc) { return sc.flipped ? 1.f / m.ior : m.ior; }
+static RGB uber_eval(const Material& m, const Shading& sc, V3 wo, V3 wi) {
+    RGB diffuse = (Pi_Inv * m.refl) * m.opacity;
+    V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
+    V3 wh = normalized(wol + wil);
+    float cos_i = dot(wil, wh);
+    float F = dielectric_fresnel(cos_i, uber_ior(m, sc));
+    float G = G_aniso(wil, wol, m.alpha_x, m.alpha_y);
+    float D = D_aniso(wh, m.alpha_x, m.alpha_y);
+    float f = (F * G * D) / (4.f * wol.z * wil.z);                                   // scattering.cpp:84
+    RGB specular = (m.ks * m.opacity) * f;
+    return diffuse + specular;
+}
+static float uber_pdf(const Material& m, const Shading& sc, V3 wo, V3 wi) {
+    float diffuse_pdf = dot(sc.n, wi) / Pi;
+    V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
+    V3 wh = normalized(wol + wil);
+    float spec_pdf = vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
+    return 0.5f * (diffuse_pdf + spec_pdf);
+}
+
 // Rough_Glass_BSDF (bsdf.cpp:268-420) with scattering.cpp:65-150 and math.h:115-127.  Material:
 // refl = Kr, trans = Kt, ior, alpha_x = GGX alpha; enter/exit from the original normal orientation.
 static void rough_glass_etas(const Material& m, const Shading& sc, float* eta_o, float* eta_i) {
@@ -967,6 +994,7 @@ static RGB bsdf_eval(const Scene& s, const Shading& sc, V3 wo, V3 wi) {
     if (m.type == YAR_MATERIAL_DIFFUSE) return Pi_Inv * m.refl;                      // bsdf.cpp:68
     if (m.type == YAR_MATERIAL_PLASTIC) return plastic_eval(m, sc, wo, wi);
     if (m.type == YAR_MATERIAL_COATED_DIFFUSE) return coated_eval(m, sc, wo, wi);
+    if (m.type == YAR_MATERIAL_PBRT3_UBER) return uber_eval(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);                               // bsdf.cpp:160
     V3 wh = normalized(wol + wil);
     float ci = dot(wil, wh);
@@ -981,6 +1009,7 @@ static float bsdf_pdf(const Scene& s, const Shading& sc, V3 wo, V3 wi) {
     if (m.type == YAR_MATERIAL_PLASTIC) return plastic_pdf(m, sc, wo, wi);
     if (m.type == YAR_MATERIAL_COATED_DIFFUSE) return coated_pdf(m, sc, wo, wi);
     if (m.type == YAR_MATERIAL_GLASS) return rough_glass_pdf(m, sc, wo, wi);
+    if (m.type == YAR_MATERIAL_PBRT3_UBER) return uber_pdf(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);                               // bsdf.cpp:190
     V3 wh = normalized(wol + wil);
     return vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
@@ -1031,6 +1060,18 @@ static RGB bsdf_sample(const Scene& s, const Shading& sc, V2 u, float u_scat, V3
         if (dot(sc.n, *wi) <= 0.f) return RGB{};
         *pdf = coated_pdf(m, sc, wo, *wi);
         return coated_eval(m, sc, wo, *wi);
+    }
+    if (m.type == YAR_MATERIAL_PBRT3_UBER) {                                        // bsdf_pbrt.cpp:56-74
+        if (u_scat < 0.5f) {
+            *wi = to_world(sc, sample_cosine(u));
+        } else {
+            V3 wol = to_local(sc, wo);
+            V3 wh = vndf_sample(u, wol, m.alpha_x, m.alpha_y);
+            *wi = to_world(sc, reflect(wol, wh));
+        }
+        if (dot(sc.n, *wi) <= 0.f) return RGB{};
+        *pdf = uber_pdf(m, sc, wo, *wi);
+        return uber_eval(m, sc, wo, *wi);
     }
     V3 wol = to_local(sc, wo);                                                       // bsdf.cpp:176
     V3 wh = vndf_sample(u, wol, m.alpha_x, m.alpha_y);
@@ -1400,6 +1441,33 @@ static RGB trace_path(const Scene& s, Ray ray, PixelSampler& ps, PathCounters& p
         V2 u_bsdf_next = ps.next2d();
         const Material& m = s.materials[sc.material];
         bool delta = (m.type == YAR_MATERIAL_GLASS && m.roughness <= 0.f) || m.type == YAR_MATERIAL_MIRROR;
+        // initialize_scattering (shading_context.cpp:174-186): the uber material picks one of its
+        // components with u_scat (get_pbrt_uber_info, delta_scattering.cpp:102-152) and hands the
+        // rescaled u_scat on; the BSDF continuation is divided by the probability of the BSDF layers
+        float bsdf_layer_probability = 1.f;
+        RGB uber_att; V3 uber_dir;
+        if (m.type == YAR_MATERIAL_PBRT3_UBER) {
+            const float fp_index = u_scat * float(m.ncomp);
+            const int ci = int(fp_index);
+            const int comp = m.comps[ci];
+            u_scat = std::clamp(fp_index - float(ci), 0.f, One_Minus_Epsilon);
+            if (comp == 2) {                                                         // delta reflection
+                const float eta = sc.flipped ? 1.f / m.ior : m.ior;
+                const float fresnel = dielectric_fresnel(dot(sc.n, sc.wo), eta);
+                uber_att = (float(m.ncomp) * fresnel) * m.kr;
+                uber_dir = reflect(sc.wo, sc.n);
+                delta = true;
+            } else if (comp == 4) {                                                  // opacity: pass through
+                uber_att = float(m.ncomp) * (splat(1.f) - m.opacity);
+                uber_dir = -sc.wo;
+                delta = true;
+            }
+            float delta_terms = 0.f;
+            for (int i = 0; i < m.ncomp; i++) if (m.comps[i] >= 2) delta_terms += 1.f;
+            bsdf_layer_probability = 1.f - delta_terms / float(m.ncomp);
+        }
+        const bool has_bsdf = !(m.type == YAR_MATERIAL_MIRROR || (m.type == YAR_MATERIAL_GLASS && m.roughness <= 0.f)) &&
+                              bsdf_layer_probability != 0.f;
         if (!delta) {
             RGB dl = direct_single(s, sc, u_sel, u_light, u_bsdf, u_scat, pc, tc);
             L = L + coeff * dl;
@@ -1408,14 +1476,21 @@ static RGB trace_path(const Scene& s, Ray ray, PixelSampler& ps, PathCounters& p
             V3 wi; float pdf;
             RGB f = bsdf_sample(s, sc, u_bsdf_next, u_scat_next, sc.wo, &wi, &pdf);
             if (is_black(f)) break;
-            coeff = coeff * (f * (std::fabs(dot(sc.n, wi)) / (pdf * 1.f)));
+            coeff = coeff * (f * (std::fabs(dot(sc.n, wi)) / (pdf * bsdf_layer_probability)));
             ray.o = sc.origin_dir(wi);
             ray.d = wi;
         } else {
+            if (has_bsdf) {                                                          // path_tracing.cpp:76-79
+                RGB dl = direct_single(s, sc, u_sel, u_light, u_bsdf, u_scat, pc, tc);
+                L = L + coeff * dl;
+            }
             RGB att; V3 dir;
             if (m.type == YAR_MATERIAL_MIRROR) {                                     // delta_scattering.cpp:27-33
                 att = m.refl;
                 dir = reflect(sc.wo, sc.n);
+            } else if (m.type == YAR_MATERIAL_PBRT3_UBER) {
+                att = uber_att;
+                dir = uber_dir;
             } else {
                 glass_scatter(sc, m, u_scat, &att, &dir);
             }
@@ -1631,6 +1706,15 @@ static bool load_scene(const yar_scene_desc* d, Scene& s, int threads) {
         };
         m.alpha_x = remap(md.u_roughness);
         m.alpha_y = remap(md.v_roughness);
+        if (md.type == YAR_MATERIAL_PBRT3_UBER) {                                    // pbrt_scene.cpp:585-608
+            m.ks = {md.specular[0], md.specular[1], md.specular[2]};
+            m.kr = {md.delta_reflectance[0], md.delta_reflectance[1], md.delta_reflectance[2]};
+            m.opacity = {md.opacity[0], md.opacity[1], md.opacity[2]};
+            if (!is_black(m.refl)) m.comps[m.ncomp++] = 0;
+            if (!is_black(m.ks)) m.comps[m.ncomp++] = 1;
+            if (!is_black(m.kr)) m.comps[m.ncomp++] = 2;
+            if (!(m.opacity.r == 1.f && m.opacity.g == 1.f && m.opacity.b == 1.f)) m.comps[m.ncomp++] = 4;
+        }
     }
     for (int i = 0; i < d->rect_light_count; i++) {
         const yar_rect_light_desc& ld = d->rect_lights[i];
@@ -1984,6 +2068,39 @@ int32_t oref_coated_sample(const float* kd, const float* r0, float ax, float ay,
     m.alpha_x = ax; m.alpha_y = ay;
     s.materials.push_back(m);
     Shading sc = probe_frame();
+    V3 wi;
+    float pdf = 0.f;
+    const RGB f = bsdf_sample(s, sc, V2{u3[0], u3[1]}, u3[2], V3(wo[0], wo[1], wo[2]), &wi, &pdf);
+    wi_out[0] = wi.x; wi_out[1] = wi.y; wi_out[2] = wi.z;
+    f_out[0] = f.r; f_out[1] = f.g; f_out[2] = f.b;
+    *pdf_out = pdf;
+    return (f.r != 0.f || f.g != 0.f || f.b != 0.f) ? 1 : 0;
+}
+
+// pbrt3 uber BRDF in the local frame (n = z, T = x, B = y): evaluate + pdf, or sample; flipped =
+// the original normal faced away (ior inverted).
+static Material uber_probe(const float* kd, const float* ks, const float* opacity, float ior, float ax, float ay) {
+    Material m{};
+    m.type = YAR_MATERIAL_PBRT3_UBER; m.ior = ior; m.alpha_x = ax; m.alpha_y = ay;
+    m.refl = RGB{kd[0], kd[1], kd[2]}; m.ks = RGB{ks[0], ks[1], ks[2]}; m.opacity = RGB{opacity[0], opacity[1], opacity[2]};
+    return m;
+}
+void oref_uber_eval(const float* kd, const float* ks, const float* opacity, float ior, float ax, float ay, int32_t flipped,
+                    const float* wo, const float* wi, float* f_out, float* pdf_out) {
+    const Material m = uber_probe(kd, ks, opacity, ior, ax, ay);
+    Shading sc = probe_frame();
+    sc.flipped = flipped != 0;
+    const V3 o(wo[0], wo[1], wo[2]), i(wi[0], wi[1], wi[2]);
+    const RGB f = uber_eval(m, sc, o, i);
+    f_out[0] = f.r; f_out[1] = f.g; f_out[2] = f.b;
+    *pdf_out = uber_pdf(m, sc, o, i);
+}
+int32_t oref_uber_sample(const float* kd, const float* ks, const float* opacity, float ior, float ax, float ay,
+                         int32_t flipped, const float* wo, const float* u3, float* wi_out, float* f_out, float* pdf_out) {
+    Scene s;
+    s.materials.push_back(uber_probe(kd, ks, opacity, ior, ax, ay));
+    Shading sc = probe_frame();
+    sc.flipped = flipped != 0;
     V3 wi;
     float pdf = 0.f;
     const RGB f = bsdf_sample(s, sc, V2{u3[0], u3[1]}, u3[2], V3(wo[0], wo[1], wo[2]), &wi, &pdf);

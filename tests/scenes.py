@@ -46,6 +46,9 @@ class DescScene:
             mt[i].v_roughness = m.get("v_roughness", m.get("u_roughness", 0.0))
             mt[i].index_of_refraction = m.get("ior", 1.5)
             mt[i].roughness_is_alpha = m.get("roughness_is_alpha", 1)
+            mt[i].specular[:] = m.get("specular", [0, 0, 0])
+            mt[i].delta_reflectance[:] = m.get("delta_reflectance", [0, 0, 0])
+            mt[i].opacity[:] = m.get("opacity", [1, 1, 1])
         ob = (_abi.ObjectDesc * len(objects))()
         for i, o in enumerate(objects):
             ob[i].mesh = o["mesh"]

@@ -241,3 +241,23 @@ def test_render_spheres_and_sphere_light_match_oracle(c1_path, oracle, tmp_path)
     assert mean_rel <= 1e-3
     assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
     assert (film["weight_sum"] == ofilm["weight_sum"]).all()
+
+
+def test_render_uber_matches_oracle(c1_path, oracle, tmp_path):
+    # the config-1 room with pbrt3 uber walls and triangles: Kd + Ks (BSDF layers only), Kd + Kr
+    # (Fresnel mirror chosen per component, direct light in both branches) and Kd + Ks + opacity
+    # (pass-through): Pbrt3_Uber_BRDF (bsdf_pbrt.cpp:14-91), get_pbrt_uber_info (delta_scattering.cpp:102-152)
+    lines = open(c1_path).read().splitlines()
+    mats = [i for i, l in enumerate(lines) if l.startswith('Material "matte"')]
+    lines[mats[0]] = 'Material "uber" "rgb Kd" [ 0.5 0.4 0.3 ] "rgb Ks" [ 0.3 0.3 0.3 ] "float roughness" [ 0.05 ]'
+    lines[mats[1]] = 'Material "uber" "rgb Kd" [ 0.2 0.3 0.4 ] "rgb Ks" [ 0 0 0 ] "rgb Kr" [ 0.8 0.8 0.8 ] "float index" [ 1.7 ]'
+    lines[mats[-1]] = ('Material "uber" "rgb Kd" [ 0.6 0.3 0.2 ] "rgb Ks" [ 0.2 0.2 0.2 ] "rgb opacity" [ 0.5 0.6 0.7 ] '
+                       '"float uroughness" [ 0.02 ] "float vroughness" [ 0.3 ]')
+    p = tmp_path / "c1_uber.pbrt"
+    p.write_text("\n".join(lines) + "\n")
+    img, film, var, st, oimg, ofilm, ovar = _render_pair(str(p), oracle, nx=4)
+    exact, close, mean_rel = _compare_images(img, oimg, film, ofilm, "c1 uber 16spp")
+    assert close >= 0.99
+    assert mean_rel <= 1e-3
+    assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
+    assert (film["weight_sum"] == ofilm["weight_sum"]).all()

@@ -364,3 +364,31 @@ AttributeEnd
     objs = [d.objects[i] for i in range(d.object_count)]
     assert [o.area_light for o in objs] == [2, 0, 1, -1, -1]
     assert objs[2].material == -1 and objs[3].material >= 0
+
+
+def test_uber_material(tmp_path):
+    """import_pbrt_material's uber branch (pbrt_scene.cpp:571-636) with pbrt-parser's defaults
+    (Kd = Ks = 0.25, Kr = Kt = 0, opacity 1, roughness 0.1, index 1.5); v roughness follows u."""
+    body = LIGHT + '''Material "uber"
+Shape "trianglemesh" "integer indices" [ 0 1 2 ] "point P" [ 0 0 0  1 0 0  0 1 0 ]
+Material "uber" "rgb Kd" [ 0.1 0.2 0.3 ] "rgb Ks" [ 0 0 0 ] "rgb Kr" [ 0.5 0.5 0.5 ] "rgb opacity" [ 0.4 0.5 0.6 ] "float index" [ 1.33 ] "float uroughness" [ 0.02 ]
+Shape "trianglemesh" "integer indices" [ 0 1 2 ] "point P" [ 0 0 1  1 0 1  0 1 1 ]
+'''
+    d = _load(tmp_path, body).desc
+    m0, m1 = d.materials[0], d.materials[1]
+    assert m0.type == m1.type == yar_amd._abi.YAR_MATERIAL_PBRT3_UBER
+    assert np.allclose(list(m0.reflectance), 0.25) and np.allclose(list(m0.specular), 0.25)
+    assert list(m0.delta_reflectance) == [0, 0, 0] and list(m0.opacity) == [1, 1, 1]
+    assert m0.u_roughness == m0.v_roughness == pytest.approx(0.1) and m0.index_of_refraction == 1.5
+    assert m0.roughness_is_alpha == 0
+    assert np.allclose(list(m1.reflectance), [0.1, 0.2, 0.3]) and list(m1.specular) == [0, 0, 0]
+    assert list(m1.delta_reflectance) == [0.5, 0.5, 0.5] and np.allclose(list(m1.opacity), [0.4, 0.5, 0.6])
+    assert m1.u_roughness == m1.v_roughness == pytest.approx(0.02) and m1.index_of_refraction == pytest.approx(1.33)
+
+
+@pytest.mark.parametrize("params", ['"rgb Kt" [ 0.5 0.5 0.5 ]', '"texture Kd" "checks"', '"float alpha" [ 0.5 ]',
+                                    '"rgb Kd" [ 0 0 0 ] "rgb Ks" [ 0 0 0 ]', '"bool remaproughness" "false"'])
+def test_uber_unsupported_forms_are_parse_errors(tmp_path, params):
+    body = LIGHT + f'Material "uber" {params}\nShape "trianglemesh" "integer indices" [ 0 1 2 ] "point P" [ 0 0 0  1 0 0  0 1 0 ]\n'
+    with pytest.raises(yar_amd.YarError):
+        _load(tmp_path, body)

@@ -790,3 +790,154 @@ def test_sphere_light_seen_directly(tmp_path):
     sc = _sphere_light_scene(tmp_path, (0, 2, 0), 0.5, spp_side=2, look_up=True)
     img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
     assert (img == 1.0).all(), (img.min(), img.max())
+
+
+def uber(kd, ks, opacity, ior, ax, ay, wo, wi=None, u=None, flipped=0):
+    L = O.lib()
+    P = C.c_void_p
+    F = C.c_float
+    L.oref_uber_eval.argtypes = [P, P, P, F, F, F, C.c_int32, P, P, P, P]
+    L.oref_uber_sample.argtypes = [P, P, P, F, F, F, C.c_int32, P, P, P, P, P]
+    L.oref_uber_sample.restype = C.c_int32
+    kd, ks, op, wo = _f32(kd), _f32(ks), _f32(opacity), _f32(wo)
+    f = np.zeros(3, np.float32)
+    pdf = C.c_float()
+    if wi is not None:
+        wi = _f32(wi)
+        L.oref_uber_eval(kd.ctypes.data, ks.ctypes.data, op.ctypes.data, ior, ax, ay, flipped, wo.ctypes.data,
+                         wi.ctypes.data, f.ctypes.data, C.byref(pdf))
+        return f.astype(np.float64), pdf.value
+    u = _f32(u)
+    out = np.zeros(3, np.float32)
+    ok = L.oref_uber_sample(kd.ctypes.data, ks.ctypes.data, op.ctypes.data, ior, ax, ay, flipped, wo.ctypes.data,
+                            u.ctypes.data, out.ctypes.data, f.ctypes.data, C.byref(pdf))
+    return bool(ok), out.astype(np.float64), f.astype(np.float64), pdf.value
+
+
+def _smith_lambda(v, ax, ay):
+    c2 = v[2] ** 2
+    s2 = 1 - c2
+    a2 = (v[0] ** 2 * ax * ax + v[1] ** 2 * ay * ay) / s2 if s2 > 0 else 0.0
+    return 0.5 * (-1 + np.sqrt(1 + a2 * s2 / c2))
+
+
+def test_uber_closed_forms():
+    # Pbrt3_Uber_BRDF (bsdf_pbrt.cpp:14-91): Ks = 0 leaves Kd * opacity / pi; Kd = 0 leaves the
+    # textbook anisotropic GGX reflection F(wi.h) G D / (4 cos_o cos_i) of a dielectric of index
+    # eta (1/eta when the surface is left), scaled by Ks * opacity; the pdf averages cosine and VNDF.
+    rng = np.random.default_rng(9)
+    kd, op = np.array([0.3, 0.5, 0.7]), np.array([0.9, 0.6, 0.4])
+    for _ in range(5):
+        wo = _hemi(rng.uniform(0, 1.4), rng.uniform(0, 2 * np.pi))
+        wi = _hemi(rng.uniform(0, 1.4), rng.uniform(0, 2 * np.pi))
+        f, _ = uber(kd, [0, 0, 0], op, 1.5, 0.2, 0.2, wo, wi)
+        assert np.allclose(f, kd * op / np.pi, rtol=1e-6)
+    ax, ay = 0.12, 0.35
+    for flipped in (0, 1):
+        eta = 1.5 if not flipped else 1 / 1.5
+        for _ in range(20):
+            wo = _hemi(rng.uniform(0, 1.4), rng.uniform(0, 2 * np.pi))
+            wi = _hemi(rng.uniform(0, 1.4), rng.uniform(0, 2 * np.pi))
+            h = (wo + wi) / np.linalg.norm(wo + wi)
+            D = 1 / (np.pi * ax * ay * (h[0] ** 2 / ax ** 2 + h[1] ** 2 / ay ** 2 + h[2] ** 2) ** 2)
+            G = 1 / (1 + _smith_lambda(wo, ax, ay) + _smith_lambda(wi, ax, ay))
+            want = _textbook_fresnel(wi.dot(h), 1.0, eta) * G * D / (4 * wo[2] * wi[2])
+            f, _ = uber([0, 0, 0], [1, 1, 1], [1, 1, 1], 1.5, ax, ay, wo, wi, flipped=flipped)
+            assert np.allclose(f, want, rtol=2e-4), (flipped, f, want)
+            # opacity scales both lobes channel by channel; f is reciprocal
+            f2, _ = uber(kd, [1, 1, 1], op, 1.5, ax, ay, wo, wi, flipped=flipped)
+            assert np.allclose(f2, op * (kd / np.pi + want), rtol=2e-4)
+            f3, _ = uber(kd, [1, 1, 1], op, 1.5, ax, ay, wi, wo, flipped=flipped)
+            assert np.allclose(f2, f3, rtol=1e-5)
+    # normal incidence: G1 = 1, D(n) = 1/(pi ax ay), so pdf = (1/pi + 1/(4 pi ax ay)) / 2
+    _, pdf = uber(kd, [1, 1, 1], op, 1.5, ax, ay, [0, 0, 1], [0, 0, 1])
+    assert pdf == pytest.approx(0.5 * (1 / np.pi + 1 / (4 * np.pi * ax * ay)), rel=2e-5)
+
+
+def test_uber_sampling_matches_quadrature():
+    rng = np.random.default_rng(21)
+    kd, ks, op = [0.4, 0.3, 0.2], [0.5, 0.5, 0.5], [0.8, 0.8, 0.8]
+    ax, ay = 0.1, 0.3
+    for theta_o in (0.3, 1.0):
+        wo = _hemi(theta_o, 0.7)
+        nt, nphi = 400, 200
+        ct = (np.arange(nt) + 0.5) / nt
+        ph = (np.arange(nphi) + 0.5) / nphi * 2 * np.pi
+        alb, mass = 0.0, 0.0
+        for c in ct:
+            st = np.sqrt(1 - c * c)
+            for p in ph[::2]:
+                wi = np.array([st * np.cos(p), st * np.sin(p), c])
+                f, pdf = uber(kd, ks, op, 1.5, ax, ay, wo, wi)
+                alb += f[1] * c
+                mass += pdf
+        dA = (1.0 / nt) * (2 * np.pi / (nphi / 2))
+        alb *= dA
+        mass *= dA
+        n = 6000
+        est, valid = 0.0, 0
+        for u in rng.uniform(size=(n, 3)):
+            ok, wi, f, pdf = uber(kd, ks, op, 1.5, ax, ay, wo, u=u)
+            if ok:
+                est += f[1] * wi[2] / pdf
+                valid += 1
+        est /= n
+        assert est == pytest.approx(alb, rel=0.03), (theta_o, est, alb)
+        assert mass == pytest.approx(valid / n, abs=0.02), (theta_o, mass, valid / n)
+
+
+def test_uber_delta_reflection_is_fresnel_weighted_mirror():
+    # an uber surface with only Kr (one component, no BSDF layer) is a mirror whose reflection is
+    # weighted by the dielectric Fresnel term at the incidence angle (get_pbrt_uber_info,
+    # delta_scattering.cpp:115-131): the 45-degree mirror scene then shows
+    # Kr * F * albedo/pi * I * cos / d^2 with F = F(cos 45, 1.5) where the quad faces the camera, and
+    # F = 1 where it faces away (the ray leaves the dielectric: 45 degrees is past the critical 41.8)
+    from tests.scenes import DescScene
+    from tests.test_oracle_fixtures import _view_from_desc
+    from yar_amd import _abi
+    s = 0.3 / np.sqrt(2)
+    c = np.array([1.0, 1.0, 0.0])
+    e1, e2 = np.array([0, 0, 0.3]), np.array([s, -s, 0])
+    plane = {"vertices": [[-50, 0, -50], [-50, 0, 50], [50, 0, 50], [50, 0, -50]], "indices": [0, 1, 2, 0, 2, 3]}
+    kr, albedo, intensity = [0.9, 0.6, 0.3], 0.5, 2.0
+    for facing, F in (([0, 2, 1, 0, 3, 2], _textbook_fresnel(np.cos(np.pi / 4), 1.0, 1.5)), ([0, 1, 2, 0, 2, 3], 1.0)):
+        quad = {"vertices": [c - e1 - e2, c + e1 - e2, c + e1 + e2, c - e1 + e2], "indices": facing}
+        sc = DescScene(
+            meshes=[plane, quad],
+            materials=[{"type": _abi.YAR_MATERIAL_DIFFUSE, "reflectance": [albedo] * 3},
+                       {"type": _abi.YAR_MATERIAL_PBRT3_UBER, "reflectance": [0, 0, 0], "specular": [0, 0, 0],
+                        "delta_reflectance": kr, "ior": 1.5, "u_roughness": 0.2}],
+            objects=[{"mesh": 0, "material": 0}, {"mesh": 1, "material": 1}],
+            points=[{"position": [4.0, 2.0, 0.0], "intensity": [intensity] * 3}],
+            camera={"c2w": [0, 0, -1, 0, 0, 1, 0, 1, 1, 0, 0, 0], "fov": 1.0},
+            config={"film_width": 16, "film_height": 16, "region": (0, 0, 16, 16), "nx": 2, "ny": 2, "maxdepth": 2})
+        img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
+        want = np.array(kr) * F * albedo / np.pi * intensity * 2.0 / 13.0 ** 1.5
+        assert np.allclose(img.mean(axis=(0, 1)), want, rtol=2e-3), (F, img.mean(axis=(0, 1)), want)
+
+
+def test_uber_opacity_passes_light_through():
+    # Kd plus opacity o: two components, each picked with probability 1/2.  The pass-through
+    # (delta_scattering.cpp:135-140) carries 2 (1 - o) of what lies behind, the BSDF layer is divided
+    # by its selection probability 1/2, and direct lighting is estimated in both branches
+    # (path_tracing.cpp:53-79), so a point-lit uber plane in front of an emitter of radiance Le shows
+    # Kd o / pi * I / d^2 + (1 - o) Le.
+    from tests.scenes import DescScene
+    from tests.test_oracle_fixtures import _view_from_desc
+    from yar_amd import _abi
+    kd, o, intensity, le = 0.5, 0.3, 2.0, 1.0
+    big = lambda y, up: {"vertices": [[-50, y, -50], [-50, y, 50], [50, y, 50], [50, y, -50]],
+                         "indices": [0, 2, 1, 0, 3, 2] if up else [0, 1, 2, 0, 2, 3]}
+    rect_l2w = [1, 0, 0, 0, 0, 0, 1, -1, 0, -1, 0, 0]                      # columns x, y, z = +y normal
+    sc = DescScene(
+        meshes=[big(0.0, True), big(-1.0, True)],
+        materials=[{"type": _abi.YAR_MATERIAL_PBRT3_UBER, "reflectance": [kd] * 3, "specular": [0, 0, 0],
+                    "opacity": [o] * 3, "ior": 1.5, "u_roughness": 0.2}],
+        objects=[{"mesh": 0, "material": 0}, {"mesh": 1, "light": 0}],
+        rects=[{"l2w": rect_l2w, "L": [le] * 3, "size": [100.0, 100.0]}],
+        points=[{"position": [0.0, 1.0, 0.0], "intensity": [intensity] * 3}],
+        camera={"c2w": [1, 0, 0, 0, 0, 0, 1, 0.5, 0, -1, 0, 0], "fov": 0.5},
+        config={"film_width": 16, "film_height": 16, "region": (0, 0, 16, 16), "nx": 8, "ny": 8, "maxdepth": 1})
+    img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
+    want = kd * o / np.pi * intensity + (1 - o) * le
+    assert img.mean() == pytest.approx(want, rel=2e-3), (img.mean(), want)

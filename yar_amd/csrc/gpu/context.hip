@@ -416,7 +416,34 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
         if (m.type == YAR_MATERIAL_COATED_DIFFUSE)               // k = Fresnel r0 per channel
             g.k = make_float4(m.specular[0], m.specular[1], m.specular[2], 0.f);
         if (m.type == YAR_MATERIAL_GLASS && m.u_roughness > 0.f) g.type = kRoughGlassType;   // Rough_Glass_BSDF
-        if (m.type < YAR_MATERIAL_DIFFUSE || m.type > YAR_MATERIAL_COATED_DIFFUSE) throw HipError{"unsupported material type", YAR_ERR_UNSUPPORTED};
+        if (m.type == YAR_MATERIAL_PBRT3_UBER) {
+            // refl = Kd | ior, trans = Ks | component code, eta = Kr, k = opacity; the components in
+            // the reference's order (pbrt_scene.cpp:585-608): diffuse 0, specular 1, delta reflection 2,
+            // opacity 4 (delta transmission 3 is not supported)
+            if (m.transmittance[0] != 0.f || m.transmittance[1] != 0.f || m.transmittance[2] != 0.f)
+                throw HipError{"uber delta transmission is not supported", YAR_ERR_UNSUPPORTED};
+            uint32_t comps[4], n = 0, ndelta = 0;
+            auto black = [](const float* c) { return c[0] == 0.f && c[1] == 0.f && c[2] == 0.f; };
+            if (!black(m.reflectance)) comps[n++] = 0;
+            if (!black(m.specular)) comps[n++] = 1;
+            if (!black(m.delta_reflectance)) comps[n++] = 2;
+            if (!(m.opacity[0] == 1.f && m.opacity[1] == 1.f && m.opacity[2] == 1.f)) comps[n++] = 4;
+            if (n == 0) throw HipError{"uber material with no scattering component", YAR_ERR_INVALID_ARGUMENT};
+            uint32_t code = n;
+            for (uint32_t j = 0; j < n; j++) {
+                code |= comps[j] << (4 + 4 * j);
+                if (comps[j] >= 2) ndelta++;
+            }
+            code |= ndelta << 24;
+            float codef;
+            std::memcpy(&codef, &code, 4);
+            g.type = kUberType;
+            g.refl = make_float4(m.reflectance[0], m.reflectance[1], m.reflectance[2], m.index_of_refraction);
+            g.trans = make_float4(m.specular[0], m.specular[1], m.specular[2], codef);
+            g.eta = make_float4(m.delta_reflectance[0], m.delta_reflectance[1], m.delta_reflectance[2], 0.f);
+            g.k = make_float4(m.opacity[0], m.opacity[1], m.opacity[2], 0.f);
+        }
+        if (m.type < YAR_MATERIAL_DIFFUSE || m.type > YAR_MATERIAL_PBRT3_UBER) throw HipError{"unsupported material type", YAR_ERR_UNSUPPORTED};
     }
     std::vector<DevRect> rl(std::max(1, d->rect_light_count));
     for (int i = 0; i < d->rect_light_count; i++) {

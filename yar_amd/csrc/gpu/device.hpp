@@ -74,8 +74,10 @@ YG_INLINE V3 xform_vector(const M34& m, V3 v) {
 // mesh's root node index and triangle base packed into the w lanes.
 struct ObjTrav { float4 w2o0, w2o1, w2o2, lo_node, hi_tri; };
 
-// Device-only material code of rough glass (yar_material_desc type GLASS with roughness > 0).
+// Device-only material codes: rough glass (yar_material_desc type GLASS with roughness > 0) and
+// the pbrt3 uber material.
 constexpr int kRoughGlassType = 7;
+constexpr int kUberType = 8;
 struct DevMaterial {      // 64 bytes
     int type; float eta_i, alpha_x, alpha_y;
     float4 refl;          // diffuse reflectance / glass Kr

@@ -828,6 +828,21 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                 next2(ubx, uby);
                 next2(unx, uny);
                 const DevMaterial m = s.materials[sc.material];
+                // initialize_scattering (shading_context.cpp:174-186): the uber material picks one of
+                // its components with u_scat and hands the rescaled u_scat on; the BSDF continuation
+                // is divided by the probability of its BSDF layers (path_tracing.cpp:68)
+                float u_s = u_scat, bsdf_layer_probability = 1.f;
+                int uber_delta = 0;                                     // 1 delta reflection, 2 pass-through
+                if (m.type == kUberType) {
+                    const uint32_t code = fbits(m.trans.w);
+                    const uint32_t ncomp = code & 15u;
+                    const float fp_index = u_scat * (float)ncomp;
+                    const int ci = (int)fp_index;
+                    const uint32_t comp = (code >> (4 + 4 * ci)) & 15u;
+                    u_s = fminf(fmaxf(fp_index - (float)ci, 0.f), kOneMinusEps);
+                    uber_delta = comp == 2u ? 1 : (comp == 4u ? 2 : 0);
+                    bsdf_layer_probability = 1.f - (float)((code >> 24) & 15u) / (float)ncomp;
+                }
                 if (m.type == 4) {
                     // perfect reflector (delta_scattering.cpp:27-33)
                     const V3 dir = reflect(sc.wo, sc.n);
@@ -864,155 +879,109 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                     flags |= kDelta;
                     want_active = true;
                 } else {
-                    // estimate_direct_lighting_from_single_sample (direct_lighting.cpp:522-578)
-                    const int li = int(u_sel * s.n_lights);
-                    pending_dl = true;
-                    int lj = li;
-                    if (lj < s.n_points) {
-                        const DevPoint pl = s.points[lj];               // direct_lighting.cpp:26-47
-                        const V3 lp = from4(pl.p);
-                        const V3 position = origin_dir(sc, lp - sc.pos);
-                        const V3 lv = lp - position;
-                        const float dist = length(lv);
-                        const V3 ldir = vdiv(lv, dist);
-                        const float ndl = dot(sc.n, ldir);
-                        if (ndl > 0.f) {
-                            const RGB f = bsdf_eval(m, sc, sc.wo, ldir);
-                            sh_c = (rgb4(pl.I) * f) * (ndl / (dist * dist));
-                            sh_o = position; sh_d = ldir; sh_tmax = dist * (1.f - 1e-5f);
-                            want_shadow = true;
-                        }
-                    } else if ((lj -= s.n_points) < s.n_spots) {
-                        const DevSpot sl = s.spots[lj];                 // direct_lighting.cpp:49-89
-                        const V3 lp = from4(sl.p);
-                        const V3 position = origin_dir(sc, lp - sc.pos);
-                        const V3 lv = lp - position;
-                        const float dist = length(lv);
-                        const V3 wi = vdiv(lv, dist);
-                        const float cone_cos = sl.dir.w;
-                        const float wi_cos = dot(-wi, from4(sl.dir));
-                        if (!(wi_cos < cone_cos)) {
-                            float penumbra = 1.f;
-                            const float penumbra_cos = sl.I.w;
-                            if (wi_cos < penumbra_cos) {
-                                const float kp = (wi_cos - cone_cos) / (penumbra_cos - cone_cos);
-                                penumbra = (kp * kp) * (kp * kp);
-                            }
-                            const float ndw = dot(sc.n, wi);
-                            if (scattering_possible(m.type, ndw)) {
-                                const RGB f = bsdf_eval(m, sc, sc.wo, wi);
-                                if (!is_black(f)) {
-                                    sh_c = (rgb4(sl.I) * f) * (penumbra * fabsf(ndw) / (dist * dist));
-                                    sh_o = position; sh_d = wi; sh_tmax = dist * (1.f - 1e-5f);
-                                    want_shadow = true;
-                                }
-                            }
-                        }
-                    } else if ((lj -= s.n_spots) < s.n_dirs) {
-                        const DevDirLight dl = s.dirs[lj];              // direct_lighting.cpp:91-107
-                        const V3 ldir = from4(dl.dir);
-                        const float ndl = dot(sc.n, ldir);
-                        if (ndl > 0.f) {
-                            const RGB f = bsdf_eval(m, sc, sc.wo, ldir);
-                            sh_c = (rgb4(dl.E) * f) * ndl;
-                            sh_o = origin_dir(sc, ldir); sh_d = ldir; sh_tmax = f_inf();
-                            want_shadow = true;
-                        }
-                    } else if ((lj -= s.n_dirs) < s.n_rects) {
-                        const int ri = lj;                              // direct_lighting.cpp:109-194
-                        const DevRect R = s.rects[ri];
-                        const M34 l2w = {R.l0, R.l1, R.l2};
-                        const V3 ln = v3(R.l0.z, R.l1.z, R.l2.z);
-                        {
-                            const V3 local = v3(R.size.x * (ulx - 0.5f), R.size.y * (uly - 0.5f), 0.0f);
-                            const V3 lp = xform_point(l2w, local);
+                    if (bsdf_layer_probability != 0.f) {
+                        // estimate_direct_lighting_from_single_sample (direct_lighting.cpp:522-578)
+                        const int li = int(u_sel * s.n_lights);
+                        pending_dl = true;
+                        int lj = li;
+                        if (lj < s.n_points) {
+                            const DevPoint pl = s.points[lj];               // direct_lighting.cpp:26-47
+                            const V3 lp = from4(pl.p);
                             const V3 position = origin_dir(sc, lp - sc.pos);
                             const V3 lv = lp - position;
-                            const float dd = length(lv);
-                            const V3 wi = vdiv(lv, dd);
-                            const float lndw = dot(ln, -wi);
-                            if (lndw > 1e-4f) {
+                            const float dist = length(lv);
+                            const V3 ldir = vdiv(lv, dist);
+                            const float ndl = dot(sc.n, ldir);
+                            if (ndl > 0.f) {
+                                const RGB f = bsdf_eval(m, sc, sc.wo, ldir);
+                                sh_c = (rgb4(pl.I) * f) * (ndl / (dist * dist));
+                                sh_o = position; sh_d = ldir; sh_tmax = dist * (1.f - 1e-5f);
+                                want_shadow = true;
+                            }
+                        } else if ((lj -= s.n_points) < s.n_spots) {
+                            const DevSpot sl = s.spots[lj];                 // direct_lighting.cpp:49-89
+                            const V3 lp = from4(sl.p);
+                            const V3 position = origin_dir(sc, lp - sc.pos);
+                            const V3 lv = lp - position;
+                            const float dist = length(lv);
+                            const V3 wi = vdiv(lv, dist);
+                            const float cone_cos = sl.dir.w;
+                            const float wi_cos = dot(-wi, from4(sl.dir));
+                            if (!(wi_cos < cone_cos)) {
+                                float penumbra = 1.f;
+                                const float penumbra_cos = sl.I.w;
+                                if (wi_cos < penumbra_cos) {
+                                    const float kp = (wi_cos - cone_cos) / (penumbra_cos - cone_cos);
+                                    penumbra = (kp * kp) * (kp * kp);
+                                }
                                 const float ndw = dot(sc.n, wi);
                                 if (scattering_possible(m.type, ndw)) {
                                     const RGB f = bsdf_eval(m, sc, sc.wo, wi);
                                     if (!is_black(f)) {
-                                        const float lpdf = (dd * dd) / (R.size.x * R.size.y * lndw);
-                                        const float bpdf = bsdf_pdf(m, sc, sc.wo, wi);
-                                        const float w = power_heuristic(lpdf, bpdf);
-                                        sh_c = (rgb4(R.L) * f) * (w * fabsf(ndw) / lpdf);
-                                        sh_o = position; sh_d = wi; sh_tmax = dd * (1.f - 1e-5f);
+                                        sh_c = (rgb4(sl.I) * f) * (penumbra * fabsf(ndw) / (dist * dist));
+                                        sh_o = position; sh_d = wi; sh_tmax = dist * (1.f - 1e-5f);
                                         want_shadow = true;
                                     }
                                 }
                             }
-                        }
-                        {
-                            V3 wi = v3(0.f, 0.f, 0.f);
-                            float bpdf = 0.f;
-                            const RGB f = bsdf_sample(m, sc, ubx, uby, u_scat, sc.wo, wi, bpdf);
-                            if (!is_black(f)) {
+                        } else if ((lj -= s.n_spots) < s.n_dirs) {
+                            const DevDirLight dl = s.dirs[lj];              // direct_lighting.cpp:91-107
+                            const V3 ldir = from4(dl.dir);
+                            const float ndl = dot(sc.n, ldir);
+                            if (ndl > 0.f) {
+                                const RGB f = bsdf_eval(m, sc, sc.wo, ldir);
+                                sh_c = (rgb4(dl.E) * f) * ndl;
+                                sh_o = origin_dir(sc, ldir); sh_d = ldir; sh_tmax = f_inf();
+                                want_shadow = true;
+                            }
+                        } else if ((lj -= s.n_dirs) < s.n_rects) {
+                            const int ri = lj;                              // direct_lighting.cpp:109-194
+                            const DevRect R = s.rects[ri];
+                            const M34 l2w = {R.l0, R.l1, R.l2};
+                            const V3 ln = v3(R.l0.z, R.l1.z, R.l2.z);
+                            {
+                                const V3 local = v3(R.size.x * (ulx - 0.5f), R.size.y * (uly - 0.5f), 0.0f);
+                                const V3 lp = xform_point(l2w, local);
+                                const V3 position = origin_dir(sc, lp - sc.pos);
+                                const V3 lv = lp - position;
+                                const float dd = length(lv);
+                                const V3 wi = vdiv(lv, dd);
                                 const float lndw = dot(ln, -wi);
                                 if (lndw > 1e-4f) {
-                                    mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
-                                    mi_ndw = dot(sc.n, wi); mi_lndw = lndw; mi_light = ri;
-                                    // only a ray that can reach the light's rectangle can contribute
-                                    want_mis = ray_may_hit_rect(R, mi_o, wi);
-                                    mis_culled = !want_mis;
+                                    const float ndw = dot(sc.n, wi);
+                                    if (scattering_possible(m.type, ndw)) {
+                                        const RGB f = bsdf_eval(m, sc, sc.wo, wi);
+                                        if (!is_black(f)) {
+                                            const float lpdf = (dd * dd) / (R.size.x * R.size.y * lndw);
+                                            const float bpdf = bsdf_pdf(m, sc, sc.wo, wi);
+                                            const float w = power_heuristic(lpdf, bpdf);
+                                            sh_c = (rgb4(R.L) * f) * (w * fabsf(ndw) / lpdf);
+                                            sh_o = position; sh_d = wi; sh_tmax = dd * (1.f - 1e-5f);
+                                            want_shadow = true;
+                                        }
+                                    }
                                 }
                             }
-                        }
-                    } else if ((lj -= s.n_rects) < s.n_spheres) {
-                        const DevSphere SL = s.spheres[lj];             // direct_lighting.cpp:196-260
-                        const SphereCone cone = sphere_cone(SL, sc.pos);
-                        {
-                            const V3 lp = sphere_cone_point(SL, cone, ulx, uly);
-                            const V3 position = origin_dir(sc, lp - sc.pos);
-                            const V3 lv = lp - position;
-                            const float dd = length(lv);
-                            const V3 wi = vdiv(lv, dd);
-                            const float ndw = dot(sc.n, wi);
-                            if (scattering_possible(m.type, ndw)) {
-                                const RGB f = bsdf_eval(m, sc, sc.wo, wi);
+                            {
+                                V3 wi = v3(0.f, 0.f, 0.f);
+                                float bpdf = 0.f;
+                                const RGB f = bsdf_sample(m, sc, ubx, uby, u_s, sc.wo, wi, bpdf);
                                 if (!is_black(f)) {
-                                    const float bpdf = bsdf_pdf(m, sc, sc.wo, wi);
-                                    const float w = power_heuristic(cone.pdf, bpdf);
-                                    sh_c = (rgb4(SL.L) * f) * (w * fabsf(ndw) / cone.pdf);
-                                    sh_o = position; sh_d = wi; sh_tmax = dd * (1.f - 1e-5f);
-                                    want_shadow = true;
+                                    const float lndw = dot(ln, -wi);
+                                    if (lndw > 1e-4f) {
+                                        mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
+                                        mi_ndw = dot(sc.n, wi); mi_lndw = lndw; mi_light = ri;
+                                        // only a ray that can reach the light's rectangle can contribute
+                                        want_mis = ray_may_hit_rect(R, mi_o, wi);
+                                        mis_culled = !want_mis;
+                                    }
                                 }
                             }
-                        }
-                        {
-                            V3 wi = v3(0.f, 0.f, 0.f);
-                            float bpdf = 0.f;
-                            const RGB f = bsdf_sample(m, sc, ubx, uby, u_scat, sc.wo, wi, bpdf);
-                            if (!is_black(f) && dot(-cone.a2, wi) >= cone.cos_max) {
-                                mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
-                                mi_ndw = dot(sc.n, wi); mi_lndw = cone.pdf; mi_light = s.n_rects + lj;
-                                want_mis = true;
-                            }
-                        }
-                    } else {
-                        lj -= s.n_spheres;                              // direct_lighting.cpp:262-322
-                        const DevMeshLight ML = s.mesh_lights[lj];
-                        {
-                            // Diffuse_Triangle_Mesh_Light_Sampler::sample (light_sampling.cpp:126-161)
-                            float u0;
-                            const float smp = sample_cdf(s.mesh_light_cdf + ML.cdf, ML.tri_count, ulx, u0);
-                            const uint32_t lt = min((uint32_t)(smp * (float)ML.tri_count), (uint32_t)ML.tri_count - 1u);
-                            const float ra = sqrtf(u0);                  // sampling.cpp:69-77
-                            V3 bl;
-                            bl.x = 1 - ra;
-                            bl.y = uly * ra;
-                            bl.z = 1.f - bl.x - bl.y;
-                            V3 lp, ln;
-                            mesh_light_point(s, (uint32_t)ML.tri_base + lt, bl, lp, ln);
-                            const V3 lv0 = lp - sc.pos;
-                            const float dsq = dot(lv0, lv0);
-                            const V3 w0 = vdiv(lv0, sqrtf(dsq));
-                            const float lndw0 = dot(ln, -w0);
-                            const float lpdf = lndw0 <= 0.f ? 0.f : dsq / (lndw0 * ML.area);
-                            if (lpdf != 0.f) {
+                        } else if ((lj -= s.n_rects) < s.n_spheres) {
+                            const DevSphere SL = s.spheres[lj];             // direct_lighting.cpp:196-260
+                            const SphereCone cone = sphere_cone(SL, sc.pos);
+                            {
+                                const V3 lp = sphere_cone_point(SL, cone, ulx, uly);
                                 const V3 position = origin_dir(sc, lp - sc.pos);
                                 const V3 lv = lp - position;
                                 const float dd = length(lv);
@@ -1022,28 +991,96 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                                     const RGB f = bsdf_eval(m, sc, sc.wo, wi);
                                     if (!is_black(f)) {
                                         const float bpdf = bsdf_pdf(m, sc, sc.wo, wi);
-                                        const float w = power_heuristic(lpdf, bpdf);
-                                        sh_c = (rgb4(ML.L) * f) * (w * fabsf(ndw) / lpdf);
+                                        const float w = power_heuristic(cone.pdf, bpdf);
+                                        sh_c = (rgb4(SL.L) * f) * (w * fabsf(ndw) / cone.pdf);
                                         sh_o = position; sh_d = wi; sh_tmax = dd * (1.f - 1e-5f);
                                         want_shadow = true;
                                     }
                                 }
                             }
-                        }
-                        {
-                            V3 wi = v3(0.f, 0.f, 0.f);
-                            float bpdf = 0.f;
-                            const RGB f = bsdf_sample(m, sc, ubx, uby, u_scat, sc.wo, wi, bpdf);
-                            if (!is_black(f)) {
-                                mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
-                                mi_ndw = dot(sc.n, wi); mi_lndw = 0.f; mi_light = s.n_rects + s.n_spheres + lj;
-                                want_mis = true;
+                            {
+                                V3 wi = v3(0.f, 0.f, 0.f);
+                                float bpdf = 0.f;
+                                const RGB f = bsdf_sample(m, sc, ubx, uby, u_s, sc.wo, wi, bpdf);
+                                if (!is_black(f) && dot(-cone.a2, wi) >= cone.cos_max) {
+                                    mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
+                                    mi_ndw = dot(sc.n, wi); mi_lndw = cone.pdf; mi_light = s.n_rects + lj;
+                                    want_mis = true;
+                                }
+                            }
+                        } else {
+                            lj -= s.n_spheres;                              // direct_lighting.cpp:262-322
+                            const DevMeshLight ML = s.mesh_lights[lj];
+                            {
+                                // Diffuse_Triangle_Mesh_Light_Sampler::sample (light_sampling.cpp:126-161)
+                                float u0;
+                                const float smp = sample_cdf(s.mesh_light_cdf + ML.cdf, ML.tri_count, ulx, u0);
+                                const uint32_t lt = min((uint32_t)(smp * (float)ML.tri_count), (uint32_t)ML.tri_count - 1u);
+                                const float ra = sqrtf(u0);                  // sampling.cpp:69-77
+                                V3 bl;
+                                bl.x = 1 - ra;
+                                bl.y = uly * ra;
+                                bl.z = 1.f - bl.x - bl.y;
+                                V3 lp, ln;
+                                mesh_light_point(s, (uint32_t)ML.tri_base + lt, bl, lp, ln);
+                                const V3 lv0 = lp - sc.pos;
+                                const float dsq = dot(lv0, lv0);
+                                const V3 w0 = vdiv(lv0, sqrtf(dsq));
+                                const float lndw0 = dot(ln, -w0);
+                                const float lpdf = lndw0 <= 0.f ? 0.f : dsq / (lndw0 * ML.area);
+                                if (lpdf != 0.f) {
+                                    const V3 position = origin_dir(sc, lp - sc.pos);
+                                    const V3 lv = lp - position;
+                                    const float dd = length(lv);
+                                    const V3 wi = vdiv(lv, dd);
+                                    const float ndw = dot(sc.n, wi);
+                                    if (scattering_possible(m.type, ndw)) {
+                                        const RGB f = bsdf_eval(m, sc, sc.wo, wi);
+                                        if (!is_black(f)) {
+                                            const float bpdf = bsdf_pdf(m, sc, sc.wo, wi);
+                                            const float w = power_heuristic(lpdf, bpdf);
+                                            sh_c = (rgb4(ML.L) * f) * (w * fabsf(ndw) / lpdf);
+                                            sh_o = position; sh_d = wi; sh_tmax = dd * (1.f - 1e-5f);
+                                            want_shadow = true;
+                                        }
+                                    }
+                                }
+                            }
+                            {
+                                V3 wi = v3(0.f, 0.f, 0.f);
+                                float bpdf = 0.f;
+                                const RGB f = bsdf_sample(m, sc, ubx, uby, u_s, sc.wo, wi, bpdf);
+                                if (!is_black(f)) {
+                                    mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
+                                    mi_ndw = dot(sc.n, wi); mi_lndw = 0.f; mi_light = s.n_rects + s.n_spheres + lj;
+                                    want_mis = true;
+                                }
                             }
                         }
+                        P.dl_coeff[slot] = f4(coeff, 0.f);
+                        P.dl_light[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        P.dl_bsdf[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
                     }
-                    P.dl_coeff[slot] = f4(coeff, 0.f);
-                    P.dl_light[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    P.dl_bsdf[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (uber_delta != 0) {
+                        // get_pbrt_uber_info (delta_scattering.cpp:102-152): Fresnel-weighted mirror
+                        // reflection of Kr, or the opacity pass-through, each scaled by the component count
+                        const float ncomp = (float)(fbits(m.trans.w) & 15u);
+                        RGB att;
+                        V3 dir;
+                        if (uber_delta == 1) {
+                            const float F = dielectric_fresnel(dot(sc.n, sc.wo), sc.flipped ? 1.f / m.refl.w : m.refl.w);
+                            att = rgb4(m.eta) * (ncomp * F);
+                            dir = reflect(sc.wo, sc.n);
+                        } else {
+                            att = (rgb(1.f, 1.f, 1.f) - rgb4(m.k)) * ncomp;
+                            dir = -sc.wo;
+                        }
+                        P.ray_o[slot] = f4(origin_dir(sc, dir), 0.f);
+                        P.ray_d[slot] = f4(dir, 0.f);
+                        coeff = coeff * att;
+                        flags |= kDelta;
+                        want_active = true;
+                    } else {
                     bounce++;
                     if (bounce == c.max_bounces) {
                         finished = true;
@@ -1105,7 +1142,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                         if (is_black(f)) {
                             finished = true;
                         } else {
-                            coeff = coeff * (f * (fabsf(dot(sc.n, wi)) / (pdf * 1.f)));
+                            coeff = coeff * (f * (fabsf(dot(sc.n, wi)) / (pdf * bsdf_layer_probability)));
                             P.ray_o[slot] = f4(origin_dir(sc, wi), 0.f);
                             P.ray_d[slot] = f4(wi, 0.f);
                             if (bounce >= c.rr_bounce_threshold) {
@@ -1119,6 +1156,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                             }
                             if (!finished) want_active = true;
                         }
+                    }
                     }
                 }
             }

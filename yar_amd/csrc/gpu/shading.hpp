@@ -364,11 +364,32 @@ YG_INLINE float rough_glass_pdf(const DevMaterial& m, const Shading& sc, V3 wo, 
     return wh_pdf * dwh_dwi * (1.f - rp);
 }
 
+// Pbrt3_Uber_BRDF (bsdf_pbrt.cpp:14-91): refl = Kd | ior, trans = Ks, k = opacity; Kd/pi and an
+// anisotropic GGX lobe with a dielectric's Fresnel (ior inverted when leaving), both scaled by the opacity.
+YG_INLINE RGB uber_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
+    const RGB diffuse = (kPiInv * rgb4(m.refl)) * rgb4(m.k);
+    const V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
+    const V3 wh = normalized(wol + wil);
+    const float cos_i = dot(wil, wh);
+    const float F = dielectric_fresnel(cos_i, sc.flipped ? 1.f / m.refl.w : m.refl.w);
+    const float G = G_aniso(wil, wol, m.alpha_x, m.alpha_y);
+    const float D = D_aniso(wh, m.alpha_x, m.alpha_y);
+    const float f = (F * G * D) / (4.f * wol.z * wil.z);                             // scattering.cpp:84
+    return diffuse + (rgb4(m.trans) * rgb4(m.k)) * f;
+}
+YG_INLINE float uber_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
+    const float diffuse_pdf = dot(sc.n, wi) / kPi;
+    const V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
+    const V3 wh = normalized(wol + wil);
+    return 0.5f * (diffuse_pdf + vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y));
+}
+
 YG_INLINE RGB bsdf_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
     if (m.type == 1) return kPiInv * rgb4(m.refl);
     if (m.type == kRoughGlassType) return rough_glass_eval(m, sc, wo, wi);
     if (m.type == 5) return plastic_eval(m, sc, wo, wi);
     if (m.type == 6) return coated_eval(m, sc, wo, wi);
+    if (m.type == kUberType) return uber_eval(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
     V3 wh = normalized(wol + wil);
     float ci = dot(wil, wh);
@@ -382,6 +403,7 @@ YG_INLINE float bsdf_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) 
     if (m.type == 5) return plastic_pdf(m, sc, wo, wi);
     if (m.type == 6) return coated_pdf(m, sc, wo, wi);
     if (m.type == kRoughGlassType) return rough_glass_pdf(m, sc, wo, wi);
+    if (m.type == kUberType) return uber_pdf(m, sc, wo, wi);
     V3 wol = to_local(sc, wo), wil = to_local(sc, wi);
     V3 wh = normalized(wol + wil);
     return vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
@@ -429,7 +451,7 @@ YG_INLINE RGB bsdf_sample(const DevMaterial& m, const Shading& sc, float u0, flo
         if (pdf == 0.f) return rgb(0.f, 0.f, 0.f);
         return rough_glass_eval(m, sc, wo, wi);
     }
-    if (m.type == 6) {                                                                 // bsdf.cpp:462-479
+    if (m.type == 6 || m.type == kUberType) {                      // bsdf.cpp:462-479, bsdf_pbrt.cpp:56-74
         if (u_scat < 0.5f) {
             wi = to_world(sc, sample_cosine(u0, u1));
         } else {
@@ -438,6 +460,10 @@ YG_INLINE RGB bsdf_sample(const DevMaterial& m, const Shading& sc, float u0, flo
             wi = to_world(sc, reflect(wol, wh));
         }
         if (dot(sc.n, wi) <= 0.f) return rgb(0.f, 0.f, 0.f);
+        if (m.type == kUberType) {
+            pdf = uber_pdf(m, sc, wo, wi);
+            return uber_eval(m, sc, wo, wi);
+        }
         pdf = coated_pdf(m, sc, wo, wi);
         return coated_eval(m, sc, wo, wi);
     }

@@ -425,6 +425,40 @@ struct Loader {
             m.u_roughness = get_f(ms->params, "uroughness", 0.1f);
             m.v_roughness = get_f(ms->params, "vroughness", 0.1f);
             m.roughness_is_alpha = get_b(ms->params, "remaproughness", true) ? 0 : 1;
+        } else if (ms->type == "uber") {                                   // pbrt_scene.cpp:571-636
+            m.type = YAR_MATERIAL_PBRT3_UBER;
+            for (const Param& e : ms->params) {                               // Materials.cpp createMaterial_uber
+                static const char* known[] = {"type", "Kd", "Ks", "Kr", "Kt", "opacity", "index", "eta", "roughness",
+                                              "uroughness", "vroughness", "alpha", "shadowalpha"};
+                bool ok = false;
+                for (const char* k : known) ok = ok || e.name == k;
+                if (!ok) throw ParseError{"unsupported uber parameter '" + e.name + "'"};
+                if (e.type == "texture") throw ParseError{"textured uber " + e.name + " is not supported"};
+            }
+            float kd[3] = {0.25f, 0.25f, 0.25f}, ks[3] = {0.25f, 0.25f, 0.25f};   // pbrt-parser UberMaterial
+            float kr[3] = {0, 0, 0}, kt[3] = {0, 0, 0}, opacity[3] = {1, 1, 1};
+            get_rgb(ms->params, "Kd", kd);
+            get_rgb(ms->params, "Ks", ks);
+            get_rgb(ms->params, "Kr", kr);
+            get_rgb(ms->params, "Kt", kt);
+            get_rgb(ms->params, "opacity", opacity);
+            if (kt[0] != 0.f || kt[1] != 0.f || kt[2] != 0.f)                 // delta_scattering.cpp:132-134
+                throw ParseError{"uber delta transmission (Kt) is not supported"};
+            if (get_f(ms->params, "alpha", 0.f) != 0.f || get_f(ms->params, "shadowalpha", 0.f) != 0.f)
+                throw ParseError{"uber alpha is not supported"};
+            for (int i = 0; i < 3; i++) {
+                m.reflectance[i] = kd[i]; m.specular[i] = ks[i];
+                m.delta_reflectance[i] = kr[i]; m.opacity[i] = opacity[i];
+            }
+            m.index_of_refraction = find(ms->params, "index") ? get_f(ms->params, "index", 1.5f)
+                                                              : get_f(ms->params, "eta", 1.5f);
+            const float r = get_f(ms->params, "roughness", 0.1f);
+            m.u_roughness = get_f(ms->params, "uroughness", r);
+            m.v_roughness = find(ms->params, "vroughness") ? get_f(ms->params, "vroughness", r) : m.u_roughness;
+            m.roughness_is_alpha = 0;
+            auto black = [](const float* c) { return c[0] == 0.f && c[1] == 0.f && c[2] == 0.f; };
+            if (black(kd) && black(ks) && black(kr) && opacity[0] == 1.f && opacity[1] == 1.f && opacity[2] == 1.f)
+                throw ParseError{"uber material with no scattering component"};
         } else {
             throw ParseError{"unsupported material type '" + ms->type + "'"};
         }
