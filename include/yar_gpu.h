@@ -145,6 +145,16 @@ typedef struct yar_sphere_light_desc {
     int32_t sample_count;               /* pbrt "nsamples" (the single-sample estimator ignores it) */
 } yar_sphere_light_desc;
 
+/* Environment_Light (src/lib/light.h:91) with a constant map: pbrt "infinite" without "mapname".
+ * Radiance `radiance` from every direction; light samples are drawn uniformly in the light frame's
+ * (phi, theta) square, as the reference's one-texel Distribution_2D does (light_sampling.cpp:27-73). */
+typedef struct yar_environment_light_desc {
+    float   light_to_world[12];
+    float   world_to_light[12];         /* get_inverse_transform of light_to_world */
+    float   radiance[3];                /* pbrt L * scale */
+    int32_t sample_count;
+} yar_environment_light_desc;
+
 typedef struct yar_render_config {      /* Raytracer_Config, src/lib/raytracer_config.h:5 */
     int32_t film_width, film_height;
     int32_t region_x0, region_y0, region_x1, region_y1;   /* render region, exclusive p1 */
@@ -177,6 +187,7 @@ typedef struct yar_scene_desc {
     const yar_directional_light_desc* directional_lights; int32_t directional_light_count;
     const yar_mesh_light_desc*        mesh_lights;        int32_t mesh_light_count;
     const yar_sphere_light_desc*      sphere_lights;      int32_t sphere_light_count;
+    const yar_environment_light_desc* environment_light;  int32_t environment_light_count;   /* 0 or 1 */
 } yar_scene_desc;
 
 /* ---- scene loading / generation -------------------------------------------------------- */
@@ -184,7 +195,8 @@ typedef struct yar_scene yar_scene;
 
 /* Loads the pbrt3 subset the RAY path tracer consumes (trianglemesh; matte, metal, glass, mirror,
  * plastic, substrate; sphere shapes, tessellated as the reference does; diffuse area lights
- * (rectangles, spheres and triangle meshes), point, spot and distant lights; perspective camera, Film, Sampler, Integrator "path", PixelFilter "box") with the
+ * (rectangles, spheres and triangle meshes), point, spot and distant lights, infinite lights without
+ * a map; perspective camera, Film, Sampler, Integrator "path", PixelFilter "box") with the
  * reference's import semantics. */
 int  yar_scene_load_pbrt(const char* path, yar_scene** out_scene);
 const yar_scene_desc* yar_scene_get_desc(const yar_scene* scene);

@@ -43,6 +43,8 @@ namespace oref {
 constexpr float Pi = 3.14159265f;                   // src/lib/math.h:9
 constexpr float Pi_Inv = 1.f / Pi;                  // src/lib/math.h:12
 constexpr float One_Minus_Epsilon = 0x1.fffffep-1;  // src/lib/math.h:14
+constexpr float Pi2 = 6.2831853f;                   // src/lib/math.h:10
+constexpr float Pi2_Inv = 1.f / Pi2;                 // src/lib/math.h:13
 constexpr float Inf = std::numeric_limits<float>::infinity();
 
 static inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
@@ -457,6 +459,7 @@ struct DirectionalLight { V3 dir; RGB E; };                                   //
 // its area and the area CDF of Distribution_1D (sampling.cpp:149-166)
 struct MeshLight { int object, mesh; RGB L; float area; std::vector<float> cdf; };
 struct SphereLight { V3 p; float radius; RGB L; };                           // light.h:78
+struct EnvLight { bool present = false; M34 l2w, w2l; RGB L; };             // light.h:91, constant map
 
 struct Counters { int64_t nodes = 0, leaf_refs = 0, tri_tests = 0, objects = 0; };
 
@@ -470,6 +473,7 @@ struct Scene {
     std::vector<DirectionalLight> dirs;
     std::vector<MeshLight> mesh_lights;
     std::vector<SphereLight> spheres;
+    EnvLight env;
     Tree scene_tree;
     yar_camera_desc camera;
     yar_render_config config;
@@ -1367,8 +1371,71 @@ static RGB light_sphere(const Scene& s, const Shading& sc, int li, V2 ul, V2 ub,
     return out;
 }
 
+// Environment_Light_Sampler over a constant (one-texel) map (light_sampling.cpp:11-73,
+// sampling.cpp:185-300): the one-cell Distribution_2D samples uv = (u1, u0) with pdf_uv 1.
+static void env_uv(V3 d, float* u, float* v) {                                       // get_uv_from_direction
+    float phi = std::atan2(d.y, d.x);
+    phi = phi < 0 ? phi + Pi2 : phi;
+    float theta = std::acos(std::clamp(d.z, -1.f, 1.f));
+    *u = std::clamp(phi * Pi2_Inv, 0.f, One_Minus_Epsilon);
+    *v = std::min(theta * Pi_Inv, One_Minus_Epsilon);
+}
+// get_filtered_radiance_for_direction: the bilinear weights of the white texel, summed in float
+static RGB env_filtered(const EnvLight& E, V3 world_dir) {
+    float u, v;
+    env_uv(xform_vector(E.w2l, world_dir), &u, &v);
+    float x = u * 1.f - 0.5f, y = v * 1.f - 0.5f;                                   // image_texture.cpp:441-480
+    float wx = x - std::floor(x), wy = y - std::floor(y);
+    float w = (((1 - wy) * (1 - wx) + (1 - wy) * wx) + wy * (1 - wx)) + wy * wx;
+    return splat(w) * E.L;
+}
+static float env_pdf(const EnvLight& E, V3 world_dir) {
+    V3 d = xform_vector(E.w2l, world_dir);
+    float sin_theta = std::sqrt(std::max(0.f, 1.f - d.z * d.z));
+    return 1.f / (2 * Pi * Pi * sin_theta);
+}
+
+static RGB light_env(const Scene& s, const Shading& sc, V2 ul, V2 ub, float us, PathCounters& pc, Counters* tc) {
+    const EnvLight& E = s.env;                                               // direct_lighting.cpp:324-384
+    RGB out;
+    {
+        const float phi = ul.y * Pi2, theta = ul.x * Pi;
+        const float sin_theta = std::sin(theta);
+        const float lpdf = 1.f / (2 * Pi * Pi * sin_theta);
+        V3 wi{sin_theta * std::cos(phi), sin_theta * std::sin(phi), std::cos(theta)};
+        wi = xform_vector(E.l2w, wi);
+        const float ndw = dot(sc.n, wi);
+        if (scattering_possible(s, sc, ndw)) {
+            RGB f = bsdf_eval(s, sc, sc.wo, wi);
+            if (!is_black(f)) {
+                const V3 position = sc.origin_dir(wi);
+                pc.any++;
+                if (!intersect_any(s, Ray{position, wi}, Inf, tc)) {
+                    const float bpdf = bsdf_pdf(s, sc, sc.wo, wi);
+                    const float w = power_heuristic(lpdf, bpdf);
+                    out = out + (E.L * f) * (w * std::fabs(ndw) / lpdf);
+                }
+            }
+        }
+    }
+    {
+        V3 wi; float bpdf;
+        RGB f = bsdf_sample(s, sc, ub, us, sc.wo, &wi, &bpdf);
+        if (!is_black(f) && !is_black(E.L)) {
+            const V3 position = sc.origin_dir(wi);
+            pc.any++;
+            if (!intersect_any(s, Ray{position, wi}, Inf, tc)) {
+                const float lpdf = env_pdf(E, wi);
+                const float w = power_heuristic(bpdf, lpdf);
+                out = out + (E.L * f) * (w * std::fabs(dot(sc.n, wi)) / bpdf);
+            }
+        }
+    }
+    return out;
+}
+
 // estimate_direct_lighting_from_single_sample (direct_lighting.cpp:522-578): lights in the order
-// point, spot, directional, rectangular, sphere, triangle mesh
+// point, spot, directional, rectangular, sphere, triangle mesh, environment
 static RGB direct_single(const Scene& s, const Shading& sc, float u_sel, V2 ul, V2 ub, float us, PathCounters& pc, Counters* tc) {
     int li = int(u_sel * s.total_lights);
     const float n = (float)s.total_lights;
@@ -1382,7 +1449,8 @@ static RGB direct_single(const Scene& s, const Shading& sc, float u_sel, V2 ul, 
     li -= (int)s.rects.size();
     if (li < (int)s.spheres.size()) return n * light_sphere(s, sc, li, ul, ub, us, pc, tc);
     li -= (int)s.spheres.size();
-    return n * light_mesh(s, sc, li, ul, ub, us, pc, tc);
+    if (li < (int)s.mesh_lights.size()) return n * light_mesh(s, sc, li, ul, ub, us, pc, tc);
+    return n * light_env(s, sc, ul, ub, us, pc, tc);
 }
 
 // Smooth glass: get_glass_info (delta_scattering.cpp:53-100), enter/exit judged by the original
@@ -1428,6 +1496,7 @@ static RGB trace_path(const Scene& s, Ray ray, PixelSampler& ps, PathCounters& p
             if (hit_found) init_shading(s, ray, hit, sc);
             if (bounce == 0) {
                 if (hit_found && sc.light >= 0) L = L + coeff * area_light_emission(s, sc);
+                else if (!hit_found && s.env.present) L = L + coeff * env_filtered(s.env, ray.d);
                 if (cfg.max_light_bounces == 0) break;
             }
             if (!hit_found || sc.light >= 0) break;
@@ -1501,6 +1570,7 @@ static RGB trace_path(const Scene& s, Ray ray, PixelSampler& ps, PathCounters& p
             if (dh) init_shading(s, dray, hit, sc);
             RGB emitted;
             if (dh && sc.light >= 0) emitted = area_light_emission(s, sc);
+            else if (!dh && s.env.present) emitted = env_filtered(s.env, dray.d);
             coeff = coeff * att;
             L = L + coeff * emitted;
             bounce++;
@@ -1766,8 +1836,15 @@ static bool load_scene(const yar_scene_desc* d, Scene& s, int threads) {
         s.spheres.push_back(SphereLight{{ld.position[0], ld.position[1], ld.position[2]}, ld.radius,
                                         {ld.emitted_radiance[0], ld.emitted_radiance[1], ld.emitted_radiance[2]}});
     }
+    if (d->environment_light_count > 0) {
+        const yar_environment_light_desc& ld = d->environment_light[0];
+        s.env.present = true;
+        s.env.l2w = load_m34(ld.light_to_world);
+        s.env.w2l = load_m34(ld.world_to_light);
+        s.env.L = {ld.radiance[0], ld.radiance[1], ld.radiance[2]};
+    }
     s.total_lights = (int)(s.points.size() + s.spots.size() + s.dirs.size() + s.rects.size() + s.spheres.size() +
-                           s.mesh_lights.size());
+                           s.mesh_lights.size()) + (s.env.present ? 1 : 0);
     s.camera = d->camera;
     s.config = d->config;
     return true;

@@ -261,3 +261,25 @@ def test_render_uber_matches_oracle(c1_path, oracle, tmp_path):
     assert mean_rel <= 1e-3
     assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
     assert (film["weight_sum"] == ofilm["weight_sum"]).all()
+
+
+def test_render_environment_light_matches_oracle(c1_path, oracle, tmp_path):
+    # the config-1 room under a turned constant sky besides its rectangle light, with a mirror wall so
+    # delta rays also leave the scene: Environment_Light_Sampler (light_sampling.cpp:11-73),
+    # direct_lighting_from_environment_light (direct_lighting.cpp:324-384), misses at the camera and
+    # after delta bounces (path_tracing.cpp:27-33, 86-96)
+    lines = open(c1_path).read().splitlines()
+    mats = [i for i, l in enumerate(lines) if l.startswith('Material "matte"')]
+    lines[mats[2]] = 'Material "mirror" "rgb Kr" [ 0.85 0.8 0.75 ]'
+    text = "\n".join(lines) + "\n"
+    text = text.replace("WorldBegin\n", 'WorldBegin\nAttributeBegin\nRotate 30 1 0.5 0\n'
+                        'LightSource "infinite" "rgb L" [ 0.3 0.4 0.5 ]\nAttributeEnd\n', 1)
+    p = tmp_path / "c1_env.pbrt"
+    p.write_text(text)
+    assert yar_amd.Scene(str(p)).desc.environment_light_count == 1
+    img, film, var, st, oimg, ofilm, ovar = _render_pair(str(p), oracle, nx=4)
+    exact, close, mean_rel = _compare_images(img, oimg, film, ofilm, "c1 environment light 16spp")
+    assert close >= 0.99
+    assert mean_rel <= 1e-3
+    assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
+    assert (film["weight_sum"] == ofilm["weight_sum"]).all()

@@ -392,3 +392,23 @@ def test_uber_unsupported_forms_are_parse_errors(tmp_path, params):
     body = LIGHT + f'Material "uber" {params}\nShape "trianglemesh" "integer indices" [ 0 1 2 ] "point P" [ 0 0 0  1 0 0  0 1 0 ]\n'
     with pytest.raises(yar_amd.YarError):
         _load(tmp_path, body)
+
+
+def test_infinite_light_without_map(tmp_path):
+    """import_pbrt_non_area_light's infinite branch (pbrt_scene.cpp:958-980): radiance L * scale, the
+    light frame is the current transform; environment maps are not supported."""
+    body = LIGHT + 'AttributeBegin\nRotate 90 1 0 0\nLightSource "infinite" "rgb L" [ 0.5 1 2 ] "rgb scale" [ 2 2 2 ] "integer nsamples" [ 4 ]\nAttributeEnd\n'
+    d = _load(tmp_path, body).desc
+    assert d.environment_light_count == 1
+    e = d.environment_light[0]
+    assert list(e.radiance) == [1, 2, 4] and e.sample_count == 4
+    l2w = np.array(list(e.light_to_world)).reshape(3, 4)
+    w2l = np.array(list(e.world_to_light)).reshape(3, 4)
+    assert np.allclose(l2w[:, :3], [[1, 0, 0], [0, 0, -1], [0, 1, 0]], atol=1e-6)
+    assert np.allclose(l2w[:, :3] @ w2l[:, :3], np.eye(3), atol=1e-6)
+    d = _load(tmp_path, 'LightSource "infinite"\n', name="only_env.pbrt").desc
+    assert d.environment_light_count == 1 and list(d.environment_light[0].radiance) == [1, 1, 1]
+    for bad in ('LightSource "infinite" "string mapname" "sky.exr"\n',
+                'LightSource "infinite"\nLightSource "infinite"\n'):
+        with pytest.raises(yar_amd.YarError):
+            _load(tmp_path, bad, name="bad.pbrt")

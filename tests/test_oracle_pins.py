@@ -941,3 +941,51 @@ def test_uber_opacity_passes_light_through():
     img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
     want = kd * o / np.pi * intensity + (1 - o) * le
     assert img.mean() == pytest.approx(want, rel=2e-3), (img.mean(), want)
+
+
+def _env_scene(tmp_path, body, look, rot="", spp=16, maxdepth=1, name="env.pbrt"):
+    text = (f'LookAt {look}\nCamera "perspective" "float fov" [ 0.5 ]\n'
+            'Film "image" "integer xresolution" [ 16 ] "integer yresolution" [ 16 ]\n'
+            f'Sampler "stratified" "integer xsamples" [ {spp} ] "integer ysamples" [ {spp} ]\n'
+            f'Integrator "path" "integer maxdepth" [ {maxdepth} ]\nWorldBegin\n'
+            f'AttributeBegin\n{rot}LightSource "infinite" "rgb L" [ 0.5 1 1.5 ] "rgb scale" [ 2 2 2 ]\nAttributeEnd\n'
+            + body + 'WorldEnd\n')
+    p = tmp_path / name
+    p.write_text(text)
+    return yar_amd.Scene(str(p))
+
+
+@pytest.mark.parametrize("rot", ["", "Rotate 70 1 0.3 0\n"], ids=["aligned", "rotated"])
+def test_environment_light_furnace(tmp_path, rot):
+    # direct_lighting.cpp:324-384, light_sampling.cpp:11-73: a diffuse plane under a constant sky of
+    # radiance L receives irradiance pi L, so it shows albedo * L; the MIS of the (phi, theta)-uniform
+    # light samples (pdf 1 / (2 pi^2 sin theta) in the light frame) and of the BSDF samples must sum
+    # to that however the light frame is turned.  A camera ray that leaves the scene sees L.
+    from tests.test_oracle_fixtures import _view_from_desc
+    L = np.array([1.0, 2.0, 3.0])
+    plane = ('AttributeBegin\nMaterial "matte" "rgb Kd" [ 0.5 0.5 0.5 ]\nShape "trianglemesh" "integer indices" '
+             '[ 0 1 2 0 2 3 ] "point P" [ -1e4 0 -1e4  -1e4 0 1e4  1e4 0 1e4  1e4 0 -1e4 ]\nAttributeEnd\n')
+    sc = _env_scene(tmp_path, plane, "0 0.5 0  0 0 0  0 0 1", rot)
+    img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
+    assert np.allclose(img.mean(axis=(0, 1)), 0.5 * L, rtol=2e-3), (img.mean(axis=(0, 1)), 0.5 * L)
+    sky = _env_scene(tmp_path, plane, "0 0.5 0  0 2 0  0 0 1", rot, spp=2, name="sky.pbrt")
+    img, _, _, _, _ = O.OracleScene(sky).render(_view_from_desc(sky.desc))
+    assert np.allclose(img, L, rtol=1e-6), (img.min(), img.max())
+
+
+def test_environment_light_convex_object_and_delta_miss(tmp_path):
+    # A convex diffuse object sees only the sky from every point, so at any path depth it shows
+    # albedo * L (the continuation rays leave the scene and add nothing after the first vertex);
+    # a mirror seen by the camera reflects the sky: Kr * L through the delta-miss branch
+    # (path_tracing.cpp:86-96).
+    from tests.test_oracle_fixtures import _view_from_desc
+    L = np.array([1.0, 2.0, 3.0])
+    ball = 'AttributeBegin\nMaterial "matte" "rgb Kd" [ 0.6 0.6 0.6 ]\nShape "sphere" "float radius" [ 1 ]\nAttributeEnd\n'
+    sc = _env_scene(tmp_path, ball, "0 0 5  0 0 0  0 1 0", maxdepth=6)
+    img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
+    assert np.allclose(img.mean(axis=(0, 1)), 0.6 * L, rtol=3e-3), (img.mean(axis=(0, 1)), 0.6 * L)
+    mirror = ('AttributeBegin\nMaterial "mirror" "rgb Kr" [ 0.8 0.5 0.25 ]\nShape "trianglemesh" "integer indices" '
+              '[ 0 1 2 0 2 3 ] "point P" [ -1e4 0 -1e4  -1e4 0 1e4  1e4 0 1e4  1e4 0 -1e4 ]\nAttributeEnd\n')
+    sc = _env_scene(tmp_path, mirror, "0 0.5 0  0.3 0 0.2  0 1 0", spp=2, maxdepth=2, name="m.pbrt")
+    img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
+    assert np.allclose(img, np.array([0.8, 0.5, 0.25]) * L, rtol=1e-5), (img.min(axis=(0, 1)), img.max(axis=(0, 1)))

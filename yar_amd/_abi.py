@@ -55,6 +55,11 @@ class SphereLightDesc(C.Structure):
                 ("emitted_radiance", C.c_float * 3), ("sample_count", C.c_int32)]
 
 
+class EnvironmentLightDesc(C.Structure):
+    _fields_ = [("light_to_world", C.c_float * 12), ("world_to_light", C.c_float * 12),
+                ("radiance", C.c_float * 3), ("sample_count", C.c_int32)]
+
+
 class RenderConfig(C.Structure):
     _fields_ = [("film_width", C.c_int32), ("film_height", C.c_int32),
                 ("region_x0", C.c_int32), ("region_y0", C.c_int32),
@@ -81,7 +86,8 @@ class SceneDesc(C.Structure):
                 ("spot_lights", C.POINTER(SpotLightDesc)), ("spot_light_count", C.c_int32),
                 ("directional_lights", C.POINTER(DirectionalLightDesc)), ("directional_light_count", C.c_int32),
                 ("mesh_lights", C.POINTER(MeshLightDesc)), ("mesh_light_count", C.c_int32),
-                ("sphere_lights", C.POINTER(SphereLightDesc)), ("sphere_light_count", C.c_int32)]
+                ("sphere_lights", C.POINTER(SphereLightDesc)), ("sphere_light_count", C.c_int32),
+                ("environment_light", C.POINTER(EnvironmentLightDesc)), ("environment_light_count", C.c_int32)]
 
 
 class GpuOptions(C.Structure):

@@ -152,6 +152,7 @@ struct yar_gpu_ctx {
     DevBuf<DevMeshLight> mesh_lights;
     DevBuf<float> mesh_light_cdf;
     DevBuf<DevSphere> spheres;
+    DevBuf<DevEnv> env;
     DevScene scene{};
     std::vector<int32_t> tri_base;        // per object (for hit conversion on the host)
 
@@ -217,7 +218,7 @@ struct yar_gpu_ctx {
         DevBuf<int>* bi[] = {&pix_view, &tile_first, &side_view};
         for (auto* b : bi) b->release();
         vtx_uv.release(); materials.release(); rects.release(); points.release();
-        spots.release(); dirs.release(); mesh_lights.release(); mesh_light_cdf.release(); spheres.release();
+        spots.release(); dirs.release(); mesh_lights.release(); mesh_light_cdf.release(); spheres.release(); env.release();
         pixels.release(); pix_first.release(); dviews.release(); chunks.release(); chunk_slot.release();
         perm.release(); jump_lo.release(); jump_hi.release(); hit_ot.release(); tr_ot.release(); state.release(); rng.release();
         DevBuf<uint32_t>* bu[] = {&q_a, &q_b, &sh_path, &mi_path, &dl_path, &counts, &q_unsorted};
@@ -523,6 +524,14 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
         for (int t = 0; t < n; t++) { sum += areas[(size_t)t]; mcdf.push_back(sum / total); }
     }
     if (mcdf.empty()) mcdf.push_back(1.f);
+    std::vector<DevEnv> envl(1);
+    std::memset(envl.data(), 0, sizeof(DevEnv));
+    if (d->environment_light_count > 0) {
+        const yar_environment_light_desc& l = d->environment_light[0];
+        m34_rows(l.light_to_world, envl[0].l2w0, envl[0].l2w1, envl[0].l2w2);
+        m34_rows(l.world_to_light, envl[0].w2l0, envl[0].w2l1, envl[0].w2l2);
+        envl[0].L = make_float4(l.radiance[0], l.radiance[1], l.radiance[2], 0.f);
+    }
     std::vector<DevSphere> sph(std::max(1, d->sphere_light_count));
     for (int i = 0; i < d->sphere_light_count; i++) {
         const yar_sphere_light_desc& l = d->sphere_lights[i];
@@ -550,6 +559,7 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
     ctx->mesh_lights.upload(ml, s);
     ctx->mesh_light_cdf.upload(mcdf, s);
     ctx->spheres.upload(sph, s);
+    ctx->env.upload(envl, s);
     HIPX(hipStreamSynchronize(s));
     auto t2 = std::chrono::steady_clock::now();
 
@@ -575,6 +585,8 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
     S.mesh_lights = ctx->mesh_lights.p;
     S.mesh_light_cdf = ctx->mesh_light_cdf.p;
     S.spheres = ctx->spheres.p;
+    S.env = ctx->env.p;
+    S.n_env = d->environment_light_count;
     S.n_rects = d->rect_light_count;
     S.n_points = d->point_light_count;
     S.n_spots = d->spot_light_count;
@@ -582,7 +594,7 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
     S.n_mesh_lights = d->mesh_light_count;
     S.n_spheres = d->sphere_light_count;
     S.n_lights = d->rect_light_count + d->point_light_count + d->spot_light_count + d->directional_light_count +
-                 d->sphere_light_count + d->mesh_light_count;                            // update_total_light_count, light.cpp:35
+                 d->sphere_light_count + d->mesh_light_count + d->environment_light_count;                            // update_total_light_count, light.cpp:35
 
     yar_gpu_info& inf = ctx->info;
     inf.scene_bytes = (int64_t)(sn.size() * 8 + mn.size() * 8 + ot.size() * sizeof(ObjTrav) + tv.size() * 16 +
@@ -1290,11 +1302,13 @@ int yar_gpu_create(const yar_scene_desc* scene, const yar_gpu_options* options, 
         scene->directional_light_count < 0 || scene->mesh_light_count < 0 || scene->sphere_light_count < 0 ||
         (scene->spot_light_count > 0 && !scene->spot_lights) ||
         (scene->sphere_light_count > 0 && !scene->sphere_lights) ||
+        scene->environment_light_count < 0 || scene->environment_light_count > 1 ||
+        (scene->environment_light_count > 0 && !scene->environment_light) ||
         (scene->directional_light_count > 0 && !scene->directional_lights) ||
         (scene->mesh_light_count > 0 && !scene->mesh_lights))
         return fail(nullptr, YAR_ERR_INVALID_ARGUMENT, "yar_gpu_create: invalid light arrays");
     if (scene->rect_light_count + scene->point_light_count + scene->spot_light_count + scene->directional_light_count +
-            scene->sphere_light_count + scene->mesh_light_count <= 0)
+            scene->sphere_light_count + scene->mesh_light_count + scene->environment_light_count <= 0)
         return fail(nullptr, YAR_ERR_UNSUPPORTED, "yar_gpu_create: scene has no lights");
     const int n_area = scene->rect_light_count + scene->sphere_light_count + scene->mesh_light_count;
     for (int i = 0; i < scene->sphere_light_count; i++) {

@@ -97,6 +97,8 @@ struct DevDirLight { float4 dir; float4 E; };          // direction towards the 
 struct DevMeshLight { int obj, tri_base, tri_count, cdf; float area, pad0, pad1, pad2; float4 L; };
 // Diffuse_Sphere_Light: world-space centre with the radius in p.w, radiance
 struct DevSphere { float4 p; float4 L; };
+// Environment_Light with a constant map: light_to_world and world_to_light rows, radiance
+struct DevEnv { float4 l2w0, l2w1, l2w2, w2l0, w2l1, w2l2, L; };
 
 struct DevScene {
     const uint2* scene_nodes;
@@ -120,8 +122,10 @@ struct DevScene {
     const DevMeshLight* mesh_lights;
     const float* mesh_light_cdf;
     const DevSphere* spheres;
-    // area lights are numbered rectangles, spheres, meshes (an object's area-light index)
-    int n_rects, n_points, n_spots, n_dirs, n_mesh_lights, n_spheres, n_lights;
+    const DevEnv* env;
+    // area lights are numbered rectangles, spheres, meshes (an object's area-light index); the
+    // environment light, when there is one, comes last in the light order
+    int n_rects, n_points, n_spots, n_dirs, n_mesh_lights, n_spheres, n_env, n_lights;
 };
 
 // ---- ray/box slab test (Bounding_Box::intersect_by_ray_without_NaNs, bounding_box.h:63) ----

@@ -180,6 +180,33 @@ YG_INLINE RGB area_emission(const DevScene& s, int light, int obj, uint32_t tri,
     return mesh_light_emission(s, light - s.n_rects - s.n_spheres, obj, tri, b, ray_d);
 }
 
+// Environment_Light_Sampler over a constant (one-texel) map (light_sampling.cpp:11-73): direction
+// to (u, v) in the light frame, the radiance a missed ray collects (bilinear weights of the white
+// texel summed in float, image_texture.cpp:441-480, times the radiance) and the solid-angle pdf of
+// the one-cell Distribution_2D (pdf_uv = 1).
+constexpr float kPi2 = 6.2831853f;                                  // src/lib/math.h:10
+constexpr float kPi2Inv = 1.f / kPi2;
+// Out of line: inlined at the two miss sites its atan2/acos expansion costs k_shade an occupancy step.
+__device__ __noinline__ RGB env_filtered(const DevEnv* __restrict__ env, V3 world_dir) {
+    const DevEnv& E = *env;
+    const V3 d = xform_vector(M34{E.w2l0, E.w2l1, E.w2l2}, world_dir);
+    float phi = atan2f(d.y, d.x);
+    phi = phi < 0 ? phi + kPi2 : phi;
+    const float theta = acosf(fminf(fmaxf(d.z, -1.f), 1.f));
+    const float u = fminf(fmaxf(phi * kPi2Inv, 0.f), kOneMinusEps);
+    const float v = fminf(theta * kPiInv, kOneMinusEps);
+    const float x = u * 1.f - 0.5f, y = v * 1.f - 0.5f;
+    const float wx = x - floorf(x), wy = y - floorf(y);
+    const float w = (((1 - wy) * (1 - wx) + (1 - wy) * wx) + wy * (1 - wx)) + wy * wx;
+    return rgb(w, w, w) * rgb4(E.L);
+}
+YG_INLINE float env_pdf(const DevScene& s, V3 world_dir) {
+    const DevEnv& E = *s.env;
+    const V3 d = xform_vector(M34{E.w2l0, E.w2l1, E.w2l2}, world_dir);
+    const float sin_theta = sqrtf(fmaxf(0.f, 1.f - d.z * d.z));
+    return 1.f / (2 * kPi * kPi * sin_theta);
+}
+
 // Guided form: a pcg32 stream per (pixel, sample), disjoint from the pixel streams (whose upper
 // 32 sequence bits are zero), for strategy choices, mixture sampling and fallback dimensions.
 YG_INLINE uint64_t guide_stream(const Chunk& c, uint2 pk) {
@@ -663,7 +690,14 @@ __global__ void k_mis(DevScene sc, Paths P, Queues q, StackCfg cfg) {
                 const HitRec& h = tr.hit;
                 const float4 o4 = q.mis_o[i], d4 = q.mis_d[i];
                 const int light = (int)fbits(o4.w);
-                if (tr.result() && sc.obj_info[h.obj].y == light) {
+                if (!tr.result() && light == sc.n_rects + sc.n_spheres + sc.n_mesh_lights) {
+                    // environment light: the BSDF sample counts when nothing blocks it (direct_lighting.cpp:355-381)
+                    const float4 f4v = q.mis_f[i];
+                    const float bpdf = f4v.w;
+                    const float w = power_heuristic(bpdf, d4.w);
+                    const RGB contrib = (rgb4(sc.env->L) * rgb4(f4v)) * (w * fabsf(q.mis_ndw[i]) / bpdf);
+                    P.dl_bsdf[q.mis_path[i]] = f4(contrib, 0.f);
+                } else if (tr.result() && sc.obj_info[h.obj].y == light) {
                     float lpdf;
                     RGB Le;
                     if (light < sc.n_rects) {
@@ -797,12 +831,14 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
             if (!delta_traced) {
                 if (bounce == 0) {
                     if (hit && light >= 0) L = L + coeff * area_emission(s, light, ot.x, (uint32_t)ot.y, v3(tb.y, tb.z, tb.w), from4(d4));
+                    else if (!hit && s.n_env) L = L + coeff * env_filtered(s.env, from4(d4));
                     if (c.max_bounces == 0) { finished = true; continue_body = false; }
                 }
                 if (continue_body && (!hit || light >= 0)) { finished = true; continue_body = false; }
             } else {
                 RGB emitted = rgb(0.f, 0.f, 0.f);
                 if (hit && light >= 0) emitted = area_emission(s, light, ot.x, (uint32_t)ot.y, v3(tb.y, tb.z, tb.w), from4(d4));
+                else if (!hit && s.n_env) emitted = env_filtered(s.env, from4(d4));
                 L = L + coeff * emitted;
                 bounce++;
                 if (bounce == c.max_bounces || !hit || light >= 0) { finished = true; continue_body = false; }
@@ -1008,9 +1044,8 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                                     want_mis = true;
                                 }
                             }
-                        } else {
-                            lj -= s.n_spheres;                              // direct_lighting.cpp:262-322
-                            const DevMeshLight ML = s.mesh_lights[lj];
+                        } else if ((lj -= s.n_spheres) < s.n_mesh_lights) {
+                            const DevMeshLight ML = s.mesh_lights[lj];      // direct_lighting.cpp:262-322
                             {
                                 // Diffuse_Triangle_Mesh_Light_Sampler::sample (light_sampling.cpp:126-161)
                                 float u0;
@@ -1053,6 +1088,39 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                                 if (!is_black(f)) {
                                     mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
                                     mi_ndw = dot(sc.n, wi); mi_lndw = 0.f; mi_light = s.n_rects + s.n_spheres + lj;
+                                    want_mis = true;
+                                }
+                            }
+                        } else {
+                            const RGB Le = rgb4(s.env->L);                  // direct_lighting.cpp:324-384
+                            {
+                                // the one-cell Distribution_2D samples uv = (u1, u0) with pdf_uv = 1
+                                const float phi = uly * kPi2, theta = ulx * kPi;
+                                const float sin_theta = sinf(theta);
+                                const float lpdf = 1.f / (2 * kPi * kPi * sin_theta);
+                                const DevEnv& E = *s.env;
+                                const V3 wi = xform_vector(M34{E.l2w0, E.l2w1, E.l2w2},
+                                                           v3(sin_theta * cosf(phi), sin_theta * sinf(phi), cosf(theta)));
+                                const float ndw = dot(sc.n, wi);
+                                if (scattering_possible(m.type, ndw)) {
+                                    const RGB f = bsdf_eval(m, sc, sc.wo, wi);
+                                    if (!is_black(f)) {
+                                        const float bpdf = bsdf_pdf(m, sc, sc.wo, wi);
+                                        const float w = power_heuristic(lpdf, bpdf);
+                                        sh_c = (Le * f) * (w * fabsf(ndw) / lpdf);
+                                        sh_o = origin_dir(sc, wi); sh_d = wi; sh_tmax = f_inf();
+                                        want_shadow = true;
+                                    }
+                                }
+                            }
+                            {
+                                V3 wi = v3(0.f, 0.f, 0.f);
+                                float bpdf = 0.f;
+                                const RGB f = bsdf_sample(m, sc, ubx, uby, u_s, sc.wo, wi, bpdf);
+                                if (!is_black(f) && !is_black(Le)) {
+                                    mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
+                                    mi_ndw = dot(sc.n, wi); mi_lndw = env_pdf(s, wi);
+                                    mi_light = s.n_rects + s.n_spheres + s.n_mesh_lights;
                                     want_mis = true;
                                 }
                             }

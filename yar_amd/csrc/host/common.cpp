@@ -87,6 +87,8 @@ void SceneData::finalize_desc() {
     desc.mesh_light_count = (int32_t)mesh_lights.size();
     desc.sphere_lights = sphere_lights.data();
     desc.sphere_light_count = (int32_t)sphere_lights.size();
+    desc.environment_light = environment_lights.empty() ? nullptr : environment_lights.data();
+    desc.environment_light_count = (int32_t)environment_lights.size();
 }
 
 }  // namespace yar

@@ -111,6 +111,7 @@ struct SceneData {
     std::vector<yar_directional_light_desc> directional_lights;
     std::vector<yar_mesh_light_desc> mesh_lights;
     std::vector<yar_sphere_light_desc> sphere_lights;
+    std::vector<yar_environment_light_desc> environment_lights;   // at most one
     yar_scene_desc desc{};
     void finalize_desc();
 };

@@ -514,6 +514,21 @@ struct Loader {
             L.direction[0] = dir.x; L.direction[1] = dir.y; L.direction[2] = dir.z;
             for (int i = 0; i < 3; i++) L.irradiance[i] = E[i] * sc[i];
             sd.directional_lights.push_back(L);
+        } else if (type == "infinite") {                                    // pbrt_scene.cpp:958-980
+            for (const Param& e : ps)                                          // Lights.cpp:15-48
+                if (e.name != "L" && e.name != "scale" && e.name != "nsamples" && e.name != "mapname")
+                    throw ParseError{"unknown 'infinite' light source param '" + e.name + "'"};
+            if (find(ps, "mapname")) throw ParseError{"environment maps (infinite light mapname) are not supported"};
+            if (!sd.environment_lights.empty()) throw ParseError{"more than one infinite light"};
+            yar_environment_light_desc L{};
+            float Lrgb[3] = {1, 1, 1};
+            get_rgb(ps, "L", Lrgb);
+            const M34 w2l = inverse_transform(gs.ctm);
+            std::memcpy(L.light_to_world, gs.ctm.a, sizeof(gs.ctm.a));
+            std::memcpy(L.world_to_light, w2l.a, sizeof(w2l.a));
+            for (int i = 0; i < 3; i++) L.radiance[i] = sc[i] * Lrgb[i];
+            L.sample_count = get_i(ps, "nsamples", 1);
+            sd.environment_lights.push_back(L);
         } else {
             throw ParseError{"unsupported light source '" + type + "'"};
         }
@@ -832,7 +847,8 @@ struct Loader {
             sd.desc.camera.fov_y_degrees = fov;
         }
         if (sd.point_lights.empty() && sd.rect_lights.empty() && sd.spot_lights.empty() &&
-            sd.directional_lights.empty() && sd.mesh_lights.empty() && sd.sphere_lights.empty())
+            sd.directional_lights.empty() && sd.mesh_lights.empty() && sd.sphere_lights.empty() &&
+            sd.environment_lights.empty())
             throw ParseError{"scene has no supported lights (the default directional light is not supported)"};
         // emitter objects index the concatenation rect lights, sphere lights, mesh lights
         const int32_t n_rects = (int32_t)sd.rect_lights.size(), n_spheres = (int32_t)sd.sphere_lights.size();
