@@ -70,6 +70,8 @@ typedef struct yar_mesh_desc {          /* Triangle_Mesh, src/lib/triangle_mesh.
 #define YAR_MATERIAL_PBRT3_UBER 7        /* Pbrt3_Uber_Material -> Pbrt3_Uber_BRDF (bsdf_pbrt.cpp:14-91) for
                                             Kd/Ks, delta reflection Kr and opacity pass-through chosen per
                                             component (delta_scattering.cpp:102-152); Kt must be black */
+#define YAR_MATERIAL_MIX     8           /* Mix_Material -> Mix_BSDF (bsdf.cpp:497-570) of two sub-materials that
+                                            both have a BSDF (matte, metal, plastic, substrate, rough glass) */
 
 typedef struct yar_material_desc {
     int32_t type;
@@ -87,6 +89,8 @@ typedef struct yar_material_desc {
                                            uber Ks */
     float   delta_reflectance[3];       /* uber Kr */
     float   opacity[3];                 /* uber opacity (1 = opaque) */
+    int32_t mix_materials[2];           /* mix: the two sub-materials (indices into materials) */
+    float   mix_amount[3];              /* mix: pbrt "amount", the weight of the first sub-material */
 } yar_material_desc;
 
 typedef struct yar_object_desc {        /* Scene_Object, src/lib/scene_object.h:8 */

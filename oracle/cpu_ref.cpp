@@ -450,6 +450,9 @@ struct Material {
     // diffuse 0, specular 1, delta reflection 2, (delta transmission 3), opacity 4
     RGB ks, kr, opacity;
     int ncomp = 0, comps[5] = {};
+    // mix (material.h:153, Mix_BSDF bsdf.cpp:497-570): sub-materials, amount
+    int sub[2] = {-1, -1};
+    RGB amount;
 };
 struct RectLight { M34 l2w; RGB L; V2 size; };
 struct PointLight { V3 p; RGB I; };
@@ -992,8 +995,77 @@ static float rough_glass_pdf(const Material& m, const Shading& sc, V3 wo, V3 wi)
     return wh_pdf * dwh_dwi * (1.f - rp);
 }
 
+static bool has_transmission(const Scene& s, const Material& m) {                 // BSDF::transmission_scattering
+    if (m.type == YAR_MATERIAL_MIX) return has_transmission(s, s.materials[m.sub[0]]) || has_transmission(s, s.materials[m.sub[1]]);
+    return m.type == YAR_MATERIAL_GLASS;                                             // Rough_Glass_BSDF
+}
+static bool sub_scatters(const Scene& s, const Material& m, float n_dot_wi) {
+    return n_dot_wi > 0.f || (n_dot_wi < 0.f && has_transmission(s, m));
+}
+static RGB bsdf_eval_m(const Scene& s, const Material& m, const Shading& sc, V3 wo, V3 wi);
+static float bsdf_pdf_m(const Scene& s, const Material& m, const Shading& sc, V3 wo, V3 wi);
+static RGB bsdf_sample_m(const Scene& s, const Material& m, const Shading& sc, V2 u, float u_scat, V3 wo, V3* wi, float* pdf);
+// Mix_BSDF: mix_coeff = White - amount weights the second BSDF, p_coeff = 1 - luminance(mix_coeff) is
+// the probability of sampling it; each BSDF counts only where it can scatter towards wi
+static RGB mix_coeff(const Material& m) { return splat(1.f) - m.amount; }
+static RGB mix_eval(const Scene& s, const Material& m, const Shading& sc, V3 wo, V3 wi) {
+    const Material& b1 = s.materials[m.sub[0]];
+    const Material& b2 = s.materials[m.sub[1]];
+    RGB f1, f2;
+    float ndw = dot(sc.n, wi);
+    if (sub_scatters(s, b1, ndw)) f1 = bsdf_eval_m(s, b1, sc, wo, wi);
+    if (sub_scatters(s, b2, ndw)) f2 = bsdf_eval_m(s, b2, sc, wo, wi);
+    const RGB mc = mix_coeff(m);
+    return (splat(1.f) - mc) * f1 + mc * f2;
+}
+static float mix_pdf(const Scene& s, const Material& m, const Shading& sc, V3 wo, V3 wi) {
+    const Material& b1 = s.materials[m.sub[0]];
+    const Material& b2 = s.materials[m.sub[1]];
+    const float p = 1.f - luminance(mix_coeff(m));
+    float ndw = dot(sc.n, wi);
+    float pdf1 = 0.f, pdf2 = 0.f;
+    if (sub_scatters(s, b1, ndw)) pdf1 = bsdf_pdf_m(s, b1, sc, wo, wi);
+    if (sub_scatters(s, b2, ndw)) pdf2 = bsdf_pdf_m(s, b2, sc, wo, wi);
+    return (1.f - p) * pdf1 + p * pdf2;
+}
+static RGB mix_sample(const Scene& s, const Material& m, const Shading& sc, V2 u, float u_scat, V3 wo, V3* wi, float* pdf) {
+    const Material& b1 = s.materials[m.sub[0]];
+    const Material& b2 = s.materials[m.sub[1]];
+    const RGB mc = mix_coeff(m);
+    const float p = 1.f - luminance(mc);
+    RGB f1, f2;
+    float pdf1 = 0.f, pdf2 = 0.f;
+    if (u_scat < p) {
+        u_scat = u_scat / p;
+        f2 = bsdf_sample_m(s, b2, sc, u, u_scat, wo, wi, &pdf2);
+        if (sub_scatters(s, b1, dot(sc.n, *wi))) {
+            pdf1 = bsdf_pdf_m(s, b1, sc, wo, *wi);
+            f1 = bsdf_eval_m(s, b1, sc, wo, *wi);
+        }
+    } else {
+        u_scat = (u_scat - p) / (1.f - p);
+        f1 = bsdf_sample_m(s, b1, sc, u, u_scat, wo, wi, &pdf1);
+        if (sub_scatters(s, b2, dot(sc.n, *wi))) {
+            pdf2 = bsdf_pdf_m(s, b2, sc, wo, *wi);
+            f2 = bsdf_eval_m(s, b2, sc, wo, *wi);
+        }
+    }
+    *pdf = (1.f - p) * pdf1 + p * pdf2;
+    return (splat(1.f) - mc) * f1 + mc * f2;
+}
+
 static RGB bsdf_eval(const Scene& s, const Shading& sc, V3 wo, V3 wi) {
-    const Material& m = s.materials[sc.material];
+    return bsdf_eval_m(s, s.materials[sc.material], sc, wo, wi);
+}
+static float bsdf_pdf(const Scene& s, const Shading& sc, V3 wo, V3 wi) {
+    return bsdf_pdf_m(s, s.materials[sc.material], sc, wo, wi);
+}
+static RGB bsdf_sample(const Scene& s, const Shading& sc, V2 u, float u_scat, V3 wo, V3* wi, float* pdf) {
+    return bsdf_sample_m(s, s.materials[sc.material], sc, u, u_scat, wo, wi, pdf);
+}
+
+static RGB bsdf_eval_m(const Scene& s, const Material& m, const Shading& sc, V3 wo, V3 wi) {
+    if (m.type == YAR_MATERIAL_MIX) return mix_eval(s, m, sc, wo, wi);
     if (m.type == YAR_MATERIAL_GLASS) return rough_glass_eval(m, sc, wo, wi);
     if (m.type == YAR_MATERIAL_DIFFUSE) return Pi_Inv * m.refl;                      // bsdf.cpp:68
     if (m.type == YAR_MATERIAL_PLASTIC) return plastic_eval(m, sc, wo, wi);
@@ -1007,8 +1079,8 @@ static RGB bsdf_eval(const Scene& s, const Shading& sc, V3 wo, V3 wi) {
     float D = D_aniso(wh, m.alpha_x, m.alpha_y);
     return F * ((G * D) / (4.f * wol.z * wil.z));
 }
-static float bsdf_pdf(const Scene& s, const Shading& sc, V3 wo, V3 wi) {
-    const Material& m = s.materials[sc.material];
+static float bsdf_pdf_m(const Scene& s, const Material& m, const Shading& sc, V3 wo, V3 wi) {
+    if (m.type == YAR_MATERIAL_MIX) return mix_pdf(s, m, sc, wo, wi);
     if (m.type == YAR_MATERIAL_DIFFUSE) return dot(sc.n, wi) / Pi;                   // bsdf.cpp:80
     if (m.type == YAR_MATERIAL_PLASTIC) return plastic_pdf(m, sc, wo, wi);
     if (m.type == YAR_MATERIAL_COATED_DIFFUSE) return coated_pdf(m, sc, wo, wi);
@@ -1018,12 +1090,12 @@ static float bsdf_pdf(const Scene& s, const Shading& sc, V3 wo, V3 wi) {
     V3 wh = normalized(wol + wil);
     return vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
 }
-static RGB bsdf_sample(const Scene& s, const Shading& sc, V2 u, float u_scat, V3 wo, V3* wi, float* pdf) {
-    const Material& m = s.materials[sc.material];
+static RGB bsdf_sample_m(const Scene& s, const Material& m, const Shading& sc, V2 u, float u_scat, V3 wo, V3* wi, float* pdf) {
+    if (m.type == YAR_MATERIAL_MIX) return mix_sample(s, m, sc, u, u_scat, wo, wi, pdf);
     if (m.type == YAR_MATERIAL_DIFFUSE) {                                           // bsdf.cpp:71
         *wi = to_world(sc, sample_cosine(u));
-        *pdf = bsdf_pdf(s, sc, wo, *wi);
-        return bsdf_eval(s, sc, wo, *wi);
+        *pdf = bsdf_pdf_m(s, m, sc, wo, *wi);
+        return bsdf_eval_m(s, m, sc, wo, *wi);
     }
     if (m.type == YAR_MATERIAL_PLASTIC) {                                           // bsdf.cpp:235-251
         if (u_scat < 0.5f) {
@@ -1083,7 +1155,7 @@ static RGB bsdf_sample(const Scene& s, const Shading& sc, V2 u, float u_scat, V3
     if (wil.z <= 0.f) return RGB{};
     *wi = to_world(sc, wil);
     *pdf = vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
-    return bsdf_eval(s, sc, wo, *wi);
+    return bsdf_eval_m(s, m, sc, wo, *wi);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1134,8 +1206,7 @@ static RGB light_rect(const Scene& s, const Shading& sc, int li, V2 ul, V2 ub, f
         float lndw = dot(ln, -wi);
         if (lndw > 1e-4f) {
             float ndw = dot(sc.n, wi);
-            const bool rough_glass = s.materials[sc.material].type == YAR_MATERIAL_GLASS;
-            if (ndw > 0.f || (ndw < 0.f && rough_glass)) {   // scattering_possible (direct_lighting.cpp:133-135)
+            if (sub_scatters(s, s.materials[sc.material], ndw)) {   // scattering_possible (direct_lighting.cpp:133-135)
                 RGB f = bsdf_eval(s, sc, sc.wo, wi);
                 if (!is_black(f)) {
                     pc.any++;
@@ -1177,8 +1248,7 @@ static RGB light_rect(const Scene& s, const Shading& sc, int li, V2 ul, V2 ub, f
 }
 
 static bool scattering_possible(const Scene& s, const Shading& sc, float n_dot_wi) {   // direct_lighting.cpp:133-135
-    const bool transmission = s.materials[sc.material].type == YAR_MATERIAL_GLASS;       // Rough_Glass_BSDF
-    return n_dot_wi > 0.f || (n_dot_wi < 0.f && transmission);
+    return sub_scatters(s, s.materials[sc.material], n_dot_wi);
 }
 
 static RGB light_spot(const Scene& s, const Shading& sc, const SpotLight& L, PathCounters& pc, Counters* tc) {
@@ -1515,6 +1585,11 @@ static RGB trace_path(const Scene& s, Ray ray, PixelSampler& ps, PathCounters& p
         // rescaled u_scat on; the BSDF continuation is divided by the probability of the BSDF layers
         float bsdf_layer_probability = 1.f;
         RGB uber_att; V3 uber_dir;
+        if (m.type == YAR_MATERIAL_MIX) {                                            // get_mix_info, delta_scattering.cpp:154-189
+            const float mix_amount = std::min(luminance(m.amount), 1.f);
+            if (u_scat < mix_amount) u_scat /= mix_amount;
+            else u_scat = (u_scat - mix_amount) / (1.f - mix_amount);
+        }
         if (m.type == YAR_MATERIAL_PBRT3_UBER) {
             const float fp_index = u_scat * float(m.ncomp);
             const int ci = int(fp_index);
@@ -1776,6 +1851,11 @@ static bool load_scene(const yar_scene_desc* d, Scene& s, int threads) {
         };
         m.alpha_x = remap(md.u_roughness);
         m.alpha_y = remap(md.v_roughness);
+        if (md.type == YAR_MATERIAL_MIX) {
+            m.sub[0] = md.mix_materials[0];
+            m.sub[1] = md.mix_materials[1];
+            m.amount = {md.mix_amount[0], md.mix_amount[1], md.mix_amount[2]};
+        }
         if (md.type == YAR_MATERIAL_PBRT3_UBER) {                                    // pbrt_scene.cpp:585-608
             m.ks = {md.specular[0], md.specular[1], md.specular[2]};
             m.kr = {md.delta_reflectance[0], md.delta_reflectance[1], md.delta_reflectance[2]};

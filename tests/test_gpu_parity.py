@@ -283,3 +283,28 @@ def test_render_environment_light_matches_oracle(c1_path, oracle, tmp_path):
     assert mean_rel <= 1e-3
     assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
     assert (film["weight_sum"] == ofilm["weight_sum"]).all()
+
+
+def test_render_mix_matches_oracle(c1_path, oracle, tmp_path):
+    # the config-1 room with mix materials (matte + plastic, substrate + GGX metal, matte + rough
+    # glass): Mix_BSDF (bsdf.cpp:497-570), get_mix_info's u_scat rescaling (delta_scattering.cpp:154-189);
+    # the renderer switches to its mix-aware shade kernel for this scene
+    lines = open(c1_path).read().splitlines()
+    mats = [i for i, l in enumerate(lines) if l.startswith('Material "matte"')]
+    named = ('MakeNamedMaterial "ma" "string type" [ "matte" ] "rgb Kd" [ 0.7 0.3 0.2 ]\n'
+             'MakeNamedMaterial "pl" "string type" [ "plastic" ] "rgb Kd" [ 0.1 0.2 0.5 ] "float roughness" [ 0.05 ]\n'
+             'MakeNamedMaterial "su" "string type" [ "substrate" ] "rgb Kd" [ 0.4 0.5 0.2 ] "float uroughness" [ 0.05 ] "float vroughness" [ 0.2 ]\n'
+             'MakeNamedMaterial "me" "string type" [ "metal" ] "rgb eta" [ 0.2 0.9 1.1 ] "rgb k" [ 3.9 2.4 2.1 ] "float roughness" [ 0.1 ]\n'
+             'MakeNamedMaterial "rg" "string type" [ "glass" ] "float uroughness" [ 0.2 ] "float vroughness" [ 0.2 ]\n')
+    lines[mats[0]] = 'Material "mix" "string namedmaterial1" [ "ma" ] "string namedmaterial2" [ "pl" ] "rgb amount" [ 0.3 0.5 0.7 ]'
+    lines[mats[1]] = 'Material "mix" "string namedmaterial1" [ "su" ] "string namedmaterial2" [ "me" ]'
+    lines[mats[-1]] = 'Material "mix" "string namedmaterial1" [ "ma" ] "string namedmaterial2" [ "rg" ] "rgb amount" [ 0.6 0.6 0.6 ]'
+    text = "\n".join(lines) + "\n"
+    p = tmp_path / "c1_mix.pbrt"
+    p.write_text(text.replace("WorldBegin\n", "WorldBegin\n" + named, 1))
+    img, film, var, st, oimg, ofilm, ovar = _render_pair(str(p), oracle, nx=4)
+    exact, close, mean_rel = _compare_images(img, oimg, film, ofilm, "c1 mix 16spp")
+    assert close >= 0.99
+    assert mean_rel <= 1e-3
+    assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
+    assert (film["weight_sum"] == ofilm["weight_sum"]).all()

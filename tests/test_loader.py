@@ -412,3 +412,34 @@ def test_infinite_light_without_map(tmp_path):
                 'LightSource "infinite"\nLightSource "infinite"\n'):
         with pytest.raises(yar_amd.YarError):
             _load(tmp_path, bad, name="bad.pbrt")
+
+
+def test_mix_material(tmp_path):
+    """import_pbrt_material's mix branch (pbrt_scene.cpp:563-569) over pbrt-parser's named
+    sub-materials (Materials.cpp:376-402), amount default 0.5."""
+    body = LIGHT + '''MakeNamedMaterial "a" "string type" [ "matte" ] "rgb Kd" [ 0.1 0.2 0.3 ]
+MakeNamedMaterial "b" "string type" [ "plastic" ]
+Material "mix" "string namedmaterial1" [ "a" ] "string namedmaterial2" [ "b" ] "rgb amount" [ 0.2 0.4 0.6 ]
+Shape "trianglemesh" "integer indices" [ 0 1 2 ] "point P" [ 0 0 0  1 0 0  0 1 0 ]
+Material "mix" "string namedmaterial1" [ "b" ] "string namedmaterial2" [ "a" ]
+Shape "trianglemesh" "integer indices" [ 0 1 2 ] "point P" [ 0 0 1  1 0 1  0 1 1 ]
+'''
+    d = _load(tmp_path, body).desc
+    mats = [d.materials[i] for i in range(d.material_count)]
+    mixes = [m for m in mats if m.type == yar_amd._abi.YAR_MATERIAL_MIX]
+    assert len(mixes) == 2
+    a, b = mixes[0].mix_materials
+    assert mats[a].type == yar_amd._abi.YAR_MATERIAL_DIFFUSE and mats[b].type == yar_amd._abi.YAR_MATERIAL_PLASTIC
+    assert np.allclose(list(mixes[0].mix_amount), [0.2, 0.4, 0.6])
+    assert list(mixes[1].mix_materials) == [b, a] and list(mixes[1].mix_amount) == [0.5, 0.5, 0.5]
+
+
+@pytest.mark.parametrize("sub", ['"mirror"', '"glass"', '"uber"'])
+def test_mix_of_delta_or_uber_is_a_parse_error(tmp_path, sub):
+    body = LIGHT + f'''MakeNamedMaterial "a" "string type" [ "matte" ]
+MakeNamedMaterial "b" "string type" [ {sub} ]
+Material "mix" "string namedmaterial1" [ "a" ] "string namedmaterial2" [ "b" ]
+Shape "trianglemesh" "integer indices" [ 0 1 2 ] "point P" [ 0 0 0  1 0 0  0 1 0 ]
+'''
+    with pytest.raises(yar_amd.YarError):
+        _load(tmp_path, body)

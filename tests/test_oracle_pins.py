@@ -989,3 +989,52 @@ def test_environment_light_convex_object_and_delta_miss(tmp_path):
     sc = _env_scene(tmp_path, mirror, "0 0.5 0  0.3 0 0.2  0 1 0", spp=2, maxdepth=2, name="m.pbrt")
     img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
     assert np.allclose(img, np.array([0.8, 0.5, 0.25]) * L, rtol=1e-5), (img.min(axis=(0, 1)), img.max(axis=(0, 1)))
+
+
+def _mix_plane(tmp_path, material_lines, light, look="0 0.5 0  0 0 0  0 0 1", spp=16, name="mix.pbrt"):
+    text = (f'LookAt {look}\nCamera "perspective" "float fov" [ 0.5 ]\n'
+            'Film "image" "integer xresolution" [ 8 ] "integer yresolution" [ 8 ]\n'
+            f'Sampler "stratified" "integer xsamples" [ {spp} ] "integer ysamples" [ {spp} ]\n'
+            'Integrator "path" "integer maxdepth" [ 1 ]\nWorldBegin\n' + light +
+            'MakeNamedMaterial "one" "string type" [ "plastic" ] "rgb Kd" [ 0.2 0.5 0.3 ] "rgb Ks" [ 0.4 0.4 0.4 ] "float roughness" [ 0.05 ]\n'
+            'MakeNamedMaterial "two" "string type" [ "substrate" ] "rgb Kd" [ 0.6 0.2 0.1 ] "float uroughness" [ 0.1 ] "float vroughness" [ 0.3 ]\n'
+            'MakeNamedMaterial "red" "string type" [ "matte" ] "rgb Kd" [ 0.8 0.1 0.3 ]\n'
+            'MakeNamedMaterial "blue" "string type" [ "matte" ] "rgb Kd" [ 0.1 0.3 0.9 ]\n'
+            + material_lines +
+            'Shape "trianglemesh" "integer indices" [ 0 1 2 0 2 3 ] "point P" [ -1e4 0 -1e4  -1e4 0 1e4  1e4 0 1e4  1e4 0 -1e4 ]\n'
+            'WorldEnd\n')
+    p = tmp_path / name
+    p.write_text(text)
+    return yar_amd.Scene(str(p))
+
+
+def test_mix_of_diffuse_point_lit_closed_form(tmp_path):
+    # Mix_BSDF (bsdf.cpp:497-521): f = amount f1 + (1 - amount) f2 per channel, so a point-lit mix of
+    # two matte materials shows (a Kd1 + (1 - a) Kd2) / pi * I / d^2
+    from tests.test_oracle_fixtures import _view_from_desc
+    a = np.array([0.3, 0.6, 0.9])
+    sc = _mix_plane(tmp_path, 'Material "mix" "string namedmaterial1" [ "red" ] "string namedmaterial2" [ "blue" ] '
+                    '"rgb amount" [ 0.3 0.6 0.9 ]\n', 'LightSource "point" "rgb I" [ 2 2 2 ] "point from" [ 0 1 0 ]\n', spp=2)
+    img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
+    want = (a * np.array([0.8, 0.1, 0.3]) + (1 - a) * np.array([0.1, 0.3, 0.9])) / np.pi * 2.0
+    assert np.allclose(img.mean(axis=(0, 1)), want, rtol=1e-4), (img.mean(axis=(0, 1)), want)
+
+
+def test_mix_is_linear_in_its_sub_materials(tmp_path):
+    # Under a constant sky, the plane's radiance is the sky-weighted albedo of its BSDF; for a mix it
+    # must equal a R1 + (1 - a) R2 with R1, R2 the renders of each sub-material alone.  This pins the
+    # sample choice (probability 1 - luminance(1 - amount) for the second BSDF), the mixed pdf and the
+    # mixed f together: any of them wrong biases the MIS estimate.
+    from tests.test_oracle_fixtures import _view_from_desc
+    sky = 'LightSource "infinite" "rgb L" [ 1 1 1 ]\n'
+    look = "0 0.5 0.4  0 0 0  0 1 0"
+    a = np.array([0.25, 0.5, 0.7])
+    means = []
+    for i, mat in enumerate(['NamedMaterial "one"\n', 'NamedMaterial "two"\n',
+                             'Material "mix" "string namedmaterial1" [ "one" ] "string namedmaterial2" [ "two" ] '
+                             '"rgb amount" [ 0.25 0.5 0.7 ]\n']):
+        sc = _mix_plane(tmp_path, mat, sky, look=look, spp=32, name=f"m{i}.pbrt")
+        img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
+        means.append(img.mean(axis=(0, 1)).astype(np.float64))
+    want = a * means[0] + (1 - a) * means[1]
+    assert np.allclose(means[2], want, rtol=5e-3), (means, want)

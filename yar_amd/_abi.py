@@ -4,6 +4,7 @@ import ctypes as C
 YAR_OK = 0
 YAR_MATERIAL_NONE, YAR_MATERIAL_DIFFUSE, YAR_MATERIAL_METAL, YAR_MATERIAL_GLASS = 0, 1, 2, 3
 YAR_MATERIAL_MIRROR, YAR_MATERIAL_PLASTIC, YAR_MATERIAL_COATED_DIFFUSE, YAR_MATERIAL_PBRT3_UBER = 4, 5, 6, 7
+YAR_MATERIAL_MIX = 8
 
 
 class MeshDesc(C.Structure):
@@ -19,7 +20,8 @@ class MaterialDesc(C.Structure):
                 ("eta", C.c_float * 3), ("k", C.c_float * 3), ("eta_i", C.c_float),
                 ("u_roughness", C.c_float), ("v_roughness", C.c_float),
                 ("index_of_refraction", C.c_float), ("specular_r0", C.c_float),
-                ("specular", C.c_float * 3), ("delta_reflectance", C.c_float * 3), ("opacity", C.c_float * 3)]
+                ("specular", C.c_float * 3), ("delta_reflectance", C.c_float * 3), ("opacity", C.c_float * 3),
+                ("mix_materials", C.c_int32 * 2), ("mix_amount", C.c_float * 3)]
 
 
 class ObjectDesc(C.Structure):

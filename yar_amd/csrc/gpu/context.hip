@@ -400,6 +400,7 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
         ctx->tri_base[i] = (int32_t)tb;
     }
     std::vector<DevMaterial> mats(std::max(1, d->material_count));
+    bool has_mix = false;
     for (int i = 0; i < d->material_count; i++) {
         const yar_material_desc& m = d->materials[i];
         DevMaterial& g = mats[i];
@@ -444,7 +445,33 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
             g.eta = make_float4(m.delta_reflectance[0], m.delta_reflectance[1], m.delta_reflectance[2], 0.f);
             g.k = make_float4(m.opacity[0], m.opacity[1], m.opacity[2], 0.f);
         }
-        if (m.type < YAR_MATERIAL_DIFFUSE || m.type > YAR_MATERIAL_PBRT3_UBER) throw HipError{"unsupported material type", YAR_ERR_UNSUPPORTED};
+        if (m.type == YAR_MATERIAL_MIX) {
+            // refl = White - amount | 1 - luminance(White - amount), trans = sub indices (bits) |
+            // min(luminance(amount), 1) | 1 when a sub-BSDF transmits (Mix_BSDF, bsdf.cpp:497-506)
+            float t = 0.f;
+            for (int j = 0; j < 2; j++) {
+                const int k = m.mix_materials[j];
+                if (k < 0 || k >= d->material_count) throw HipError{"mix sub-material index out of range", YAR_ERR_INVALID_ARGUMENT};
+                const yar_material_desc& b = d->materials[k];
+                const bool bsdf = b.type == YAR_MATERIAL_DIFFUSE || b.type == YAR_MATERIAL_METAL ||
+                                  b.type == YAR_MATERIAL_PLASTIC || b.type == YAR_MATERIAL_COATED_DIFFUSE ||
+                                  (b.type == YAR_MATERIAL_GLASS && b.u_roughness > 0.f);
+                if (!bsdf) throw HipError{"mix sub-materials must both have a BSDF (no delta layer, uber or mix)", YAR_ERR_UNSUPPORTED};
+                if (b.type == YAR_MATERIAL_GLASS) t = 1.f;
+            }
+            auto lum = [](float r, float g, float bl) { return 0.2126729f * r + 0.7151522f * g + 0.0721750f * bl; };
+            const float c0 = 1.f - m.mix_amount[0], c1 = 1.f - m.mix_amount[1], c2 = 1.f - m.mix_amount[2];
+            uint32_t i0 = (uint32_t)m.mix_materials[0], i1 = (uint32_t)m.mix_materials[1];
+            float f0, f1;
+            std::memcpy(&f0, &i0, 4);
+            std::memcpy(&f1, &i1, 4);
+            g.type = kMixType;
+            g.refl = make_float4(c0, c1, c2, 1.f - lum(c0, c1, c2));
+            g.trans = make_float4(f0, f1, std::min(lum(m.mix_amount[0], m.mix_amount[1], m.mix_amount[2]), 1.f), t);
+            g.eta = g.k = make_float4(0.f, 0.f, 0.f, 0.f);
+            has_mix = true;
+        }
+        if (m.type < YAR_MATERIAL_DIFFUSE || m.type > YAR_MATERIAL_MIX) throw HipError{"unsupported material type", YAR_ERR_UNSUPPORTED};
     }
     std::vector<DevRect> rl(std::max(1, d->rect_light_count));
     for (int i = 0; i < d->rect_light_count; i++) {
@@ -585,6 +612,7 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
     S.mesh_lights = ctx->mesh_lights.p;
     S.mesh_light_cdf = ctx->mesh_light_cdf.p;
     S.spheres = ctx->spheres.p;
+    S.has_mix = has_mix ? 1 : 0;
     S.env = ctx->env.p;
     S.n_env = d->environment_light_count;
     S.n_rects = d->rect_light_count;

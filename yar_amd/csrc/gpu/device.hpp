@@ -74,10 +74,11 @@ YG_INLINE V3 xform_vector(const M34& m, V3 v) {
 // mesh's root node index and triangle base packed into the w lanes.
 struct ObjTrav { float4 w2o0, w2o1, w2o2, lo_node, hi_tri; };
 
-// Device-only material codes: rough glass (yar_material_desc type GLASS with roughness > 0) and
-// the pbrt3 uber material.
+// Device-only material codes: rough glass (yar_material_desc type GLASS with roughness > 0), the
+// pbrt3 uber material and the mix material.
 constexpr int kRoughGlassType = 7;
 constexpr int kUberType = 8;
+constexpr int kMixType = 9;
 struct DevMaterial {      // 64 bytes
     int type; float eta_i, alpha_x, alpha_y;
     float4 refl;          // diffuse reflectance / glass Kr
@@ -126,6 +127,7 @@ struct DevScene {
     // area lights are numbered rectangles, spheres, meshes (an object's area-light index); the
     // environment light, when there is one, comes last in the light order
     int n_rects, n_points, n_spots, n_dirs, n_mesh_lights, n_spheres, n_env, n_lights;
+    int has_mix;                   // some material is a mix
 };
 
 // ---- ray/box slab test (Bounding_Box::intersect_by_ray_without_NaNs, bounding_box.h:63) ----

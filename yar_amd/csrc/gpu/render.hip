@@ -769,7 +769,7 @@ YG_INLINE void finalize(const Chunk& c, const Paths& P, uint2 pk, RGB L) {      
 
 // GUIDED selects the guided form's variant at compile time, so the plain kernel keeps its own
 // (smaller) register allocation.
-template <bool GUIDED>
+template <bool GUIDED, bool MIX>
 __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restrict__ cp, Paths P, Queues q, int parity) {
     const Chunk& c = *cp;
     const uint32_t* in = parity ? q.active1 : q.active0;
@@ -869,6 +869,12 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                 // is divided by the probability of its BSDF layers (path_tracing.cpp:68)
                 float u_s = u_scat, bsdf_layer_probability = 1.f;
                 int uber_delta = 0;                                     // 1 delta reflection, 2 pass-through
+                if (MIX && m.type == kMixType) {
+                    // get_mix_info (delta_scattering.cpp:154-189) with two BSDF sub-materials: no delta
+                    // event, u_scat rescaled to the chosen sub-material's interval
+                    const float mix_amount = m.trans.z;
+                    u_s = u_scat < mix_amount ? u_scat / mix_amount : (u_scat - mix_amount) / (1.f - mix_amount);
+                }
                 if (m.type == kUberType) {
                     const uint32_t code = fbits(m.trans.w);
                     const uint32_t ncomp = code & 15u;
@@ -929,7 +935,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                             const V3 ldir = vdiv(lv, dist);
                             const float ndl = dot(sc.n, ldir);
                             if (ndl > 0.f) {
-                                const RGB f = bsdf_eval(m, sc, sc.wo, ldir);
+                                const RGB f = bsdf_eval<MIX>(m, sc, sc.wo, ldir, s.materials);
                                 sh_c = (rgb4(pl.I) * f) * (ndl / (dist * dist));
                                 sh_o = position; sh_d = ldir; sh_tmax = dist * (1.f - 1e-5f);
                                 want_shadow = true;
@@ -951,8 +957,8 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                                     penumbra = (kp * kp) * (kp * kp);
                                 }
                                 const float ndw = dot(sc.n, wi);
-                                if (scattering_possible(m.type, ndw)) {
-                                    const RGB f = bsdf_eval(m, sc, sc.wo, wi);
+                                if (scattering_possible(m, ndw)) {
+                                    const RGB f = bsdf_eval<MIX>(m, sc, sc.wo, wi, s.materials);
                                     if (!is_black(f)) {
                                         sh_c = (rgb4(sl.I) * f) * (penumbra * fabsf(ndw) / (dist * dist));
                                         sh_o = position; sh_d = wi; sh_tmax = dist * (1.f - 1e-5f);
@@ -965,7 +971,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                             const V3 ldir = from4(dl.dir);
                             const float ndl = dot(sc.n, ldir);
                             if (ndl > 0.f) {
-                                const RGB f = bsdf_eval(m, sc, sc.wo, ldir);
+                                const RGB f = bsdf_eval<MIX>(m, sc, sc.wo, ldir, s.materials);
                                 sh_c = (rgb4(dl.E) * f) * ndl;
                                 sh_o = origin_dir(sc, ldir); sh_d = ldir; sh_tmax = f_inf();
                                 want_shadow = true;
@@ -985,11 +991,11 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                                 const float lndw = dot(ln, -wi);
                                 if (lndw > 1e-4f) {
                                     const float ndw = dot(sc.n, wi);
-                                    if (scattering_possible(m.type, ndw)) {
-                                        const RGB f = bsdf_eval(m, sc, sc.wo, wi);
+                                    if (scattering_possible(m, ndw)) {
+                                        const RGB f = bsdf_eval<MIX>(m, sc, sc.wo, wi, s.materials);
                                         if (!is_black(f)) {
                                             const float lpdf = (dd * dd) / (R.size.x * R.size.y * lndw);
-                                            const float bpdf = bsdf_pdf(m, sc, sc.wo, wi);
+                                            const float bpdf = bsdf_pdf<MIX>(m, sc, sc.wo, wi, s.materials);
                                             const float w = power_heuristic(lpdf, bpdf);
                                             sh_c = (rgb4(R.L) * f) * (w * fabsf(ndw) / lpdf);
                                             sh_o = position; sh_d = wi; sh_tmax = dd * (1.f - 1e-5f);
@@ -1001,7 +1007,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                             {
                                 V3 wi = v3(0.f, 0.f, 0.f);
                                 float bpdf = 0.f;
-                                const RGB f = bsdf_sample(m, sc, ubx, uby, u_s, sc.wo, wi, bpdf);
+                                const RGB f = bsdf_sample<MIX>(m, sc, ubx, uby, u_s, sc.wo, wi, bpdf, s.materials);
                                 if (!is_black(f)) {
                                     const float lndw = dot(ln, -wi);
                                     if (lndw > 1e-4f) {
@@ -1023,10 +1029,10 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                                 const float dd = length(lv);
                                 const V3 wi = vdiv(lv, dd);
                                 const float ndw = dot(sc.n, wi);
-                                if (scattering_possible(m.type, ndw)) {
-                                    const RGB f = bsdf_eval(m, sc, sc.wo, wi);
+                                if (scattering_possible(m, ndw)) {
+                                    const RGB f = bsdf_eval<MIX>(m, sc, sc.wo, wi, s.materials);
                                     if (!is_black(f)) {
-                                        const float bpdf = bsdf_pdf(m, sc, sc.wo, wi);
+                                        const float bpdf = bsdf_pdf<MIX>(m, sc, sc.wo, wi, s.materials);
                                         const float w = power_heuristic(cone.pdf, bpdf);
                                         sh_c = (rgb4(SL.L) * f) * (w * fabsf(ndw) / cone.pdf);
                                         sh_o = position; sh_d = wi; sh_tmax = dd * (1.f - 1e-5f);
@@ -1037,7 +1043,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                             {
                                 V3 wi = v3(0.f, 0.f, 0.f);
                                 float bpdf = 0.f;
-                                const RGB f = bsdf_sample(m, sc, ubx, uby, u_s, sc.wo, wi, bpdf);
+                                const RGB f = bsdf_sample<MIX>(m, sc, ubx, uby, u_s, sc.wo, wi, bpdf, s.materials);
                                 if (!is_black(f) && dot(-cone.a2, wi) >= cone.cos_max) {
                                     mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
                                     mi_ndw = dot(sc.n, wi); mi_lndw = cone.pdf; mi_light = s.n_rects + lj;
@@ -1069,10 +1075,10 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                                     const float dd = length(lv);
                                     const V3 wi = vdiv(lv, dd);
                                     const float ndw = dot(sc.n, wi);
-                                    if (scattering_possible(m.type, ndw)) {
-                                        const RGB f = bsdf_eval(m, sc, sc.wo, wi);
+                                    if (scattering_possible(m, ndw)) {
+                                        const RGB f = bsdf_eval<MIX>(m, sc, sc.wo, wi, s.materials);
                                         if (!is_black(f)) {
-                                            const float bpdf = bsdf_pdf(m, sc, sc.wo, wi);
+                                            const float bpdf = bsdf_pdf<MIX>(m, sc, sc.wo, wi, s.materials);
                                             const float w = power_heuristic(lpdf, bpdf);
                                             sh_c = (rgb4(ML.L) * f) * (w * fabsf(ndw) / lpdf);
                                             sh_o = position; sh_d = wi; sh_tmax = dd * (1.f - 1e-5f);
@@ -1084,7 +1090,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                             {
                                 V3 wi = v3(0.f, 0.f, 0.f);
                                 float bpdf = 0.f;
-                                const RGB f = bsdf_sample(m, sc, ubx, uby, u_s, sc.wo, wi, bpdf);
+                                const RGB f = bsdf_sample<MIX>(m, sc, ubx, uby, u_s, sc.wo, wi, bpdf, s.materials);
                                 if (!is_black(f)) {
                                     mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
                                     mi_ndw = dot(sc.n, wi); mi_lndw = 0.f; mi_light = s.n_rects + s.n_spheres + lj;
@@ -1102,10 +1108,10 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                                 const V3 wi = xform_vector(M34{E.l2w0, E.l2w1, E.l2w2},
                                                            v3(sin_theta * cosf(phi), sin_theta * sinf(phi), cosf(theta)));
                                 const float ndw = dot(sc.n, wi);
-                                if (scattering_possible(m.type, ndw)) {
-                                    const RGB f = bsdf_eval(m, sc, sc.wo, wi);
+                                if (scattering_possible(m, ndw)) {
+                                    const RGB f = bsdf_eval<MIX>(m, sc, sc.wo, wi, s.materials);
                                     if (!is_black(f)) {
-                                        const float bpdf = bsdf_pdf(m, sc, sc.wo, wi);
+                                        const float bpdf = bsdf_pdf<MIX>(m, sc, sc.wo, wi, s.materials);
                                         const float w = power_heuristic(lpdf, bpdf);
                                         sh_c = (Le * f) * (w * fabsf(ndw) / lpdf);
                                         sh_o = origin_dir(sc, wi); sh_d = wi; sh_tmax = f_inf();
@@ -1116,7 +1122,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                             {
                                 V3 wi = v3(0.f, 0.f, 0.f);
                                 float bpdf = 0.f;
-                                const RGB f = bsdf_sample(m, sc, ubx, uby, u_s, sc.wo, wi, bpdf);
+                                const RGB f = bsdf_sample<MIX>(m, sc, ubx, uby, u_s, sc.wo, wi, bpdf, s.materials);
                                 if (!is_black(f) && !is_black(Le)) {
                                     mi_o = origin_dir(sc, wi); mi_d = wi; mi_f = f; mi_pdf = bpdf;
                                     mi_ndw = dot(sc.n, wi); mi_lndw = env_pdf(s, wi);
@@ -1181,7 +1187,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                         } else {
                             V3 wi = v3(0.f, 0.f, 0.f);
                             float pdf = 0.f;
-                            const RGB f = bsdf_sample(m, sc, unx, uny, u_scat_next, sc.wo, wi, pdf);
+                            const RGB f = bsdf_sample<MIX>(m, sc, unx, uny, u_scat_next, sc.wo, wi, pdf, s.materials);
                             if (is_black(f)) {
                                 finished = true;
                             } else {
@@ -1206,7 +1212,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                     } else {
                         V3 wi = v3(0.f, 0.f, 0.f);
                         float pdf = 0.f;
-                        const RGB f = bsdf_sample(m, sc, unx, uny, u_scat_next, sc.wo, wi, pdf);
+                        const RGB f = bsdf_sample<MIX>(m, sc, unx, uny, u_scat_next, sc.wo, wi, pdf, s.materials);
                         if (is_black(f)) {
                             finished = true;
                         } else {
@@ -1319,11 +1325,11 @@ __global__ void k_guide_sample(DevScene s, const Chunk* __restrict__ cp, Paths P
                 wi = mixture_sample(lobes, u0, u1, u2);
                 const bool upper = dot(sc.n, wi) > 0.f && (m.type == 1 || to_local(sc, wi).z > 0.f);
                 if (upper) {
-                    f = bsdf_eval(m, sc, sc.wo, wi);
-                    pb = bsdf_pdf(m, sc, sc.wo, wi);
+                    f = bsdf_eval_base(m, sc, sc.wo, wi);
+                    pb = bsdf_pdf_base(m, sc, sc.wo, wi);
                 }
             } else {
-                f = bsdf_sample(m, sc, unx, uny, u_scat_next, sc.wo, wi, pb);
+                f = bsdf_sample_base(m, sc, unx, uny, u_scat_next, sc.wo, wi, pb);
                 if (!(dot(sc.n, wi) > 0.f)) f = rgb(0.f, 0.f, 0.f);
             }
             const float pm = is_black(f) ? 0.f : a * mixture_pdf(lobes, wi) + (1.f - a) * pb;
@@ -1512,8 +1518,11 @@ void launch_stage(int stage, const DevScene& sc, const Chunk* c, const Paths& p,
         break;
     case kStageExtend: YG_LAUNCH_TRAV(k_extend, cfg.trav_grid, sc, p, q, parity); break;
     case kStageShade:
-        if (cfg.guided) hipLaunchKernelGGL(k_shade<true>, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q, parity);
-        else hipLaunchKernelGGL(k_shade<false>, dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q, parity);
+        // scenes with mix materials take the variant whose BSDF code evaluates sub-materials
+        if (cfg.guided && sc.has_mix) hipLaunchKernelGGL((k_shade<true, true>), dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q, parity);
+        else if (cfg.guided) hipLaunchKernelGGL((k_shade<true, false>), dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q, parity);
+        else if (sc.has_mix) hipLaunchKernelGGL((k_shade<false, true>), dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q, parity);
+        else hipLaunchKernelGGL((k_shade<false, false>), dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q, parity);
         break;
     case kStageShadow: YG_LAUNCH_TRAV(k_shadow, cfg.trav_grid, sc, p, q); break;
     case kStageMis: YG_LAUNCH_TRAV(k_mis, cfg.trav_grid, sc, p, q); break;

@@ -12,8 +12,8 @@ namespace yg {
 
 // Whether a BSDF can scatter light arriving from direction wi: reflection above the shading
 // normal, transmission below it (direct_lighting.cpp:133-135).
-YG_INLINE bool scattering_possible(int type, float n_dot_wi) {
-    return n_dot_wi > 0.f || (n_dot_wi < 0.f && type == kRoughGlassType);
+YG_INLINE bool scattering_possible(const DevMaterial& m, float n_dot_wi) {
+    return n_dot_wi > 0.f || (n_dot_wi < 0.f && (m.type == kRoughGlassType || (m.type == kMixType && m.trans.w != 0.f)));
 }
 
 YG_INLINE float power_heuristic(float a, float b) {             // direct_lighting.cpp:18
@@ -384,7 +384,7 @@ YG_INLINE float uber_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) 
     return 0.5f * (diffuse_pdf + vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y));
 }
 
-YG_INLINE RGB bsdf_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
+YG_INLINE RGB bsdf_eval_base(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
     if (m.type == 1) return kPiInv * rgb4(m.refl);
     if (m.type == kRoughGlassType) return rough_glass_eval(m, sc, wo, wi);
     if (m.type == 5) return plastic_eval(m, sc, wo, wi);
@@ -398,7 +398,7 @@ YG_INLINE RGB bsdf_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
     float D = D_aniso(wh, m.alpha_x, m.alpha_y);
     return F * ((G * D) / (4.f * wol.z * wil.z));
 }
-YG_INLINE float bsdf_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
+YG_INLINE float bsdf_pdf_base(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi) {
     if (m.type == 1) return dot(sc.n, wi) / kPi;
     if (m.type == 5) return plastic_pdf(m, sc, wo, wi);
     if (m.type == 6) return coated_pdf(m, sc, wo, wi);
@@ -416,12 +416,12 @@ YG_INLINE V3 sample_cosine(float u0, float u1) {                                
 }
 
 // u_scat selects the lobe of layered BSDFs (plastic, coated diffuse: diffuse below 0.5, specular above).
-YG_INLINE RGB bsdf_sample(const DevMaterial& m, const Shading& sc, float u0, float u1, float u_scat, V3 wo, V3& wi,
+YG_INLINE RGB bsdf_sample_base(const DevMaterial& m, const Shading& sc, float u0, float u1, float u_scat, V3 wo, V3& wi,
                           float& pdf) {
     if (m.type == 1) {
         wi = to_world(sc, sample_cosine(u0, u1));
-        pdf = bsdf_pdf(m, sc, wo, wi);
-        return bsdf_eval(m, sc, wo, wi);
+        pdf = bsdf_pdf_base(m, sc, wo, wi);
+        return bsdf_eval_base(m, sc, wo, wi);
     }
     if (m.type == 5) {                                                                 // bsdf.cpp:235-251
         if (u_scat < 0.5f) {
@@ -473,7 +473,66 @@ YG_INLINE RGB bsdf_sample(const DevMaterial& m, const Shading& sc, float u0, flo
     if (wil.z <= 0.f) return rgb(0.f, 0.f, 0.f);
     wi = to_world(sc, wil);
     pdf = vndf_wi_pdf(wol, wh, m.alpha_x, m.alpha_y);
-    return bsdf_eval(m, sc, wo, wi);
+    return bsdf_eval_base(m, sc, wo, wi);
+}
+
+// Mix_BSDF (bsdf.cpp:497-570) over two sub-materials that both have a BSDF.  Record: refl = mix_coeff
+// (White - amount) | p_coeff = 1 - luminance(mix_coeff), trans = sub-material indices (bits) | the
+// clamped luminance of amount | whether a sub-BSDF transmits.  f = (White - mix_coeff) f1 + mix_coeff f2
+// and pdf = (1 - p) pdf1 + p pdf2, each sub-BSDF counted only where it can scatter towards wi; a sample
+// comes from the second BSDF with probability p.  With MIX false (no mix material in the scene) these
+// are the base functions; with MIX true one copy of the base code serves both sub-materials.
+template <bool MIX>
+YG_INLINE RGB bsdf_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi, const DevMaterial* mats) {
+    if (!MIX) return bsdf_eval_base(m, sc, wo, wi);
+    const bool mix = m.type == kMixType;
+    const float ndw = dot(sc.n, wi);
+    RGB f = rgb(0.f, 0.f, 0.f);
+#pragma unroll 1
+    for (int j = 0; j < (mix ? 2 : 1); j++) {
+        const DevMaterial b = mix ? mats[fbits(j ? m.trans.y : m.trans.x)] : m;
+        RGB fj = rgb(0.f, 0.f, 0.f);
+        if (!mix || scattering_possible(b, ndw)) fj = bsdf_eval_base(b, sc, wo, wi);
+        const RGB w = !mix ? rgb(1.f, 1.f, 1.f) : (j ? rgb4(m.refl) : rgb(1.f, 1.f, 1.f) - rgb4(m.refl));
+        f = f + w * fj;
+    }
+    return f;
+}
+template <bool MIX>
+YG_INLINE float bsdf_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi, const DevMaterial* mats) {
+    if (!MIX) return bsdf_pdf_base(m, sc, wo, wi);
+    const bool mix = m.type == kMixType;
+    const float ndw = dot(sc.n, wi);
+    float pdf = 0.f;
+#pragma unroll 1
+    for (int j = 0; j < (mix ? 2 : 1); j++) {
+        const DevMaterial b = mix ? mats[fbits(j ? m.trans.y : m.trans.x)] : m;
+        float pj = 0.f;
+        if (!mix || scattering_possible(b, ndw)) pj = bsdf_pdf_base(b, sc, wo, wi);
+        const float w = !mix ? 1.f : (j ? m.refl.w : 1.f - m.refl.w);
+        pdf = pdf + w * pj;
+    }
+    return pdf;
+}
+template <bool MIX>
+YG_INLINE RGB bsdf_sample(const DevMaterial& m, const Shading& sc, float u0, float u1, float u_scat, V3 wo, V3& wi,
+                          float& pdf, const DevMaterial* mats) {
+    if (!MIX || m.type != kMixType) return bsdf_sample_base(m, sc, u0, u1, u_scat, wo, wi, pdf);
+    const float p = m.refl.w;
+    const bool second = u_scat < p;
+    const float us = second ? u_scat / p : (u_scat - p) / (1.f - p);
+    float pdf_s = 0.f, pdf_o = 0.f;
+    const RGB f_s = bsdf_sample_base(mats[fbits(second ? m.trans.y : m.trans.x)], sc, u0, u1, us, wo, wi, pdf_s);
+    RGB f_o = rgb(0.f, 0.f, 0.f);
+    const DevMaterial bo = mats[fbits(second ? m.trans.x : m.trans.y)];
+    if (scattering_possible(bo, dot(sc.n, wi))) {
+        pdf_o = bsdf_pdf_base(bo, sc, wo, wi);
+        f_o = bsdf_eval_base(bo, sc, wo, wi);
+    }
+    const float pdf1 = second ? pdf_o : pdf_s, pdf2 = second ? pdf_s : pdf_o;
+    const RGB f1 = second ? f_o : f_s, f2 = second ? f_s : f_o;
+    pdf = (1.f - p) * pdf1 + p * pdf2;
+    return (rgb(1.f, 1.f, 1.f) - rgb4(m.refl)) * f1 + rgb4(m.refl) * f2;
 }
 
 }  // namespace yg

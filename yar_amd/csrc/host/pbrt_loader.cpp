@@ -459,6 +459,26 @@ struct Loader {
             auto black = [](const float* c) { return c[0] == 0.f && c[1] == 0.f && c[2] == 0.f; };
             if (black(kd) && black(ks) && black(kr) && opacity[0] == 1.f && opacity[1] == 1.f && opacity[2] == 1.f)
                 throw ParseError{"uber material with no scattering component"};
+        } else if (ms->type == "mix") {                                    // pbrt_scene.cpp:563-569
+            m.type = YAR_MATERIAL_MIX;                                         // Materials.cpp:376-402
+            m.mix_amount[0] = m.mix_amount[1] = m.mix_amount[2] = 0.5f;
+            if (find(ms->params, "amount") && find(ms->params, "amount")->type == "texture")
+                throw ParseError{"textured mix amount is not supported"};
+            if (find(ms->params, "amount") && find(ms->params, "amount")->fnums.size() != 3)
+                throw ParseError{"mix amount needs 3 components"};
+            get_rgb(ms->params, "amount", m.mix_amount);
+            for (int j = 0; j < 2; j++) {
+                const char* key = j == 0 ? "namedmaterial1" : "namedmaterial2";
+                const Param* pn = find(ms->params, key);
+                if (!pn || pn->strs.empty()) throw ParseError{std::string("mix material without '") + key + "'"};
+                auto it = named.find(pn->strs[0]);
+                if (it == named.end()) throw ParseError{"unknown named material '" + pn->strs[0] + "'"};
+                const std::string& t = it->second->type;
+                const bool delta_glass = t == "glass" && get_f(it->second->params, "uroughness", 0.f) == 0.f;
+                if (t == "mix" || t == "uber" || t == "mirror" || delta_glass)
+                    throw ParseError{"mix of a '" + t + "' sub-material (delta layer, uber or nested mix) is not supported"};
+                m.mix_materials[j] = import_material(it->second);
+            }
         } else {
             throw ParseError{"unsupported material type '" + ms->type + "'"};
         }
