@@ -813,12 +813,15 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         for (auto& pr : gr->passes) pass_max = std::max(pass_max, pr.second - pr.first);
 
     // chunks of whole tiles, bounded by paths in flight, sample-table and radiance sizes.
-    // Guided jobs revisit every chunk once per pass, so by default they keep up to 16M paths in one
-    // chunk to generate each sample table once.  Path state is about 280 B per path (plain: 14
-    // float4 rays / throughputs / direct-lighting halves / queue payloads, hit, state and 5 queue
-    // slots) and about 800 B guided (+ training vertices, network inputs and outputs): 16M guided
-    // paths hold about 13 GB.  The default never takes more than a quarter of the free device memory.
-    int64_t max_paths = ctx->opt.max_paths_in_flight > 0 ? ctx->opt.max_paths_in_flight : (int64_t)1 << (guided ? 24 : 22);
+    // Plain jobs default to 32M paths in flight: the persistent traversal kernels need wide queues in
+    // the late bounces, where most paths have ended (config 2, 1024x1024 at 64 spp: 262 Msamples/s at
+    // 4M paths, 329 at 8M, 390 at 16M, 416 at 32M, flat beyond, where the sample-table bound already
+    // splits the frame in two chunks).  Guided jobs revisit every chunk once per pass and keep up to
+    // 16M paths in one chunk.  Path state is about 280 B per path (plain: 14 float4 rays /
+    // throughputs / direct-lighting halves / queue payloads, hit, state and 5 queue slots; 32M paths
+    // hold about 9 GB) and about 800 B guided (+ training vertices, network inputs and outputs).  The
+    // default never takes more than a quarter of the free device memory.
+    int64_t max_paths = ctx->opt.max_paths_in_flight > 0 ? ctx->opt.max_paths_in_flight : (int64_t)1 << (guided ? 24 : 25);
     if (ctx->opt.max_paths_in_flight <= 0) {
         size_t free_b = 0, total_b = 0;
         const int64_t per_path = guided ? 800 : 280;
