@@ -813,15 +813,15 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         for (auto& pr : gr->passes) pass_max = std::max(pass_max, pr.second - pr.first);
 
     // chunks of whole tiles, bounded by paths in flight, sample-table and radiance sizes.
-    // Plain jobs default to 32M paths in flight: the persistent traversal kernels need wide queues in
+    // Plain jobs default to 64M paths in flight: the persistent traversal kernels need wide queues in
     // the late bounces, where most paths have ended (config 2, 1024x1024 at 64 spp: 262 Msamples/s at
-    // 4M paths, 329 at 8M, 390 at 16M, 416 at 32M, flat beyond, where the sample-table bound already
-    // splits the frame in two chunks).  Guided jobs revisit every chunk once per pass and keep up to
-    // 16M paths in one chunk.  Path state is about 280 B per path (plain: 14 float4 rays /
-    // throughputs / direct-lighting halves / queue payloads, hit, state and 5 queue slots; 32M paths
-    // hold about 9 GB) and about 800 B guided (+ training vertices, network inputs and outputs).  The
-    // default never takes more than a quarter of the free device memory.
-    int64_t max_paths = ctx->opt.max_paths_in_flight > 0 ? ctx->opt.max_paths_in_flight : (int64_t)1 << (guided ? 24 : 25);
+    // 4M paths, 329 at 8M, 390 at 16M, 416 at 32M in two chunks, 449 at 64M in one chunk).  Guided
+    // jobs revisit every chunk once per pass and keep up to 16M paths in one chunk.  Path state is
+    // about 280 B per path (plain: 14 float4 rays / throughputs / direct-lighting halves / queue
+    // payloads, hit, state and 5 queue slots; 64M paths hold about 19 GB) and about 800 B guided
+    // (+ training vertices, network inputs and outputs).  The default never takes more than a
+    // quarter of the free device memory.
+    int64_t max_paths = ctx->opt.max_paths_in_flight > 0 ? ctx->opt.max_paths_in_flight : (int64_t)1 << (guided ? 24 : 26);
     if (ctx->opt.max_paths_in_flight <= 0) {
         size_t free_b = 0, total_b = 0;
         const int64_t per_path = guided ? 800 : 280;
@@ -831,7 +831,8 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     const int perm_bytes = job_max_spp <= 256 ? 1 : 2;
     const int64_t draws = 2 * (int64_t)job_max_spp + d1 * (2 * (int64_t)job_max_spp - 1) + d2 * (3 * (int64_t)job_max_spp - 1);
     if (draws >= (int64_t)kJumpLo * kJumpHi) throw HipError{"samples per pixel too large for the sampler's jump tables", YAR_ERR_CAPACITY};
-    const int64_t max_samples = std::min<int64_t>(((int64_t)1 << 31) / std::max(1, n_dims * perm_bytes), (int64_t)1 << 27);
+    // sample table of a chunk at most 8 GB (config 2: 67M samples x 56 dimensions x 1 B = 3.8 GB)
+    const int64_t max_samples = std::min<int64_t>(((int64_t)1 << 33) / std::max(1, n_dims * perm_bytes), (int64_t)1 << 27);
     struct ChunkInfo { int64_t start, pixels, samples, paths; int max_spp, min_spp; };
     std::vector<ChunkInfo> cks;
     {
