@@ -375,6 +375,17 @@ __global__ void k_sample_perms(const Chunk* __restrict__ cp, int n_dims) {
         }
     }
     if (!STAGED) return;
+    if (c.uniform_spp > 0 && (c.uniform_spp & 3) == 0 && sizeof(T) == 1) {
+        // every run of the chunk has the same 4-aligned length: each lane stores its own run as
+        // packed 32-bit words, reading its own LDS column (one bank per lane, no conflicts)
+        if (g < total) {
+            const uint32_t* a = perm_lds + threadIdx.x;
+            uint32_t* o = reinterpret_cast<uint32_t*>(out + off);
+            for (int k = 0; k < spp; k += 4)
+                o[k >> 2] = a[k * nt] | (a[(k + 1) * nt] << 8) | (a[(k + 2) * nt] << 16) | (a[(k + 3) * nt] << 24);
+        }
+        return;
+    }
     // coalesced write-out of the block's contiguous range [run_off[0], end)
     run_off[threadIdx.x] = g < total ? off : -1;
     __syncthreads();
