@@ -70,8 +70,11 @@ typedef struct yar_mesh_desc {          /* Triangle_Mesh, src/lib/triangle_mesh.
 #define YAR_MATERIAL_PBRT3_UBER 7        /* Pbrt3_Uber_Material -> Pbrt3_Uber_BRDF (bsdf_pbrt.cpp:14-91) for
                                             Kd/Ks, delta reflection Kr and opacity pass-through chosen per
                                             component (delta_scattering.cpp:102-152); Kt must be black */
-#define YAR_MATERIAL_MIX     8           /* Mix_Material -> Mix_BSDF (bsdf.cpp:497-570) of two sub-materials that
-                                            both have a BSDF (matte, metal, plastic, substrate, rough glass) */
+#define YAR_MATERIAL_MIX     8           /* Mix_Material of two sub-materials (matte, metal, plastic, substrate,
+                                            glass, mirror; at most one of them a delta layer): Mix_BSDF
+                                            (bsdf.cpp:497-570), or with a delta layer that layer picked per
+                                            sample (delta_scattering.cpp:154-189) and the other one's BSDF
+                                            scaled (Scaled_BSDF, bsdf.cpp:572-629) */
 
 typedef struct yar_material_desc {
     int32_t type;

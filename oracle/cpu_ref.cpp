@@ -995,8 +995,16 @@ static float rough_glass_pdf(const Material& m, const Shading& sc, V3 wo, V3 wi)
     return wh_pdf * dwh_dwi * (1.f - rp);
 }
 
+// material_has_bsdf (material.h): the delta layers (perfect reflector, smooth glass) have none
+static bool has_bsdf(const Material& m) {
+    return !(m.type == YAR_MATERIAL_MIRROR || (m.type == YAR_MATERIAL_GLASS && m.roughness <= 0.f));
+}
 static bool has_transmission(const Scene& s, const Material& m) {                 // BSDF::transmission_scattering
-    if (m.type == YAR_MATERIAL_MIX) return has_transmission(s, s.materials[m.sub[0]]) || has_transmission(s, s.materials[m.sub[1]]);
+    if (m.type == YAR_MATERIAL_MIX) {
+        const Material& b1 = s.materials[m.sub[0]];
+        const Material& b2 = s.materials[m.sub[1]];
+        return (has_bsdf(b1) && has_transmission(s, b1)) || (has_bsdf(b2) && has_transmission(s, b2));
+    }
     return m.type == YAR_MATERIAL_GLASS;                                             // Rough_Glass_BSDF
 }
 static bool sub_scatters(const Scene& s, const Material& m, float n_dot_wi) {
@@ -1008,7 +1016,20 @@ static RGB bsdf_sample_m(const Scene& s, const Material& m, const Shading& sc, V
 // Mix_BSDF: mix_coeff = White - amount weights the second BSDF, p_coeff = 1 - luminance(mix_coeff) is
 // the probability of sampling it; each BSDF counts only where it can scatter towards wi
 static RGB mix_coeff(const Material& m) { return splat(1.f) - m.amount; }
+// create_mix_bsdf (bsdf.cpp:599-629): with a delta sub-material the BSDF is the other one scaled by
+// its weight, luminance(amount) for the first, 1 - luminance(amount) for the second (unclamped)
+static bool mix_scaled(const Scene& s, const Material& m, const Material** b, float* scale) {
+    const Material& b1 = s.materials[m.sub[0]];
+    const Material& b2 = s.materials[m.sub[1]];
+    if (has_bsdf(b1) && has_bsdf(b2)) return false;
+    const float mix_amount = luminance(m.amount);
+    if (!has_bsdf(b1)) { *b = &b2; *scale = 1.f - mix_amount; }
+    else { *b = &b1; *scale = mix_amount; }
+    return true;
+}
 static RGB mix_eval(const Scene& s, const Material& m, const Shading& sc, V3 wo, V3 wi) {
+    const Material* bs; float scale;
+    if (mix_scaled(s, m, &bs, &scale)) return scale * bsdf_eval_m(s, *bs, sc, wo, wi);   // Scaled_BSDF
     const Material& b1 = s.materials[m.sub[0]];
     const Material& b2 = s.materials[m.sub[1]];
     RGB f1, f2;
@@ -1019,6 +1040,8 @@ static RGB mix_eval(const Scene& s, const Material& m, const Shading& sc, V3 wo,
     return (splat(1.f) - mc) * f1 + mc * f2;
 }
 static float mix_pdf(const Scene& s, const Material& m, const Shading& sc, V3 wo, V3 wi) {
+    const Material* bs; float scale;
+    if (mix_scaled(s, m, &bs, &scale)) return bsdf_pdf_m(s, *bs, sc, wo, wi);
     const Material& b1 = s.materials[m.sub[0]];
     const Material& b2 = s.materials[m.sub[1]];
     const float p = 1.f - luminance(mix_coeff(m));
@@ -1029,6 +1052,8 @@ static float mix_pdf(const Scene& s, const Material& m, const Shading& sc, V3 wo
     return (1.f - p) * pdf1 + p * pdf2;
 }
 static RGB mix_sample(const Scene& s, const Material& m, const Shading& sc, V2 u, float u_scat, V3 wo, V3* wi, float* pdf) {
+    const Material* bs; float scale;
+    if (mix_scaled(s, m, &bs, &scale)) return scale * bsdf_sample_m(s, *bs, sc, u, u_scat, wo, wi, pdf);
     const Material& b1 = s.materials[m.sub[0]];
     const Material& b2 = s.materials[m.sub[1]];
     const RGB mc = mix_coeff(m);
@@ -1585,10 +1610,19 @@ static RGB trace_path(const Scene& s, Ray ray, PixelSampler& ps, PathCounters& p
         // rescaled u_scat on; the BSDF continuation is divided by the probability of the BSDF layers
         float bsdf_layer_probability = 1.f;
         RGB uber_att; V3 uber_dir;
+        const Material* mix_delta = nullptr;                                         // the delta sub-material picked
         if (m.type == YAR_MATERIAL_MIX) {                                            // get_mix_info, delta_scattering.cpp:154-189
             const float mix_amount = std::min(luminance(m.amount), 1.f);
-            if (u_scat < mix_amount) u_scat /= mix_amount;
+            const bool first = u_scat < mix_amount;
+            if (first) u_scat /= mix_amount;
             else u_scat = (u_scat - mix_amount) / (1.f - mix_amount);
+            const Material& b1 = s.materials[m.sub[0]];
+            const Material& b2 = s.materials[m.sub[1]];
+            if (!has_bsdf(b1) || !has_bsdf(b2)) {
+                const Material& picked = first ? b1 : b2;
+                if (!has_bsdf(picked)) { mix_delta = &picked; delta = true; }
+                bsdf_layer_probability = 1.f - (has_bsdf(b1) ? 1.f - mix_amount : mix_amount);
+            }
         }
         if (m.type == YAR_MATERIAL_PBRT3_UBER) {
             const float fp_index = u_scat * float(m.ncomp);
@@ -1610,8 +1644,7 @@ static RGB trace_path(const Scene& s, Ray ray, PixelSampler& ps, PathCounters& p
             for (int i = 0; i < m.ncomp; i++) if (m.comps[i] >= 2) delta_terms += 1.f;
             bsdf_layer_probability = 1.f - delta_terms / float(m.ncomp);
         }
-        const bool has_bsdf = !(m.type == YAR_MATERIAL_MIRROR || (m.type == YAR_MATERIAL_GLASS && m.roughness <= 0.f)) &&
-                              bsdf_layer_probability != 0.f;
+        const bool layer_bsdf = has_bsdf(m) && bsdf_layer_probability != 0.f;
         if (!delta) {
             RGB dl = direct_single(s, sc, u_sel, u_light, u_bsdf, u_scat, pc, tc);
             L = L + coeff * dl;
@@ -1624,19 +1657,20 @@ static RGB trace_path(const Scene& s, Ray ray, PixelSampler& ps, PathCounters& p
             ray.o = sc.origin_dir(wi);
             ray.d = wi;
         } else {
-            if (has_bsdf) {                                                          // path_tracing.cpp:76-79
+            if (layer_bsdf) {                                                        // path_tracing.cpp:76-79
                 RGB dl = direct_single(s, sc, u_sel, u_light, u_bsdf, u_scat, pc, tc);
                 L = L + coeff * dl;
             }
             RGB att; V3 dir;
-            if (m.type == YAR_MATERIAL_MIRROR) {                                     // delta_scattering.cpp:27-33
-                att = m.refl;
+            const Material& dm = mix_delta ? *mix_delta : m;
+            if (dm.type == YAR_MATERIAL_MIRROR) {                                    // delta_scattering.cpp:27-33
+                att = dm.refl;
                 dir = reflect(sc.wo, sc.n);
-            } else if (m.type == YAR_MATERIAL_PBRT3_UBER) {
+            } else if (dm.type == YAR_MATERIAL_PBRT3_UBER) {
                 att = uber_att;
                 dir = uber_dir;
             } else {
-                glass_scatter(sc, m, u_scat, &att, &dir);
+                glass_scatter(sc, dm, u_scat, &att, &dir);
             }
             Ray dray{sc.origin_dir(dir), dir};
             hit = Hit{Inf, -1, 0, {}};

@@ -49,6 +49,8 @@ class DescScene:
             mt[i].specular[:] = m.get("specular", [0, 0, 0])
             mt[i].delta_reflectance[:] = m.get("delta_reflectance", [0, 0, 0])
             mt[i].opacity[:] = m.get("opacity", [1, 1, 1])
+            mt[i].mix_materials[:] = m.get("mix_materials", [0, 0])
+            mt[i].mix_amount[:] = m.get("mix_amount", [0.5, 0.5, 0.5])
         ob = (_abi.ObjectDesc * len(objects))()
         for i, o in enumerate(objects):
             ob[i].mesh = o["mesh"]

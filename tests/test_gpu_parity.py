@@ -308,3 +308,26 @@ def test_render_mix_matches_oracle(c1_path, oracle, tmp_path):
     assert mean_rel <= 1e-3
     assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
     assert (film["weight_sum"] == ofilm["weight_sum"]).all()
+
+
+def test_render_mix_with_delta_layers_matches_oracle(c1_path, oracle, tmp_path):
+    # mix materials with one delta sub-material: mirror + matte on a wall, plastic + smooth glass on
+    # the triangles (the delta layer picked with its mix weight, the other BSDF scaled: get_mix_info,
+    # delta_scattering.cpp:154-189; create_mix_bsdf / Scaled_BSDF, bsdf.cpp:572-629)
+    lines = open(c1_path).read().splitlines()
+    mats = [i for i, l in enumerate(lines) if l.startswith('Material "matte"')]
+    named = ('MakeNamedMaterial "mi" "string type" [ "mirror" ] "rgb Kr" [ 0.9 0.85 0.8 ]\n'
+             'MakeNamedMaterial "ma" "string type" [ "matte" ] "rgb Kd" [ 0.5 0.3 0.2 ]\n'
+             'MakeNamedMaterial "gl" "string type" [ "glass" ] "float index" [ 1.5 ]\n'
+             'MakeNamedMaterial "pl" "string type" [ "plastic" ] "rgb Kd" [ 0.2 0.4 0.6 ] "float roughness" [ 0.1 ]\n')
+    lines[mats[2]] = 'Material "mix" "string namedmaterial1" [ "mi" ] "string namedmaterial2" [ "ma" ] "rgb amount" [ 0.6 0.6 0.6 ]'
+    lines[mats[-1]] = 'Material "mix" "string namedmaterial1" [ "pl" ] "string namedmaterial2" [ "gl" ] "rgb amount" [ 0.3 0.4 0.5 ]'
+    text = "\n".join(lines) + "\n"
+    p = tmp_path / "c1_mix_delta.pbrt"
+    p.write_text(text.replace("WorldBegin\n", "WorldBegin\n" + named, 1))
+    img, film, var, st, oimg, ofilm, ovar = _render_pair(str(p), oracle, nx=4)
+    exact, close, mean_rel = _compare_images(img, oimg, film, ofilm, "c1 mix with delta layers 16spp")
+    assert close >= 0.99
+    assert mean_rel <= 1e-3
+    assert abs(var - ovar) / max(ovar, 1e-12) <= 1e-2
+    assert (film["weight_sum"] == ofilm["weight_sum"]).all()

@@ -434,12 +434,24 @@ Shape "trianglemesh" "integer indices" [ 0 1 2 ] "point P" [ 0 0 1  1 0 1  0 1 1
     assert list(mixes[1].mix_materials) == [b, a] and list(mixes[1].mix_amount) == [0.5, 0.5, 0.5]
 
 
-@pytest.mark.parametrize("sub", ['"mirror"', '"glass"', '"uber"'])
-def test_mix_of_delta_or_uber_is_a_parse_error(tmp_path, sub):
-    body = LIGHT + f'''MakeNamedMaterial "a" "string type" [ "matte" ]
-MakeNamedMaterial "b" "string type" [ {sub} ]
+@pytest.mark.parametrize("subs", [('"matte"', '"uber"'), ('"mix"', '"matte"'), ('"mirror"', '"glass"')],
+                         ids=["uber", "nested-mix", "two-delta-layers"])
+def test_mix_unsupported_sub_materials_are_parse_errors(tmp_path, subs):
+    body = LIGHT + f'''MakeNamedMaterial "a" "string type" [ {subs[0]} ]
+MakeNamedMaterial "b" "string type" [ {subs[1]} ]
 Material "mix" "string namedmaterial1" [ "a" ] "string namedmaterial2" [ "b" ]
 Shape "trianglemesh" "integer indices" [ 0 1 2 ] "point P" [ 0 0 0  1 0 0  0 1 0 ]
 '''
     with pytest.raises(yar_amd.YarError):
         _load(tmp_path, body)
+
+
+def test_mix_with_one_delta_layer_loads(tmp_path):
+    body = LIGHT + '''MakeNamedMaterial "a" "string type" [ "mirror" ]
+MakeNamedMaterial "b" "string type" [ "matte" ]
+Material "mix" "string namedmaterial1" [ "a" ] "string namedmaterial2" [ "b" ]
+Shape "trianglemesh" "integer indices" [ 0 1 2 ] "point P" [ 0 0 0  1 0 0  0 1 0 ]
+'''
+    d = _load(tmp_path, body).desc
+    mix = [d.materials[i] for i in range(d.material_count) if d.materials[i].type == yar_amd._abi.YAR_MATERIAL_MIX][0]
+    assert d.materials[mix.mix_materials[0]].type == yar_amd._abi.YAR_MATERIAL_MIRROR

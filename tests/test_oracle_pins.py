@@ -1038,3 +1038,52 @@ def test_mix_is_linear_in_its_sub_materials(tmp_path):
         means.append(img.mean(axis=(0, 1)).astype(np.float64))
     want = a * means[0] + (1 - a) * means[1]
     assert np.allclose(means[2], want, rtol=5e-3), (means, want)
+
+
+def test_mix_with_a_delta_layer(tmp_path):
+    # get_mix_info / create_mix_bsdf (delta_scattering.cpp:154-189, bsdf.cpp:599-629): with a mirror as
+    # the first sub-material and matte as the second, the mirror is picked with probability
+    # m = luminance(amount) and its reflection is not reweighted, and the matte BSDF is scaled by
+    # 1 - luminance(amount) (the luminance, not per channel) and estimated in both branches.
+    # (1) Seen straight down and point-lit: (1 - lum(a)) Kd / pi * I per channel.
+    from tests.test_oracle_fixtures import _view_from_desc
+    a = np.array([0.2, 0.5, 0.9])
+    lum = 0.2126729 * a[0] + 0.7151522 * a[1] + 0.0721750 * a[2]
+    text = ('LookAt 0 0.5 0  0 0 0  0 0 1\nCamera "perspective" "float fov" [ 0.5 ]\n'
+            'Film "image" "integer xresolution" [ 8 ] "integer yresolution" [ 8 ]\n'
+            'Sampler "stratified" "integer xsamples" [ 16 ] "integer ysamples" [ 16 ]\n'
+            'Integrator "path" "integer maxdepth" [ 1 ]\nWorldBegin\n'
+            'LightSource "point" "rgb I" [ 2 2 2 ] "point from" [ 0 1 0 ]\n'
+            'MakeNamedMaterial "mi" "string type" [ "mirror" ] "rgb Kr" [ 0.9 0.9 0.9 ]\n'
+            'MakeNamedMaterial "ma" "string type" [ "matte" ] "rgb Kd" [ 0.8 0.4 0.2 ]\n'
+            'Material "mix" "string namedmaterial1" [ "mi" ] "string namedmaterial2" [ "ma" ] "rgb amount" [ 0.2 0.5 0.9 ]\n'
+            'Shape "trianglemesh" "integer indices" [ 0 1 2 0 2 3 ] "point P" [ -1e4 0 -1e4  -1e4 0 1e4  1e4 0 1e4  1e4 0 -1e4 ]\n'
+            'WorldEnd\n')
+    p = tmp_path / "mixd.pbrt"
+    p.write_text(text)
+    sc = yar_amd.Scene(str(p))
+    img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
+    want = (1 - lum) * np.array([0.8, 0.4, 0.2]) / np.pi * 2.0
+    assert np.allclose(img.mean(axis=(0, 1)), want, rtol=1e-3), (img.mean(axis=(0, 1)), want)
+    # (2) The same mix as the 45-degree mirror of the mirror closed form, with a black matte layer:
+    # only the picked mirror carries light, Kr * lum(a) * albedo/pi * I * cos / d^2
+    from tests.scenes import DescScene
+    from yar_amd import _abi
+    s = 0.3 / np.sqrt(2)
+    c = np.array([1.0, 1.0, 0.0])
+    e1, e2 = np.array([0, 0, 0.3]), np.array([s, -s, 0])
+    quad = {"vertices": [c - e1 - e2, c + e1 - e2, c + e1 + e2, c - e1 + e2], "indices": [0, 1, 2, 0, 2, 3]}
+    plane = {"vertices": [[-50, 0, -50], [-50, 0, 50], [50, 0, 50], [50, 0, -50]], "indices": [0, 1, 2, 0, 2, 3]}
+    sc = DescScene(
+        meshes=[plane, quad],
+        materials=[{"type": _abi.YAR_MATERIAL_DIFFUSE, "reflectance": [0.5] * 3},
+                   {"type": _abi.YAR_MATERIAL_MIRROR, "reflectance": [0.9, 0.6, 0.3]},
+                   {"type": _abi.YAR_MATERIAL_DIFFUSE, "reflectance": [0.0] * 3},
+                   {"type": _abi.YAR_MATERIAL_MIX, "mix_materials": [1, 2], "mix_amount": list(a)}],
+        objects=[{"mesh": 0, "material": 0}, {"mesh": 1, "material": 3}],
+        points=[{"position": [4.0, 2.0, 0.0], "intensity": [2.0] * 3}],
+        camera={"c2w": [0, 0, -1, 0, 0, 1, 0, 1, 1, 0, 0, 0], "fov": 1.0},
+        config={"film_width": 16, "film_height": 16, "region": (0, 0, 16, 16), "nx": 16, "ny": 16, "maxdepth": 2})
+    img, _, _, _, _ = O.OracleScene(sc).render(_view_from_desc(sc.desc))
+    want = np.array([0.9, 0.6, 0.3]) * lum * 0.5 / np.pi * 2.0 * 2.0 / 13.0 ** 1.5
+    assert np.allclose(img.mean(axis=(0, 1)), want, rtol=3e-3), (img.mean(axis=(0, 1)), want)

@@ -436,9 +436,10 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
                 code |= comps[j] << (4 + 4 * j);
                 if (comps[j] >= 2) ndelta++;
             }
-            code |= ndelta << 24;
-            float codef;
-            std::memcpy(&codef, &code, 4);
+            code |= ndelta << 20;
+            // stored as the integer's value (exact below 2^24), not its bit pattern: small integers'
+            // bit patterns are FP32 denormals, which instructions that flush denormals would zero
+            const float codef = (float)code;
             g.type = kUberType;
             g.refl = make_float4(m.reflectance[0], m.reflectance[1], m.reflectance[2], m.index_of_refraction);
             g.trans = make_float4(m.specular[0], m.specular[1], m.specular[2], codef);
@@ -446,9 +447,9 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
             g.k = make_float4(m.opacity[0], m.opacity[1], m.opacity[2], 0.f);
         }
         if (m.type == YAR_MATERIAL_MIX) {
-            // refl = White - amount | 1 - luminance(White - amount), trans = sub indices (bits) |
+            // refl = White - amount | 1 - luminance(White - amount), trans = sub indices (values) |
             // min(luminance(amount), 1) | 1 when a sub-BSDF transmits (Mix_BSDF, bsdf.cpp:497-506)
-            float t = 0.f;
+            float t = 0.f, delta_sub = -1.f;
             for (int j = 0; j < 2; j++) {
                 const int k = m.mix_materials[j];
                 if (k < 0 || k >= d->material_count) throw HipError{"mix sub-material index out of range", YAR_ERR_INVALID_ARGUMENT};
@@ -456,19 +457,25 @@ static void upload_scene(yar_gpu_ctx* ctx, const yar_scene_desc* d) {
                 const bool bsdf = b.type == YAR_MATERIAL_DIFFUSE || b.type == YAR_MATERIAL_METAL ||
                                   b.type == YAR_MATERIAL_PLASTIC || b.type == YAR_MATERIAL_COATED_DIFFUSE ||
                                   (b.type == YAR_MATERIAL_GLASS && b.u_roughness > 0.f);
-                if (!bsdf) throw HipError{"mix sub-materials must both have a BSDF (no delta layer, uber or mix)", YAR_ERR_UNSUPPORTED};
-                if (b.type == YAR_MATERIAL_GLASS) t = 1.f;
+                const bool delta = b.type == YAR_MATERIAL_MIRROR || (b.type == YAR_MATERIAL_GLASS && b.u_roughness <= 0.f);
+                if (!bsdf && !delta) throw HipError{"mix sub-materials must be BSDF or delta materials (no uber or mix)", YAR_ERR_UNSUPPORTED};
+                if (delta) {
+                    if (delta_sub >= 0.f) throw HipError{"mix of two delta sub-materials is not supported", YAR_ERR_UNSUPPORTED};
+                    delta_sub = (float)j;
+                }
+                if (bsdf && b.type == YAR_MATERIAL_GLASS) t = 1.f;
             }
             auto lum = [](float r, float g, float bl) { return 0.2126729f * r + 0.7151522f * g + 0.0721750f * bl; };
             const float c0 = 1.f - m.mix_amount[0], c1 = 1.f - m.mix_amount[1], c2 = 1.f - m.mix_amount[2];
-            uint32_t i0 = (uint32_t)m.mix_materials[0], i1 = (uint32_t)m.mix_materials[1];
-            float f0, f1;
-            std::memcpy(&f0, &i0, 4);
-            std::memcpy(&f1, &i1, 4);
+            const float f0 = (float)m.mix_materials[0], f1 = (float)m.mix_materials[1];   // exact values
             g.type = kMixType;
             g.refl = make_float4(c0, c1, c2, 1.f - lum(c0, c1, c2));
             g.trans = make_float4(f0, f1, std::min(lum(m.mix_amount[0], m.mix_amount[1], m.mix_amount[2]), 1.f), t);
-            g.eta = g.k = make_float4(0.f, 0.f, 0.f, 0.f);
+            // Scaled_BSDF factor of the BSDF sub-material next to a delta one (bsdf.cpp:617-624)
+            const float lum_amount = lum(m.mix_amount[0], m.mix_amount[1], m.mix_amount[2]);
+            const float scale = delta_sub == 0.f ? 1.f - lum_amount : lum_amount;
+            g.eta = make_float4(delta_sub, delta_sub >= 0.f ? scale : 0.f, 0.f, 0.f);
+            g.k = make_float4(0.f, 0.f, 0.f, 0.f);
             has_mix = true;
         }
         if (m.type < YAR_MATERIAL_DIFFUSE || m.type > YAR_MATERIAL_MIX) throw HipError{"unsupported material type", YAR_ERR_UNSUPPORTED};

@@ -880,21 +880,29 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                 // is divided by the probability of its BSDF layers (path_tracing.cpp:68)
                 float u_s = u_scat, bsdf_layer_probability = 1.f;
                 int uber_delta = 0;                                     // 1 delta reflection, 2 pass-through
+                bool mix_delta = false;
                 if (MIX && m.type == kMixType) {
-                    // get_mix_info (delta_scattering.cpp:154-189) with two BSDF sub-materials: no delta
-                    // event, u_scat rescaled to the chosen sub-material's interval
+                    // get_mix_info (delta_scattering.cpp:154-189): u_scat rescaled to the picked
+                    // sub-material's interval; picking a delta sub-material (mirror, smooth glass) is a
+                    // delta event, and the BSDF layer's probability is the other one's weight
                     const float mix_amount = m.trans.z;
-                    u_s = u_scat < mix_amount ? u_scat / mix_amount : (u_scat - mix_amount) / (1.f - mix_amount);
+                    const bool first = u_scat < mix_amount;
+                    u_s = first ? u_scat / mix_amount : (u_scat - mix_amount) / (1.f - mix_amount);
+                    if (m.eta.x >= 0.f) {
+                        const bool delta_first = m.eta.x == 0.f;
+                        mix_delta = first == delta_first;
+                        bsdf_layer_probability = 1.f - (delta_first ? mix_amount : 1.f - mix_amount);
+                    }
                 }
                 if (m.type == kUberType) {
-                    const uint32_t code = fbits(m.trans.w);
+                    const uint32_t code = (uint32_t)m.trans.w;
                     const uint32_t ncomp = code & 15u;
                     const float fp_index = u_scat * (float)ncomp;
                     const int ci = (int)fp_index;
                     const uint32_t comp = (code >> (4 + 4 * ci)) & 15u;
                     u_s = fminf(fmaxf(fp_index - (float)ci, 0.f), kOneMinusEps);
                     uber_delta = comp == 2u ? 1 : (comp == 4u ? 2 : 0);
-                    bsdf_layer_probability = 1.f - (float)((code >> 24) & 15u) / (float)ncomp;
+                    bsdf_layer_probability = 1.f - (float)((code >> 20) & 15u) / (float)ncomp;
                 }
                 if (m.type == 4) {
                     // perfect reflector (delta_scattering.cpp:27-33)
@@ -1149,7 +1157,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                     if (uber_delta != 0) {
                         // get_pbrt_uber_info (delta_scattering.cpp:102-152): Fresnel-weighted mirror
                         // reflection of Kr, or the opacity pass-through, each scaled by the component count
-                        const float ncomp = (float)(fbits(m.trans.w) & 15u);
+                        const float ncomp = (float)((uint32_t)m.trans.w & 15u);
                         RGB att;
                         V3 dir;
                         if (uber_delta == 1) {
@@ -1159,6 +1167,29 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, const Chunk* __restri
                         } else {
                             att = (rgb(1.f, 1.f, 1.f) - rgb4(m.k)) * ncomp;
                             dir = -sc.wo;
+                        }
+                        P.ray_o[slot] = f4(origin_dir(sc, dir), 0.f);
+                        P.ray_d[slot] = f4(dir, 0.f);
+                        coeff = coeff * att;
+                        flags |= kDelta;
+                        want_active = true;
+                    } else if (MIX && mix_delta) {
+                        // the mix's delta sub-material with the rescaled u_scat: perfect reflector
+                        // (delta_scattering.cpp:27-33) or smooth glass (get_glass_info, :53-100)
+                        const DevMaterial dm = s.materials[(int)(m.eta.x == 0.f ? m.trans.x : m.trans.y)];
+                        V3 dir = reflect(sc.wo, sc.n);
+                        RGB att = rgb4(dm.refl);
+                        if (dm.type == 3) {
+                            const float ior = dm.trans.w;
+                            const float etaT_over_etaI = !sc.flipped ? ior : 1.f / ior;
+                            const float cos_i = dot(sc.n, sc.wo);
+                            if (!(u_s < dielectric_fresnel(cos_i, etaT_over_etaI))) {
+                                const float eta = 1.f / etaT_over_etaI;
+                                att = rgb4(dm.trans) * (eta * eta);
+                                const float st2 = eta * eta * fmaxf(0.f, 1.f - cos_i * cos_i);
+                                const float ct = sqrtf(1.f - st2);
+                                dir = (-eta) * sc.wo + (eta * cos_i - ct) * sc.n;
+                            }
                         }
                         P.ray_o[slot] = f4(origin_dir(sc, dir), 0.f);
                         P.ray_d[slot] = f4(dir, 0.f);

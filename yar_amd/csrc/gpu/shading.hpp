@@ -477,23 +477,30 @@ YG_INLINE RGB bsdf_sample_base(const DevMaterial& m, const Shading& sc, float u0
 }
 
 // Mix_BSDF (bsdf.cpp:497-570) over two sub-materials that both have a BSDF.  Record: refl = mix_coeff
-// (White - amount) | p_coeff = 1 - luminance(mix_coeff), trans = sub-material indices (bits) | the
-// clamped luminance of amount | whether a sub-BSDF transmits.  f = (White - mix_coeff) f1 + mix_coeff f2
-// and pdf = (1 - p) pdf1 + p pdf2, each sub-BSDF counted only where it can scatter towards wi; a sample
-// comes from the second BSDF with probability p.  With MIX false (no mix material in the scene) these
-// are the base functions; with MIX true one copy of the base code serves both sub-materials.
+// (White - amount) | p_coeff = 1 - luminance(mix_coeff), trans = sub-material indices (values) | the
+// clamped luminance of amount | whether a BSDF sub-material transmits, eta.x = index of a delta
+// sub-material (-1 when none) | eta.y = the Scaled_BSDF factor.  f = (White - mix_coeff) f1 +
+// mix_coeff f2 and pdf = (1 - p) pdf1 + p pdf2, each sub-BSDF counted only where it can scatter
+// towards wi; a sample comes from the second BSDF with probability p.  With a delta sub-material the
+// BSDF is the other sub-material's scaled by eta.y, its pdf unscaled (Scaled_BSDF, bsdf.cpp:572-629).
+// With MIX false (no mix material in the scene) these are the base functions; with MIX true one
+// copy of the base code serves every sub-material.
 template <bool MIX>
 YG_INLINE RGB bsdf_eval(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi, const DevMaterial* mats) {
     if (!MIX) return bsdf_eval_base(m, sc, wo, wi);
     const bool mix = m.type == kMixType;
+    const bool scaled = mix && m.eta.x >= 0.f;
     const float ndw = dot(sc.n, wi);
     RGB f = rgb(0.f, 0.f, 0.f);
 #pragma unroll 1
-    for (int j = 0; j < (mix ? 2 : 1); j++) {
-        const DevMaterial b = mix ? mats[fbits(j ? m.trans.y : m.trans.x)] : m;
+    for (int j = 0; j < (mix && !scaled ? 2 : 1); j++) {
+        const int sub = scaled ? (m.eta.x == 0.f ? 1 : 0) : j;
+        const DevMaterial b = mix ? mats[(int)(sub ? m.trans.y : m.trans.x)] : m;
         RGB fj = rgb(0.f, 0.f, 0.f);
-        if (!mix || scattering_possible(b, ndw)) fj = bsdf_eval_base(b, sc, wo, wi);
-        const RGB w = !mix ? rgb(1.f, 1.f, 1.f) : (j ? rgb4(m.refl) : rgb(1.f, 1.f, 1.f) - rgb4(m.refl));
+        if (!mix || scaled || scattering_possible(b, ndw)) fj = bsdf_eval_base(b, sc, wo, wi);
+        const RGB w = !mix ? rgb(1.f, 1.f, 1.f)
+                           : scaled ? rgb(m.eta.y, m.eta.y, m.eta.y)
+                                    : (j ? rgb4(m.refl) : rgb(1.f, 1.f, 1.f) - rgb4(m.refl));
         f = f + w * fj;
     }
     return f;
@@ -502,14 +509,16 @@ template <bool MIX>
 YG_INLINE float bsdf_pdf(const DevMaterial& m, const Shading& sc, V3 wo, V3 wi, const DevMaterial* mats) {
     if (!MIX) return bsdf_pdf_base(m, sc, wo, wi);
     const bool mix = m.type == kMixType;
+    const bool scaled = mix && m.eta.x >= 0.f;
     const float ndw = dot(sc.n, wi);
     float pdf = 0.f;
 #pragma unroll 1
-    for (int j = 0; j < (mix ? 2 : 1); j++) {
-        const DevMaterial b = mix ? mats[fbits(j ? m.trans.y : m.trans.x)] : m;
+    for (int j = 0; j < (mix && !scaled ? 2 : 1); j++) {
+        const int sub = scaled ? (m.eta.x == 0.f ? 1 : 0) : j;
+        const DevMaterial b = mix ? mats[(int)(sub ? m.trans.y : m.trans.x)] : m;
         float pj = 0.f;
-        if (!mix || scattering_possible(b, ndw)) pj = bsdf_pdf_base(b, sc, wo, wi);
-        const float w = !mix ? 1.f : (j ? m.refl.w : 1.f - m.refl.w);
+        if (!mix || scaled || scattering_possible(b, ndw)) pj = bsdf_pdf_base(b, sc, wo, wi);
+        const float w = !mix || scaled ? 1.f : (j ? m.refl.w : 1.f - m.refl.w);
         pdf = pdf + w * pj;
     }
     return pdf;
@@ -518,13 +527,17 @@ template <bool MIX>
 YG_INLINE RGB bsdf_sample(const DevMaterial& m, const Shading& sc, float u0, float u1, float u_scat, V3 wo, V3& wi,
                           float& pdf, const DevMaterial* mats) {
     if (!MIX || m.type != kMixType) return bsdf_sample_base(m, sc, u0, u1, u_scat, wo, wi, pdf);
+    if (m.eta.x >= 0.f) {
+        const RGB f = bsdf_sample_base(mats[(int)(m.eta.x == 0.f ? m.trans.y : m.trans.x)], sc, u0, u1, u_scat, wo, wi, pdf);
+        return rgb(m.eta.y, m.eta.y, m.eta.y) * f;
+    }
     const float p = m.refl.w;
     const bool second = u_scat < p;
     const float us = second ? u_scat / p : (u_scat - p) / (1.f - p);
     float pdf_s = 0.f, pdf_o = 0.f;
-    const RGB f_s = bsdf_sample_base(mats[fbits(second ? m.trans.y : m.trans.x)], sc, u0, u1, us, wo, wi, pdf_s);
+    const RGB f_s = bsdf_sample_base(mats[(int)(second ? m.trans.y : m.trans.x)], sc, u0, u1, us, wo, wi, pdf_s);
     RGB f_o = rgb(0.f, 0.f, 0.f);
-    const DevMaterial bo = mats[fbits(second ? m.trans.x : m.trans.y)];
+    const DevMaterial bo = mats[(int)(second ? m.trans.x : m.trans.y)];
     if (scattering_possible(bo, dot(sc.n, wi))) {
         pdf_o = bsdf_pdf_base(bo, sc, wo, wi);
         f_o = bsdf_eval_base(bo, sc, wo, wi);

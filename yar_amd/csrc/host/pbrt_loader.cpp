@@ -467,6 +467,7 @@ struct Loader {
             if (find(ms->params, "amount") && find(ms->params, "amount")->fnums.size() != 3)
                 throw ParseError{"mix amount needs 3 components"};
             get_rgb(ms->params, "amount", m.mix_amount);
+            int delta_subs = 0;
             for (int j = 0; j < 2; j++) {
                 const char* key = j == 0 ? "namedmaterial1" : "namedmaterial2";
                 const Param* pn = find(ms->params, key);
@@ -474,11 +475,13 @@ struct Loader {
                 auto it = named.find(pn->strs[0]);
                 if (it == named.end()) throw ParseError{"unknown named material '" + pn->strs[0] + "'"};
                 const std::string& t = it->second->type;
-                const bool delta_glass = t == "glass" && get_f(it->second->params, "uroughness", 0.f) == 0.f;
-                if (t == "mix" || t == "uber" || t == "mirror" || delta_glass)
-                    throw ParseError{"mix of a '" + t + "' sub-material (delta layer, uber or nested mix) is not supported"};
+                if (t == "mix" || t == "uber")
+                    throw ParseError{"mix of a '" + t + "' sub-material (uber or nested mix) is not supported"};
+                if (t == "mirror" || (t == "glass" && get_f(it->second->params, "uroughness", 0.f) == 0.f)) delta_subs++;
                 m.mix_materials[j] = import_material(it->second);
             }
+            if (delta_subs == 2)                                               // bsdf.cpp:615, delta_scattering.cpp:164
+                throw ParseError{"mix of two delta sub-materials is not supported"};
         } else {
             throw ParseError{"unsupported material type '" + ms->type + "'"};
         }
