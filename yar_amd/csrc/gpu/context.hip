@@ -54,6 +54,24 @@ struct DevBuf {
 
 static int pow2_at_least(int v) { int p = 1; while (p < v) p <<= 1; return p; }
 
+// body(i) for i in [0, n) on up to 16 host threads, contiguous ranges per thread
+template <class Body>
+static void parallel_for(size_t n, Body&& body) {
+    const size_t nt = std::min<size_t>({n, (size_t)16, (size_t)std::max(1u, std::thread::hardware_concurrency())});
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; i++) body(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    auto range = [&](size_t t) {
+        const size_t a = n * t / nt, b = n * (t + 1) / nt;
+        for (size_t i = a; i < b; i++) body(i);
+    };
+    for (size_t t = 1; t < nt; t++) pool.emplace_back(range, t);
+    range(0);
+    for (auto& th : pool) th.join();
+}
+
 // Per-kernel HIP-event timing (kind: 0 extend, 1 shade, 2 shadow, 3 mis, 4 other, 5 guide network
 // and sampling, 6 guide training).
 constexpr int kTimerKinds = 7;
@@ -185,6 +203,9 @@ struct yar_gpu_ctx {
     DevBuf<double> m1, m2;
     uint32_t* h_poll = nullptr;           // pinned: active counts read back while the loop runs
     size_t h_poll_n = 0;
+    float4* h_film = nullptr;             // pinned: per-pixel film sums and variances read back per job
+    double* h_var = nullptr;
+    size_t h_film_n = 0;
     // bounce-loop graph (captured per job shape)
     hipGraphExec_t graph = nullptr;
     long long graph_paths = -1;
@@ -227,6 +248,8 @@ struct yar_gpu_ctx {
         grng.release(); rec.release(); guide_in.release(); guide_out.release(); guide_path.release(); train.release();
         m1.release(); m2.release();
         if (h_poll) (void)hipHostFree(h_poll);
+        if (h_film) (void)hipHostFree(h_film);
+        if (h_var) (void)hipHostFree(h_var);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -1155,11 +1178,21 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         }
     }
     HIPX(hipEventRecord(e1, s));
-    std::vector<float4> film_own(total_pixels);
-    std::vector<double> var_own(total_pixels);
+    if ((size_t)total_pixels > ctx->h_film_n) {         // pinned, so the read-back runs at link speed
+        if (ctx->h_film) HIPX(hipHostFree(ctx->h_film));
+        if (ctx->h_var) HIPX(hipHostFree(ctx->h_var));
+        ctx->h_film = nullptr;
+        ctx->h_var = nullptr;
+        ctx->h_film_n = 0;
+        HIPX(hipHostMalloc((void**)&ctx->h_film, (size_t)total_pixels * sizeof(float4), hipHostMallocDefault));
+        HIPX(hipHostMalloc((void**)&ctx->h_var, (size_t)total_pixels * sizeof(double), hipHostMallocDefault));
+        ctx->h_film_n = (size_t)total_pixels;
+    }
+    const float4* film_own = ctx->h_film;
+    const double* var_own = ctx->h_var;
     if (total_pixels > 0) {
-        HIPX(hipMemcpyAsync(film_own.data(), ctx->out_film.p, total_pixels * sizeof(float4), hipMemcpyDeviceToHost, s));
-        HIPX(hipMemcpyAsync(var_own.data(), ctx->out_var.p, total_pixels * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPX(hipMemcpyAsync(ctx->h_film, ctx->out_film.p, total_pixels * sizeof(float4), hipMemcpyDeviceToHost, s));
+        HIPX(hipMemcpyAsync(ctx->h_var, ctx->out_var.p, total_pixels * sizeof(double), hipMemcpyDeviceToHost, s));
     }
     unsigned long long tot[24];
     uint32_t cnt[16];
@@ -1196,13 +1229,15 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         const int tx = (side_info[i].x - v.sx0) >> 6, ty = (side_info[i].y - v.sy0) >> 6;
         side_of_tile[(size_t)v.tile_base + (size_t)ty * v.tgx + tx].push_back(i);
     }
-    for (auto& l : side_of_tile)
+    parallel_for(side_of_tile.size(), [&](size_t t) {
+        std::vector<size_t>& l = side_of_tile[t];
         std::sort(l.begin(), l.end(), [&](size_t a, size_t b) {
             const int4 &A = side_info[a], &B = side_info[b];
             if (A.y != B.y) return A.y < B.y;
             if (A.x != B.x) return A.x < B.x;
             return A.z < B.z;
         });
+    });
 
     // Assemble Film_Tiles: in-tile values from the device gather, margins from the side list.
     out.assign(parts.size(), {});
@@ -1211,11 +1246,17 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         out[pi].tile_var.assign(parts[pi].tiles.size(), 0.0);
     }
     {
-        std::vector<size_t> next(parts.size(), 0);
-        for (const Unit& u : units) {
+        // every unit (tile) fills its own Film_Tile: independent, so the units run on all host cores
+        std::vector<size_t> unit_ti(units.size());
+        {
+            std::vector<size_t> next(parts.size(), 0);
+            for (size_t ui = 0; ui < units.size(); ui++) unit_ti[ui] = next[units[ui].part]++;
+        }
+        parallel_for(units.size(), [&](size_t ui) {
+            const Unit& u = units[ui];
             const yar_view& v = ctx->views[parts[u.part].view];
             const TileGeom g = tile_geom(v, u.tile);
-            const size_t ti = next[u.part]++;
+            const size_t ti = unit_ti[ui];
             const int pw = g.px1 - g.px0, ph = g.py1 - g.py0;
             std::vector<yar_film_pixel>& tp = out[u.part].tiles[ti];
             tp.assign((size_t)std::max(0, pw) * std::max(0, ph), yar_film_pixel{{0, 0, 0}, 0});
@@ -1243,7 +1284,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
                         fp.weight_sum += 1.f;
                     }
             }
-        }
+        });
     }
     int64_t samples = 0;
     for (const Unit& u : units) samples += u.count * dv[u.part].spp;
@@ -1283,17 +1324,24 @@ static void merge_view(const yar_view& v, const PartResult& r, yar_film_pixel* o
     const int n = (int)r.tiles.size();
     const int rw = v.region_x1 - v.region_x0, rh = v.region_y1 - v.region_y0;
     std::vector<yar_film_pixel> film((size_t)rw * rh, yar_film_pixel{{0, 0, 0}, 0});
-    for (int t = 0; t < n; t++) {
-        const TileGeom g = tile_geom(v, t);
-        const int pw = g.px1 - g.px0;
-        for (int y = g.py0; y < g.py1; y++)
-            for (int x = g.px0; x < g.px1; x++) {
-                yar_film_pixel& fo = film[(size_t)(y - v.region_y0) * rw + (x - v.region_x0)];
-                const yar_film_pixel& p = r.tiles[t][(size_t)(y - g.py0) * pw + (x - g.px0)];
-                fo.color_sum[0] += p.color_sum[0]; fo.color_sum[1] += p.color_sum[1]; fo.color_sum[2] += p.color_sum[2];
-                fo.weight_sum += p.weight_sum;
-            }
-    }
+    std::vector<TileGeom> geo((size_t)n);
+    for (int t = 0; t < n; t++) geo[(size_t)t] = tile_geom(v, t);
+    // bands of 16 rows in parallel; within a band every pixel still takes the tiles in index order
+    const int bands = (rh + 15) / 16;
+    parallel_for((size_t)bands, [&](size_t b) {
+        const int ry0 = v.region_y0 + (int)b * 16, ry1 = std::min(v.region_y1, ry0 + 16);
+        for (int t = 0; t < n; t++) {
+            const TileGeom& g = geo[(size_t)t];
+            const int pw = g.px1 - g.px0;
+            for (int y = std::max(g.py0, ry0); y < std::min(g.py1, ry1); y++)
+                for (int x = g.px0; x < g.px1; x++) {
+                    yar_film_pixel& fo = film[(size_t)(y - v.region_y0) * rw + (x - v.region_x0)];
+                    const yar_film_pixel& p = r.tiles[t][(size_t)(y - g.py0) * pw + (x - g.px0)];
+                    fo.color_sum[0] += p.color_sum[0]; fo.color_sum[1] += p.color_sum[1]; fo.color_sum[2] += p.color_sum[2];
+                    fo.weight_sum += p.weight_sum;
+                }
+        }
+    });
     if (out_film) std::memcpy(out_film, film.data(), film.size() * sizeof(yar_film_pixel));
     if (out_image) {
         for (size_t i = 0; i < film.size(); i++) {
