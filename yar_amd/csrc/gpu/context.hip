@@ -206,6 +206,8 @@ struct yar_gpu_ctx {
     float4* h_film = nullptr;             // pinned: per-pixel film sums and variances read back per job
     double* h_var = nullptr;
     size_t h_film_n = 0;
+    char* h_stage = nullptr;              // pinned: pixel list, pixel views and spp prefix sums sent per job
+    size_t h_stage_n = 0;
     // bounce-loop graph (captured per job shape)
     hipGraphExec_t graph = nullptr;
     long long graph_paths = -1;
@@ -250,6 +252,7 @@ struct yar_gpu_ctx {
         if (h_poll) (void)hipHostFree(h_poll);
         if (h_film) (void)hipHostFree(h_film);
         if (h_var) (void)hipHostFree(h_var);
+        if (h_stage) (void)hipHostFree(h_stage);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -808,8 +811,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
     // views, tile offsets and the pixel list
     std::vector<DevView> dv;
     std::vector<int> tile_first;
-    std::vector<int2> plist;
-    std::vector<int> pview;
+    int64_t total_pixels = 0;
     struct Unit { int part, tile; int64_t start, count; };
     std::vector<Unit> units;
     int job_max_spp = 1;
@@ -825,18 +827,14 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         tile_first.resize(tile_first.size() + (size_t)tgx * tgy, -1);
         for (int t : parts[pi].tiles) {
             const TileGeom g = tile_geom(v, t);
-            if (plist.size() + (size_t)(g.x1 - g.x0) * (g.y1 - g.y0) > (size_t)0x7fffffff)
+            const int64_t count = (int64_t)(g.x1 - g.x0) * (g.y1 - g.y0);
+            if (total_pixels + count > (int64_t)0x7fffffff)
                 throw HipError{"more than 2^31 pixels in one job", YAR_ERR_CAPACITY};
-            tile_first[base + t] = (int)plist.size();
-            units.push_back({(int)pi, t, (int64_t)plist.size(), (int64_t)(g.x1 - g.x0) * (g.y1 - g.y0)});
-            for (int y = g.y0; y < g.y1; y++)
-                for (int x = g.x0; x < g.x1; x++) {
-                    plist.push_back(make_int2(x, y));
-                    pview.push_back((int)pi);
-                }
+            tile_first[base + t] = (int)total_pixels;
+            units.push_back({(int)pi, t, total_pixels, count});
+            total_pixels += count;
         }
     }
-    const int64_t total_pixels = (int64_t)plist.size();
     if (guided) gr->passes = guided_passes(job_max_spp, guide_config(gr->guide).first_pass_spp, guide_config(gr->guide).pass_count);
     int pass_max = 1;                                                          // widest guided pass
     if (guided)
@@ -883,18 +881,13 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         }
         if (cur.pixels > 0) cks.push_back(cur);
     }
-    // chunk-local spp prefix sums, one (n_pixels + 1)-array per chunk
-    std::vector<long long> pf;
+    // chunk-local spp prefix sums, one (n_pixels + 1)-array per chunk: chunk ci's array starts at
+    // entry ck.start + ci
     std::vector<size_t> pf_off;
     int64_t max_chunk_pixels = 1, max_chunk_samples = 1, max_chunk_paths = 1;
-    for (const ChunkInfo& ck : cks) {
-        pf_off.push_back(pf.size());
-        long long acc = 0;
-        for (int64_t i = 0; i < ck.pixels; i++) {
-            pf.push_back(acc);
-            acc += dv[pview[ck.start + i]].spp;
-        }
-        pf.push_back(acc);
+    for (size_t ci = 0; ci < cks.size(); ci++) {
+        const ChunkInfo& ck = cks[ci];
+        pf_off.push_back((size_t)ck.start + ci);
         max_chunk_pixels = std::max(max_chunk_pixels, ck.pixels);
         max_chunk_samples = std::max(max_chunk_samples, ck.samples);
         const int64_t paths = guided ? ck.pixels * pass_max : ck.paths;
@@ -946,9 +939,62 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         ctx->m2.reserve((size_t)std::max<int64_t>(1, total_pixels));
         guide_prepare_device();
     }
-    ctx->pixels.upload(plist, s);
-    ctx->pix_view.upload(pview, s);
-    ctx->pix_first.upload(pf, s);
+    {
+        // pixel list, pixel views and prefix sums written in place into pinned staging by all host
+        // cores, one tile per task, then sent at link speed
+        const size_t n = (size_t)total_pixels, n_pf = n + cks.size();
+        const size_t o_view = (n * sizeof(int2) + 255) & ~(size_t)255;
+        const size_t o_pf = (o_view + n * sizeof(int) + 255) & ~(size_t)255;
+        const size_t bytes = o_pf + n_pf * sizeof(long long);
+        HIPX(hipStreamSynchronize(s));                    // the previous job's sends have left the staging
+        if (bytes > ctx->h_stage_n) {
+            if (ctx->h_stage) HIPX(hipHostFree(ctx->h_stage));
+            ctx->h_stage = nullptr;
+            ctx->h_stage_n = 0;
+            HIPX(hipHostMalloc((void**)&ctx->h_stage, bytes, hipHostMallocDefault));
+            ctx->h_stage_n = bytes;
+        }
+        int2* hpix = (int2*)ctx->h_stage;
+        int* hview = (int*)(ctx->h_stage + o_view);
+        long long* hpf = (long long*)(ctx->h_stage + o_pf);
+        // each unit's chunk and the prefix sum at its first pixel
+        std::vector<size_t> u_chunk(units.size());
+        std::vector<long long> u_acc(units.size());
+        {
+            size_t ci = 0;
+            long long acc = 0;
+            for (size_t ui = 0; ui < units.size(); ui++) {
+                while (ci + 1 < cks.size() && units[ui].start >= cks[ci + 1].start) { ci++; acc = 0; }
+                u_chunk[ui] = ci;
+                u_acc[ui] = acc;
+                acc += (long long)units[ui].count * dv[units[ui].part].spp;
+            }
+        }
+        parallel_for(units.size(), [&](size_t ui) {
+            const Unit& u = units[ui];
+            const TileGeom g = tile_geom(ctx->views[parts[u.part].view], u.tile);
+            const ChunkInfo& ck = cks[u_chunk[ui]];
+            const long long spp = dv[u.part].spp;
+            int64_t off = u.start;
+            long long* pfc = hpf + pf_off[u_chunk[ui]] - ck.start;
+            long long acc = u_acc[ui];
+            for (int y = g.y0; y < g.y1; y++)
+                for (int x = g.x0; x < g.x1; x++, off++, acc += spp) {
+                    hpix[off] = make_int2(x, y);
+                    hview[off] = u.part;
+                    pfc[off] = acc;
+                }
+        });
+        for (size_t ci = 0; ci < cks.size(); ci++) hpf[pf_off[ci] + cks[ci].pixels] = cks[ci].samples;
+        ctx->pixels.reserve(n);
+        ctx->pix_view.reserve(n);
+        ctx->pix_first.reserve(n_pf);
+        if (n > 0) {
+            HIPX(hipMemcpyAsync(ctx->pixels.p, hpix, n * sizeof(int2), hipMemcpyHostToDevice, s));
+            HIPX(hipMemcpyAsync(ctx->pix_view.p, hview, n * sizeof(int), hipMemcpyHostToDevice, s));
+        }
+        if (n_pf > 0) HIPX(hipMemcpyAsync(ctx->pix_first.p, hpf, n_pf * sizeof(long long), hipMemcpyHostToDevice, s));
+    }
     ctx->tile_first.upload(tile_first, s);
     ctx->dviews.upload(dv, s);
 
@@ -1341,19 +1387,20 @@ static void merge_view(const yar_view& v, const PartResult& r, yar_film_pixel* o
                     fo.weight_sum += p.weight_sum;
                 }
         }
-    });
-    if (out_film) std::memcpy(out_film, film.data(), film.size() * sizeof(yar_film_pixel));
-    if (out_image) {
-        for (size_t i = 0; i < film.size(); i++) {
-            const yar_film_pixel& fp = film[i];
-            float cr = 0, cg = 0, cb = 0;
-            if (fp.weight_sum != 0.f) {
-                const float inv = 1.f / fp.weight_sum;
-                cr = fp.color_sum[0] * inv; cg = fp.color_sum[1] * inv; cb = fp.color_sum[2] * inv;
+        if (out_film)
+            std::memcpy(out_film + (size_t)(ry0 - v.region_y0) * rw, film.data() + (size_t)(ry0 - v.region_y0) * rw,
+                        (size_t)(ry1 - ry0) * rw * sizeof(yar_film_pixel));
+        if (out_image)
+            for (size_t i = (size_t)(ry0 - v.region_y0) * rw; i < (size_t)(ry1 - v.region_y0) * rw; i++) {
+                const yar_film_pixel& fp = film[i];
+                float cr = 0, cg = 0, cb = 0;
+                if (fp.weight_sum != 0.f) {
+                    const float inv = 1.f / fp.weight_sum;
+                    cr = fp.color_sum[0] * inv; cg = fp.color_sum[1] * inv; cb = fp.color_sum[2] * inv;
+                }
+                out_image[3 * i] = std::max(0.f, cr); out_image[3 * i + 1] = std::max(0.f, cg); out_image[3 * i + 2] = std::max(0.f, cb);
             }
-            out_image[3 * i] = std::max(0.f, cr); out_image[3 * i + 1] = std::max(0.f, cg); out_image[3 * i + 2] = std::max(0.f, cb);
-        }
-    }
+    });
     if (out_variance) {
         const int spp = v.x_pixel_sample_count * v.y_pixel_sample_count;
         double acc = 0.0;
