@@ -208,6 +208,7 @@ struct yar_gpu_ctx {
     size_t h_film_n = 0;
     char* h_stage = nullptr;              // pinned: pixel list, pixel views and spp prefix sums sent per job
     size_t h_stage_n = 0;
+    std::vector<yar_film_pixel> merge_work;  // one view's merged film, kept to avoid fresh pages per render
     // bounce-loop graph (captured per job shape)
     hipGraphExec_t graph = nullptr;
     long long graph_paths = -1;
@@ -1365,17 +1366,20 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
 
 // Film::merge_tile of every tile in index order, then Film::get_image (film.cpp:88-118), and the
 // variance estimate of render_scene (reference_renderer.cpp:566-576).
+// `film` is the caller's reusable work buffer: each band clears its own rows before merging.
 static void merge_view(const yar_view& v, const PartResult& r, yar_film_pixel* out_film, float* out_image,
-                       double* out_variance) {
+                       double* out_variance, std::vector<yar_film_pixel>& film) {
     const int n = (int)r.tiles.size();
     const int rw = v.region_x1 - v.region_x0, rh = v.region_y1 - v.region_y0;
-    std::vector<yar_film_pixel> film((size_t)rw * rh, yar_film_pixel{{0, 0, 0}, 0});
+    if (film.size() < (size_t)rw * rh) film.resize((size_t)rw * rh);
     std::vector<TileGeom> geo((size_t)n);
     for (int t = 0; t < n; t++) geo[(size_t)t] = tile_geom(v, t);
     // bands of 16 rows in parallel; within a band every pixel still takes the tiles in index order
     const int bands = (rh + 15) / 16;
     parallel_for((size_t)bands, [&](size_t b) {
         const int ry0 = v.region_y0 + (int)b * 16, ry1 = std::min(v.region_y1, ry0 + 16);
+        std::fill(film.begin() + (size_t)(ry0 - v.region_y0) * rw, film.begin() + (size_t)(ry1 - v.region_y0) * rw,
+                  yar_film_pixel{{0, 0, 0}, 0});
         for (int t = 0; t < n; t++) {
             const TileGeom& g = geo[(size_t)t];
             const int pw = g.px1 - g.px0;
@@ -1592,7 +1596,7 @@ int yar_gpu_render(yar_gpu_ctx* ctx, int32_t view, yar_film_pixel* out_film, flo
         yar_render_stats st{};
         HIPX(hipSetDevice(ctx->device));
         render_job(ctx, parts, rr, st);
-        merge_view(ctx->views[view], rr[0], out_film, out_image, out_variance);
+        merge_view(ctx->views[view], rr[0], out_film, out_image, out_variance, ctx->merge_work);
         if (out_stats) *out_stats = st;
     });
 }
@@ -1650,7 +1654,7 @@ int yar_gpu_render_checkpointed(yar_gpu_ctx* ctx, int32_t view, const char* chec
             total.any_rays += st.any_rays;
             total.gpu_seconds += st.gpu_seconds;
         }
-        merge_view(v, all, out_film, out_image, out_variance);
+        merge_view(v, all, out_film, out_image, out_variance, ctx->merge_work);
         const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         total.render_seconds = elapsed;
         total.passes = 1;
@@ -1676,7 +1680,7 @@ int yar_gpu_render_batch(yar_gpu_ctx* ctx, const int32_t* views, int32_t count, 
         render_job(ctx, parts, rr, st);
         for (int i = 0; i < count; i++)
             merge_view(ctx->views[views[i]], rr[i], nullptr, out_images ? out_images[i] : nullptr,
-                       out_variances ? &out_variances[i] : nullptr);
+                       out_variances ? &out_variances[i] : nullptr, ctx->merge_work);
         if (out_stats) *out_stats = st;
     });
 }
@@ -1703,7 +1707,7 @@ int yar_gpu_render_guided(yar_gpu_ctx* ctx, yar_guide* guide, const int32_t* vie
         render_job(ctx, parts, rr, st, &gr);
         for (int i = 0; i < count; i++)
             merge_view(ctx->views[views[i]], rr[i], nullptr, out_images ? out_images[i] : nullptr,
-                       out_variances ? &out_variances[i] : nullptr);
+                       out_variances ? &out_variances[i] : nullptr, ctx->merge_work);
         if (out_stats) *out_stats = st;
     });
 }
