@@ -329,9 +329,12 @@ __global__ __launch_bounds__(64) void k_sample_perms_large(const Chunk* __restri
 
 // Shuffle permutations: one work item per (pixel, 1D or 2D dimension) applies the run's Fisher-Yates
 // swaps (math.h:138-144, the swap draws follow the run's value draws) to an index array and writes
-// the spp resulting stratum indices.  STAGED: the index arrays live in LDS ([entry][thread]) and the
-// block's runs, adjacent in the table, are written as one contiguous range; otherwise the swaps run in
-// place on the output (large spp).
+// the spp resulting stratum indices.  STAGED: the index arrays live in LDS as entries of type T,
+// 4 / sizeof(T) consecutive entries of one lane per 32-bit word and words laid out [word][thread], so
+// a wave's accesses fall in distinct banks while a 64-entry byte run takes 4 KB per block instead of
+// 16 KB (more resident waves to hide the serial draw-and-swap chain); the block's runs, adjacent in
+// the table, are written as one contiguous range.  Otherwise the swaps run in place on the output
+// (large spp).
 template <class T, bool STAGED>
 __global__ void k_sample_perms(const Chunk* __restrict__ cp, int n_dims) {
     extern __shared__ uint32_t perm_lds[];
@@ -355,13 +358,17 @@ __global__ void k_sample_perms(const Chunk* __restrict__ cp, int n_dims) {
                                         : dim2_base((uint32_t)spp, (uint32_t)c.d1, (uint32_t)(i - c.d1)) + 2u * (uint32_t)spp;
         uint64_t st = pcg_jump(c.jt, s0, inc, first);
         if (STAGED) {
-            uint32_t* a = perm_lds + threadIdx.x;
-            for (int k = 0; k < spp; k++) a[k * nt] = (uint32_t)k;
+            constexpr int E = 4 / (int)sizeof(T);
+            T* a = reinterpret_cast<T*>(perm_lds + threadIdx.x);
+            const int row = nt * E;                 // entry k at a[(k / E) * row + k % E]
+            for (int k = 0; k < spp; k++) a[(k / E) * row + (k % E)] = (T)k;
             for (int n = spp; n > 1; n--) {
                 const int k = (int)(((uint64_t)pcg_next(st, inc) * (uint64_t)n) >> 32);
-                const uint32_t x = a[(n - 1) * nt];
-                a[(n - 1) * nt] = a[k * nt];
-                a[k * nt] = x;
+                T* an = a + ((n - 1) / E) * row + ((n - 1) % E);
+                T* ak = a + (k / E) * row + (k % E);
+                const T x = *an;
+                *an = *ak;
+                *ak = x;
             }
         } else {
             T* a = out + off;
@@ -376,13 +383,12 @@ __global__ void k_sample_perms(const Chunk* __restrict__ cp, int n_dims) {
     }
     if (!STAGED) return;
     if (c.uniform_spp > 0 && (c.uniform_spp & 3) == 0 && sizeof(T) == 1) {
-        // every run of the chunk has the same 4-aligned length: each lane stores its own run as
-        // packed 32-bit words, reading its own LDS column (one bank per lane, no conflicts)
+        // every run of the chunk has the same 4-aligned length: each lane copies its own run's LDS
+        // words, which already hold four consecutive entries each, to the table
         if (g < total) {
             const uint32_t* a = perm_lds + threadIdx.x;
             uint32_t* o = reinterpret_cast<uint32_t*>(out + off);
-            for (int k = 0; k < spp; k += 4)
-                o[k >> 2] = a[k * nt] | (a[(k + 1) * nt] << 8) | (a[(k + 2) * nt] << 16) | (a[(k + 3) * nt] << 24);
+            for (int w = 0; w < (spp >> 2); w++) o[w] = a[w * nt];
         }
         return;
     }
@@ -410,7 +416,8 @@ __global__ void k_sample_perms(const Chunk* __restrict__ cp, int n_dims) {
             t = lo;
             k = (int)(base + j - run_off[t]);
         }
-        out[base + j] = (T)perm_lds[k * nt + t];
+        constexpr int E = 4 / (int)sizeof(T);
+        out[base + j] = reinterpret_cast<const T*>(perm_lds)[((k / E) * nt + t) * E + (k % E)];
     }
 }
 
@@ -423,7 +430,8 @@ void launch_sample_perms(const Chunk& hc, const Chunk* dc, hipStream_t s) {
     const long long items = (long long)hc.n_pixels * n_dims;
     const unsigned blocks = (unsigned)((items + kPermBlock - 1) / kPermBlock);
     if (hc.max_spp <= 128) {
-        const size_t shm = (size_t)kPermBlock * hc.max_spp * sizeof(uint32_t);
+        const int per_word = hc.perm_bytes == 1 ? 4 : 2;
+        const size_t shm = (size_t)kPermBlock * ((hc.max_spp + per_word - 1) / per_word) * sizeof(uint32_t);
         if (hc.perm_bytes == 1) hipLaunchKernelGGL((k_sample_perms<uint8_t, true>), dim3(blocks), dim3(kPermBlock), shm, s, dc, n_dims);
         else hipLaunchKernelGGL((k_sample_perms<uint16_t, true>), dim3(blocks), dim3(kPermBlock), shm, s, dc, n_dims);
         return;
