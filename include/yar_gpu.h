@@ -240,7 +240,7 @@ typedef struct yar_gpu_ctx yar_gpu_ctx;
 
 typedef struct yar_gpu_options {
     int32_t device;                     /* HIP device ordinal */
-    int32_t block_size;                 /* traversal threads per block, 0 = 64 */
+    int32_t block_size;                 /* traversal threads per block: 0 (= 64), 64, 128, 192 or 256 */
     int64_t max_paths_in_flight;        /* 0 = default: 64M plain, 16M guided, at most a quarter of free HBM */
     int32_t traversal_stack_depth;      /* short-stack entries per tree level, 0 = 8 (kd-restart beyond) */
     int32_t enable_counters;            /* count node visits / triangle tests per ray */

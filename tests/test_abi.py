@@ -61,3 +61,15 @@ def test_generator_is_deterministic(scene_dir):
     c = yar_amd.generate_scene(1, str(scene_dir / "c.pbrt"), seed=6)
     assert open(a, "rb").read() == open(b, "rb").read()
     assert open(a, "rb").read() != open(c, "rb").read()
+
+
+def test_traversal_block_size_is_checked_before_any_device_call(scene_dir):
+    p = yar_amd.generate_scene(1, str(scene_dir / "blk.pbrt"), seed=3)
+    s = yar_amd.Scene(p)
+    for bad in (32, 96, 512, 1024):
+        try:
+            yar_amd.Context(s, block_size=bad)
+        except Exception as e:
+            assert "block_size" in str(e)
+        else:
+            raise AssertionError(f"block_size {bad} accepted")

@@ -265,7 +265,7 @@ struct yar_gpu_ctx {
         lc.priv_stack = opt.stack_mode == 2;
         lc.refill = opt.lane_refill == 1;
         lc.counters = opt.enable_counters != 0;
-        lc.trav_grid = 1;
+        lc.trav_grid = lc.shadow_grid = lc.mis_grid = 1;
         lc.path_grid = 1;
         lc.stream = stream;
         lc.guided = false;
@@ -1092,7 +1092,9 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         f.m2 = ctx->m2.p;
     }
     LaunchCfg lc = ctx->launch_cfg();
-    lc.trav_grid = ctx->cu_count * traversal_blocks_per_cu(lc);
+    lc.trav_grid = ctx->cu_count * traversal_blocks_per_cu(lc, 0);
+    lc.shadow_grid = ctx->cu_count * traversal_blocks_per_cu(lc, 1);
+    lc.mis_grid = ctx->cu_count * traversal_blocks_per_cu(lc, 2);
     lc.path_grid = (int)std::max<int64_t>(1, std::min<int64_t>((max_chunk_paths + 255) / 256, (int64_t)ctx->cu_count * 8));
     if (guided) {
         lc.guided = true;
@@ -1484,6 +1486,8 @@ int yar_gpu_create(const yar_scene_desc* scene, const yar_gpu_options* options, 
     const int rc = guarded(nullptr, "yar_gpu_create", [&] {
         ctx.reset(new yar_gpu_ctx);
         if (options) ctx->opt = *options;
+        if (ctx->opt.block_size != 0 && (ctx->opt.block_size < 64 || ctx->opt.block_size > 256 || ctx->opt.block_size % 64 != 0))
+            throw HipError{"block_size must be 0 (64), 64, 128, 192 or 256", YAR_ERR_INVALID_ARGUMENT};
         ctx->device = ctx->opt.device;
         ctx->cfg = scene->config;
         int ndev = 0;

@@ -130,7 +130,8 @@ struct LaunchCfg {
     bool priv_stack;          // stacks in private (scratch) memory instead of LDS
     bool refill;              // per-lane refill in the persistent traversal kernels
     bool counters;
-    int trav_grid;            // persistent traversal grid (blocks)
+    int trav_grid;            // persistent traversal grids (blocks): k_extend, k_shadow, k_mis
+    int shadow_grid, mis_grid;
     int path_grid;            // grid-stride grid of per-path kernels (blocks of 256)
     hipStream_t stream;
     bool guided;              // launch the guide stages
@@ -150,6 +151,6 @@ void launch_iteration(const DevScene& sc, const Chunk* c, const Paths& p, Queues
 void launch_film(const Chunk* c, int n_pixels, const Paths& p, FilmParams f, Queues q, hipStream_t s);
 void launch_trace_rays(const DevScene& sc, const float4* rays, int64_t n, int any, float4* out_tb, int2* out_ot,
                        unsigned long long* totals, const LaunchCfg& cfg);
-int traversal_blocks_per_cu(const LaunchCfg& cfg);
+int traversal_blocks_per_cu(const LaunchCfg& cfg, int kernel);
 
 }  // namespace yg

@@ -644,8 +644,14 @@ YG_INLINE void trace_queue(const DevScene& sc, uint32_t n, uint32_t* cursor, SSt
     }
 }
 
+// The closest-hit and shadow traversals are latency bound: registers capped for five waves per SIMD
+// (96 VGPRs; four waves at the uncapped 105) measured faster on config 2 (k_shadow 28.4 -> 27.0 ms
+// per step with no spills, k_extend 74.9 -> 74.3 ms with 36 B of spills); k_mis keeps its registers
+// (40 B of spills there cost more than the wave gains).  Blocks of at most 256 threads.
+#define YG_TRAV_BOUNDS __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+
 template <bool COUNT, bool PRIV>
-__global__ void k_extend(DevScene sc, Paths P, Queues q, int parity, StackCfg cfg) {
+__global__ void YG_TRAV_BOUNDS k_extend(DevScene sc, Paths P, Queues q, int parity, StackCfg cfg) {
     const uint32_t* in = parity ? q.active1 : q.active0;
     const uint32_t n = q.counts[parity];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -672,7 +678,7 @@ __global__ void k_extend(DevScene sc, Paths P, Queues q, int parity, StackCfg cf
 }
 
 template <bool COUNT, bool PRIV>
-__global__ void k_shadow(DevScene sc, Paths P, Queues q, StackCfg cfg) {
+__global__ void YG_TRAV_BOUNDS k_shadow(DevScene sc, Paths P, Queues q, StackCfg cfg) {
     const uint32_t n = q.counts[2];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&q.totals[17], (unsigned long long)n);
     TravCounters cnt = {0, 0, 0, 0};
@@ -693,7 +699,7 @@ __global__ void k_shadow(DevScene sc, Paths P, Queues q, StackCfg cfg) {
 }
 
 template <bool COUNT, bool PRIV>
-__global__ void k_mis(DevScene sc, Paths P, Queues q, StackCfg cfg) {
+__global__ void __launch_bounds__(256) k_mis(DevScene sc, Paths P, Queues q, StackCfg cfg) {
     const uint32_t n = q.counts[3];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&q.totals[18], (unsigned long long)n);
     TravCounters cnt = {0, 0, 0, 0};
@@ -1546,12 +1552,17 @@ static size_t stack_lds(const LaunchCfg& cfg) {
         }                                                                                                  \
     } while (0)
 
-int traversal_blocks_per_cu(const LaunchCfg& cfg) {
+// Resident blocks per CU of a persistent traversal kernel (0 = k_extend, 1 = k_shadow, 2 = k_mis).
+int traversal_blocks_per_cu(const LaunchCfg& cfg, int kernel) {
     int nb = 0;
     const size_t shm = stack_lds(cfg);
     hipError_t e;
-    if (cfg.priv_stack) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_extend<false, true>), cfg.block, shm);
-    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (k_extend<false, false>), cfg.block, shm);
+#define YG_OCC(K) (cfg.priv_stack ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (K<false, true>), cfg.block, shm) \
+                                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (K<false, false>), cfg.block, shm))
+    if (kernel == 1) e = YG_OCC(k_shadow);
+    else if (kernel == 2) e = YG_OCC(k_mis);
+    else e = YG_OCC(k_extend);
+#undef YG_OCC
     if (e != hipSuccess || nb <= 0) nb = 1;
     return nb;
 }
@@ -1574,8 +1585,8 @@ void launch_stage(int stage, const DevScene& sc, const Chunk* c, const Paths& p,
         else if (sc.has_mix) hipLaunchKernelGGL((k_shade<false, true>), dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q, parity);
         else hipLaunchKernelGGL((k_shade<false, false>), dim3(cfg.path_grid), dim3(256), 0, cfg.stream, sc, c, p, q, parity);
         break;
-    case kStageShadow: YG_LAUNCH_TRAV(k_shadow, cfg.trav_grid, sc, p, q); break;
-    case kStageMis: YG_LAUNCH_TRAV(k_mis, cfg.trav_grid, sc, p, q); break;
+    case kStageShadow: YG_LAUNCH_TRAV(k_shadow, cfg.shadow_grid, sc, p, q); break;
+    case kStageMis: YG_LAUNCH_TRAV(k_mis, cfg.mis_grid, sc, p, q); break;
     case kStageGuideForward:
         if (cfg.guided) guide_forward_dev(cfg.net, q.guide_in, 16, &q.counts[9], q.guide_out, kGOut, cfg.guide_grid, cfg.stream);
         break;
