@@ -14,6 +14,7 @@
 #include <mutex>
 #include <cstdio>
 #include <string>
+#include <exception>
 #include <thread>
 #include <vector>
 
@@ -54,22 +55,38 @@ struct DevBuf {
 
 static int pow2_at_least(int v) { int p = 1; while (p < v) p <<= 1; return p; }
 
-// body(i) for i in [0, n) on up to 16 host threads, contiguous ranges per thread
+// body(i) for i in [0, n) on up to 16 host threads, contiguous ranges per thread, each thread taking
+// at least `grain` items (starting a thread costs about as much as a few 64x64 tiles of work, so
+// small jobs stay on the calling thread)
 template <class Body>
-static void parallel_for(size_t n, Body&& body) {
-    const size_t nt = std::min<size_t>({n, (size_t)16, (size_t)std::max(1u, std::thread::hardware_concurrency())});
+static void parallel_for(size_t n, Body&& body, size_t grain = 4) {
+    const size_t nt = std::min<size_t>({n / std::max<size_t>(grain, 1), (size_t)16,
+                                        (size_t)std::max(1u, std::thread::hardware_concurrency())});
     if (nt <= 1) {
         for (size_t i = 0; i < n; i++) body(i);
         return;
     }
-    std::vector<std::thread> pool;
+    // an exception in a worker (std::bad_alloc) is carried to the calling thread, not std::terminate
+    std::vector<std::exception_ptr> err(nt);
     auto range = [&](size_t t) {
-        const size_t a = n * t / nt, b = n * (t + 1) / nt;
-        for (size_t i = a; i < b; i++) body(i);
+        try {
+            const size_t a = n * t / nt, b = n * (t + 1) / nt;
+            for (size_t i = a; i < b; i++) body(i);
+        } catch (...) {
+            err[t] = std::current_exception();
+        }
     };
-    for (size_t t = 1; t < nt; t++) pool.emplace_back(range, t);
+    std::vector<std::thread> pool;
+    pool.reserve(nt - 1);
+    try {
+        for (size_t t = 1; t < nt; t++) pool.emplace_back(range, t);
+    } catch (...) {                          // no more threads: the ranges not started run here
+    }
+    for (size_t t = 1 + pool.size(); t < nt; t++) range(t);
     range(0);
     for (auto& th : pool) th.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
 }
 
 // Per-kernel HIP-event timing (kind: 0 extend, 1 shade, 2 shadow, 3 mis, 4 other, 5 guide network
@@ -1286,7 +1303,7 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
             if (A.x != B.x) return A.x < B.x;
             return A.z < B.z;
         });
-    });
+    }, 64);
 
     // Assemble Film_Tiles: in-tile values from the device gather, margins from the side list.
     out.assign(parts.size(), {});
