@@ -53,6 +53,36 @@ struct DevBuf {
     void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
 };
 
+// Host staging kept across jobs: pinned memory, so copies run at link speed, or pageable memory
+// when no pinned memory of that size can be had (the copies then stage through the driver).
+struct HostBuf {
+    char* p = nullptr;
+    size_t n = 0;
+    bool pinned = false;
+    std::vector<char> pageable;
+    void reserve(size_t bytes) {
+        if (bytes <= n) return;
+        release();
+        if (hipHostMalloc((void**)&p, bytes, hipHostMallocDefault) == hipSuccess) {
+            pinned = true;
+        } else {
+            (void)hipGetLastError();          // clear the failed call's error state
+            p = nullptr;
+            pageable.resize(bytes);
+            p = pageable.data();
+        }
+        n = bytes;
+    }
+    void release() {
+        if (pinned && p) (void)hipHostFree(p);
+        std::vector<char>().swap(pageable);
+        p = nullptr;
+        n = 0;
+        pinned = false;
+    }
+    ~HostBuf() { release(); }
+};
+
 static int pow2_at_least(int v) { int p = 1; while (p < v) p <<= 1; return p; }
 
 // body(i) for i in [0, n) on up to 16 host threads, contiguous ranges per thread, each thread taking
@@ -220,11 +250,8 @@ struct yar_gpu_ctx {
     DevBuf<double> m1, m2;
     uint32_t* h_poll = nullptr;           // pinned: active counts read back while the loop runs
     size_t h_poll_n = 0;
-    float4* h_film = nullptr;             // pinned: per-pixel film sums and variances read back per job
-    double* h_var = nullptr;
-    size_t h_film_n = 0;
-    char* h_stage = nullptr;              // pinned: pixel list, pixel views and spp prefix sums sent per job
-    size_t h_stage_n = 0;
+    HostBuf h_film, h_var;                // per-pixel film sums and variances read back per job
+    HostBuf h_stage;                      // pixel list, pixel views and spp prefix sums sent per job
     std::vector<yar_film_pixel> merge_work;  // one view's merged film, kept to avoid fresh pages per render
     // bounce-loop graph (captured per job shape)
     hipGraphExec_t graph = nullptr;
@@ -268,9 +295,9 @@ struct yar_gpu_ctx {
         grng.release(); rec.release(); guide_in.release(); guide_out.release(); guide_path.release(); train.release();
         m1.release(); m2.release();
         if (h_poll) (void)hipHostFree(h_poll);
-        if (h_film) (void)hipHostFree(h_film);
-        if (h_var) (void)hipHostFree(h_var);
-        if (h_stage) (void)hipHostFree(h_stage);
+        h_film.release();
+        h_var.release();
+        h_stage.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -965,16 +992,10 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         const size_t o_pf = (o_view + n * sizeof(int) + 255) & ~(size_t)255;
         const size_t bytes = o_pf + n_pf * sizeof(long long);
         HIPX(hipStreamSynchronize(s));                    // the previous job's sends have left the staging
-        if (bytes > ctx->h_stage_n) {
-            if (ctx->h_stage) HIPX(hipHostFree(ctx->h_stage));
-            ctx->h_stage = nullptr;
-            ctx->h_stage_n = 0;
-            HIPX(hipHostMalloc((void**)&ctx->h_stage, bytes, hipHostMallocDefault));
-            ctx->h_stage_n = bytes;
-        }
-        int2* hpix = (int2*)ctx->h_stage;
-        int* hview = (int*)(ctx->h_stage + o_view);
-        long long* hpf = (long long*)(ctx->h_stage + o_pf);
+        ctx->h_stage.reserve(bytes);
+        int2* hpix = (int2*)ctx->h_stage.p;
+        int* hview = (int*)(ctx->h_stage.p + o_view);
+        long long* hpf = (long long*)(ctx->h_stage.p + o_pf);
         // each unit's chunk and the prefix sum at its first pixel
         std::vector<size_t> u_chunk(units.size());
         std::vector<long long> u_acc(units.size());
@@ -1244,21 +1265,13 @@ static void render_job(yar_gpu_ctx* ctx, const std::vector<JobPart>& parts, std:
         }
     }
     HIPX(hipEventRecord(e1, s));
-    if ((size_t)total_pixels > ctx->h_film_n) {         // pinned, so the read-back runs at link speed
-        if (ctx->h_film) HIPX(hipHostFree(ctx->h_film));
-        if (ctx->h_var) HIPX(hipHostFree(ctx->h_var));
-        ctx->h_film = nullptr;
-        ctx->h_var = nullptr;
-        ctx->h_film_n = 0;
-        HIPX(hipHostMalloc((void**)&ctx->h_film, (size_t)total_pixels * sizeof(float4), hipHostMallocDefault));
-        HIPX(hipHostMalloc((void**)&ctx->h_var, (size_t)total_pixels * sizeof(double), hipHostMallocDefault));
-        ctx->h_film_n = (size_t)total_pixels;
-    }
-    const float4* film_own = ctx->h_film;
-    const double* var_own = ctx->h_var;
+    ctx->h_film.reserve((size_t)total_pixels * sizeof(float4));
+    ctx->h_var.reserve((size_t)total_pixels * sizeof(double));
+    const float4* film_own = (const float4*)ctx->h_film.p;
+    const double* var_own = (const double*)ctx->h_var.p;
     if (total_pixels > 0) {
-        HIPX(hipMemcpyAsync(ctx->h_film, ctx->out_film.p, total_pixels * sizeof(float4), hipMemcpyDeviceToHost, s));
-        HIPX(hipMemcpyAsync(ctx->h_var, ctx->out_var.p, total_pixels * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPX(hipMemcpyAsync(ctx->h_film.p, ctx->out_film.p, total_pixels * sizeof(float4), hipMemcpyDeviceToHost, s));
+        HIPX(hipMemcpyAsync(ctx->h_var.p, ctx->out_var.p, total_pixels * sizeof(double), hipMemcpyDeviceToHost, s));
     }
     unsigned long long tot[24];
     uint32_t cnt[16];
