@@ -83,12 +83,13 @@ def test_render_tiles_subset_matches_full(c1_path):
 
 
 def test_graph_replay_and_launch_options_equal_eager_c2(c2_small_path):
-    """hipGraph replay, chunk size, per-lane refill and stack storage change no bit of the image."""
+    """hipGraph replay, chunk size, per-lane refill, stack storage and traversal block size (three- and
+    two-wave blocks on the per-kernel occupancy grids) change no bit of the image."""
     scene = yar_amd.Scene(c2_small_path)
     region = (432, 432, 592, 592)
     res = []
     for opts in ({}, {"use_graph": 1}, {"use_graph": 1, "max_paths_in_flight": 20000}, {"lane_refill": 1},
-                 {"lane_refill": 1, "stack_mode": 2, "block_size": 256}):
+                 {"lane_refill": 1, "stack_mode": 2, "block_size": 256}, {"block_size": 192}, {"block_size": 128}):
         ctx = yar_amd.Context(scene, **opts)
         v = ctx.view(0)
         v.x_pixel_sample_count = v.y_pixel_sample_count = 2
