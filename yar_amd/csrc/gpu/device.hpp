@@ -316,7 +316,7 @@ struct Tracer {
     // mesh level
     bool in_mesh;
     uint32_t mroot, mnode, oi;
-    float mtmin, mtmax, mt1, mrestart, mesh_tmax;
+    float mtmin, mtmax, mt1, mrestart;
     uint32_t mss;                   // packed short-stack state
     V3 lo, ld, linv;
     uint32_t kxyz;                  // watertight axis permutation kx | ky << 2 | kz << 4
@@ -397,7 +397,6 @@ struct Tracer {
             kxyz = (uint32_t)kx | ((uint32_t)ky << 2) | ((uint32_t)kz << 4);
             const float dx = ld[kx], dy = ld[ky], dz = ld[kz];
             sx = -dx / dz; sy = -dy / dz; sz = linv[kz];           // linv[kz] is 1.f / dz
-            mesh_tmax = hit.t;
             mnode = mroot; mtmin = mt0; mtmax = mt1; mrestart = -1.f;
             mss = 0u;
             in_mesh = true;
@@ -435,14 +434,13 @@ struct Tracer {
                     return true;
                 } else if (t < hit.t) {
                     if (degenerate(p0, p1, p2)) continue;
-                    hit.t = t; hit.tri = ti; hit.b = b;
+                    hit.t = t; hit.tri = ti; hit.b = b; hit.obj = (int)oi;
                 }
             }
             done = !kd_next(mnode, mroot, mtmin, mtmax, mt1, mst, mdepth - 1, mss, mrestart);
         }
         if (done) {
             in_mesh = false;
-            if (!ANY && hit.t < mesh_tmax) hit.obj = (int)oi;
         }
         return false;
     }
