@@ -646,7 +646,7 @@ YG_INLINE void trace_queue(const DevScene& sc, uint32_t n, uint32_t* cursor, SSt
 
 // The closest-hit and shadow traversals are latency bound: registers capped for five waves per SIMD
 // (96 VGPRs; four waves at the uncapped 105) measured faster on config 2 (k_shadow 28.4 -> 27.0 ms
-// per step with no spills, k_extend 74.9 -> 74.3 ms with 36 B of spills); k_mis keeps its registers
+// per step with no spills, k_extend 74.9 -> 74.3 ms with 28-36 B of spills); k_mis keeps its registers
 // (40 B of spills there cost more than the wave gains).  Blocks of at most 256 threads.
 #define YG_TRAV_BOUNDS __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 
